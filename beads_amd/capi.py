@@ -1,0 +1,267 @@
+"""ctypes view of the C ABI in include/beads_amd.h.
+
+``Engine`` drives one run through the engine-handle entry points
+(create / initialize / advance / read back) so tests and the bench can step a
+run and inspect neighbour lists, cells, forces and state between steps. The
+library must be built; nothing here falls back to host code.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .workloads import Workload
+
+_LIB_PATH = Path(__file__).resolve().parent / "libbeads_amd.so"
+
+STATUS_NAMES = {0: "OK", 1: "VALUE", 2: "TYPE", 3: "OVERFLOW", 4: "INDEX", 5: "RUNTIME",
+                6: "INTERRUPTED", 7: "DEVICE"}
+
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class Spec(C.Structure):
+    _fields_ = [
+        ("units", C.c_int32), ("n_particles", C.c_uint32),
+        ("box_bound", _f32p), ("positions", _f32p), ("velocities", _f32p), ("masses", _f32p),
+        ("types", _i32p), ("tags", _u32p), ("molecule_ids", _u32p), ("images", _i32p),
+        ("n_bonds", C.c_uint32), ("bonds", _u32p), ("n_angles", C.c_uint32), ("angles", _u32p),
+        ("n_dihedrals", C.c_uint32), ("dihedrals", _u32p),
+        ("n_types", C.c_int32), ("lj_cutoff", C.c_float), ("lj_shift", C.c_int32),
+        ("lj_epsilon", _f32p), ("lj_sigma", _f32p),
+        ("n_bond_types", C.c_int32), ("bond_k", _f32p), ("bond_r0", _f32p),
+        ("n_angle_types", C.c_int32), ("angle_k", _f32p), ("angle_theta0_deg", _f32p),
+        ("n_dihedral_types", C.c_int32), ("dihedral_k", _f32p), ("dihedral_d", _i32p),
+        ("dihedral_n", _i32p), ("exclusion_distance", C.c_int32),
+        ("dt", C.c_float), ("berendsen", C.c_int32), ("berendsen_temperature", C.c_float),
+        ("berendsen_tau", C.c_float),
+        ("cutoff_buffer", C.c_float), ("rebuild_check_every", C.c_uint64),
+        ("sort_every_rebuild", C.c_uint64), ("max_neighbors", C.c_uint32),
+        ("thermo_path", C.c_char_p), ("thermo_every", C.c_uint64),
+        ("trajectory_path", C.c_char_p), ("trajectory_every", C.c_uint64),
+        ("trajectory_fields", C.c_uint32), ("final_state_path", C.c_char_p),
+        ("log_echo", C.c_int32), ("log_path", C.c_char_p),
+        ("runsteps", C.c_uint64),
+        ("interrupt", C.c_void_p), ("interrupt_context", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("step", C.c_uint64), ("build_count", C.c_uint64), ("dangerous_rebuild_count", C.c_uint64),
+        ("reorder_count", C.c_uint64), ("overflow", C.c_int32), ("path", C.c_int32),
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("max_neighbors", C.c_uint32),
+        ("mean_neighbors", C.c_double), ("ke", C.c_double), ("lambda_", C.c_double),
+        ("pe", C.c_double * 4), ("virial", C.c_double * 4), ("temperature", C.c_double),
+        ("pressure", C.c_double),
+    ]
+
+
+class StateOut(C.Structure):
+    _fields_ = [("positions", _f32p), ("velocities", _f32p), ("forces", _f32p), ("images", _i32p),
+                ("tags", _u32p), ("types", _i32p), ("molecule_ids", _u32p), ("masses", _f32p)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("total_s", C.c_double), ("integrate_s", C.c_double),
+                ("rebuild_s", C.c_double), ("force_s", C.c_double),
+                ("force_launches", C.c_uint64)]
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [(k, C.c_char_p) for k in ("real_precision", "real_dtype", "type_dtype",
+                                          "index_dtype", "image_dtype", "runstep_dtype",
+                                          "device_arch")]
+
+
+# Every symbol include/beads_amd.h declares, with its ctypes signature.
+SIGNATURES = {
+    "beads_last_error": (C.c_char_p, []),
+    "beads_abi_version": (C.c_int, []),
+    "beads_build_info": (C.c_int, [C.POINTER(BuildInfo)]),
+    "beads_execute_simulation": (C.c_int, [C.POINTER(Spec)]),
+    "beads_engine_create": (C.c_int, [C.POINTER(Spec), C.POINTER(C.c_void_p)]),
+    "beads_engine_initialize": (C.c_int, [C.c_void_p, C.c_int32]),
+    "beads_engine_advance": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32]),
+    "beads_engine_synchronize": (C.c_int, [C.c_void_p]),
+    "beads_engine_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "beads_engine_read_state": (C.c_int, [C.c_void_p, C.POINTER(StateOut)]),
+    "beads_engine_read_neighbors": (C.c_int, [C.c_void_p, _u32p, _u32p, C.c_uint64]),
+    "beads_engine_read_cells": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p, _u32p, C.c_uint64]),
+    "beads_engine_profile_advance": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32,
+                                               C.POINTER(PhaseTimes)]),
+    "beads_engine_destroy": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libbeads_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(f"{_LIB_PATH} is not built; run __graft_entry__.build().")
+        lib = C.CDLL(os.fspath(_LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class BeadsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{STATUS_NAMES.get(status, status)}] {message}")
+        self.status = status
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise BeadsError(status, library().beads_last_error().decode())
+
+
+def _ptr(arr: np.ndarray, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_spec(w: Workload) -> tuple[Spec, list]:
+    """Fill a Spec from a Workload; the returned list keeps its arrays alive."""
+    keep = []
+
+    def arr(a, dtype):
+        out = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(out)
+        return out
+
+    s = Spec()
+    s.units = 0 if w.units == "reduced" else 1
+    s.n_particles = w.n
+    s.box_bound = _ptr(arr(w.box_bound, np.float32), C.c_float)
+    s.positions = _ptr(arr(w.positions, np.float32), C.c_float)
+    s.velocities = _ptr(arr(w.velocities, np.float32), C.c_float)
+    s.masses = _ptr(arr(w.masses, np.float32), C.c_float)
+    s.types = _ptr(arr(w.types, np.int32), C.c_int32)
+    s.tags = _ptr(arr(w.tags, np.uint32), C.c_uint32)
+    s.molecule_ids = _ptr(arr(w.molecule_ids, np.uint32), C.c_uint32)
+    s.images = _ptr(arr(w.images, np.int32), C.c_int32)
+    s.n_bonds = len(w.bonds)
+    s.bonds = _ptr(arr(w.bonds.reshape(-1, 3), np.uint32), C.c_uint32)
+    s.n_angles = len(w.angles)
+    s.angles = _ptr(arr(w.angles.reshape(-1, 4), np.uint32), C.c_uint32)
+    s.n_dihedrals = len(w.dihedrals)
+    s.dihedrals = _ptr(arr(w.dihedrals.reshape(-1, 5), np.uint32), C.c_uint32)
+    s.n_types = w.n_types
+    s.lj_cutoff = w.cutoff
+    s.lj_shift = 1 if w.shift else 0
+    s.lj_epsilon = _ptr(arr(w.lj_epsilon, np.float32), C.c_float)
+    s.lj_sigma = _ptr(arr(w.lj_sigma, np.float32), C.c_float)
+    s.n_bond_types = len(w.bond_k) if len(w.bonds) else 0
+    s.bond_k = _ptr(arr(w.bond_k, np.float32), C.c_float)
+    s.bond_r0 = _ptr(arr(w.bond_r0, np.float32), C.c_float)
+    s.n_angle_types = len(w.angle_k) if len(w.angles) else 0
+    s.angle_k = _ptr(arr(w.angle_k, np.float32), C.c_float)
+    s.angle_theta0_deg = _ptr(arr(w.angle_theta0, np.float32), C.c_float)
+    s.n_dihedral_types = len(w.dihedral_k) if len(w.dihedrals) else 0
+    s.dihedral_k = _ptr(arr(w.dihedral_k, np.float32), C.c_float)
+    s.dihedral_d = _ptr(arr(w.dihedral_d, np.int32), C.c_int32)
+    s.dihedral_n = _ptr(arr(w.dihedral_n, np.int32), C.c_int32)
+    s.exclusion_distance = int(w.exclusion_distance) if len(w.bonds) else 0
+    s.dt = w.dt
+    s.berendsen = 1 if w.berendsen else 0
+    s.berendsen_temperature = w.temperature
+    s.berendsen_tau = w.tau
+    s.cutoff_buffer = w.cutoff_buffer
+    s.rebuild_check_every = w.rebuild_check_every
+    s.sort_every_rebuild = w.sort_every_rebuild
+    s.max_neighbors = w.max_neighbors
+    s.log_echo = 3
+    s.runsteps = w.steps
+    return s, keep
+
+
+class Engine:
+    """One run on the device, stepped from Python."""
+
+    def __init__(self, workload: Workload):
+        self.lib = library()
+        self.workload = workload
+        self.n = workload.n
+        self._spec, self._keep = make_spec(workload)
+        handle = C.c_void_p()
+        _check(self.lib.beads_engine_create(C.byref(self._spec), C.byref(handle)))
+        self.handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.beads_engine_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self, observables: bool = True) -> None:
+        _check(self.lib.beads_engine_initialize(self.handle, 1 if observables else 0))
+
+    def advance(self, steps: int, observables_on_last: bool = False) -> None:
+        _check(self.lib.beads_engine_advance(self.handle, steps, 1 if observables_on_last else 0))
+
+    def synchronize(self) -> None:
+        _check(self.lib.beads_engine_synchronize(self.handle))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(self.lib.beads_engine_stats(self.handle, C.byref(s)))
+        return s
+
+    def state(self) -> dict[str, np.ndarray]:
+        n = self.n
+        out = {"positions": np.zeros((n, 3), np.float32), "velocities": np.zeros((n, 3), np.float32),
+               "forces": np.zeros((n, 3), np.float32), "images": np.zeros((n, 3), np.int32),
+               "tags": np.zeros(n, np.uint32), "types": np.zeros(n, np.int32),
+               "molecule_ids": np.zeros(n, np.uint32), "masses": np.zeros(n, np.float32)}
+        so = StateOut(_ptr(out["positions"], C.c_float), _ptr(out["velocities"], C.c_float),
+                      _ptr(out["forces"], C.c_float), _ptr(out["images"], C.c_int32),
+                      _ptr(out["tags"], C.c_uint32), _ptr(out["types"], C.c_int32),
+                      _ptr(out["molecule_ids"], C.c_uint32), _ptr(out["masses"], C.c_float))
+        _check(self.lib.beads_engine_read_state(self.handle, C.byref(so)))
+        return out
+
+    def neighbors(self) -> tuple[np.ndarray, np.ndarray]:
+        """(counts[n], list[max_neighbors][n]) in the engine's slot-major layout."""
+        m = self.workload.max_neighbors
+        counts = np.zeros(self.n, np.uint32)
+        lst = np.zeros((m, self.n), np.uint32)
+        _check(self.lib.beads_engine_read_neighbors(self.handle, _ptr(counts, C.c_uint32),
+                                                    _ptr(lst, C.c_uint32), lst.size))
+        return counts, lst
+
+    def cells(self, n_cells: int) -> dict[str, np.ndarray]:
+        out = {"cell_of": np.zeros(self.n, np.uint32), "cell_atoms": np.zeros(self.n, np.uint32),
+               "cell_start": np.zeros(n_cells, np.uint32), "cell_count": np.zeros(n_cells, np.uint32)}
+        _check(self.lib.beads_engine_read_cells(
+            self.handle, _ptr(out["cell_of"], C.c_uint32), _ptr(out["cell_atoms"], C.c_uint32),
+            _ptr(out["cell_start"], C.c_uint32), _ptr(out["cell_count"], C.c_uint32), n_cells))
+        return out
+
+    def profile(self, steps: int, per_phase: bool = False) -> PhaseTimes:
+        t = PhaseTimes()
+        _check(self.lib.beads_engine_profile_advance(self.handle, steps, 1 if per_phase else 0,
+                                                     C.byref(t)))
+        return t

@@ -1,0 +1,169 @@
+// Parameter blocks shared by the host engine and the gfx950 kernels.
+//
+// Device layout (slot order, N atoms, all 16-byte vectors coalesced):
+//   pos    float4[N]  x, y, z, w = bit pattern of the 0-based type id
+//   vel    float4[N]  vx, vy, vz, w = mass
+//   force  float4[N]  fx, fy, fz, w unused
+//   image  int4[N]    ix, iy, iz, unused
+//   nbr_count u32[N], nbr u32[max_neighbors * N] slot-major
+//     (neighbor_list.cuh:12-19: entry k of atom i at k * N + i)
+//   per-slot listed-term roles (rebuilt on every reorder):
+//     topo_count u32[N] = bonds | angles << 8 | dihedrals << 16 | exclusions << 24
+//     bond_rec  uint2[bcap * N]  {partner slot, type | first-role << 31}
+//     angle_rec uint4[acap * N]  {other slot, other slot, type, role 0..2}
+//     dih_rec   uint4[dcap * N]  {other, other, other, type | role << 30}
+//     excl      u32[ecap * N]    excluded partner slots
+#pragma once
+
+#include <cstdint>
+
+namespace beads_amd {
+
+constexpr int kMaxLdsTypes = 16;  // LJ type tables up to 16x16 live in LDS
+
+struct BoxView {
+  float lo[3];
+  float L[3];
+  float hL[3];
+  float invL[3];
+};
+
+struct CellGridView {
+  uint32_t nx, ny, nz, n_cells;
+  float inv_cell[3];
+  int32_t path;  // 0 direct, 1 regular, 2 small dimension
+};
+
+// Device-resident control block. Written only by kernels (and zeroed once at
+// set-up), read back asynchronously by the host.
+struct alignas(256) Ctrl {
+  uint32_t rebuild;         // a rebuild is due in this step
+  uint32_t reorder;         // the rebuild also sorts particles into cell order
+  uint32_t checked;         // a scheduled check ran since the last build
+  uint32_t overflow;        // a neighbour list overflowed
+  uint64_t build_count;
+  uint64_t dangerous;
+  uint64_t reorder_count;
+  uint64_t neighbor_total;  // sum of neighbour counts of the current list
+  uint32_t arrive_force;    // last-block arrival counters
+  uint32_t arrive_scale;
+  uint32_t arrive_build;
+  uint32_t pad0;
+  double lambda;            // Berendsen factor still to be applied to vel
+  double ke_pre;            // kinetic energy after the post-force half-kick
+  double ke;                // kinetic energy of the velocities as stored
+  double pe[4];             // pair, bond, angle, dihedral
+  double virial[4];
+};
+
+struct NeighborControl {
+  uint64_t check_every;
+  uint64_t sort_every;
+  int32_t path;
+};
+
+struct TopologyView {
+  const uint32_t* count;
+  const uint2* bond;
+  const uint4* angle;
+  const uint4* dih;
+  const uint32_t* excl;
+  uint32_t bcap, acap, dcap, ecap;
+  const float2* bond_coeff;   // k, r0
+  const float2* angle_coeff;  // k, theta0 (rad)
+  const float4* dih_coeff;    // k, d, n, unused
+  int32_t has_listed;
+};
+
+struct ForceParams {
+  uint32_t n;
+  const float4* pos;
+  float4* vel;
+  float4* force;
+  const uint32_t* nbr_count;
+  const uint32_t* nbr;
+  BoxView box;
+  float rc2;
+  int32_t n_types;
+  const float4* lj;  // [T*T] sigma^2, 24 eps, 4 eps, shift
+  TopologyView topo;
+  float dt;
+  int32_t berendsen;
+  double target_thermal;  // kB * T0
+  double dt_over_tau;
+  double dof;
+  double* partials;       // per block: ke, pe[4], vir[4]
+  Ctrl* ctrl;
+  uint32_t* cell_count;   // zeroed after a rebuild consumed it
+  uint32_t n_cells;
+};
+
+struct IntegrateParams {
+  uint32_t n;
+  float4* pos;
+  float4* vel;
+  const float4* force;
+  int4* image;
+  const float4* ref_pos;
+  const int4* ref_image;
+  BoxView box;
+  float dt;
+  float threshold_sq;
+  Ctrl* ctrl;
+};
+
+struct BuildParams {
+  uint32_t n;
+  const float4* pos;
+  const int4* image;
+  float4* ref_pos;
+  int4* ref_image;
+  BoxView box;
+  CellGridView grid;
+  float rs2;
+  const uint32_t* cell_of;
+  const uint32_t* cell_start;
+  const uint32_t* cell_count;
+  const uint32_t* cell_atoms;
+  const uint32_t* topo_count;
+  const uint32_t* excl;
+  uint32_t ecap;
+  uint32_t max_neighbors;
+  uint32_t* nbr_count;
+  uint32_t* nbr;
+  Ctrl* ctrl;
+};
+
+struct ReorderParams {
+  uint32_t n;
+  // primary arrays
+  float4* pos;
+  float4* vel;
+  int4* image;
+  uint32_t* tag;
+  uint32_t* molid;
+  uint32_t* cell_of;
+  uint32_t* topo_count;
+  uint2* bond;
+  uint4* angle;
+  uint4* dih;
+  uint32_t* excl;
+  // scratch
+  float4* s_pos;
+  float4* s_vel;
+  int4* s_image;
+  uint32_t* s_tag;
+  uint32_t* s_molid;
+  uint32_t* s_cell_of;
+  uint32_t* s_topo_count;
+  uint2* s_bond;
+  uint4* s_angle;
+  uint4* s_dih;
+  uint32_t* s_excl;
+  uint32_t bcap, acap, dcap, ecap;
+  uint32_t* cell_atoms;
+  uint32_t* new_of_old;
+  Ctrl* ctrl;
+};
+
+}  // namespace beads_amd

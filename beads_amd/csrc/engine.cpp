@@ -1,0 +1,1150 @@
+// Host engine implementation. See engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <numeric>
+#include <sstream>
+
+namespace beads_amd {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr uint64_t kOverflowPollSteps = 256;
+
+bool finite(float v) { return std::isfinite(v); }
+
+std::string fmt9(double v) {
+  std::ostringstream s;
+  s << std::setprecision(9) << static_cast<float>(v);
+  return s.str();
+}
+
+template <typename T>
+void write_raw(std::ofstream& out, const T* data, size_t count, const std::string& what) {
+  out.write(reinterpret_cast<const char*>(data), static_cast<std::streamsize>(count * sizeof(T)));
+  if (!out) {
+    fail(BEADS_E_RUNTIME, "failed to write " + what + ".");
+  }
+}
+
+template <typename T>
+void write_scalar(std::ofstream& out, T value, const std::string& what) {
+  write_raw(out, &value, 1, what);
+}
+
+}  // namespace
+
+UnitSystem unit_system_from(int32_t units) {
+  if (units == BEADS_UNITS_REDUCED) {
+    return {"reduced", 1.0, 1.0};
+  }
+  if (units == BEADS_UNITS_NM_KJMOL) {
+    return {"nm_kjmol", static_cast<double>(static_cast<float>(0.00831446261815324)),
+            static_cast<double>(static_cast<float>(16.60539067))};
+  }
+  fail(BEADS_E_VALUE, "system.units must be \"reduced\" or \"nm_kjmol\".");
+}
+
+BoxView make_box_view(const float* bb) {
+  BoxView b{};
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = bb[a];
+    b.L[a] = bb[3 + a] - bb[a];
+    b.hL[a] = 0.5f * b.L[a];
+    b.invL[a] = 1.0f / b.L[a];
+  }
+  return b;
+}
+
+NeighborPlan make_neighbor_plan(const BoxView& box, float rc, float buffer, uint32_t n,
+                                uint32_t max_neighbors, uint64_t check_every,
+                                uint64_t sort_every) {
+  NeighborPlan p;
+  require(n > 0, "neighbor n_particles must be positive.");
+  require(finite(rc) && rc > 0.f, "neighbor force_cutoff must be finite and positive.");
+  require(finite(buffer) && buffer >= 0.f,
+          "neighbor cutoff_buffer must be finite and non-negative.");
+  require(max_neighbors > 0, "neighbor max_neighbors must be positive.");
+  require(check_every > 0, "neighbor rebuild_check_every must be positive.");
+  require(sort_every > 0, "neighbor.sort_every_rebuild must be positive.");
+  p.force_cutoff = rc;
+  p.cutoff_buffer = buffer;
+  p.search_cutoff = rc + buffer;
+  p.search_cutoff_sq = p.search_cutoff * p.search_cutoff;
+  p.max_neighbors = max_neighbors;
+  p.check_every = check_every;
+  p.sort_every = sort_every;
+  const float half_shortest = 0.5f * std::min(box.L[0], std::min(box.L[1], box.L[2]));
+  require(p.search_cutoff <= half_shortest,
+          "neighbor search_cutoff must not exceed half shortest box length.");
+  uint32_t counts[3];
+  for (int a = 0; a < 3; ++a) {
+    const double raw =
+        std::floor(static_cast<double>(box.L[a]) / static_cast<double>(p.search_cutoff));
+    require(std::isfinite(raw), "neighbor box length cell count must be finite.");
+    require(raw <= static_cast<double>(std::numeric_limits<uint32_t>::max()),
+            "neighbor cell count exceeds index_t range.");
+    counts[a] = raw < 1.0 ? 1u : static_cast<uint32_t>(raw);
+    p.cell_size[a] = box.L[a] / static_cast<float>(counts[a]);
+    p.inv_cell[a] = 1.0f / p.cell_size[a];
+    if (counts[a] >= 3) {
+      const float tol = std::max(1.0e-6f, p.search_cutoff * 1.0e-5f);
+      require(p.cell_size[a] + tol >= p.search_cutoff,
+              "neighbor cell size must be at least neighbor search_cutoff.");
+    }
+  }
+  const uint64_t cells = static_cast<uint64_t>(counts[0]) * counts[1] * counts[2];
+  require(cells <= std::numeric_limits<uint32_t>::max() - 1,
+          "neighbor n_cells exceeds index_t range.");
+  p.nx = counts[0];
+  p.ny = counts[1];
+  p.nz = counts[2];
+  p.n_cells = static_cast<uint32_t>(cells);
+  if (p.nx < 3 && p.ny < 3 && p.nz < 3) {
+    p.path = 0;
+  } else if (p.nx >= 3 && p.ny >= 3 && p.nz >= 3) {
+    p.path = 1;
+  } else {
+    p.path = 2;
+  }
+  return p;
+}
+
+std::vector<std::pair<uint32_t, uint32_t>> bond_graph_exclusions(uint32_t n, uint32_t n_bonds,
+                                                                 const uint32_t* bonds,
+                                                                 int32_t distance) {
+  std::vector<std::pair<uint32_t, uint32_t>> out;
+  if (distance <= 0 || n_bonds == 0) {
+    return out;
+  }
+  // Tag-indexed adjacency in CSR form.
+  std::vector<uint32_t> deg(static_cast<size_t>(n) + 2, 0);
+  for (uint32_t b = 0; b < n_bonds; ++b) {
+    ++deg[bonds[3 * b]];
+    ++deg[bonds[3 * b + 1]];
+  }
+  std::vector<uint32_t> off(static_cast<size_t>(n) + 2, 0);
+  for (uint32_t t = 1; t <= n + 1; ++t) {
+    off[t] = off[t - 1] + deg[t - 1];
+  }
+  std::vector<uint32_t> adj(off[n + 1]);
+  std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+  for (uint32_t b = 0; b < n_bonds; ++b) {
+    const uint32_t a = bonds[3 * b], c = bonds[3 * b + 1];
+    adj[fill[a]++] = c;
+    adj[fill[c]++] = a;
+  }
+  std::vector<uint32_t> mark(static_cast<size_t>(n) + 1, 0);
+  std::vector<std::pair<uint32_t, int>> frontier;
+  for (uint32_t start = 1; start <= n; ++start) {
+    if (deg[start] == 0) {
+      continue;
+    }
+    frontier.clear();
+    frontier.emplace_back(start, 0);
+    mark[start] = start;
+    for (size_t r = 0; r < frontier.size(); ++r) {
+      const auto [t, depth] = frontier[r];
+      if (depth == distance) {
+        continue;
+      }
+      for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+        const uint32_t q = adj[e];
+        if (mark[q] == start) {
+          continue;
+        }
+        mark[q] = start;
+        if (q > start) {
+          out.emplace_back(start, q);
+        }
+        frontier.emplace_back(q, depth + 1);
+      }
+    }
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+HostTopology build_host_topology(const beads_spec_t& s) {
+  HostTopology h;
+  const uint32_t n = s.n_particles;
+  std::vector<uint32_t> slot_of_tag(static_cast<size_t>(n) + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    slot_of_tag[s.tags[i]] = i;
+  }
+  std::vector<uint32_t> nb(n, 0), na(n, 0), nd(n, 0), ne(n, 0);
+  for (uint32_t b = 0; b < s.n_bonds; ++b) {
+    ++nb[slot_of_tag[s.bonds[3 * b]]];
+    ++nb[slot_of_tag[s.bonds[3 * b + 1]]];
+  }
+  for (uint32_t a = 0; a < s.n_angles; ++a) {
+    for (int r = 0; r < 3; ++r) ++na[slot_of_tag[s.angles[4 * a + r]]];
+  }
+  for (uint32_t d = 0; d < s.n_dihedrals; ++d) {
+    for (int r = 0; r < 4; ++r) ++nd[slot_of_tag[s.dihedrals[5 * d + r]]];
+  }
+  const auto pairs = bond_graph_exclusions(n, s.n_bonds, s.bonds, s.exclusion_distance);
+  h.excluded_pairs = pairs.size();
+  for (const auto& p : pairs) {
+    ++ne[slot_of_tag[p.first]];
+    ++ne[slot_of_tag[p.second]];
+  }
+  auto max_of = [](const std::vector<uint32_t>& v) {
+    return v.empty() ? 0u : *std::max_element(v.begin(), v.end());
+  };
+  h.bcap = max_of(nb);
+  h.acap = max_of(na);
+  h.dcap = max_of(nd);
+  h.ecap = max_of(ne);
+  require(h.bcap <= 255 && h.acap <= 255 && h.dcap <= 255 && h.ecap <= 255,
+          "a particle takes part in more than 255 bonds, angles, dihedrals or exclusions; "
+          "this engine supports at most 255 of each per particle.");
+  h.count.assign(n, 0);
+  h.bond.assign(static_cast<size_t>(h.bcap) * n, uint2{0, 0});
+  h.angle.assign(static_cast<size_t>(h.acap) * n, uint4{0, 0, 0, 0});
+  h.dih.assign(static_cast<size_t>(h.dcap) * n, uint4{0, 0, 0, 0});
+  h.excl.assign(static_cast<size_t>(h.ecap) * n, 0u);
+  std::vector<uint32_t> cb(n, 0), ca(n, 0), cd(n, 0), ce(n, 0);
+  const size_t N = n;
+  for (uint32_t b = 0; b < s.n_bonds; ++b) {
+    const uint32_t si = slot_of_tag[s.bonds[3 * b]], sj = slot_of_tag[s.bonds[3 * b + 1]];
+    const uint32_t ty = s.bonds[3 * b + 2] - 1;
+    h.bond[cb[si]++ * N + si] = uint2{sj, ty | 0x80000000u};
+    h.bond[cb[sj]++ * N + sj] = uint2{si, ty};
+  }
+  for (uint32_t a = 0; a < s.n_angles; ++a) {
+    const uint32_t si = slot_of_tag[s.angles[4 * a]], sj = slot_of_tag[s.angles[4 * a + 1]],
+                   sk = slot_of_tag[s.angles[4 * a + 2]];
+    const uint32_t ty = s.angles[4 * a + 3] - 1;
+    h.angle[ca[si]++ * N + si] = uint4{sj, sk, ty, 0};
+    h.angle[ca[sj]++ * N + sj] = uint4{si, sk, ty, 1};
+    h.angle[ca[sk]++ * N + sk] = uint4{si, sj, ty, 2};
+  }
+  for (uint32_t d = 0; d < s.n_dihedrals; ++d) {
+    uint32_t sl[4];
+    for (int r = 0; r < 4; ++r) sl[r] = slot_of_tag[s.dihedrals[5 * d + r]];
+    const uint32_t ty = s.dihedrals[5 * d + 4] - 1;
+    for (uint32_t r = 0; r < 4; ++r) {
+      uint32_t others[3];
+      int m = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (q != static_cast<int>(r)) others[m++] = sl[q];
+      }
+      h.dih[cd[sl[r]]++ * N + sl[r]] = uint4{others[0], others[1], others[2], ty | (r << 30)};
+    }
+  }
+  for (const auto& p : pairs) {
+    const uint32_t a = slot_of_tag[p.first], b = slot_of_tag[p.second];
+    h.excl[ce[a]++ * N + a] = b;
+    h.excl[ce[b]++ * N + b] = a;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    h.count[i] = nb[i] | (na[i] << 8) | (nd[i] << 16) | (ne[i] << 24);
+  }
+  return h;
+}
+
+namespace {
+
+// Shell validation of the typed spec (native_spec_validation.hpp:23-716,
+// restated for the fields this engine consumes).
+void validate_spec(const beads_spec_t& s) {
+  (void)unit_system_from(s.units);
+  require(s.n_particles > 0, "system.n_particles must be positive.");
+  require(s.box_bound && s.positions && s.velocities && s.masses && s.types && s.tags &&
+              s.molecule_ids && s.images,
+          "system arrays must not be null.", BEADS_E_TYPE);
+  const uint32_t n = s.n_particles;
+  for (int a = 0; a < 3; ++a) {
+    require(finite(s.box_bound[a]) && finite(s.box_bound[3 + a]), "system.box_bound must be finite.");
+    require(s.box_bound[3 + a] > s.box_bound[a],
+            "system.box_bound upper row must be greater than lower row.");
+  }
+  std::vector<uint8_t> seen(static_cast<size_t>(n) + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      const float x = s.positions[3 * i + a];
+      require(finite(x), "system.positions must be finite.");
+      require(x >= s.box_bound[a] && x < s.box_bound[3 + a],
+              "system.positions must be inside box_bound.");
+      require(finite(s.velocities[3 * i + a]), "system.velocities must be finite.");
+    }
+    require(finite(s.masses[i]) && s.masses[i] > 0.f, "system.masses must be finite and positive.");
+    require(s.types[i] >= 1 && s.types[i] <= s.n_types,
+            "system.types must be covered by the pair coefficient table.");
+    const uint32_t t = s.tags[i];
+    require(t >= 1 && t <= n && !seen[t], "system.tags must be a permutation of 1..N.");
+    seen[t] = 1;
+    require(s.molecule_ids[i] >= 1, "system.molecule_ids must be one-based.");
+  }
+  require(s.n_types >= 1 && s.lj_epsilon && s.lj_sigma,
+          "forcefield requires pair_style \"lj\" with coefficients.");
+  require(finite(s.lj_cutoff) && s.lj_cutoff > 0.f,
+          "Pair(\"lj\") pair_style.cutoff must be finite and positive.");
+  for (int32_t k = 0; k < s.n_types * s.n_types; ++k) {
+    require(finite(s.lj_epsilon[k]) && s.lj_epsilon[k] > 0.f,
+            "Pair(\"lj\") pair_coeff.epsilon must be finite and positive.");
+    require(finite(s.lj_sigma[k]) && s.lj_sigma[k] > 0.f,
+            "Pair(\"lj\") pair_coeff.sigma must be finite and positive.");
+  }
+  auto check_tag = [&](uint32_t t, const char* what) {
+    require(t >= 1 && t <= n, std::string(what) + " references a tag outside the system.");
+  };
+  require(s.n_bonds == 0 || s.n_bond_types >= 1, "System topology bonds require an active bond_style.");
+  for (uint32_t b = 0; b < s.n_bonds; ++b) {
+    check_tag(s.bonds[3 * b], "bond");
+    check_tag(s.bonds[3 * b + 1], "bond");
+    require(s.bonds[3 * b] != s.bonds[3 * b + 1], "bond entries must reference distinct tags.");
+    require(s.bonds[3 * b + 2] >= 1 && s.bonds[3 * b + 2] <= static_cast<uint32_t>(s.n_bond_types),
+            "bond type is not covered by bond_coeffs.");
+  }
+  require(s.n_angles == 0 || s.n_angle_types >= 1, "System topology angles require an active angle_style.");
+  for (uint32_t a = 0; a < s.n_angles; ++a) {
+    for (int r = 0; r < 3; ++r) check_tag(s.angles[4 * a + r], "angle");
+    require(s.angles[4 * a + 3] >= 1 && s.angles[4 * a + 3] <= static_cast<uint32_t>(s.n_angle_types),
+            "angle type is not covered by angle_coeffs.");
+  }
+  require(s.n_dihedrals == 0 || s.n_dihedral_types >= 1,
+          "System topology dihedrals require an active dihedral_style.");
+  for (uint32_t d = 0; d < s.n_dihedrals; ++d) {
+    for (int r = 0; r < 4; ++r) check_tag(s.dihedrals[5 * d + r], "dihedral");
+    require(s.dihedrals[5 * d + 4] >= 1 &&
+                s.dihedrals[5 * d + 4] <= static_cast<uint32_t>(s.n_dihedral_types),
+            "dihedral type is not covered by dihedral_coeffs.");
+  }
+  for (int32_t t = 0; t < s.n_bond_types; ++t) {
+    require(finite(s.bond_k[t]) && s.bond_k[t] >= 0.f, "Bond(\"harmonic\") bond_coeff.k must be non-negative.");
+    require(finite(s.bond_r0[t]) && s.bond_r0[t] >= 0.f, "Bond(\"harmonic\") bond_coeff.r0 must be non-negative.");
+  }
+  for (int32_t t = 0; t < s.n_angle_types; ++t) {
+    require(finite(s.angle_k[t]) && s.angle_k[t] >= 0.f, "Angle(\"harmonic\") angle_coeff.k must be non-negative.");
+    require(finite(s.angle_theta0_deg[t]) && s.angle_theta0_deg[t] >= 0.f && s.angle_theta0_deg[t] <= 180.f,
+            "Angle(\"harmonic\") angle_coeff.theta0 must be between 0 and 180 degrees.");
+  }
+  for (int32_t t = 0; t < s.n_dihedral_types; ++t) {
+    require(finite(s.dihedral_k[t]) && s.dihedral_k[t] >= 0.f,
+            "Dihedral(\"harmonic\") dihedral_coeff.k must be non-negative.");
+    require(s.dihedral_d[t] == 1 || s.dihedral_d[t] == -1,
+            "Dihedral(\"harmonic\") dihedral_coeff.d must be +1 or -1.");
+    require(s.dihedral_n[t] >= 0, "Dihedral(\"harmonic\") dihedral_coeff.n must be non-negative.");
+  }
+  require(s.exclusion_distance >= 0 && s.exclusion_distance <= 3,
+          "bond graph exclusion distance must be 1, 2, or 3.");
+  require(s.exclusion_distance == 0 || s.n_bonds > 0,
+          "ForceField.exclude_bonded(...) requires System topology bonds.");
+  require(finite(s.dt) && s.dt > 0.f, "VelocityVerletIntegrator dt must be finite and positive.");
+  if (s.berendsen) {
+    require(finite(s.berendsen_temperature) && s.berendsen_temperature >= 0.f,
+            "Thermostat(\"berendsen\").temperature must be non-negative.");
+    require(finite(s.berendsen_tau) && s.berendsen_tau > 0.f,
+            "Thermostat(\"berendsen\").tau must be positive.");
+  }
+  require(!(s.cutoff_buffer == 0.f && s.rebuild_check_every != 1),
+          "neighbor cutoff_buffer=0 requires rebuild_check_every=1");
+  // neighbor_list.cu:13-28: slot-major indices are 32-bit.
+  if (static_cast<uint64_t>(s.n_particles) * s.max_neighbors > 0xffffffffull) {
+    fail(BEADS_E_OVERFLOW, "NeighborList n_particles * max_neighbors exceeds index_t range.");
+  }
+  require(s.thermo_path == nullptr || s.thermo_every > 0, "output.thermo.every must be positive.");
+  require(s.trajectory_path == nullptr || s.trajectory_every > 0,
+          "output.trajectory.every must be positive.");
+  require(s.log_echo >= 0 && s.log_echo <= 3, "output.log.echo is not supported.");
+  require(!((s.log_echo == BEADS_LOG_FILE || s.log_echo == BEADS_LOG_BOTH) && s.log_path == nullptr),
+          "output.log.prefix is required for echo=log/both.");
+}
+
+}  // namespace
+
+Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
+  validate_spec(spec);
+  if (spec.thermo_path) thermo_path_ = spec.thermo_path;
+  if (spec.trajectory_path) trajectory_path_ = spec.trajectory_path;
+  if (spec.final_state_path) final_state_path_ = spec.final_state_path;
+  if (spec.log_path) log_path_ = spec.log_path;
+  // The spec only borrows caller memory for the duration of construction.
+  spec_.thermo_path = spec_.trajectory_path = spec_.final_state_path = spec_.log_path = nullptr;
+  units_ = unit_system_from(spec.units);
+  box_ = make_box_view(spec.box_bound);
+  for (int a = 0; a < 3; ++a) spec_box_hi_[a] = spec.box_bound[3 + a];
+  n_ = spec.n_particles;
+  thermostat_ = spec.berendsen != 0;
+  has_bond_style_ = spec.n_bond_types > 0;
+  has_angle_style_ = spec.n_angle_types > 0;
+  has_dihedral_style_ = spec.n_dihedral_types > 0;
+  plan_ = make_neighbor_plan(box_, spec.lj_cutoff, spec.cutoff_buffer, n_, spec.max_neighbors,
+                             spec.rebuild_check_every, spec.sort_every_rebuild);
+  interrupt_interval_ = std::max<uint64_t>(1, 100000 / std::max<uint32_t>(1, n_));
+
+  BEADS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  upload_state();
+  set_up_params();
+  // The spec's arrays belong to the caller; drop them.
+  spec_.box_bound = nullptr;
+  spec_.positions = spec_.velocities = spec_.masses = nullptr;
+  spec_.types = nullptr;
+  spec_.tags = spec_.molecule_ids = nullptr;
+  spec_.images = nullptr;
+  spec_.bonds = spec_.angles = spec_.dihedrals = nullptr;
+  spec_.lj_epsilon = spec_.lj_sigma = nullptr;
+  spec_.bond_k = spec_.bond_r0 = spec_.angle_k = spec_.angle_theta0_deg = nullptr;
+  spec_.dihedral_k = nullptr;
+  spec_.dihedral_d = spec_.dihedral_n = nullptr;
+}
+
+Engine::~Engine() {
+  if (stream_ != nullptr) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Engine::upload_state() {
+  const beads_spec_t& s = spec_;
+  const size_t n = n_;
+  std::vector<float4> pos(n), vel(n);
+  std::vector<int4> img(n);
+  for (size_t i = 0; i < n; ++i) {
+    int32_t t = s.types[i] - 1;
+    float tw;
+    std::memcpy(&tw, &t, sizeof(float));
+    pos[i] = make_float4(s.positions[3 * i], s.positions[3 * i + 1], s.positions[3 * i + 2], tw);
+    vel[i] = make_float4(s.velocities[3 * i], s.velocities[3 * i + 1], s.velocities[3 * i + 2],
+                         s.masses[i]);
+    img[i] = make_int4(s.images[3 * i], s.images[3 * i + 1], s.images[3 * i + 2], 0);
+  }
+  pos_.upload(pos.data(), n, stream_);
+  vel_.upload(vel.data(), n, stream_);
+  image_.upload(img.data(), n, stream_);
+  tag_.upload(s.tags, n, stream_);
+  molid_.upload(s.molecule_ids, n, stream_);
+  force_.resize(n);
+  force_.zero(stream_);
+  ref_pos_.resize(n);
+  ref_image_.resize(n);
+  nbr_count_.resize(n);
+  nbr_count_.zero(stream_);
+  nbr_.resize(static_cast<size_t>(plan_.max_neighbors) * n);
+  cell_of_.resize(n);
+  cell_atoms_.resize(n);
+  cell_count_.resize(plan_.n_cells);
+  cell_count_.zero(stream_);
+  cell_start_.resize(static_cast<size_t>(plan_.n_cells) + 1);
+  cell_start_.zero(stream_);
+  cell_cursor_.resize(plan_.n_cells);
+  // Until the first binning every atom sits in cell order = slot order.
+  std::vector<uint32_t> ident(n);
+  std::iota(ident.begin(), ident.end(), 0u);
+  cell_atoms_.upload(ident.data(), n, stream_);
+  cell_of_.zero(stream_);
+
+  // Listed terms and exclusions, per slot.
+  const HostTopology topo = build_host_topology(s);
+  excluded_pairs_ = topo.excluded_pairs;
+  bcap_ = topo.bcap;
+  acap_ = topo.acap;
+  dcap_ = topo.dcap;
+  ecap_ = topo.ecap;
+  has_topology_ = (bcap_ | acap_ | dcap_ | ecap_) != 0;
+  topo_count_.upload(topo.count.data(), n, stream_);
+  bond_.upload(topo.bond.data(), topo.bond.size(), stream_);
+  angle_.upload(topo.angle.data(), topo.angle.size(), stream_);
+  dih_.upload(topo.dih.data(), topo.dih.size(), stream_);
+  excl_.upload(topo.excl.data(), topo.excl.size(), stream_);
+
+  if (plan_.path != 0) {
+    s_pos_.resize(n);
+    s_vel_.resize(n);
+    s_image_.resize(n);
+    s_tag_.resize(n);
+    s_molid_.resize(n);
+    s_cell_of_.resize(n);
+    s_topo_count_.resize(n);
+    s_bond_.resize(topo.bond.size());
+    s_angle_.resize(topo.angle.size());
+    s_dih_.resize(topo.dih.size());
+    s_excl_.resize(topo.excl.size());
+    new_of_old_.resize(n);
+  }
+
+  // pair_lj.cu:171-192 coefficient packing in binary32.
+  const int T = s.n_types;
+  std::vector<float4> lj(static_cast<size_t>(T) * T);
+  const float rc = s.lj_cutoff;
+  for (int a = 0; a < T; ++a) {
+    for (int b = 0; b < T; ++b) {
+      const float eps = s.lj_epsilon[a * T + b], sig = s.lj_sigma[a * T + b];
+      const float sigma2 = sig * sig;
+      const float eps4 = 4.f * eps;
+      float shift = 0.f;
+      if (s.lj_shift) {
+        const float sr2 = sigma2 / (rc * rc);
+        const float sr6 = sr2 * sr2 * sr2;
+        shift = eps4 * (sr6 * sr6 - sr6);
+      }
+      lj[static_cast<size_t>(a) * T + b] = make_float4(sigma2, 6.f * eps4, eps4, shift);
+    }
+  }
+  lj_.upload(lj.data(), lj.size(), stream_);
+  std::vector<float2> bc(std::max(1, s.n_bond_types)), ac(std::max(1, s.n_angle_types));
+  std::vector<float4> dc(std::max(1, s.n_dihedral_types));
+  for (int t = 0; t < s.n_bond_types; ++t) bc[t] = make_float2(s.bond_k[t], s.bond_r0[t]);
+  for (int t = 0; t < s.n_angle_types; ++t) {
+    ac[t] = make_float2(s.angle_k[t], s.angle_theta0_deg[t] * static_cast<float>(kPi) / 180.f);
+  }
+  for (int t = 0; t < s.n_dihedral_types; ++t) {
+    dc[t] = make_float4(s.dihedral_k[t], static_cast<float>(s.dihedral_d[t]),
+                        static_cast<float>(s.dihedral_n[t]), 0.f);
+  }
+  bond_coeff_.upload(bc.data(), bc.size(), stream_);
+  angle_coeff_.upload(ac.data(), ac.size(), stream_);
+  dih_coeff_.upload(dc.data(), dc.size(), stream_);
+
+  partials_.resize(9 * static_cast<size_t>(grid_for(n_)));
+  Ctrl c{};
+  c.lambda = 1.0;
+  ctrl_.upload(&c, 1, stream_);
+  ctrl_host_.resize(1);
+  BEADS_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_up_params() {
+  const size_t n = n_;
+  (void)n;
+  TopologyView topo{};
+  topo.count = topo_count_.data();
+  topo.bond = bond_.data();
+  topo.angle = angle_.data();
+  topo.dih = dih_.data();
+  topo.excl = excl_.data();
+  topo.bcap = bcap_;
+  topo.acap = acap_;
+  topo.dcap = dcap_;
+  topo.ecap = ecap_;
+  topo.bond_coeff = bond_coeff_.data();
+  topo.angle_coeff = angle_coeff_.data();
+  topo.dih_coeff = dih_coeff_.data();
+  topo.has_listed = (bcap_ | acap_ | dcap_) != 0;
+
+  ForceParams& f = force_params_;
+  f.n = n_;
+  f.pos = pos_.data();
+  f.vel = vel_.data();
+  f.force = force_.data();
+  f.nbr_count = nbr_count_.data();
+  f.nbr = nbr_.data();
+  f.box = box_;
+  f.rc2 = plan_.force_cutoff * plan_.force_cutoff;
+  f.n_types = spec_.n_types;
+  f.lj = lj_.data();
+  f.topo = topo;
+  f.dt = spec_.dt;
+  f.berendsen = thermostat_ ? 1 : 0;
+  f.target_thermal = units_.kb * static_cast<double>(spec_.berendsen_temperature);
+  f.dt_over_tau = thermostat_ ? static_cast<double>(spec_.dt) / spec_.berendsen_tau : 0.0;
+  f.dof = std::max(0.0, 3.0 * static_cast<double>(n_) - 3.0);
+  f.partials = partials_.data();
+  f.ctrl = ctrl_.data();
+  f.cell_count = cell_count_.data();
+  f.n_cells = plan_.n_cells;
+
+  IntegrateParams& in = integrate_params_;
+  in.n = n_;
+  in.pos = pos_.data();
+  in.vel = vel_.data();
+  in.force = force_.data();
+  in.image = image_.data();
+  in.ref_pos = ref_pos_.data();
+  in.ref_image = ref_image_.data();
+  in.box = box_;
+  in.dt = spec_.dt;
+  const float half_buffer = 0.5f * plan_.cutoff_buffer;
+  in.threshold_sq = half_buffer * half_buffer;
+  in.ctrl = ctrl_.data();
+
+  CellGridView grid{};
+  grid.nx = plan_.nx;
+  grid.ny = plan_.ny;
+  grid.nz = plan_.nz;
+  grid.n_cells = plan_.n_cells;
+  for (int a = 0; a < 3; ++a) grid.inv_cell[a] = plan_.inv_cell[a];
+  grid.path = plan_.path;
+
+  BuildParams& b = build_params_;
+  b.n = n_;
+  b.pos = pos_.data();
+  b.image = image_.data();
+  b.ref_pos = ref_pos_.data();
+  b.ref_image = ref_image_.data();
+  b.box = box_;
+  b.grid = grid;
+  b.rs2 = plan_.search_cutoff_sq;
+  b.cell_of = cell_of_.data();
+  b.cell_start = cell_start_.data();
+  b.cell_count = cell_count_.data();
+  b.cell_atoms = cell_atoms_.data();
+  b.topo_count = topo_count_.data();
+  b.excl = excl_.data();
+  b.ecap = ecap_;
+  b.max_neighbors = plan_.max_neighbors;
+  b.nbr_count = nbr_count_.data();
+  b.nbr = nbr_.data();
+  b.ctrl = ctrl_.data();
+
+  BinParams& bp = bin_params_;
+  bp.n = n_;
+  bp.pos = pos_.data();
+  bp.box = box_;
+  bp.grid = grid;
+  bp.cell_of = cell_of_.data();
+  bp.cell_count = cell_count_.data();
+  bp.ctrl = ctrl_.data();
+
+  CellScanParams& sp = scan_params_;
+  sp.n = n_;
+  sp.n_cells = plan_.n_cells;
+  sp.cell_of = cell_of_.data();
+  sp.cell_count = cell_count_.data();
+  sp.cell_start = cell_start_.data();
+  sp.cell_cursor = cell_cursor_.data();
+  sp.cell_atoms = cell_atoms_.data();
+  sp.ctrl = ctrl_.data();
+
+  ReorderParams& r = reorder_params_;
+  r.n = n_;
+  r.pos = pos_.data();
+  r.vel = vel_.data();
+  r.image = image_.data();
+  r.tag = tag_.data();
+  r.molid = molid_.data();
+  r.cell_of = cell_of_.data();
+  r.topo_count = topo_count_.data();
+  r.bond = bond_.data();
+  r.angle = angle_.data();
+  r.dih = dih_.data();
+  r.excl = excl_.data();
+  r.s_pos = s_pos_.data();
+  r.s_vel = s_vel_.data();
+  r.s_image = s_image_.data();
+  r.s_tag = s_tag_.data();
+  r.s_molid = s_molid_.data();
+  r.s_cell_of = s_cell_of_.data();
+  r.s_topo_count = s_topo_count_.data();
+  r.s_bond = s_bond_.data();
+  r.s_angle = s_angle_.data();
+  r.s_dih = s_dih_.data();
+  r.s_excl = s_excl_.data();
+  r.bcap = bcap_;
+  r.acap = acap_;
+  r.dcap = dcap_;
+  r.ecap = ecap_;
+  r.cell_atoms = cell_atoms_.data();
+  r.new_of_old = new_of_old_.data();
+  r.ctrl = ctrl_.data();
+
+  ScaleParams& sc = scale_params_;
+  sc.n = n_;
+  sc.vel = vel_.data();
+  sc.partials = partials_.data();
+  sc.ctrl = ctrl_.data();
+
+  control_.check_every = plan_.check_every;
+  control_.sort_every = plan_.sort_every;
+  control_.path = plan_.path;
+}
+
+// NeighborSystem::rebuild_now (neighbor_system.cpp:107-146) as a predicated
+// launch sequence.
+void Engine::rebuild_sequence(bool forced) {
+  launch_control(ctrl_.data(), control_, forced, stream_);
+  if (plan_.path != 0) {
+    launch_bin(bin_params_, stream_);
+    launch_cell_scan(scan_params_, stream_);
+    launch_cell_scatter(scan_params_, stream_);
+    launch_cell_sort(scan_params_, stream_);
+    launch_reorder(reorder_params_, stream_);
+  }
+  launch_build(build_params_, stream_);
+}
+
+void Engine::initialize(bool observables) {
+  rebuild_sequence(true);
+  launch_force(force_params_, false, observables, stream_);
+  BEADS_HIP(hipGetLastError());
+  initialized_ = true;
+}
+
+// SimulationRun::advance_dynamics_step (simulation_run.cpp:212-235): pending
+// thermostat factor + pre-force, neighbour update, forces + post-force +
+// Berendsen factor.
+void Engine::step_once(bool observables) {
+  const uint64_t physical = step_ + 1;
+  const bool every = plan_.cutoff_buffer == 0.f;
+  const bool check = !every && physical % plan_.check_every == 0;
+  launch_integrate(integrate_params_, thermostat_, check, false, stream_);
+  if (every || check) {
+    rebuild_sequence(every);
+  }
+  launch_force(force_params_, true, observables, stream_);
+  step_ = physical;
+}
+
+void Engine::advance(uint64_t n_steps, bool observables_on_last) {
+  require(initialized_, "engine must be initialized before advancing.", BEADS_E_RUNTIME);
+  for (uint64_t k = 0; k < n_steps; ++k) {
+    step_once(observables_on_last && k + 1 == n_steps);
+  }
+  BEADS_HIP(hipGetLastError());
+}
+
+void Engine::materialize_velocities() {
+  launch_scale(scale_params_, thermostat_, stream_);
+}
+
+void Engine::synchronize() { BEADS_HIP(hipStreamSynchronize(stream_)); }
+
+Ctrl Engine::fetch_ctrl() {
+  BEADS_HIP(hipMemcpyAsync(ctrl_host_.data(), ctrl_.data(), sizeof(Ctrl), hipMemcpyDeviceToHost,
+                           stream_));
+  BEADS_HIP(hipStreamSynchronize(stream_));
+  return *ctrl_host_.data();
+}
+
+void Engine::check_overflow(const Ctrl& c) {
+  if (c.overflow != 0) {
+    std::ostringstream m;
+    m << "NeighborList overflow"
+      << "\npath=" << (plan_.path == 0 ? "direct" : (plan_.path == 1 ? "regular_cell" : "small_dimension"))
+      << "\nn_particles=" << n_ << "\nmax_neighbors=" << plan_.max_neighbors
+      << "\nforce_cutoff=" << plan_.force_cutoff << "\ncutoff_buffer=" << plan_.cutoff_buffer
+      << "\nsearch_cutoff=" << plan_.search_cutoff
+      << ". Increase max_neighbors before consuming this neighbor list.";
+    fail(BEADS_E_RUNTIME, m.str());
+  }
+}
+
+void Engine::stats(beads_engine_stats_t* out) {
+  const Ctrl c = fetch_ctrl();
+  std::memset(out, 0, sizeof(*out));
+  out->step = step_;
+  out->build_count = c.build_count;
+  out->dangerous_rebuild_count = c.dangerous;
+  out->reorder_count = c.reorder_count;
+  out->overflow = static_cast<int32_t>(c.overflow);
+  out->path = plan_.path;
+  out->nx = plan_.nx;
+  out->ny = plan_.ny;
+  out->nz = plan_.nz;
+  out->max_neighbors = plan_.max_neighbors;
+  out->mean_neighbors = static_cast<double>(c.neighbor_total) / n_;
+  out->ke = c.ke;
+  out->lambda = c.lambda;
+  for (int q = 0; q < 4; ++q) {
+    out->pe[q] = c.pe[q];
+    out->virial[q] = c.virial[q];
+  }
+  const double dof = std::max(0.0, 3.0 * n_ - 3.0);
+  out->temperature = dof > 0 ? 2.0 * c.ke / (dof * units_.kb) : 0.0;
+  const double vol = static_cast<double>(box_.L[0]) * box_.L[1] * box_.L[2];
+  const double w = c.virial[0] + c.virial[1] + c.virial[2] + c.virial[3];
+  out->pressure = units_.pressure_scale * (2.0 * c.ke + w) / (3.0 * vol);
+}
+
+void Engine::read_state(const beads_state_out_t& o) {
+  const size_t n = n_;
+  std::vector<float4> pos(n), vel(n), force(n);
+  std::vector<int4> img(n);
+  pos_.download(pos.data(), n, stream_);
+  vel_.download(vel.data(), n, stream_);
+  force_.download(force.data(), n, stream_);
+  image_.download(img.data(), n, stream_);
+  if (o.tags) tag_.download(o.tags, n, stream_);
+  if (o.molecule_ids) molid_.download(o.molecule_ids, n, stream_);
+  synchronize();
+  for (size_t i = 0; i < n; ++i) {
+    if (o.positions) {
+      o.positions[3 * i] = pos[i].x;
+      o.positions[3 * i + 1] = pos[i].y;
+      o.positions[3 * i + 2] = pos[i].z;
+    }
+    if (o.types) {
+      int32_t t;
+      std::memcpy(&t, &pos[i].w, sizeof(int32_t));
+      o.types[i] = t + 1;
+    }
+    if (o.velocities) {
+      o.velocities[3 * i] = vel[i].x;
+      o.velocities[3 * i + 1] = vel[i].y;
+      o.velocities[3 * i + 2] = vel[i].z;
+    }
+    if (o.masses) o.masses[i] = vel[i].w;
+    if (o.forces) {
+      o.forces[3 * i] = force[i].x;
+      o.forces[3 * i + 1] = force[i].y;
+      o.forces[3 * i + 2] = force[i].z;
+    }
+    if (o.images) {
+      o.images[3 * i] = img[i].x;
+      o.images[3 * i + 1] = img[i].y;
+      o.images[3 * i + 2] = img[i].z;
+    }
+  }
+}
+
+void Engine::read_neighbors(uint32_t* counts, uint32_t* list, uint64_t capacity) {
+  require(capacity >= static_cast<uint64_t>(plan_.max_neighbors) * n_,
+          "neighbor list capacity must be max_neighbors * n_particles.");
+  nbr_count_.download(counts, n_, stream_);
+  nbr_.download(list, static_cast<size_t>(plan_.max_neighbors) * n_, stream_);
+  synchronize();
+}
+
+void Engine::read_cells(uint32_t* cell_of, uint32_t* cell_atoms, uint32_t* cell_start,
+                        uint32_t* cell_count, uint64_t capacity) {
+  require(capacity >= plan_.n_cells, "cell capacity must be at least n_cells.");
+  std::vector<uint32_t> start(static_cast<size_t>(plan_.n_cells) + 1);
+  cell_of_.download(cell_of, n_, stream_);
+  cell_atoms_.download(cell_atoms, n_, stream_);
+  cell_start_.download(start.data(), start.size(), stream_);
+  synchronize();
+  for (uint32_t c = 0; c < plan_.n_cells; ++c) {
+    cell_start[c] = start[c];
+    cell_count[c] = start[c + 1] - start[c];
+  }
+}
+
+void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times_t* out) {
+  require(initialized_, "engine must be initialized before advancing.", BEADS_E_RUNTIME);
+  std::memset(out, 0, sizeof(*out));
+  hipEvent_t t0, t1;
+  BEADS_HIP(hipEventCreate(&t0));
+  BEADS_HIP(hipEventCreate(&t1));
+  std::vector<hipEvent_t> ev;
+  if (per_phase) {
+    ev.resize(4 * n_steps);
+    for (auto& e : ev) BEADS_HIP(hipEventCreate(&e));
+  }
+  BEADS_HIP(hipEventRecord(t0, stream_));
+  for (uint64_t k = 0; k < n_steps; ++k) {
+    if (!per_phase) {
+      step_once(false);
+      continue;
+    }
+    const uint64_t physical = step_ + 1;
+    const bool every = plan_.cutoff_buffer == 0.f;
+    const bool check = !every && physical % plan_.check_every == 0;
+    BEADS_HIP(hipEventRecord(ev[4 * k], stream_));
+    launch_integrate(integrate_params_, thermostat_, check, false, stream_);
+    BEADS_HIP(hipEventRecord(ev[4 * k + 1], stream_));
+    if (every || check) rebuild_sequence(every);
+    BEADS_HIP(hipEventRecord(ev[4 * k + 2], stream_));
+    launch_force(force_params_, true, false, stream_);
+    BEADS_HIP(hipEventRecord(ev[4 * k + 3], stream_));
+    step_ = physical;
+  }
+  BEADS_HIP(hipEventRecord(t1, stream_));
+  BEADS_HIP(hipEventSynchronize(t1));
+  BEADS_HIP(hipGetLastError());
+  float ms = 0.f;
+  BEADS_HIP(hipEventElapsedTime(&ms, t0, t1));
+  out->steps = n_steps;
+  out->total_s = ms * 1e-3;
+  for (uint64_t k = 0; k < n_steps && per_phase; ++k) {
+    float a = 0, b = 0, c = 0;
+    BEADS_HIP(hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]));
+    BEADS_HIP(hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 2]));
+    BEADS_HIP(hipEventElapsedTime(&c, ev[4 * k + 2], ev[4 * k + 3]));
+    out->integrate_s += a * 1e-3;
+    out->rebuild_s += b * 1e-3;
+    out->force_s += c * 1e-3;
+    out->force_launches += 1;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+}
+
+// ----------------------------------------------------------------- outputs --
+
+bool Engine::output_due(uint64_t step) const {
+  const bool thermo = !thermo_path_.empty() && step % spec_.thermo_every == 0;
+  const bool traj = !trajectory_path_.empty() && step % spec_.trajectory_every == 0;
+  return thermo || traj;
+}
+
+void Engine::stage_outputs(uint64_t step) {
+  materialize_velocities();
+  const Ctrl c = fetch_ctrl();
+  check_overflow(c);
+  if (!thermo_path_.empty() && step % spec_.thermo_every == 0) {
+    write_thermo_row(step, c);
+  }
+  if (!trajectory_path_.empty() && step % spec_.trajectory_every == 0) {
+    write_trajectory_frame(step);
+  }
+}
+
+// thermo_output.cu:437-509.
+void Engine::write_thermo_row(uint64_t step, const Ctrl& c) {
+  if (!thermo_stream_.is_open()) {
+    thermo_stream_.open(thermo_path_);
+    if (!thermo_stream_) {
+      fail(BEADS_E_RUNTIME, "failed to open thermo output file " + thermo_path_ + ".");
+    }
+    thermo_stream_ << "step,pe,pair_pe";
+    if (has_bond_style_) thermo_stream_ << ",bond_pe";
+    if (has_angle_style_) thermo_stream_ << ",angle_pe";
+    if (has_dihedral_style_) thermo_stream_ << ",dihedral_pe";
+    thermo_stream_ << ",ke,temp,press\n";
+  }
+  const float pair = static_cast<float>(c.pe[0]);
+  const float bond = has_bond_style_ ? static_cast<float>(c.pe[1]) : 0.f;
+  const float angle = has_angle_style_ ? static_cast<float>(c.pe[2]) : 0.f;
+  const float dih = has_dihedral_style_ ? static_cast<float>(c.pe[3]) : 0.f;
+  const float pe = pair + bond + angle + dih;
+  const double dof = std::max(0.0, 3.0 * n_ - 3.0);
+  const double temp = dof > 0 ? 2.0 * c.ke / (dof * units_.kb) : 0.0;
+  const double vol = static_cast<double>(box_.L[0]) * box_.L[1] * box_.L[2];
+  const double w = c.virial[0] + c.virial[1] + c.virial[2] + c.virial[3];
+  const double press = vol > 0 ? units_.pressure_scale * (2.0 * c.ke + w) / (3.0 * vol) : 0.0;
+  std::ostringstream row;
+  row << step << ',' << fmt9(pe) << ',' << fmt9(pair);
+  if (has_bond_style_) row << ',' << fmt9(bond);
+  if (has_angle_style_) row << ',' << fmt9(angle);
+  if (has_dihedral_style_) row << ',' << fmt9(dih);
+  row << ',' << fmt9(c.ke) << ',' << fmt9(temp) << ',' << fmt9(press);
+  thermo_stream_ << row.str() << '\n';
+  if (!thermo_stream_) {
+    fail(BEADS_E_RUNTIME, "failed to write thermo output file " + thermo_path_ + ".");
+  }
+  if (spec_.log_echo != BEADS_LOG_NONE) {
+    log("THERMO", row.str());
+  }
+}
+
+// trajectory_output.cu:33-46, 148-250, 555-595 (BEADS_TRAJECTORY v1).
+void Engine::write_trajectory_frame(uint64_t step) {
+  const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
+  size_t rec = sizeof(uint32_t);
+  if (mask & BEADS_TRAJ_TYPE) rec += sizeof(int32_t);
+  if (mask & BEADS_TRAJ_POSITION) rec += 3 * sizeof(float);
+  if (mask & BEADS_TRAJ_IMAGE) rec += 3 * sizeof(int32_t);
+  if (mask & BEADS_TRAJ_VELOCITY) rec += 3 * sizeof(float);
+  if (mask & BEADS_TRAJ_FORCE) rec += 3 * sizeof(float);
+  if (!trajectory_stream_.is_open()) {
+    trajectory_stream_.open(trajectory_path_, std::ios::binary | std::ios::trunc);
+    if (!trajectory_stream_) {
+      fail(BEADS_E_RUNTIME, "failed to open trajectory output file " + trajectory_path_ + ".");
+    }
+    auto& o = trajectory_stream_;
+    write_raw(o, "BEADS_TRAJECTORY", 16, "trajectory magic");
+    write_scalar<uint32_t>(o, 1, "trajectory version");
+    write_scalar<uint32_t>(o, 0x01020304u, "trajectory endian marker");
+    write_scalar<uint32_t>(o, sizeof(float), "trajectory real size");
+    write_scalar<uint32_t>(o, sizeof(uint32_t), "trajectory index size");
+    write_scalar<uint32_t>(o, sizeof(int32_t), "trajectory image size");
+    write_scalar<uint32_t>(o, sizeof(int32_t), "trajectory type size");
+    write_scalar<uint32_t>(o, sizeof(uint32_t), "trajectory tag size");
+    write_scalar<uint64_t>(o, units_.name.size(), "trajectory units length");
+    write_raw(o, units_.name.data(), units_.name.size(), "trajectory units");
+    write_scalar<uint32_t>(o, mask, "trajectory field mask");
+    write_scalar<uint64_t>(o, n_, "trajectory particle count");
+    write_scalar<uint64_t>(o, rec, "trajectory record bytes");
+  }
+  const size_t n = n_;
+  std::vector<float> pos(3 * n), vel(3 * n), force(3 * n);
+  std::vector<int32_t> img(3 * n), type(n);
+  std::vector<uint32_t> tag(n);
+  beads_state_out_t out{};
+  out.positions = pos.data();
+  out.velocities = vel.data();
+  out.forces = force.data();
+  out.images = img.data();
+  out.tags = tag.data();
+  out.types = type.data();
+  read_state(out);
+  std::vector<unsigned char> buf(rec * n);
+  for (size_t i = 0; i < n; ++i) {
+    unsigned char* r = buf.data() + i * rec;
+    size_t off = 0;
+    auto put = [&](const void* src, size_t bytes) {
+      std::memcpy(r + off, src, bytes);
+      off += bytes;
+    };
+    put(&tag[i], 4);
+    if (mask & BEADS_TRAJ_TYPE) put(&type[i], 4);
+    if (mask & BEADS_TRAJ_POSITION) put(&pos[3 * i], 12);
+    if (mask & BEADS_TRAJ_IMAGE) put(&img[3 * i], 12);
+    if (mask & BEADS_TRAJ_VELOCITY) put(&vel[3 * i], 12);
+    if (mask & BEADS_TRAJ_FORCE) put(&force[3 * i], 12);
+  }
+  auto& o = trajectory_stream_;
+  write_scalar<uint64_t>(o, step, "trajectory frame step");
+  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
+  write_raw(o, lo, 3, "trajectory box lower");
+  write_raw(o, hi, 3, "trajectory box upper");
+  write_raw(o, buf.data(), buf.size(), "trajectory records");
+}
+
+// final_state_output.cu:103-185 (BEADS_SNAPSHOT v1, SoA in slot order).
+void Engine::write_final_state() {
+  const size_t n = n_;
+  std::vector<float> pos(3 * n), vel(3 * n), mass(n);
+  std::vector<int32_t> img(3 * n), type(n);
+  std::vector<uint32_t> tag(n), molid(n);
+  beads_state_out_t out{};
+  out.positions = pos.data();
+  out.velocities = vel.data();
+  out.images = img.data();
+  out.tags = tag.data();
+  out.types = type.data();
+  out.molecule_ids = molid.data();
+  out.masses = mass.data();
+  read_state(out);
+  std::ofstream o(final_state_path_, std::ios::binary | std::ios::trunc);
+  if (!o) {
+    fail(BEADS_E_RUNTIME, "failed to open final-state output file " + final_state_path_ + ".");
+  }
+  write_raw(o, "BEADS_SNAPSHOT", 14, "snapshot magic");
+  write_scalar<uint32_t>(o, 1, "snapshot version");
+  write_scalar<uint32_t>(o, sizeof(float), "snapshot real size");
+  write_scalar<uint32_t>(o, sizeof(uint32_t), "snapshot index size");
+  write_scalar<uint32_t>(o, sizeof(int32_t), "snapshot image size");
+  write_scalar<uint32_t>(o, sizeof(int32_t), "snapshot type size");
+  write_scalar<uint32_t>(o, sizeof(uint32_t), "snapshot tag size");
+  write_scalar<uint32_t>(o, 0x01020304u, "snapshot endian marker");
+  write_scalar<uint64_t>(o, units_.name.size(), "snapshot units length");
+  write_raw(o, units_.name.data(), units_.name.size(), "snapshot units");
+  write_scalar<uint64_t>(o, step_, "snapshot step");
+  write_scalar<uint64_t>(o, n, "snapshot particle count");
+  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
+  write_raw(o, lo, 3, "snapshot box lower");
+  write_raw(o, hi, 3, "snapshot box upper");
+  write_raw(o, tag.data(), n, "snapshot tag");
+  write_raw(o, type.data(), n, "snapshot type");
+  write_raw(o, molid.data(), n, "snapshot molecule_id");
+  write_raw(o, mass.data(), n, "snapshot mass");
+  std::vector<float> col(n);
+  for (int a = 0; a < 3; ++a) {
+    for (size_t i = 0; i < n; ++i) col[i] = pos[3 * i + a];
+    write_raw(o, col.data(), n, "snapshot position");
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (size_t i = 0; i < n; ++i) col[i] = vel[3 * i + a];
+    write_raw(o, col.data(), n, "snapshot velocity");
+  }
+  std::vector<int32_t> icol(n);
+  for (int a = 0; a < 3; ++a) {
+    for (size_t i = 0; i < n; ++i) icol[i] = img[3 * i + a];
+    write_raw(o, icol.data(), n, "snapshot image");
+  }
+  o.flush();
+  if (!o) {
+    fail(BEADS_E_RUNTIME, "failed to write final-state output file " + final_state_path_ + ".");
+  }
+}
+
+// log_output.cpp:233-258: "[BEADS] LEVEL message", blank line between blocks.
+void Engine::log(const std::string& level, const std::string& message) {
+  const int echo = spec_.log_echo;
+  if (echo == BEADS_LOG_NONE) return;
+  const bool screen = echo == BEADS_LOG_SCREEN || echo == BEADS_LOG_BOTH;
+  const bool file = echo == BEADS_LOG_FILE || echo == BEADS_LOG_BOTH;
+  const bool separate = log_wrote_any_ && level != "THERMO";
+  const std::string line = "[BEADS] " + level + " " + message;
+  if (screen) {
+    if (separate) std::cout << '\n';
+    std::cout << line << '\n';
+  }
+  if (file) {
+    if (!log_stream_.is_open()) {
+      log_stream_.open(log_path_);
+      if (!log_stream_) fail(BEADS_E_RUNTIME, "failed to open log output file " + log_path_ + ".");
+    }
+    if (separate) log_stream_ << '\n';
+    log_stream_ << line << '\n';
+  }
+  log_wrote_any_ = true;
+}
+
+void Engine::poll_interrupt() {
+  if (spec_.interrupt != nullptr && spec_.interrupt(spec_.interrupt_context) != 0) {
+    (void)hipStreamSynchronize(stream_);
+    fail(BEADS_E_INTERRUPTED, "interrupted");
+  }
+}
+
+// SimulationRun::execute (simulation_run.cpp:158-244).
+void Engine::run() {
+  const uint64_t runsteps = spec_.runsteps;
+  const bool any_output = !thermo_path_.empty() || !trajectory_path_.empty() ||
+                          !final_state_path_.empty() || spec_.log_echo != BEADS_LOG_NONE;
+  if (runsteps == 0 && !any_output) {
+    return;
+  }
+  run_start_ = std::chrono::steady_clock::now();
+  {
+    std::ostringstream m;
+    m << "execute start\n  units: " << units_.name << "\n  particles: " << n_
+      << "\n  runsteps: " << runsteps;
+    log("INFO", m.str());
+    std::ostringstream nm;
+    nm << "neighbor summary\n  cutoff_buffer: " << plan_.cutoff_buffer << "\n  path: "
+       << (plan_.path == 0 ? "direct" : (plan_.path == 1 ? "regular_cell" : "small_dimension"))
+       << "\n  rebuild_check_every: " << plan_.check_every
+       << "\n  sort_every_rebuild: " << plan_.sort_every
+       << "\n  max_neighbors: " << plan_.max_neighbors;
+    log("INFO", nm.str());
+  }
+  const bool thermo0 = !thermo_path_.empty();
+  const bool traj_force0 = !trajectory_path_.empty() && (spec_.trajectory_fields & BEADS_TRAJ_FORCE);
+  if (runsteps > 0 || thermo0 || traj_force0) {
+    initialize(thermo0);
+  }
+  if (output_due(0)) {
+    stage_outputs(0);
+  }
+  for (uint64_t s = 0; s < runsteps; ++s) {
+    if (s % interrupt_interval_ == 0) {
+      poll_interrupt();
+    }
+    const uint64_t physical = step_ + 1;
+    const bool thermo_due = !thermo_path_.empty() && physical % spec_.thermo_every == 0;
+    step_once(thermo_due);
+    if (output_due(step_)) {
+      stage_outputs(step_);
+    } else if (step_ % kOverflowPollSteps == 0) {
+      check_overflow(fetch_ctrl());
+    }
+  }
+  BEADS_HIP(hipGetLastError());
+  materialize_velocities();
+  const Ctrl c = fetch_ctrl();
+  if (initialized_) check_overflow(c);
+  if (!final_state_path_.empty()) {
+    write_final_state();
+  }
+  if (thermo_stream_.is_open()) thermo_stream_.flush();
+  if (trajectory_stream_.is_open()) trajectory_stream_.flush();
+  const double wall =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - run_start_).count();
+  std::ostringstream m;
+  m << "execute end\n  runsteps: " << runsteps << "\n  run_wall_s: " << std::setprecision(9)
+    << wall << "\n  steps_per_second: ";
+  if (runsteps) m << std::setprecision(9) << static_cast<double>(runsteps) / wall;
+  else m << "not_applicable";
+  m << "\n  neighbor_build_count: " << c.build_count
+    << "\n  dangerous_rebuild_count: " << c.dangerous;
+  log("INFO", m.str());
+  if (log_stream_.is_open()) log_stream_.flush();
+  std::cout.flush();
+}
+
+}  // namespace beads_amd

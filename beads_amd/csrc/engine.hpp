@@ -1,0 +1,156 @@
+// Host engine: owns device state for one simulation and drives the gfx950
+// kernels. Counterpart of the reference's SimulationRun together with the
+// NeighborSystem, ForceFieldModel and DynamicsProgram it holds
+// (native/src/simulation/simulation_run.hpp:20-64).
+#pragma once
+
+#include <chrono>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace beads_amd {
+
+// unit_system.hpp:27-49, unit_system.cpp:9-40.
+struct UnitSystem {
+  std::string name;
+  double kb;
+  double pressure_scale;
+};
+
+UnitSystem unit_system_from(int32_t units);
+
+// neighbor_plan.cpp:182-236: search cutoff, cell grid, path choice.
+struct NeighborPlan {
+  float force_cutoff = 0.f;
+  float cutoff_buffer = 0.f;
+  float search_cutoff = 0.f;
+  float search_cutoff_sq = 0.f;
+  uint32_t nx = 1, ny = 1, nz = 1, n_cells = 1;
+  float cell_size[3] = {0, 0, 0};
+  float inv_cell[3] = {0, 0, 0};
+  int32_t path = 0;  // 0 direct, 1 regular cell, 2 small dimension
+  uint32_t max_neighbors = 0;
+  uint64_t check_every = 1;
+  uint64_t sort_every = 10;
+};
+
+NeighborPlan make_neighbor_plan(const BoxView& box, float force_cutoff, float buffer,
+                                uint32_t n, uint32_t max_neighbors, uint64_t check_every,
+                                uint64_t sort_every);
+BoxView make_box_view(const float* box_bound);
+
+// Per-slot listed-term roles and exclusion partners built on the host from the
+// tag-based topology (the device keeps them current across reorders).
+struct HostTopology {
+  uint32_t bcap = 0, acap = 0, dcap = 0, ecap = 0;
+  std::vector<uint32_t> count;  // packed counts per slot
+  std::vector<uint2> bond;      // [bcap][n]
+  std::vector<uint4> angle;     // [acap][n]
+  std::vector<uint4> dih;       // [dcap][n]
+  std::vector<uint32_t> excl;   // [ecap][n]
+  uint64_t excluded_pairs = 0;
+};
+
+HostTopology build_host_topology(const beads_spec_t& spec);
+
+// Bond-graph exclusion pairs (tag_a < tag_b) within the given distance
+// (exclusions.cu:514-598).
+std::vector<std::pair<uint32_t, uint32_t>> bond_graph_exclusions(uint32_t n, uint32_t n_bonds,
+                                                                 const uint32_t* bonds,
+                                                                 int32_t distance);
+
+class Engine {
+ public:
+  explicit Engine(const beads_spec_t& spec);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  void initialize(bool observables);
+  void advance(uint64_t n_steps, bool observables_on_last);
+  void run();  // the whole simulation with outputs
+  void synchronize();
+  void stats(beads_engine_stats_t* out);
+  void read_state(const beads_state_out_t& out);
+  void read_neighbors(uint32_t* counts, uint32_t* list, uint64_t capacity);
+  void read_cells(uint32_t* cell_of, uint32_t* cell_atoms, uint32_t* cell_start,
+                  uint32_t* cell_count, uint64_t capacity);
+  void profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times_t* out);
+
+ private:
+  void upload_state();
+  void set_up_params();
+  void rebuild_sequence(bool forced);
+  void step_once(bool observables);
+  void materialize_velocities();
+  Ctrl fetch_ctrl();
+  void check_overflow(const Ctrl& c);
+  bool output_due(uint64_t step) const;
+  void stage_outputs(uint64_t step);
+  void write_thermo_row(uint64_t step, const Ctrl& c);
+  void write_trajectory_frame(uint64_t step);
+  void write_final_state();
+  void log(const std::string& level, const std::string& message);
+  void poll_interrupt();
+
+  // configuration
+  beads_spec_t spec_{};
+  std::string thermo_path_, trajectory_path_, final_state_path_, log_path_;
+  UnitSystem units_;
+  BoxView box_{};
+  float spec_box_hi_[3] = {0.f, 0.f, 0.f};  // upper bounds exactly as given
+  NeighborPlan plan_;
+  uint32_t n_ = 0;
+  bool thermostat_ = false;
+  bool has_bond_style_ = false, has_angle_style_ = false, has_dihedral_style_ = false;
+  uint64_t step_ = 0;
+  bool initialized_ = false;
+  uint64_t excluded_pairs_ = 0;
+
+  hipStream_t stream_ = nullptr;
+
+  // device state
+  DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
+  DeviceBuffer<int4> image_, ref_image_, s_image_;
+  DeviceBuffer<uint32_t> tag_, molid_, s_tag_, s_molid_;
+  DeviceBuffer<uint32_t> nbr_count_, nbr_;
+  DeviceBuffer<uint32_t> cell_of_, s_cell_of_, cell_count_, cell_start_, cell_cursor_, cell_atoms_;
+  DeviceBuffer<uint32_t> new_of_old_;
+  DeviceBuffer<uint32_t> topo_count_, s_topo_count_, excl_, s_excl_;
+  DeviceBuffer<uint2> bond_, s_bond_;
+  DeviceBuffer<uint4> angle_, s_angle_, dih_, s_dih_;
+  DeviceBuffer<float4> lj_;
+  DeviceBuffer<float2> bond_coeff_, angle_coeff_;
+  DeviceBuffer<float4> dih_coeff_;
+  DeviceBuffer<double> partials_;
+  DeviceBuffer<Ctrl> ctrl_;
+  PinnedBuffer<Ctrl> ctrl_host_;
+  uint32_t bcap_ = 0, acap_ = 0, dcap_ = 0, ecap_ = 0;
+  bool has_topology_ = false;
+
+  // kernel parameter blocks
+  ForceParams force_params_{};
+  IntegrateParams integrate_params_{};
+  BuildParams build_params_{};
+  BinParams bin_params_{};
+  CellScanParams scan_params_{};
+  ReorderParams reorder_params_{};
+  ScaleParams scale_params_{};
+  NeighborControl control_{};
+
+  // outputs
+  std::ofstream thermo_stream_;
+  std::ofstream trajectory_stream_;
+  std::ofstream log_stream_;
+  bool log_wrote_any_ = false;
+  uint64_t interrupt_interval_ = 1;
+  std::chrono::steady_clock::time_point run_start_;
+};
+
+}  // namespace beads_amd

@@ -1,0 +1,61 @@
+// Host-side launchers for the gfx950 kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace beads_amd {
+
+constexpr int kBlock = 256;
+
+struct BinParams {
+  uint32_t n;
+  const float4* pos;
+  BoxView box;
+  CellGridView grid;
+  uint32_t* cell_of;
+  uint32_t* cell_count;
+  Ctrl* ctrl;
+  NeighborControl control;
+};
+
+struct CellScanParams {
+  uint32_t n;
+  uint32_t n_cells;
+  const uint32_t* cell_of;
+  uint32_t* cell_count;
+  uint32_t* cell_start;
+  uint32_t* cell_cursor;
+  uint32_t* cell_atoms;
+  Ctrl* ctrl;
+};
+
+struct ScaleParams {
+  uint32_t n;
+  float4* vel;
+  double* partials;
+  Ctrl* ctrl;
+};
+
+inline unsigned grid_for(uint32_t n, int block = kBlock) {
+  return static_cast<unsigned>((static_cast<uint64_t>(n) + block - 1) / block);
+}
+
+// Neighbour-rebuild sequence; every kernel is predicated on ctrl->rebuild so
+// the host never waits for the displacement check.
+void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, hipStream_t stream);
+void launch_bin(const BinParams& p, hipStream_t stream);
+void launch_cell_scan(const CellScanParams& p, hipStream_t stream);
+void launch_cell_scatter(const CellScanParams& p, hipStream_t stream);
+void launch_cell_sort(const CellScanParams& p, hipStream_t stream);
+void launch_reorder(const ReorderParams& p, hipStream_t stream);
+void launch_build(const BuildParams& p, hipStream_t stream);
+
+// One step's work.
+void launch_force(const ForceParams& p, bool kick, bool observables, hipStream_t stream);
+void launch_integrate(const IntegrateParams& p, bool thermostat, bool check,
+                      bool force_rebuild, hipStream_t stream);
+void launch_scale(const ScaleParams& p, bool thermostat, hipStream_t stream);
+
+}  // namespace beads_amd

@@ -1,0 +1,217 @@
+/*
+ * beads_amd C ABI -- the boundary between the Python front end and the
+ * MI355X (gfx950) engine. Plain C types only: host pointers, sizes, status
+ * codes. No exceptions cross this boundary; every entry point returns a
+ * beads_status_t and leaves a thread-local message for beads_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   beads_build_info        <- native/src/bindings/module.cpp:15-27 (_native.build_info)
+ *   beads_execute_simulation<- native/src/bindings/module.cpp:36-44 (_native.execute_simulation),
+ *                              i.e. parse_native_spec + SimulationRun::execute
+ *                              (native/src/simulation/simulation_run.cpp:237-244)
+ *   beads_engine_*          <- SimulationRun's construction, initialize_force_state and
+ *                              advance_dynamics_step (simulation_run.cpp:60-133, 212-235),
+ *                              split so tests and the bench can drive and inspect a run
+ *   beads_spec_t            <- input::SimulationSpec (native/src/input/native_spec.hpp:16-176)
+ */
+#ifndef BEADS_AMD_H
+#define BEADS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEADS_AMD_ABI_VERSION 1
+
+typedef enum beads_status {
+  BEADS_OK = 0,
+  BEADS_E_VALUE = 1,       /* -> ValueError  (std::invalid_argument in the reference) */
+  BEADS_E_TYPE = 2,        /* -> TypeError */
+  BEADS_E_OVERFLOW = 3,    /* -> OverflowError (std::overflow_error) */
+  BEADS_E_INDEX = 4,       /* -> IndexError (std::out_of_range) */
+  BEADS_E_RUNTIME = 5,     /* -> RuntimeError (std::runtime_error / logic_error) */
+  BEADS_E_INTERRUPTED = 6, /* the interrupt callback asked to stop (KeyboardInterrupt) */
+  BEADS_E_DEVICE = 7       /* HIP runtime error -> RuntimeError */
+} beads_status_t;
+
+typedef enum beads_units { BEADS_UNITS_REDUCED = 0, BEADS_UNITS_NM_KJMOL = 1 } beads_units_t;
+
+typedef enum beads_log_echo {
+  BEADS_LOG_SCREEN = 0,
+  BEADS_LOG_FILE = 1,
+  BEADS_LOG_BOTH = 2,
+  BEADS_LOG_NONE = 3
+} beads_log_echo_t;
+
+/* Trajectory field bits (trajectory_output.cu:39-46). */
+enum {
+  BEADS_TRAJ_TAG = 1u << 0,
+  BEADS_TRAJ_TYPE = 1u << 1,
+  BEADS_TRAJ_POSITION = 1u << 2,
+  BEADS_TRAJ_IMAGE = 1u << 3,
+  BEADS_TRAJ_VELOCITY = 1u << 4,
+  BEADS_TRAJ_FORCE = 1u << 5
+};
+
+/* Returns 0 to continue, nonzero to stop the run with BEADS_E_INTERRUPTED. */
+typedef int (*beads_interrupt_fn)(void* context);
+
+/*
+ * One simulation, fully typed. Arrays are host memory borrowed for the
+ * duration of the call (the engine copies them in). Shapes in brackets.
+ */
+typedef struct beads_spec {
+  /* system */
+  int32_t units;                 /* beads_units_t */
+  uint32_t n_particles;
+  const float* box_bound;        /* [2][3] lower row, upper row */
+  const float* positions;        /* [n][3] */
+  const float* velocities;       /* [n][3] */
+  const float* masses;           /* [n] */
+  const int32_t* types;          /* [n], dense 1-based */
+  const uint32_t* tags;          /* [n], permutation of 1..n */
+  const uint32_t* molecule_ids;  /* [n] */
+  const int32_t* images;         /* [n][3] */
+  uint32_t n_bonds;
+  const uint32_t* bonds;         /* [n_bonds][3]: tag_i, tag_j, type */
+  uint32_t n_angles;
+  const uint32_t* angles;        /* [n_angles][4]: tag_i, tag_j, tag_k, type */
+  uint32_t n_dihedrals;
+  const uint32_t* dihedrals;     /* [n_dihedrals][5]: tag_i..tag_l, type */
+
+  /* pair style "lj" */
+  int32_t n_types;
+  float lj_cutoff;
+  int32_t lj_shift;
+  const float* lj_epsilon;       /* [n_types][n_types], symmetric */
+  const float* lj_sigma;         /* [n_types][n_types], symmetric */
+
+  /* listed styles "harmonic"; a count of 0 means the style is absent */
+  int32_t n_bond_types;
+  const float* bond_k;           /* [n_bond_types] */
+  const float* bond_r0;
+  int32_t n_angle_types;
+  const float* angle_k;
+  const float* angle_theta0_deg;
+  int32_t n_dihedral_types;
+  const float* dihedral_k;
+  const int32_t* dihedral_d;     /* +1 or -1 */
+  const int32_t* dihedral_n;     /* >= 0 */
+  int32_t exclusion_distance;    /* 0 none, else 1..3 */
+
+  /* dynamics "velocity_verlet" + optional thermostat "berendsen" */
+  float dt;
+  int32_t berendsen;
+  float berendsen_temperature;
+  float berendsen_tau;
+
+  /* neighbor */
+  float cutoff_buffer;
+  uint64_t rebuild_check_every;
+  uint64_t sort_every_rebuild;
+  uint32_t max_neighbors;
+
+  /* outputs (NULL path = disabled) */
+  const char* thermo_path;
+  uint64_t thermo_every;
+  const char* trajectory_path;
+  uint64_t trajectory_every;
+  uint32_t trajectory_fields;    /* BEADS_TRAJ_* mask, TAG always set */
+  const char* final_state_path;
+  int32_t log_echo;              /* beads_log_echo_t */
+  const char* log_path;
+
+  uint64_t runsteps;
+
+  beads_interrupt_fn interrupt;  /* may be NULL */
+  void* interrupt_context;
+} beads_spec_t;
+
+typedef struct beads_build_info {
+  const char* real_precision;  /* "single" */
+  const char* real_dtype;      /* "float32" */
+  const char* type_dtype;      /* "int32" */
+  const char* index_dtype;     /* "uint32" */
+  const char* image_dtype;     /* "int32" */
+  const char* runstep_dtype;   /* "uint64" */
+  const char* device_arch;     /* "gfx950" */
+} beads_build_info_t;
+
+const char* beads_last_error(void);
+int beads_abi_version(void);
+beads_status_t beads_build_info(beads_build_info_t* out);
+
+/* Validate, run and write outputs: the whole of _native.execute_simulation. */
+beads_status_t beads_execute_simulation(const beads_spec_t* spec);
+
+/* ---- engine handle: a run that tests and the bench can step and inspect ---- */
+typedef struct beads_engine beads_engine_t;
+
+typedef struct beads_engine_stats {
+  uint64_t step;                 /* physical steps advanced */
+  uint64_t build_count;          /* neighbour builds, initial one included */
+  uint64_t dangerous_rebuild_count;
+  uint64_t reorder_count;
+  int32_t overflow;              /* neighbour-list overflow seen */
+  int32_t path;                  /* 0 direct, 1 regular cell, 2 small dimension */
+  uint32_t nx, ny, nz;           /* cell grid */
+  uint32_t max_neighbors;
+  double mean_neighbors;         /* sum(neighbor_count)/N of the current list */
+  double ke;                     /* kinetic energy after the last completed step */
+  double lambda;                 /* last Berendsen factor */
+  double pe[4];                  /* pair, bond, angle, dihedral (last observable step) */
+  double virial[4];
+  double temperature;
+  double pressure;
+} beads_engine_stats_t;
+
+/* Host-side views filled by beads_engine_read_state (each pointer may be NULL). */
+typedef struct beads_state_out {
+  float* positions;    /* [n][3] slot order */
+  float* velocities;   /* [n][3] */
+  float* forces;       /* [n][3] */
+  int32_t* images;     /* [n][3] */
+  uint32_t* tags;      /* [n] */
+  int32_t* types;      /* [n] */
+  uint32_t* molecule_ids; /* [n] */
+  float* masses;       /* [n] */
+} beads_state_out_t;
+
+beads_status_t beads_engine_create(const beads_spec_t* spec, beads_engine_t** out);
+/* Initial neighbour build and step-0 forces (simulation_run.cpp:113-119). */
+beads_status_t beads_engine_initialize(beads_engine_t* engine, int32_t with_observables);
+/* Advance n steps; observables are computed on the last one if requested. */
+beads_status_t beads_engine_advance(beads_engine_t* engine, uint64_t n_steps,
+                                    int32_t observables_on_last);
+beads_status_t beads_engine_synchronize(beads_engine_t* engine);
+beads_status_t beads_engine_stats(beads_engine_t* engine, beads_engine_stats_t* out);
+beads_status_t beads_engine_read_state(beads_engine_t* engine, const beads_state_out_t* out);
+/* counts[n]; list is slot-major [max_neighbors][n] (neighbor_list.cuh:12-19). */
+beads_status_t beads_engine_read_neighbors(beads_engine_t* engine, uint32_t* counts,
+                                           uint32_t* list, uint64_t list_capacity);
+/* cell_of[n], cell_atoms[n] (atoms in cell order), cell_start/cell_count[n_cells]. */
+beads_status_t beads_engine_read_cells(beads_engine_t* engine, uint32_t* cell_of,
+                                       uint32_t* cell_atoms, uint32_t* cell_start,
+                                       uint32_t* cell_count, uint64_t n_cells_capacity);
+/* Device time of n steps measured with HIP events on the engine's stream.
+ * Per-phase times are summed over the steps (events around each launch group). */
+typedef struct beads_phase_times {
+  uint64_t steps;
+  double total_s;       /* whole advance, events at both ends */
+  double integrate_s;   /* Berendsen factor + first half-kick + drift + wrap + check */
+  double rebuild_s;     /* control + binning + sort + reorder + neighbour build */
+  double force_s;       /* fused force + second half-kick + kinetic energy */
+  uint64_t force_launches;
+} beads_phase_times_t;
+beads_status_t beads_engine_profile_advance(beads_engine_t* engine, uint64_t n_steps,
+                                            int32_t per_phase, beads_phase_times_t* out);
+void beads_engine_destroy(beads_engine_t* engine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,959 @@
+// FP64 host oracle for the BEADS per-step MD path.
+//
+// TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker. The product path never links or
+// calls it.
+//
+// Every function restates the reference's algorithm; the citation next to it
+// names the reference file:line (paths relative to the reference's native/src).
+// Build with -ffp-contract=off: the binary32 decision expressions below must be
+// evaluated exactly in the written order, like the reference's source text.
+//
+// Parity pins (what checks each function; all in tests/test_oracle_*.py):
+//   plan            closed-form cell counts / path choice / error on r_s > L/2
+//   cell ids, sort  brute-force recount, permutation + stability invariants
+//   exclusions      brute-force all-pairs shortest path on small graphs
+//   neighbour build three paths agree with each other and with an O(N^2) scan
+//   forces          known answers (LJ minimum, U(sigma)=0, shift), central FD of
+//                   the energy, sum of forces = 0, f64-vs-f32-input agreement
+//   vv / berendsen  closed forms, lambda = 1 at T = T0, NVE energy drift
+#include "oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_error;
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// box_geometry.cpp:9-17 (binary32 like the reference's real_t).
+struct BoxF {
+  float lo[3];
+  float L[3];
+  float hL[3];
+  float invL[3];
+};
+
+BoxF make_box(const float lo[3], const float hi[3]) {
+  BoxF b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = lo[a];
+    b.L[a] = hi[a] - lo[a];
+    b.hL[a] = 0.5f * b.L[a];
+    b.invL[a] = 1.0f / b.L[a];
+  }
+  return b;
+}
+
+// box_geometry_view.cuh:50-62, minimum image for wrapped coordinates.
+inline float mic_f(float d, float L, float hL) {
+  if (d > hL) {
+    d -= L;
+  } else if (d < -hL) {
+    d += L;
+  }
+  return d;
+}
+
+inline double mic_d(double d, double L, double hL) {
+  if (d > hL) {
+    d -= L;
+  } else if (d < -hL) {
+    d += L;
+  }
+  return d;
+}
+
+template <typename Real>
+inline Real mic_any(Real d, Real L, Real hL) {
+  if (d > hL) {
+    d -= L;
+  } else if (d < -hL) {
+    d += L;
+  }
+  return d;
+}
+
+// neighbor_plan.cpp:29-49 (count computed in double), :133-166, :107-115,
+// :90-105 and the constructor checks :182-236.
+bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
+               uint32_t n, orc_plan_t* p) {
+  const BoxF b = make_box(lo, hi);
+  if (n == 0) {
+    g_error = "neighbor n_particles must be positive.";
+    return false;
+  }
+  if (!std::isfinite(rc) || rc <= 0.f) {
+    g_error = "neighbor force_cutoff must be finite and positive.";
+    return false;
+  }
+  if (!std::isfinite(buffer) || buffer < 0.f) {
+    g_error = "neighbor cutoff_buffer must be finite and non-negative.";
+    return false;
+  }
+  const float rs = rc + buffer;
+  const float rs2 = rs * rs;
+  const float half_shortest = 0.5f * std::min(b.L[0], std::min(b.L[1], b.L[2]));
+  if (rs > half_shortest) {
+    g_error = "neighbor search_cutoff must not exceed half shortest box length.";
+    return false;
+  }
+  uint32_t counts[3];
+  for (int a = 0; a < 3; ++a) {
+    const double raw = std::floor(static_cast<double>(b.L[a]) / static_cast<double>(rs));
+    counts[a] = raw < 1.0 ? 1u : static_cast<uint32_t>(raw);
+    p->cell_size[a] = b.L[a] / static_cast<float>(counts[a]);
+    p->inv_cell[a] = 1.0f / p->cell_size[a];
+    if (counts[a] >= 3) {
+      const float tol = std::max(1.0e-6f, rs * 1.0e-5f);
+      if (p->cell_size[a] + tol < rs) {
+        g_error = "neighbor cell size must be at least neighbor search_cutoff.";
+        return false;
+      }
+    }
+  }
+  p->nx = counts[0];
+  p->ny = counts[1];
+  p->nz = counts[2];
+  p->n_cells = counts[0] * counts[1] * counts[2];
+  p->force_cutoff = rc;
+  p->search_cutoff = rs;
+  p->search_cutoff_sq = rs2;
+  if (counts[0] < 3 && counts[1] < 3 && counts[2] < 3) {
+    p->path = ORC_PATH_DIRECT;
+  } else if (counts[0] >= 3 && counts[1] >= 3 && counts[2] >= 3) {
+    p->path = ORC_PATH_REGULAR;
+  } else {
+    p->path = ORC_PATH_SMALL;
+  }
+  return true;
+}
+
+// cell_list_build.cu:110-123.
+inline uint32_t axis_cell(float x, float lo, float inv, uint32_t count) {
+  const float s = (x - lo) * inv;
+  uint32_t i = s <= 0.f ? 0u : static_cast<uint32_t>(s);
+  if (i >= count) {
+    i = count - 1;
+  }
+  return i;
+}
+
+// regular_cell_neighbor_builder.cu:40-53 (wrapped neighbour cell).
+inline uint32_t wrapped_axis(uint32_t c, int delta, uint32_t count) {
+  if (delta < 0) {
+    return c == 0 ? count - 1 : c - 1;
+  }
+  if (delta > 0) {
+    const uint32_t next = c + 1;
+    return next == count ? 0 : next;
+  }
+  return c;
+}
+
+// small_dimension_neighbor_builder.cu:56-76 (per-axis unique cells).
+inline int unique_axis(uint32_t c, uint32_t count, uint32_t out[3]) {
+  if (count <= 1) {
+    out[0] = 0;
+    return 1;
+  }
+  if (count == 2) {
+    out[0] = c == 0 ? 1 : 0;
+    out[1] = c;
+    return 2;
+  }
+  out[0] = wrapped_axis(c, -1, count);
+  out[1] = c;
+  out[2] = wrapped_axis(c, 1, count);
+  return 3;
+}
+
+// Stable counting sort by (cell, slot). The reference's scatter
+// (cell_list_build.cu:168-181) is atomic and unordered inside a cell; the
+// engine under test orders each cell by slot, which this restates.
+void stable_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
+                  std::vector<uint32_t>& order, std::vector<uint32_t>& counts,
+                  std::vector<uint32_t>& offsets) {
+  counts.assign(n_cells, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    ++counts[cell[i]];
+  }
+  offsets.assign(n_cells, 0);
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < n_cells; ++c) {
+    offsets[c] = run;
+    run += counts[c];
+  }
+  std::vector<uint32_t> cursor = offsets;
+  order.assign(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    order[cursor[cell[i]]++] = i;
+  }
+}
+
+// exclusions.cu:514-598: breadth-first search on the bond graph from every tag;
+// every partner first reached at depth <= distance forms an excluded pair.
+std::vector<std::pair<uint32_t, uint32_t>> exclusion_pairs(
+    uint32_t n, uint32_t n_bonds, const uint32_t* bonds, int distance) {
+  std::vector<std::vector<uint32_t>> adj(static_cast<size_t>(n) + 1);
+  for (uint32_t b = 0; b < n_bonds; ++b) {
+    const uint32_t a = bonds[3 * b];
+    const uint32_t c = bonds[3 * b + 1];
+    adj[a].push_back(c);
+    adj[c].push_back(a);
+  }
+  for (auto& v : adj) {
+    std::sort(v.begin(), v.end());
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> out;
+  if (distance <= 0) {
+    return out;
+  }
+  std::vector<uint32_t> seen(static_cast<size_t>(n) + 1, 0);
+  std::vector<std::pair<uint32_t, int>> queue;
+  uint32_t gen = 0;
+  for (uint32_t start = 1; start <= n; ++start) {
+    ++gen;
+    queue.clear();
+    queue.emplace_back(start, 0);
+    seen[start] = gen;
+    for (size_t r = 0; r < queue.size(); ++r) {
+      const uint32_t t = queue[r].first;
+      const int depth = queue[r].second;
+      if (depth == distance) {
+        continue;
+      }
+      for (uint32_t p : adj[t]) {
+        if (seen[p] == gen) {
+          continue;
+        }
+        seen[p] = gen;
+        out.emplace_back(std::min(start, p), std::max(start, p));
+        queue.emplace_back(p, depth + 1);
+      }
+    }
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+// Slot-level exclusion lookup (exclusions.cuh:151-192 restated with a
+// per-slot sorted partner vector).
+struct SlotExclusions {
+  std::vector<std::vector<uint32_t>> partners;  // by slot, sorted slots
+  bool any = false;
+  bool excluded(uint32_t i, uint32_t j) const {
+    if (!any) {
+      return false;
+    }
+    const auto& v = partners[i];
+    return std::binary_search(v.begin(), v.end(), j);
+  }
+};
+
+SlotExclusions slot_exclusions(const orc_system_t* s, const uint32_t* tags) {
+  SlotExclusions e;
+  if (s->exclusion_distance <= 0 || s->n_bonds == 0) {
+    return e;
+  }
+  std::vector<uint32_t> slot_of_tag(static_cast<size_t>(s->n) + 1, 0);
+  for (uint32_t i = 0; i < s->n; ++i) {
+    slot_of_tag[tags[i]] = i;
+  }
+  const auto pairs = exclusion_pairs(s->n, s->n_bonds, s->bonds, s->exclusion_distance);
+  e.partners.assign(s->n, {});
+  for (const auto& p : pairs) {
+    const uint32_t a = slot_of_tag[p.first];
+    const uint32_t b = slot_of_tag[p.second];
+    e.partners[a].push_back(b);
+    e.partners[b].push_back(a);
+  }
+  for (auto& v : e.partners) {
+    std::sort(v.begin(), v.end());
+  }
+  e.any = true;
+  return e;
+}
+
+// Candidate enumeration of the three full-list builders
+// (regular_cell_neighbor_builder.cu:80-107, small_dimension_neighbor_builder.cu:
+// 100-131, direct_neighbor_builder.cu:33-47) with the append rule of
+// full_neighbor_path_common.cuh:49-94. Pos is any accessor giving x,y,z of a
+// slot; Real is the decision arithmetic.
+template <typename Real, typename Pos>
+void build_lists(uint32_t n, const Pos& pos, const BoxF& box, const orc_plan_t& plan,
+                 int path, const SlotExclusions& excl, Real rs2,
+                 std::vector<std::vector<uint32_t>>& lists) {
+  lists.assign(n, {});
+  std::vector<uint32_t> cell(n), order, counts, offsets;
+  if (path != ORC_PATH_DIRECT) {
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t ix = axis_cell(static_cast<float>(pos.x(i)), box.lo[0], plan.inv_cell[0], plan.nx);
+      const uint32_t iy = axis_cell(static_cast<float>(pos.y(i)), box.lo[1], plan.inv_cell[1], plan.ny);
+      const uint32_t iz = axis_cell(static_cast<float>(pos.z(i)), box.lo[2], plan.inv_cell[2], plan.nz);
+      cell[i] = ix + plan.nx * (iy + plan.ny * iz);
+    }
+    stable_order(n, cell.data(), plan.n_cells, order, counts, offsets);
+  }
+  const Real L[3] = {static_cast<Real>(box.L[0]), static_cast<Real>(box.L[1]), static_cast<Real>(box.L[2])};
+  const Real hL[3] = {static_cast<Real>(box.hL[0]), static_cast<Real>(box.hL[1]), static_cast<Real>(box.hL[2])};
+
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+    const uint32_t i = static_cast<uint32_t>(ii);
+    const Real xi = pos.x(i), yi = pos.y(i), zi = pos.z(i);
+    auto& out = lists[i];
+    auto visit = [&](uint32_t j) {
+      if (i == j || excl.excluded(i, j)) {
+        return;
+      }
+      const Real dx = mic_any<Real>(xi - pos.x(j), L[0], hL[0]);
+      const Real dy = mic_any<Real>(yi - pos.y(j), L[1], hL[1]);
+      const Real dz = mic_any<Real>(zi - pos.z(j), L[2], hL[2]);
+      const Real r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > rs2) {
+        return;
+      }
+      out.push_back(j);
+    };
+    if (path == ORC_PATH_DIRECT) {
+      for (uint32_t j = 0; j < n; ++j) {
+        visit(j);
+      }
+      continue;
+    }
+    const uint32_t c = cell[i];
+    const uint32_t cx = c % plan.nx;
+    const uint32_t cy = (c / plan.nx) % plan.ny;
+    const uint32_t cz = (c / plan.nx) / plan.ny;
+    uint32_t xs[3], ys[3], zs[3];
+    int nxs, nys, nzs;
+    if (path == ORC_PATH_REGULAR) {
+      for (int d = -1; d <= 1; ++d) {
+        xs[d + 1] = wrapped_axis(cx, d, plan.nx);
+        ys[d + 1] = wrapped_axis(cy, d, plan.ny);
+        zs[d + 1] = wrapped_axis(cz, d, plan.nz);
+      }
+      nxs = nys = nzs = 3;
+    } else {
+      nxs = unique_axis(cx, plan.nx, xs);
+      nys = unique_axis(cy, plan.ny, ys);
+      nzs = unique_axis(cz, plan.nz, zs);
+    }
+    for (int a = 0; a < nzs; ++a) {
+      for (int b = 0; b < nys; ++b) {
+        for (int d = 0; d < nxs; ++d) {
+          const uint32_t nc = xs[d] + plan.nx * (ys[b] + plan.ny * zs[a]);
+          for (uint32_t k = 0; k < counts[nc]; ++k) {
+            visit(order[offsets[nc] + k]);
+          }
+        }
+      }
+    }
+  }
+}
+
+
+// Position accessors: binary32 AoS input, or binary64 AoS.
+struct PosF {
+  const float* p;
+  float x(uint32_t i) const { return p[3 * i]; }
+  float y(uint32_t i) const { return p[3 * i + 1]; }
+  float z(uint32_t i) const { return p[3 * i + 2]; }
+};
+struct PosD {
+  const double* p;
+  double x(uint32_t i) const { return p[3 * i]; }
+  double y(uint32_t i) const { return p[3 * i + 1]; }
+  double z(uint32_t i) const { return p[3 * i + 2]; }
+};
+
+// pair_lj.cu:171-192 (packing) evaluated in binary64 from the binary32
+// parameters the engine receives.
+struct LjCoeff {
+  double sigma2, eps4, eps24, shift;
+};
+
+std::vector<LjCoeff> lj_table(const orc_system_t* s) {
+  const int T = s->n_types;
+  std::vector<LjCoeff> t(static_cast<size_t>(T) * T);
+  const double rc = s->lj_cutoff;
+  for (int a = 0; a < T; ++a) {
+    for (int b = 0; b < T; ++b) {
+      const double eps = s->lj_eps[a * T + b];
+      const double sig = s->lj_sig[a * T + b];
+      LjCoeff c;
+      c.sigma2 = sig * sig;
+      c.eps4 = 4.0 * eps;
+      c.eps24 = 24.0 * eps;
+      c.shift = 0.0;
+      if (s->lj_shift) {
+        const double sr2 = c.sigma2 / (rc * rc);
+        const double sr6 = sr2 * sr2 * sr2;
+        c.shift = c.eps4 * (sr6 * sr6 - sr6);
+      }
+      t[static_cast<size_t>(a) * T + b] = c;
+    }
+  }
+  return t;
+}
+
+struct Vec {
+  double x, y, z;
+};
+inline Vec vsub(Vec a, Vec b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline Vec vadd(Vec a, Vec b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline Vec vscale(Vec a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double vdot(Vec a, Vec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec vcross(Vec a, Vec b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+template <typename Pos>
+Vec mic_vec(const Pos& pos, uint32_t i, uint32_t j, const double L[3], const double hL[3]) {
+  return {mic_d(static_cast<double>(pos.x(i)) - static_cast<double>(pos.x(j)), L[0], hL[0]),
+          mic_d(static_cast<double>(pos.y(i)) - static_cast<double>(pos.y(j)), L[1], hL[1]),
+          mic_d(static_cast<double>(pos.z(i)) - static_cast<double>(pos.z(j)), L[2], hL[2])};
+}
+
+// Forces + observables. lists[i] holds the stored neighbours of slot i;
+// the pair is evaluated if the decision arithmetic (Real) gives r2 < rc2
+// (pair_lj.cu:54), and its value in binary64 (pair_lj.cu:56-66, :128-150).
+// Listed terms follow bond_harmonic.cu:41-79, angle_harmonic.cu:61-158 and
+// dihedral_harmonic.cu:83-206; their coordinates are minimum-imaged in binary64.
+template <typename Real, typename Pos>
+void compute_forces(const orc_system_t* s, const Pos& pos,
+                    const std::vector<std::vector<uint32_t>>& lists, double* f,
+                    orc_observables_t* obs) {
+  const uint32_t n = s->n;
+  const BoxF box = make_box(s->lo, s->hi);
+  const double Ld[3] = {box.L[0], box.L[1], box.L[2]};
+  const double hLd[3] = {box.hL[0], box.hL[1], box.hL[2]};
+  const Real Lr[3] = {static_cast<Real>(box.L[0]), static_cast<Real>(box.L[1]), static_cast<Real>(box.L[2])};
+  const Real hLr[3] = {static_cast<Real>(box.hL[0]), static_cast<Real>(box.hL[1]), static_cast<Real>(box.hL[2])};
+  const Real rc = static_cast<Real>(s->lj_cutoff);
+  const Real rc2 = rc * rc;
+  const auto table = lj_table(s);
+  const int T = s->n_types;
+  double pair_pe = 0.0, pair_vir = 0.0;
+
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : pair_pe, pair_vir)
+  for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+    const uint32_t i = static_cast<uint32_t>(ii);
+    double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    const int ti = s->type[i] - 1;
+    for (uint32_t j : lists[i]) {
+      const Real dxr = mic_any<Real>(pos.x(i) - pos.x(j), Lr[0], hLr[0]);
+      const Real dyr = mic_any<Real>(pos.y(i) - pos.y(j), Lr[1], hLr[1]);
+      const Real dzr = mic_any<Real>(pos.z(i) - pos.z(j), Lr[2], hLr[2]);
+      const Real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+      if (r2r >= rc2) {
+        continue;
+      }
+      const LjCoeff& c = table[static_cast<size_t>(ti) * T + (s->type[j] - 1)];
+      const Vec d = mic_vec(pos, i, j, Ld, hLd);
+      const double r2 = vdot(d, d);
+      const double sr2 = c.sigma2 / r2;
+      const double sr6 = sr2 * sr2 * sr2;
+      const double sr12 = sr6 * sr6;
+      const double sc = c.eps24 / r2 * (2.0 * sr12 - sr6);
+      fx += sc * d.x;
+      fy += sc * d.y;
+      fz += sc * d.z;
+      pe += c.eps4 * (sr12 - sr6) - c.shift;
+      vir += sc * r2;
+    }
+    f[3 * i] = fx;
+    f[3 * i + 1] = fy;
+    f[3 * i + 2] = fz;
+    pair_pe += 0.5 * pe;
+    pair_vir += 0.5 * vir;
+  }
+
+  std::vector<uint32_t> slot_of_tag(static_cast<size_t>(n) + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    slot_of_tag[s->tag[i]] = i;
+  }
+  auto add = [&](uint32_t slot, Vec v) {
+    f[3 * slot] += v.x;
+    f[3 * slot + 1] += v.y;
+    f[3 * slot + 2] += v.z;
+  };
+
+  double bond_pe = 0, bond_vir = 0;
+  for (uint32_t b = 0; b < s->n_bonds; ++b) {
+    const uint32_t si = slot_of_tag[s->bonds[3 * b]];
+    const uint32_t sj = slot_of_tag[s->bonds[3 * b + 1]];
+    const int ty = static_cast<int>(s->bonds[3 * b + 2]) - 1;
+    const double k = s->bond_k[ty], r0 = s->bond_r0[ty];
+    const Vec d = mic_vec(pos, si, sj, Ld, hLd);
+    const double r = std::sqrt(vdot(d, d));
+    const double stretch = r - r0;
+    bond_pe += 0.5 * k * stretch * stretch;
+    if (r == 0.0) {
+      continue;
+    }
+    const Vec fi = vscale(d, -k * stretch / r);
+    add(si, fi);
+    add(sj, vscale(fi, -1.0));
+    bond_vir += vdot(d, fi);
+  }
+
+  const double sin_floor = static_cast<double>(0.001f);
+  double angle_pe = 0, angle_vir = 0;
+  for (uint32_t a = 0; a < s->n_angles; ++a) {
+    const uint32_t si = slot_of_tag[s->angles[4 * a]];
+    const uint32_t sj = slot_of_tag[s->angles[4 * a + 1]];
+    const uint32_t sk = slot_of_tag[s->angles[4 * a + 2]];
+    const int ty = static_cast<int>(s->angles[4 * a + 3]) - 1;
+    const double k = s->angle_k[ty];
+    const double theta0 = static_cast<double>(s->angle_theta0_deg[ty]) * kPi / 180.0;
+    const Vec va = mic_vec(pos, si, sj, Ld, hLd);
+    const Vec vb = mic_vec(pos, sk, sj, Ld, hLd);
+    const double ra2 = vdot(va, va), rb2 = vdot(vb, vb);
+    if (ra2 == 0.0 || rb2 == 0.0) {
+      angle_pe += 0.5 * k * theta0 * theta0;
+      continue;
+    }
+    const double ra = std::sqrt(ra2), rb = std::sqrt(rb2);
+    double c = vdot(va, vb) / (ra * rb);
+    c = std::max(-1.0, std::min(1.0, c));
+    const double theta = c == 1.0 ? 0.0 : (c == -1.0 ? kPi : std::acos(c));
+    const double dth = theta - theta0;
+    angle_pe += 0.5 * k * dth * dth;
+    const double sin_sq = 1.0 - c * c;
+    const double sn = sin_sq <= sin_floor * sin_floor ? sin_floor : std::sqrt(sin_sq);
+    const double A = -k * dth / sn;
+    const double a11 = A * c / ra2, a12 = -A / (ra * rb), a22 = A * c / rb2;
+    const Vec fi = vadd(vscale(va, a11), vscale(vb, a12));
+    const Vec fk = vadd(vscale(vb, a22), vscale(va, a12));
+    add(si, fi);
+    add(sk, fk);
+    add(sj, vscale(vadd(fi, fk), -1.0));
+    angle_vir += vdot(va, fi) + vdot(vb, fk);
+  }
+
+  double dih_pe = 0, dih_vir = 0;
+  for (uint32_t q = 0; q < s->n_dihedrals; ++q) {
+    const uint32_t si = slot_of_tag[s->dihedrals[5 * q]];
+    const uint32_t sj = slot_of_tag[s->dihedrals[5 * q + 1]];
+    const uint32_t sk = slot_of_tag[s->dihedrals[5 * q + 2]];
+    const uint32_t sl = slot_of_tag[s->dihedrals[5 * q + 3]];
+    const int ty = static_cast<int>(s->dihedrals[5 * q + 4]) - 1;
+    const double k = s->dihedral_k[ty];
+    const int dd = s->dihedral_d[ty], nn = s->dihedral_n[ty];
+    const Vec v0 = mic_vec(pos, si, sj, Ld, hLd);
+    const Vec v1 = mic_vec(pos, sk, sj, Ld, hLd);
+    const Vec v2 = mic_vec(pos, sk, sl, Ld, hLd);
+    const Vec c0 = vcross(v0, v1), c1 = vcross(v1, v2);
+    const double n1 = vdot(v1, v1), nc0 = vdot(c0, c0), nc1 = vdot(c1, c1);
+    if (n1 == 0.0 || nc0 == 0.0 || nc1 == 0.0) {
+      dih_pe += nn == 0 ? k * (1.0 + dd) : k;
+      continue;
+    }
+    double c = vdot(c0, c1) / std::sqrt(nc0 * nc1);
+    c = std::max(-1.0, std::min(1.0, c));
+    double phi = c == 1.0 ? 0.0 : (c == -1.0 ? kPi : std::acos(c));
+    if (vdot(v0, c1) < 0.0) {
+      phi = -phi;
+    }
+    const double arg = nn * phi;
+    dih_pe += k * (1.0 + dd * std::cos(arg));
+    const double dE = -k * dd * nn * std::sin(arg);
+    const double nb = std::sqrt(n1);
+    const Vec fi = vscale(c0, -dE * nb / nc0);
+    const Vec fl = vscale(c1, dE * nb / nc1);
+    const Vec sv = vsub(vscale(fi, vdot(v0, v1) / n1), vscale(fl, vdot(v2, v1) / n1));
+    const Vec fj = vsub(sv, fi);
+    const Vec fk = vscale(vadd(sv, fl), -1.0);
+    add(si, fi);
+    add(sj, fj);
+    add(sk, fk);
+    add(sl, fl);
+    dih_vir += vdot(v0, fi) + vdot(v1, fk) + vdot(vsub(v1, v2), fl);
+  }
+
+  if (obs != nullptr) {
+    obs->pair_pe = pair_pe;
+    obs->pair_virial = pair_vir;
+    obs->bond_pe = bond_pe;
+    obs->bond_virial = bond_vir;
+    obs->angle_pe = angle_pe;
+    obs->angle_virial = angle_vir;
+    obs->dihedral_pe = dih_pe;
+    obs->dihedral_virial = dih_vir;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_error.c_str(); }
+
+int orc_plan(const float lo[3], const float hi[3], float force_cutoff,
+             float cutoff_buffer, uint32_t n, orc_plan_t* out) {
+  return make_plan(lo, hi, force_cutoff, cutoff_buffer, n, out) ? 0 : 1;
+}
+
+void orc_cell_ids(uint32_t n, const float* pos, const float lo[3],
+                  const orc_plan_t* plan, uint32_t* cell) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t ix = axis_cell(pos[3 * i], lo[0], plan->inv_cell[0], plan->nx);
+    const uint32_t iy = axis_cell(pos[3 * i + 1], lo[1], plan->inv_cell[1], plan->ny);
+    const uint32_t iz = axis_cell(pos[3 * i + 2], lo[2], plan->inv_cell[2], plan->nz);
+    cell[i] = ix + plan->nx * (iy + plan->ny * iz);
+  }
+}
+
+void orc_stable_cell_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
+                           uint32_t* order, uint32_t* counts, uint32_t* offsets) {
+  std::vector<uint32_t> o, c, off;
+  stable_order(n, cell, n_cells, o, c, off);
+  std::copy(o.begin(), o.end(), order);
+  std::copy(c.begin(), c.end(), counts);
+  std::copy(off.begin(), off.end(), offsets);
+}
+
+uint64_t orc_exclusion_pairs(uint32_t n, uint32_t n_bonds, const uint32_t* bonds,
+                             int32_t distance, uint32_t* pairs, uint64_t cap) {
+  const auto p = exclusion_pairs(n, n_bonds, bonds, distance);
+  for (uint64_t k = 0; k < p.size() && k < cap; ++k) {
+    pairs[2 * k] = p[k].first;
+    pairs[2 * k + 1] = p[k].second;
+  }
+  return p.size();
+}
+
+int orc_neighbor_build(const orc_system_t* sys, float cutoff_buffer,
+                       uint32_t max_neighbors, int32_t path, uint32_t* counts,
+                       uint32_t* list, int32_t* overflow) {
+  orc_plan_t plan;
+  if (!make_plan(sys->lo, sys->hi, sys->lj_cutoff, cutoff_buffer, sys->n, &plan)) {
+    return 1;
+  }
+  if (path < 0) {
+    path = plan.path;
+  }
+  if (path == ORC_PATH_REGULAR && (plan.nx < 3 || plan.ny < 3 || plan.nz < 3)) {
+    g_error = "regular path requires at least 3 cells per axis.";
+    return 1;
+  }
+  const BoxF box = make_box(sys->lo, sys->hi);
+  const SlotExclusions excl = slot_exclusions(sys, sys->tag);
+  std::vector<std::vector<uint32_t>> lists;
+  build_lists<float>(sys->n, PosF{sys->pos}, box, plan, path, excl, plan.search_cutoff_sq, lists);
+  *overflow = 0;
+  for (uint32_t i = 0; i < sys->n; ++i) {
+    const auto& v = lists[i];
+    const uint32_t m = static_cast<uint32_t>(std::min<size_t>(v.size(), max_neighbors));
+    if (v.size() > max_neighbors) {
+      *overflow = 1;
+    }
+    counts[i] = m;
+    std::copy(v.begin(), v.begin() + m, list + static_cast<size_t>(i) * max_neighbors);
+  }
+  return 0;
+}
+
+int orc_forces(const orc_system_t* sys, const uint32_t* counts, const uint32_t* list,
+               uint32_t max_neighbors, double* forces, orc_observables_t* obs) {
+  std::vector<std::vector<uint32_t>> lists;
+  if (counts != nullptr && list != nullptr) {
+    lists.assign(sys->n, {});
+    for (uint32_t i = 0; i < sys->n; ++i) {
+      const uint32_t* row = list + static_cast<size_t>(i) * max_neighbors;
+      lists[i].assign(row, row + counts[i]);
+    }
+  } else {
+    orc_plan_t plan;
+    if (!make_plan(sys->lo, sys->hi, sys->lj_cutoff, 0.0f, sys->n, &plan)) {
+      return 1;
+    }
+    const BoxF box = make_box(sys->lo, sys->hi);
+    const SlotExclusions excl = slot_exclusions(sys, sys->tag);
+    build_lists<float>(sys->n, PosF{sys->pos}, box, plan, plan.path, excl,
+                       plan.search_cutoff_sq, lists);
+  }
+  compute_forces<float>(sys, PosF{sys->pos}, lists, forces, obs);
+  return 0;
+}
+
+int orc_forces_f64(const orc_system_t* sys, const double* pos64, double* forces,
+                   orc_observables_t* obs) {
+  orc_plan_t plan;
+  if (!make_plan(sys->lo, sys->hi, sys->lj_cutoff, 0.0f, sys->n, &plan)) {
+    return 1;
+  }
+  const BoxF box = make_box(sys->lo, sys->hi);
+  const SlotExclusions excl = slot_exclusions(sys, sys->tag);
+  std::vector<std::vector<uint32_t>> lists;
+  const double rc = sys->lj_cutoff;
+  build_lists<double>(sys->n, PosD{pos64}, box, plan, plan.path, excl, rc * rc, lists);
+  compute_forces<double>(sys, PosD{pos64}, lists, forces, obs);
+  return 0;
+}
+
+// velocity_verlet.cu:59-95; wrap box_geometry_view.cuh:92-102.
+void orc_vv_pre(uint32_t n, const float* pos, const float* vel, const float* force,
+                const float* mass, const int32_t* image, const float lo[3],
+                const float hi[3], double dt, double* pos_out, double* vel_out,
+                int32_t* image_out) {
+  const BoxF box = make_box(lo, hi);
+  for (uint32_t i = 0; i < n; ++i) {
+    const double h = 0.5 * dt / static_cast<double>(mass[i]);
+    for (int a = 0; a < 3; ++a) {
+      const double v = static_cast<double>(vel[3 * i + a]) + h * force[3 * i + a];
+      double x = static_cast<double>(pos[3 * i + a]) + dt * v;
+      const double c = std::floor((x - box.lo[a]) / static_cast<double>(box.L[a]));
+      x -= c * static_cast<double>(box.L[a]);
+      vel_out[3 * i + a] = v;
+      pos_out[3 * i + a] = x;
+      image_out[3 * i + a] = image[3 * i + a] + static_cast<int32_t>(c);
+    }
+  }
+}
+
+// velocity_verlet.cu:97-111.
+void orc_vv_post(uint32_t n, const float* vel, const float* force, const float* mass,
+                 double dt, double* vel_out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const double h = 0.5 * dt / static_cast<double>(mass[i]);
+    for (int a = 0; a < 3; ++a) {
+      vel_out[3 * i + a] = static_cast<double>(vel[3 * i + a]) + h * force[3 * i + a];
+    }
+  }
+}
+
+// berendsen_thermostat.cu:109-133 (kinetic energy).
+double orc_kinetic_energy(uint32_t n, const float* vel, const float* mass) {
+  double ke = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+    ke += 0.5 * static_cast<double>(mass[i]) * (vx * vx + vy * vy + vz * vz);
+  }
+  return ke;
+}
+
+// berendsen_thermostat.cu:104-107 (dof) and :135-163 (lambda).
+double orc_berendsen_lambda(double ke, uint32_t n, double kb, double temperature,
+                            double dt, double tau) {
+  const double dof = std::max(0.0, 3.0 * n - 3.0);
+  if (!(ke > 0.0) || !(dof > 0.0)) {
+    return 1.0;
+  }
+  const double current = 2.0 * ke / dof;
+  const double target = kb * temperature;
+  const double arg = 1.0 + (dt / tau) * (target / current - 1.0);
+  return std::sqrt(std::max(0.0, arg));
+}
+
+}  // extern "C"
+
+// A whole run in binary64 (simulation_run.cpp:189-235 step order; neighbour
+// policy neighbor_system.cpp:79-146, 198-216; displacement test
+// rebuild_tracker.cu:55-94; reorder cadence counted in rebuilds). Used as the
+// CPU baseline and for trajectory-level invariants.
+extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
+                       double* pos_out, double* vel_out, int32_t* image_out,
+                       uint32_t* tags_out, orc_run_stats_t* st) {
+#ifdef _OPENMP
+  if (p->threads > 0) {
+    omp_set_num_threads(p->threads);
+  }
+  st->threads_used = omp_get_max_threads();
+#else
+  st->threads_used = 1;
+#endif
+  const uint32_t n = sys->n;
+  orc_plan_t plan;
+  if (!make_plan(sys->lo, sys->hi, sys->lj_cutoff, p->cutoff_buffer, n, &plan)) {
+    return 1;
+  }
+  const BoxF box = make_box(sys->lo, sys->hi);
+  std::vector<double> x(3 * static_cast<size_t>(n)), v(3 * static_cast<size_t>(n)), m(n);
+  std::vector<int32_t> img(3 * static_cast<size_t>(n)), type(sys->type, sys->type + n);
+  std::vector<uint32_t> tag(sys->tag, sys->tag + n);
+  std::vector<float> mass_f(sys->mass, sys->mass + n);
+  for (size_t k = 0; k < 3 * static_cast<size_t>(n); ++k) {
+    x[k] = sys->pos[k];
+    v[k] = sys->vel[k];
+    img[k] = sys->image[k];
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    m[i] = sys->mass[i];
+  }
+  orc_system_t cur = *sys;
+  cur.type = type.data();
+  cur.tag = tag.data();
+  cur.mass = mass_f.data();
+
+  std::vector<std::vector<uint32_t>> lists;
+  std::vector<double> f(3 * static_cast<size_t>(n), 0.0), xref, imgref_dummy;
+  std::vector<int32_t> imgref;
+  SlotExclusions excl;
+  bool excl_valid = false;
+  uint64_t build_count = 0, dangerous = 0;
+  bool checked_since_build = false;
+  double neighbor_sum = 0.0;
+  orc_observables_t obs{};
+
+  auto rebuild = [&]() {
+    ++build_count;
+    if (plan.path != ORC_PATH_DIRECT && build_count % p->sort_every_rebuild == 0) {
+      std::vector<uint32_t> cell(n), order, counts, offsets;
+      for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t ix = axis_cell(static_cast<float>(x[3 * i]), box.lo[0], plan.inv_cell[0], plan.nx);
+        const uint32_t iy = axis_cell(static_cast<float>(x[3 * i + 1]), box.lo[1], plan.inv_cell[1], plan.ny);
+        const uint32_t iz = axis_cell(static_cast<float>(x[3 * i + 2]), box.lo[2], plan.inv_cell[2], plan.nz);
+        cell[i] = ix + plan.nx * (iy + plan.ny * iz);
+      }
+      stable_order(n, cell.data(), plan.n_cells, order, counts, offsets);
+      auto permute3 = [&](auto& a) {
+        auto b = a;
+        for (uint32_t s = 0; s < n; ++s) {
+          for (int c = 0; c < 3; ++c) {
+            a[3 * s + c] = b[3 * order[s] + c];
+          }
+        }
+      };
+      auto permute1 = [&](auto& a) {
+        auto b = a;
+        for (uint32_t s = 0; s < n; ++s) {
+          a[s] = b[order[s]];
+        }
+      };
+      permute3(x);
+      permute3(v);
+      permute3(img);
+      permute1(m);
+      permute1(type);
+      permute1(tag);
+      permute1(mass_f);
+      excl_valid = false;
+    }
+    if (!excl_valid) {
+      excl = slot_exclusions(&cur, tag.data());
+      excl_valid = true;
+    }
+    build_lists<double>(n, PosD{x.data()}, box, plan, plan.path, excl,
+                        static_cast<double>(plan.search_cutoff) * plan.search_cutoff, lists);
+    size_t total = 0;
+    for (const auto& l : lists) {
+      total += std::min<size_t>(l.size(), p->max_neighbors);
+      if (l.size() > p->max_neighbors) {
+        g_error = "NeighborList overflow";
+        return false;
+      }
+    }
+    neighbor_sum += static_cast<double>(total) / n;
+    xref = x;
+    imgref = img;
+    checked_since_build = false;
+    return true;
+  };
+
+  const double dt = p->dt;
+  const double half_skin = 0.5 * static_cast<double>(p->cutoff_buffer);
+  const double thr2 = half_skin * half_skin;
+
+  if (!rebuild()) {
+    return 2;
+  }
+  compute_forces<double>(&cur, PosD{x.data()}, lists, f.data(), &obs);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t step = 1; step <= p->steps; ++step) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+      const size_t i = static_cast<size_t>(ii);
+      const double h = 0.5 * dt / m[i];
+      for (int a = 0; a < 3; ++a) {
+        v[3 * i + a] += h * f[3 * i + a];
+        double xa = x[3 * i + a] + dt * v[3 * i + a];
+        const double c = std::floor((xa - box.lo[a]) / static_cast<double>(box.L[a]));
+        xa -= c * static_cast<double>(box.L[a]);
+        x[3 * i + a] = xa;
+        img[3 * i + a] += static_cast<int32_t>(c);
+      }
+    }
+    bool need = false;
+    if (p->cutoff_buffer == 0.0f) {
+      need = true;
+    } else if (step % p->rebuild_check_every == 0) {
+      const bool first = !checked_since_build;
+      checked_since_build = true;
+      int exceeded = 0;
+#pragma omp parallel for schedule(static) reduction(| : exceeded)
+      for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+        const size_t i = static_cast<size_t>(ii);
+        double d2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          const double d = (x[3 * i + a] - xref[3 * i + a]) +
+                           static_cast<double>(img[3 * i + a] - imgref[3 * i + a]) * box.L[a];
+          d2 += d * d;
+        }
+        exceeded |= d2 > thr2 ? 1 : 0;
+      }
+      if (exceeded) {
+        need = true;
+        if (p->rebuild_check_every > 1 && first) {
+          ++dangerous;
+        }
+      }
+    }
+    if (need && !rebuild()) {
+      return 2;
+    }
+    compute_forces<double>(&cur, PosD{x.data()}, lists, f.data(), &obs);
+    double ke = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : ke)
+    for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+      const size_t i = static_cast<size_t>(ii);
+      const double h = 0.5 * dt / m[i];
+      double v2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        v[3 * i + a] += h * f[3 * i + a];
+        v2 += v[3 * i + a] * v[3 * i + a];
+      }
+      ke += 0.5 * m[i] * v2;
+    }
+    if (p->thermostat == 1) {
+      const double lam = orc_berendsen_lambda(ke, n, p->kb, p->temperature, dt, p->tau);
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < static_cast<int64_t>(3 * static_cast<size_t>(n)); ++k) {
+        v[static_cast<size_t>(k)] *= lam;
+      }
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  st->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
+  st->build_count = build_count;
+  st->dangerous_rebuild_count = dangerous;
+  st->mean_neighbors = build_count ? neighbor_sum / static_cast<double>(build_count) : 0.0;
+  double ke = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double v2 = v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2];
+    ke += 0.5 * m[i] * v2;
+  }
+  st->final_ke = ke;
+  st->final_obs = obs;
+  std::copy(x.begin(), x.end(), pos_out);
+  std::copy(v.begin(), v.end(), vel_out);
+  std::copy(img.begin(), img.end(), image_out);
+  std::copy(tag.begin(), tag.end(), tags_out);
+  return 0;
+}
