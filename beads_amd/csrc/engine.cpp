@@ -676,6 +676,7 @@ void Engine::initialize(bool observables) {
   rebuild_sequence(true);
   launch_force(force_params_, false, observables, stream_);
   BEADS_HIP(hipGetLastError());
+  ke_from_scale_ = false;
   initialized_ = true;
 }
 
@@ -691,6 +692,7 @@ void Engine::step_once(bool observables) {
     rebuild_sequence(every);
   }
   launch_force(force_params_, true, observables, stream_);
+  ke_from_scale_ = false;
   step_ = physical;
 }
 
@@ -704,6 +706,7 @@ void Engine::advance(uint64_t n_steps, bool observables_on_last) {
 
 void Engine::materialize_velocities() {
   launch_scale(scale_params_, thermostat_, stream_);
+  ke_from_scale_ = true;
 }
 
 void Engine::synchronize() { BEADS_HIP(hipStreamSynchronize(stream_)); }
@@ -742,17 +745,18 @@ void Engine::stats(beads_engine_stats_t* out) {
   out->nz = plan_.nz;
   out->max_neighbors = plan_.max_neighbors;
   out->mean_neighbors = static_cast<double>(c.neighbor_total) / n_;
-  out->ke = c.ke;
+  // Kinetic energy of the velocities as stored (before any pending factor).
+  out->ke = ke_from_scale_ ? c.ke : c.ke_pre;
   out->lambda = c.lambda;
   for (int q = 0; q < 4; ++q) {
     out->pe[q] = c.pe[q];
     out->virial[q] = c.virial[q];
   }
   const double dof = std::max(0.0, 3.0 * n_ - 3.0);
-  out->temperature = dof > 0 ? 2.0 * c.ke / (dof * units_.kb) : 0.0;
+  out->temperature = dof > 0 ? 2.0 * out->ke / (dof * units_.kb) : 0.0;
   const double vol = static_cast<double>(box_.L[0]) * box_.L[1] * box_.L[2];
   const double w = c.virial[0] + c.virial[1] + c.virial[2] + c.virial[3];
-  out->pressure = units_.pressure_scale * (2.0 * c.ke + w) / (3.0 * vol);
+  out->pressure = units_.pressure_scale * (2.0 * out->ke + w) / (3.0 * vol);
 }
 
 void Engine::read_state(const beads_state_out_t& o) {

@@ -111,6 +111,7 @@ class Engine {
   bool has_bond_style_ = false, has_angle_style_ = false, has_dihedral_style_ = false;
   uint64_t step_ = 0;
   bool initialized_ = false;
+  bool ke_from_scale_ = false;  // which reduction last measured the stored velocities
   uint64_t excluded_pairs_ = 0;
 
   hipStream_t stream_ = nullptr;

@@ -449,7 +449,12 @@ __device__ __forceinline__ float3 mic3(float3 d, const BoxView& b) {
 }
 __device__ __forceinline__ float clamp_cos(float c) { return c > 1.f ? 1.f : (c < -1.f ? -1.f : c); }
 
-// Harmonic angle (angle_harmonic.cu:61-158) evaluated for one role.
+// Harmonic angle U = k/2 (theta - theta0)^2 (angle_harmonic.cu:61-158) for one
+// role. Same mathematics as the reference, evaluated in a binary32-stable form:
+// theta = atan2(|a x b|, a.b) and sin(theta) = |a x b| / (|a||b|) instead of
+// acos(c) and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
+// instead of differences of nearly parallel vectors. The reference's floor
+// (sin theta replaced by 0.001 at or below 0.001) is kept.
 __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 xk, float2 kc,
                                                    int role, const BoxView& box, float& energy,
                                                    float& virial) {
@@ -463,16 +468,16 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
     return make_float3(0.f, 0.f, 0.f);
   }
   const float ra = sqrtf(ra2), rb = sqrtf(rb2);
-  const float c = clamp_cos(dot3(a, b) / (ra * rb));
-  const float theta = c == 1.f ? 0.f : (c == -1.f ? kPiF : acosf(c));
+  const float3 nrm = cross3(a, b);
+  const float nn = sqrtf(dot3(nrm, nrm));
+  const float theta = atan2f(nn, dot3(a, b));
   const float dth = theta - kc.y;
   energy = 0.5f * kc.x * dth * dth;
-  const float sin_sq = 1.f - c * c;
-  const float sn = sin_sq <= 0.001f * 0.001f ? 0.001f : sqrtf(sin_sq);
+  const float sn_true = nn / (ra * rb);
+  const float sn = sn_true <= 0.001f ? 0.001f : sn_true;
   const float A = -kc.x * dth / sn;
-  const float a11 = A * c / ra2, a12 = -A / (ra * rb), a22 = A * c / rb2;
-  const float3 fi = add3(scale3(a, a11), scale3(b, a12));
-  const float3 fk = add3(scale3(b, a22), scale3(a, a12));
+  const float3 fi = scale3(cross3(a, nrm), A / (ra2 * ra * rb));
+  const float3 fk = scale3(cross3(b, nrm), -A / (rb2 * rb * ra));
   virial = dot3(a, fi) + dot3(b, fk);
   if (role == 0) return fi;
   if (role == 2) return fk;
@@ -480,6 +485,8 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
 }
 
 // Harmonic dihedral k[1 + d cos(n phi)] (dihedral_harmonic.cu:83-206), one role.
+// phi = atan2(|v1| v0.c1, c0.c1) carries the reference's sign convention
+// (negative when v0.c1 < 0) without the ill-conditioned acos near 0 and pi.
 __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, float3 xk, float3 xl,
                                                       float4 kc, int role, const BoxView& box,
                                                       float& energy, float& virial) {
@@ -494,9 +501,7 @@ __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, floa
     virial = 0.f;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const float c = clamp_cos(dot3(c0, c1) * (1.f / sqrtf(nc0 * nc1)));
-  float phi = c == 1.f ? 0.f : (c == -1.f ? kPiF : acosf(c));
-  if (dot3(v0, c1) < 0.f) phi = -phi;
+  const float phi = atan2f(sqrtf(n1) * dot3(v0, c1), dot3(c0, c1));
   const float arg = kc.z * phi;
   float sn, cs;
   sincosf(arg, &sn, &cs);
@@ -544,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const size_t n = p.n;
   double ke = 0.0;
-  float e_pair = 0.f, w_pair = 0.f;
+  double e_pair = 0.0, w_pair = 0.0;
   double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
   if (i < p.n) {
     const float4 xi4 = p.pos[i];
@@ -552,6 +557,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
     const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
     const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     float fx = 0.f, fy = 0.f, fz = 0.f;
+    double fxd = 0.0, fyd = 0.0, fzd = 0.0;
     const uint32_t cnt = p.nbr_count[i];
     const uint32_t* nb = p.nbr + i;
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -571,20 +577,30 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
           const float4 g = p.lj[ti * T + __float_as_int(xj.w)];
           c = LjCoeff{g.x, g.y, g.z, g.w};
         }
-        const float inv_r2 = __builtin_amdgcn_rcpf(r2);
-        const float sr2 = c.sigma2 * inv_r2;
-        const float sr6 = sr2 * sr2 * sr2;
-        const float sr12 = sr6 * sr6;
-        const float s = c.eps24 * inv_r2 * (2.f * sr12 - sr6);
-        fx += s * dx;
-        fy += s * dy;
-        fz += s * dz;
+        // The pair value in FP64 (full-rate VALU on CDNA4): dx is exact in
+        // binary32, so the only rounding left is FP64's; binary32 would
+        // amplify the rounding of r^2 six-fold through (sigma^2/r^2)^6.
+        const double ddx = dx, ddy = dy, ddz = dz;
+        const double rr = ddx * ddx + ddy * ddy + ddz * ddz;
+        double inv = __builtin_amdgcn_rcp(rr);
+        inv = inv * (2.0 - rr * inv);
+        inv = inv * (2.0 - rr * inv);
+        const double sr2 = static_cast<double>(c.sigma2) * inv;
+        const double sr6 = sr2 * sr2 * sr2;
+        const double sr12 = sr6 * sr6;
+        const double s = static_cast<double>(c.eps24) * inv * (2.0 * sr12 - sr6);
+        fxd += s * ddx;
+        fyd += s * ddy;
+        fzd += s * ddz;
         if (OBS) {
-          e_pair += c.eps4 * (sr12 - sr6) - c.shift;
-          w_pair += s * r2;
+          e_pair += static_cast<double>(c.eps4) * (sr12 - sr6) - static_cast<double>(c.shift);
+          w_pair += s * rr;
         }
       }
     }
+    fx = static_cast<float>(fxd);
+    fy = static_cast<float>(fyd);
+    fz = static_cast<float>(fzd);
     if (p.topo.has_listed) {
       const TopologyView& tp = p.topo;
       const uint32_t cw = tp.count[i];

@@ -1,0 +1,306 @@
+"""Pin the FP64 oracle (CPU only).
+
+The reference ships no tests or golden vectors, so every oracle function is
+pinned here by something independent of its own source: closed forms and
+known answers, brute force on small inputs, central finite differences of
+the energy, and invariants (Newton's third law, path agreement, scale
+invariance of the angle/dihedral virial).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from beads_amd import workloads as W
+
+
+def two_body(r: float, eps: float = 1.0, sig: float = 1.0, cutoff: float = 2.5,
+             shift: bool = False) -> W.Workload:
+    L = 10.0
+    pos = np.array([[2.0, 5.0, 5.0], [2.0 + r, 5.0, 5.0]])
+    return W.Workload(name="pair", units="reduced", positions=pos, velocities=np.zeros((2, 3)),
+                      masses=np.ones(2), types=np.ones(2, np.int32),
+                      box_bound=np.array([[0, 0, 0], [L, L, L]]), lj_epsilon=[[eps]],
+                      lj_sigma=[[sig]], cutoff=cutoff, shift=shift, cutoff_buffer=0.0)
+
+
+def energy(obs: dict) -> float:
+    return obs["pair_pe"] + obs["bond_pe"] + obs["angle_pe"] + obs["dihedral_pe"]
+
+
+# ----------------------------------------------------------------- plan ----
+
+def test_plan_closed_form_fcc():
+    w = W.lj_fluid_fcc()  # L = 10 a, a = (4/0.8442)^(1/3)
+    p = O.plan(w)
+    L = 10 * (4 / 0.8442) ** (1 / 3)
+    n = math.floor(L / 2.8)
+    assert (p.nx, p.ny, p.nz) == (n, n, n) == (5, 5, 5)
+    assert p.path == O.PATH_REGULAR
+    assert p.search_cutoff == np.float32(np.float32(2.5) + np.float32(0.3))
+
+
+def test_plan_paths_and_errors():
+    w = W.lj_fluid_fcc(cells=5)
+    assert O.plan(w).path == O.PATH_DIRECT  # 8.4 / 2.8 < 3 cells
+    tall = w.with_(box_bound=np.array([[0, 0, 0], [8.4, 8.4, 30.0]], np.float32))
+    assert O.plan(tall).path == O.PATH_SMALL
+    with pytest.raises(O.OracleError, match="half shortest box length"):
+        O.plan(w.with_(cutoff=4.5))
+
+
+# ---------------------------------------------------------- cells, sort ----
+
+def test_cell_ids_match_binary32_formula():
+    w = W.lj_box(3000, 0.8, 1.0, seed=2)
+    p = O.plan(w)
+    x = w.positions
+    lo = w.box_bound[0]
+    inv = np.array(p.inv_cell, np.float32)
+    s = ((x - lo).astype(np.float32) * inv).astype(np.float32)
+    idx = np.where(s <= 0, 0, s.astype(np.uint32))
+    idx = np.minimum(idx, np.array([p.nx - 1, p.ny - 1, p.nz - 1], np.uint32))
+    want = idx[:, 0] + p.nx * (idx[:, 1] + p.ny * idx[:, 2])
+    np.testing.assert_array_equal(O.cell_ids(w), want)
+
+
+def test_stable_cell_order_is_a_stable_sort():
+    rng = np.random.default_rng(0)
+    cells = rng.integers(0, 50, 2000).astype(np.uint32)
+    order, counts, offsets = O.stable_cell_order(cells, 50)
+    np.testing.assert_array_equal(order, np.argsort(cells, kind="stable"))
+    np.testing.assert_array_equal(counts, np.bincount(cells, minlength=50))
+    np.testing.assert_array_equal(offsets, np.concatenate([[0], np.cumsum(counts)[:-1]]))
+
+
+# ------------------------------------------------------------ exclusions ---
+
+def _graph_distance_pairs(n: int, bonds: np.ndarray, d: int) -> set:
+    INF = 10 ** 9
+    dist = np.full((n + 1, n + 1), INF, np.int64)
+    for t in range(1, n + 1):
+        dist[t, t] = 0
+    for a, b, _ in bonds:
+        dist[a, b] = dist[b, a] = 1
+    for k in range(1, n + 1):  # Floyd-Warshall
+        dist = np.minimum(dist, dist[:, k:k + 1] + dist[k:k + 1, :])
+    return {(a, b) for a in range(1, n + 1) for b in range(a + 1, n + 1) if dist[a, b] <= d}
+
+
+@pytest.mark.parametrize("distance", [1, 2, 3])
+def test_exclusions_equal_graph_distance(distance):
+    # an 8-bead chain with a branch and a ring closure
+    bonds = np.array([[1, 2, 1], [2, 3, 1], [3, 4, 1], [4, 5, 1], [5, 6, 1], [6, 7, 1], [7, 8, 1],
+                      [3, 9, 1], [9, 10, 1], [10, 4, 1]], np.uint32)
+    got = {tuple(int(v) for v in p) for p in O.exclusion_pairs(10, bonds, distance)}
+    assert got == _graph_distance_pairs(10, bonds, distance)
+
+
+# --------------------------------------------------------- neighbours ------
+
+def _brute_sets(w: W.Workload, rs2: np.float32, excluded: set) -> list[set]:
+    x = w.positions
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float32)
+    hL = (np.float32(0.5) * L).astype(np.float32)
+    tag = w.tags
+    out = []
+    for i in range(w.n):
+        d = (x[i] - x).astype(np.float32)
+        d = np.where(d > hL, d - L, np.where(d < -hL, d + L, d)).astype(np.float32)
+        r2 = ((d[:, 0] * d[:, 0]).astype(np.float32) + (d[:, 1] * d[:, 1]).astype(np.float32)
+              ).astype(np.float32) + (d[:, 2] * d[:, 2]).astype(np.float32)
+        r2 = r2.astype(np.float32)
+        js = np.nonzero(r2 <= rs2)[0]
+        out.append({int(j) for j in js if j != i and
+                    (min(tag[i], tag[j]), max(tag[i], tag[j])) not in excluded})
+    return out
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: W.lj_box(1200, 0.8, 1.0, seed=4),
+    lambda: W.polymer_melt(n_chains=24, beads_per_chain=50, seed=2),
+])
+def test_neighbor_paths_agree_with_brute_force(maker):
+    w = maker()
+    p = O.plan(w)
+    excluded = set()
+    if w.exclusion_distance:
+        excluded = {tuple(int(v) for v in q) for q in O.exclusion_pairs(w.n, w.bonds, w.exclusion_distance)}
+    brute = _brute_sets(w, np.float32(p.search_cutoff_sq), excluded)
+    paths = [O.PATH_DIRECT, O.PATH_SMALL] + ([O.PATH_REGULAR] if p.path == O.PATH_REGULAR else [])
+    for path in paths:
+        c, l, ovf = O.neighbor_lists(w, path=path)
+        assert not ovf
+        for i in range(w.n):
+            assert set(l[i, : c[i]].tolist()) == brute[i], (path, i)
+
+
+def test_neighbor_overflow_is_reported():
+    w = W.lj_box(1200, 0.8, 1.0, seed=4)
+    c, _, ovf = O.neighbor_lists(w, max_neighbors=8)
+    assert ovf and c.max() == 8
+
+
+# -------------------------------------------------------------- forces -----
+
+def test_lj_known_answers():
+    rmin = 2.0 ** (1.0 / 6.0)
+    f, obs = O.forces(two_body(rmin))
+    assert abs(f[0, 0]) < 1e-6  # force minimum (binary32 input rounding only)
+    assert obs["pair_pe"] == pytest.approx(-1.0, abs=1e-6)
+    f, obs = O.forces(two_body(1.0))
+    assert obs["pair_pe"] == pytest.approx(0.0, abs=1e-12)
+    assert f[1, 0] == pytest.approx(24.0, rel=1e-12) and f[0, 0] == pytest.approx(-24.0, rel=1e-12)
+    # shifted: U(r) - U(rc); at r just inside rc the energy is ~0
+    f, obs = O.forces(two_body(2.4999, shift=True))
+    assert abs(obs["pair_pe"]) < 1e-5
+    _, o2 = O.forces(two_body(1.5, shift=True))
+    _, o3 = O.forces(two_body(1.5, shift=False))
+    rc = np.float64(np.float32(2.5))
+    assert o3["pair_pe"] - o2["pair_pe"] == pytest.approx(4 * (rc ** -12 - rc ** -6), rel=1e-9)
+    # beyond the cutoff nothing acts
+    f, obs = O.forces(two_body(2.6))
+    assert not f.any() and obs["pair_pe"] == 0.0
+
+
+def _single_chain(theta0=120.0, k_angle=25.0, d=1, n=1, k_dih=1.0, positions=None):
+    pos = positions if positions is not None else np.array(
+        [[1.0, 1.0, 1.0], [1.47, 1.0, 1.0], [1.7, 1.4, 1.0], [2.1, 1.5, 1.35]])
+    return W.Workload(
+        name="chain", units="nm_kjmol", positions=pos, velocities=np.zeros((4, 3)),
+        masses=np.ones(4), types=np.ones(4, np.int32),
+        box_bound=np.array([[0, 0, 0], [6, 6, 6]]), lj_epsilon=[[1e-6]], lj_sigma=[[0.1]],
+        cutoff=0.2, bonds=np.array([[1, 2, 1], [2, 3, 1], [3, 4, 1]]),
+        angles=np.array([[1, 2, 3, 1], [2, 3, 4, 1]]), dihedrals=np.array([[1, 2, 3, 4, 1]]),
+        bond_k=[1250.0], bond_r0=[0.47], angle_k=[k_angle], angle_theta0=[theta0],
+        dihedral_k=[k_dih], dihedral_d=[d], dihedral_n=[n], exclusion_distance=3,
+        cutoff_buffer=0.0)
+
+
+def test_listed_terms_known_answers():
+    # bond at r0, angle at theta0, dihedral n=1 d=1 at phi=pi -> zero energy and force
+    w = _single_chain()
+    p = w.positions.astype(np.float64)
+    b1 = p[1] - p[0]
+    b2 = p[2] - p[1]
+    theta = math.degrees(math.acos(np.dot(-b1, b2) / np.linalg.norm(b1) / np.linalg.norm(b2)))
+    ref = _single_chain(theta0=np.float32(theta))
+    _, obs = O.forces(ref)
+    assert obs["bond_pe"] > 0  # bonds 2-3 and 3-4 are not at r0
+    # trans planar chain: phi = pi -> k (1 + cos(pi)) = 0
+    trans = np.array([[1.0, 1.0, 1.0], [1.47, 1.0, 1.0], [1.7, 1.41, 1.0], [2.17, 1.41, 1.0]])
+    _, obs = O.forces(_single_chain(positions=trans))
+    assert obs["dihedral_pe"] == pytest.approx(0.0, abs=1e-10)
+    # n = 0: constant k(1 + d)
+    _, obs = O.forces(_single_chain(n=0, d=-1))
+    assert obs["dihedral_pe"] == pytest.approx(0.0, abs=1e-12)
+    # angle exactly at theta0 carries no force on the end beads
+    ang = np.array([[1.0, 1.0, 1.0], [1.47, 1.0, 1.0], [1.47 + 0.47 * math.cos(math.radians(60)),
+                                                        1.0 + 0.47 * math.sin(math.radians(60)), 1.0],
+                    [3.0, 3.0, 3.0]])
+    w = _single_chain(positions=ang, k_dih=0.0)
+    w = w.with_(angles=w.angles[:1], dihedrals=w.dihedrals[:0], bonds=w.bonds[:2],
+                dihedral_k=np.zeros(0), dihedral_d=np.zeros(0), dihedral_n=np.zeros(0))
+    _, obs = O.forces(w)
+    assert obs["angle_pe"] == pytest.approx(0.0, abs=1e-10)
+    assert obs["bond_pe"] == pytest.approx(0.0, abs=1e-8)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: W.lj_mixture(n=1500, seed=3).with_(cutoff=1.1),
+    lambda: W.polymer_melt(n_chains=30, beads_per_chain=50, seed=5),
+    lambda: _single_chain(),
+])
+def test_forces_are_minus_energy_gradient(maker):
+    w = maker()
+    pos = w.positions.astype(np.float64)
+    f, _ = O.forces_f64(w, pos)
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for i in rng.choice(w.n, size=min(6, w.n), replace=False):
+        for a in range(3):
+            pp = pos.copy(); pp[i, a] += h
+            pm = pos.copy(); pm[i, a] -= h
+            fd = -(energy(O.forces_f64(w, pp)[1]) - energy(O.forces_f64(w, pm)[1])) / (2 * h)
+            assert fd == pytest.approx(f[i, a], rel=1e-5, abs=1e-4 * (1 + abs(f[i, a])))
+
+
+def test_newton_third_law_and_scale_invariant_virials():
+    w = W.polymer_melt(n_chains=30, beads_per_chain=50, seed=5)
+    f, obs = O.forces(w)
+    scale = np.abs(f).max()
+    assert np.abs(f.sum(axis=0)).max() < 1e-9 * scale * w.n
+    # angle and dihedral energies are invariant under uniform scaling of the
+    # coordinates, so their virials vanish
+    assert abs(obs["angle_virial"]) < 1e-9 * abs(obs["angle_pe"]) + 1e-9
+    assert abs(obs["dihedral_virial"]) < 1e-9 * abs(obs["dihedral_pe"]) + 1e-9
+
+
+def test_f32_and_f64_inputs_agree():
+    w = W.lj_mixture(n=2000, seed=6)
+    f32, o32 = O.forces(w)
+    f64, o64 = O.forces_f64(w, w.positions.astype(np.float64))
+    np.testing.assert_allclose(f32, f64, rtol=1e-12, atol=1e-9)
+    assert energy(o32) == pytest.approx(energy(o64), rel=1e-12)
+
+
+# ------------------------------------------------------ integrator etc. ----
+
+def test_velocity_verlet_closed_forms():
+    w = W.lj_box(200, 0.5, 1.0, seed=8)
+    f = np.random.default_rng(2).standard_normal((w.n, 3)).astype(np.float32)
+    po, vo, io = O.vv_pre(w, w.positions, w.velocities, f, w.images)
+    v_half = w.velocities + 0.5 * w.dt * f.astype(np.float64) / w.masses[:, None]
+    np.testing.assert_allclose(vo, v_half, rtol=1e-12)
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    unwrapped = w.positions + w.dt * v_half
+    np.testing.assert_allclose(po + io * L, unwrapped, rtol=0, atol=1e-12)
+    assert (po >= 0).all() and (po < L).all()
+    v_full = O.vv_post(w, vo.astype(np.float32), f)
+    np.testing.assert_allclose(v_full, vo.astype(np.float32) + 0.5 * w.dt * f / w.masses[:, None], rtol=1e-6)
+
+
+def test_berendsen_lambda_closed_form():
+    n, kb = 1000, 0.00831446261815324
+    dof = 3 * n - 3
+    ke_at_300 = 0.5 * dof * kb * 300.0
+    assert O.berendsen_lambda(ke_at_300, n, kb, 300.0, 0.01, 1.0) == pytest.approx(1.0, abs=1e-12)
+    lam = O.berendsen_lambda(ke_at_300, n, kb, 330.0, 0.01, 1.0)
+    assert lam == pytest.approx(math.sqrt(1 + 0.01 * (330.0 / 300.0 - 1)), rel=1e-12)
+    assert O.berendsen_lambda(0.0, n, kb, 300.0, 0.01, 1.0) == 1.0
+    assert O.berendsen_lambda(ke_at_300, 1, kb, 300.0, 0.01, 1.0) == 1.0  # dof = 0
+
+
+def test_kinetic_energy():
+    v = np.array([[1.0, 2.0, 2.0], [0.0, 0.0, 1.0]], np.float32)
+    m = np.array([2.0, 4.0], np.float32)
+    assert O.kinetic_energy(v, m) == pytest.approx(0.5 * 2 * 9 + 0.5 * 4 * 1)
+
+
+def test_fp64_run_conserves_energy_and_rebuilds():
+    w = W.lj_fluid_fcc(cells=6, steps=400).with_(rebuild_check_every=1)
+    _, st0 = O.run(w, steps=0)
+    e0 = st0.final_ke + energy({k: getattr(st0.final_obs, k) for k in
+                                ("pair_pe", "bond_pe", "angle_pe", "dihedral_pe")})
+    state, st = O.run(w, steps=400)
+    e1 = st.final_ke + energy({k: getattr(st.final_obs, k) for k in
+                               ("pair_pe", "bond_pe", "angle_pe", "dihedral_pe")})
+    assert st.build_count > 1
+    assert abs(e1 - e0) < 2e-4 * abs(e0)
+    assert sorted(state["tags"].tolist()) == list(range(1, w.n + 1))
+
+
+def test_planted_mistake_is_caught_by_finite_differences():
+    # A force formula with a dropped factor must fail the FD pin: compare the
+    # oracle energy gradient with a deliberately wrong LJ force (24 -> 12).
+    w = two_body(1.1)
+    f, _ = O.forces_f64(w, w.positions.astype(np.float64))
+    r = float(w.positions[1, 0]) - float(w.positions[0, 0])  # binary32 inputs
+    wrong = 12.0 / r * (2 * r ** -12 - r ** -6)
+    right = 24.0 / r * (2 * r ** -12 - r ** -6)
+    assert f[1, 0] == pytest.approx(right, rel=1e-6)
+    assert abs(f[1, 0] - wrong) > 0.4 * abs(right)
