@@ -43,6 +43,16 @@ __device__ __forceinline__ float mic(float d, float L, float hL) {
   return d > hL ? d - L : (d < -hL ? d + L : d);
 }
 
+// Minimum-image displacement xi - xj with exact intermediates: when the raw
+// difference crosses half the box, xi - L (xi > L/2) or xj - L (xj > L/2) is
+// exact by Sterbenz, so only the final small difference rounds. mic(xi - xj)
+// instead rounds at the scale of L before folding back (about 1e-6 nm at
+// L ~ 50 nm), which the r^-13 force amplifies past 1e-5.
+__device__ __forceinline__ float mic_diff(float xi, float xj, float L, float hL) {
+  const float d = xi - xj;
+  return d > hL ? (xi - L) - xj : (d < -hL ? xi - (xj - L) : d);
+}
+
 // Squared distance exactly as written in the reference source (no FMA), so the
 // r^2 <= r_s^2 and r^2 < r_c^2 decisions match a binary32 host evaluation.
 __device__ __forceinline__ float r2_strict(float dx, float dy, float dz) {
@@ -444,8 +454,10 @@ __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a
 __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
   return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float3 mic3(float3 d, const BoxView& b) {
-  return make_float3(mic(d.x, b.L[0], b.hL[0]), mic(d.y, b.L[1], b.hL[1]), mic(d.z, b.L[2], b.hL[2]));
+// Minimum-image displacement p - q (wrapped coordinates), see mic_diff.
+__device__ __forceinline__ float3 mic3(float3 p, float3 q, const BoxView& b) {
+  return make_float3(mic_diff(p.x, q.x, b.L[0], b.hL[0]), mic_diff(p.y, q.y, b.L[1], b.hL[1]),
+                     mic_diff(p.z, q.z, b.L[2], b.hL[2]));
 }
 __device__ __forceinline__ float clamp_cos(float c) { return c > 1.f ? 1.f : (c < -1.f ? -1.f : c); }
 
@@ -458,8 +470,8 @@ __device__ __forceinline__ float clamp_cos(float c) { return c > 1.f ? 1.f : (c 
 __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 xk, float2 kc,
                                                    int role, const BoxView& box, float& energy,
                                                    float& virial) {
-  const float3 a = mic3(sub3(xi, xj), box);
-  const float3 b = mic3(sub3(xk, xj), box);
+  const float3 a = mic3(xi, xj, box);
+  const float3 b = mic3(xk, xj, box);
   const float ra2 = dot3(a, a), rb2 = dot3(b, b);
   if (ra2 == 0.f || rb2 == 0.f) {
     const float d = 0.f - kc.y;
@@ -490,9 +502,9 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
 __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, float3 xk, float3 xl,
                                                       float4 kc, int role, const BoxView& box,
                                                       float& energy, float& virial) {
-  const float3 v0 = mic3(sub3(xi, xj), box);
-  const float3 v1 = mic3(sub3(xk, xj), box);
-  const float3 v2 = mic3(sub3(xk, xl), box);
+  const float3 v0 = mic3(xi, xj, box);
+  const float3 v1 = mic3(xk, xj, box);
+  const float3 v2 = mic3(xk, xl, box);
   const float3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
   const float n1 = dot3(v1, v1), nc0 = dot3(c0, c0), nc1 = dot3(c1, c1);
   const int nn = static_cast<int>(kc.z);
@@ -563,9 +575,9 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
     for (uint32_t k = 0; k < cnt; ++k) {
       const uint32_t j = nb[static_cast<size_t>(k) * n];
       const float4 xj = p.pos[j];
-      const float dx = mic(xi4.x - xj.x, L0, h0);
-      const float dy = mic(xi4.y - xj.y, L1, h1);
-      const float dz = mic(xi4.z - xj.z, L2, h2);
+      const float dx = mic_diff(xi4.x, xj.x, L0, h0);
+      const float dy = mic_diff(xi4.y, xj.y, L1, h1);
+      const float dz = mic_diff(xi4.z, xj.z, L2, h2);
       const float r2 = r2_strict(dx, dy, dz);
       if (r2 < p.rc2) {
         LjCoeff c;
@@ -609,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
       for (uint32_t k = 0; k < nbond; ++k) {
         const uint2 r = tp.bond[k * n + i];
         const float2 kc = tp.bond_coeff[r.y & 0x7fffffffu];
-        const float3 d = mic3(sub3(xi, f3(p.pos[r.x])), p.box);
+        const float3 d = mic3(xi, f3(p.pos[r.x]), p.box);
         const float r2 = dot3(d, d);
         const float rr = sqrtf(r2);
         const float stretch = rr - kc.y;

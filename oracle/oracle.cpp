@@ -86,6 +86,18 @@ inline Real mic_any(Real d, Real L, Real hL) {
   return d;
 }
 
+template <typename Real>
+inline Real mic_diff_any(Real xi, Real xj, Real L, Real hL) {
+  const Real d = xi - xj;
+  if (d > hL) {
+    return (xi - L) - xj;
+  }
+  if (d < -hL) {
+    return xi - (xj - L);
+  }
+  return d;
+}
+
 // neighbor_plan.cpp:29-49 (count computed in double), :133-166, :107-115,
 // :90-105 and the constructor checks :182-236.
 bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
@@ -456,9 +468,14 @@ void compute_forces(const orc_system_t* s, const Pos& pos,
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
     const int ti = s->type[i] - 1;
     for (uint32_t j : lists[i]) {
-      const Real dxr = mic_any<Real>(pos.x(i) - pos.x(j), Lr[0], hLr[0]);
-      const Real dyr = mic_any<Real>(pos.y(i) - pos.y(j), Lr[1], hLr[1]);
-      const Real dzr = mic_any<Real>(pos.z(i) - pos.z(j), Lr[2], hLr[2]);
+      // The r^2 < r_c^2 decision (pair_lj.cu:54) on the minimum-image
+      // displacement with exact intermediates: (xi - L) - xj or xi - (xj - L)
+      // when the raw difference crosses half the box. Same value as the
+      // reference's MIC(xi - xj) in exact arithmetic, rounded once instead of
+      // at the scale of L; the device's force kernel evaluates it identically.
+      const Real dxr = mic_diff_any<Real>(pos.x(i), pos.x(j), Lr[0], hLr[0]);
+      const Real dyr = mic_diff_any<Real>(pos.y(i), pos.y(j), Lr[1], hLr[1]);
+      const Real dzr = mic_diff_any<Real>(pos.z(i), pos.z(j), Lr[2], hLr[2]);
       const Real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
       if (r2r >= rc2) {
         continue;
