@@ -1,0 +1,273 @@
+"""Differentiable forces on the gfx950 engine (frame mode + adjoint kernel).
+
+No reference counterpart: the reference computes forces only inside its
+dynamics loop (native/src/simulation/simulation_run.cpp:212-235) and has no
+gradients. This module puts the engine's frame-mode entry points
+(``beads_engine_frame_forces`` / ``beads_engine_frame_backward``) behind
+``torch.autograd.Function`` so that
+
+* ``FrameModel.forces(x, params)`` returns forces and the total potential
+  energy; gradients of any loss of them flow to the positions (J^T g - gE F,
+  the Hessian-vector product of the adjoint kernel) and to the per-type-pair LJ
+  epsilon/sigma and per-type bond k/r0 (FP64 accumulation on the device);
+* ``FrameModel.trajectory(x0, v0, params, steps, checkpoint_every)`` runs
+  velocity Verlet (NVE) with engine forces and backpropagates through it by a
+  hand-written reverse sweep that recomputes each segment of ``checkpoint_every``
+  steps from its stored (x, v) checkpoint.
+
+Forces are binary32 as in the dynamics path; the adjoint kernel evaluates in
+FP64 from the binary32 coordinates. Every engine call synchronises: torch's
+current stream is drained before the engine reads a tensor, and the engine's
+stream is drained before a result is handed back.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .capi import Engine
+from .workloads import Workload
+
+
+@dataclass
+class Params:
+    """Differentiable force-field parameters: epsilon/sigma [T, T] (symmetric),
+    bond k/r0 [n_bond_types]."""
+
+    epsilon: torch.Tensor
+    sigma: torch.Tensor
+    bond_k: torch.Tensor
+    bond_r0: torch.Tensor
+
+    def tensors(self) -> tuple[torch.Tensor, ...]:
+        return self.epsilon, self.sigma, self.bond_k, self.bond_r0
+
+
+class FrameModel:
+    """One topology (types, bonds, exclusions, box, cutoff) evaluated at
+    caller-supplied positions. Positions are [n, 3] in the workload's slot
+    order; the engine never reorders them."""
+
+    def __init__(self, workload: Workload, device: str | torch.device = "cuda"):
+        if workload.angles.shape[0] or workload.dihedrals.shape[0]:
+            raise ValueError("the adjoint covers LJ pairs and harmonic bonds; remove angle and "
+                             "dihedral terms")
+        self.workload = workload
+        self.device = torch.device(device)
+        self.engine = Engine(workload)
+        self.n = workload.n
+        self.n_types = workload.n_types
+        self.n_bond_types = int(workload.bond_k.shape[0])
+        self.masses = torch.as_tensor(workload.masses, dtype=torch.float32,
+                                      device=self.device).reshape(-1, 1)
+        self._params_key: bytes | None = None
+        self._loaded = 0   # token of the frame whose neighbour list the engine holds
+        self._tokens = 0
+        self.forward_calls = 0
+        self.backward_calls = 0
+
+    def close(self) -> None:
+        self.engine.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- params
+
+    def initial_params(self, requires_grad: bool = True) -> Params:
+        w = self.workload
+
+        def t(a):
+            return torch.tensor(np.asarray(a, np.float32), dtype=torch.float32,
+                                device=self.device, requires_grad=requires_grad)
+
+        return Params(t(w.lj_epsilon), t(w.lj_sigma), t(w.bond_k), t(w.bond_r0))
+
+    def _set_params(self, eps, sigma, k, r0) -> None:
+        arrs = [np.ascontiguousarray(p.detach().to("cpu", torch.float32).numpy())
+                for p in (eps, sigma, k, r0)]
+        key = b"".join(a.tobytes() for a in arrs)
+        if key == self._params_key:
+            return
+        self.engine.set_lj_params(arrs[0], arrs[1])
+        if self.n_bond_types:
+            self.engine.set_bond_params(arrs[2], arrs[3])
+        self._params_key = key
+
+    # ------------------------------------------------------------ raw calls
+
+    def _forward_raw(self, x: torch.Tensor, reuse_list: bool = True):
+        torch.cuda.current_stream(self.device).synchronize()
+        f = torch.empty_like(x)
+        e = self.engine.frame_forces(x.data_ptr(), f.data_ptr(), reuse_list=reuse_list)
+        self._tokens += 1
+        self._loaded = self._tokens
+        self.forward_calls += 1
+        return f, e, self._tokens
+
+    def _backward_raw(self, g_forces: torch.Tensor, g_energy: float):
+        n_p = 2 * self.n_types * self.n_types + 2 * self.n_bond_types
+        gx = torch.empty((self.n, 3), dtype=torch.float32, device=self.device)
+        gp = torch.empty(max(n_p, 1), dtype=torch.float64, device=self.device)
+        torch.cuda.current_stream(self.device).synchronize()
+        self.engine.frame_backward(g_forces.data_ptr(), g_energy, gx.data_ptr(), gp.data_ptr())
+        self.backward_calls += 1
+        return gx, gp
+
+    def _split(self, gp: torch.Tensor, like: Params | tuple) -> tuple[torch.Tensor, ...]:
+        T, nb = self.n_types, self.n_bond_types
+        eps, sig, k, r0 = like
+        parts = (gp[: T * T].reshape(T, T), gp[T * T: 2 * T * T].reshape(T, T),
+                 gp[2 * T * T: 2 * T * T + nb], gp[2 * T * T + nb: 2 * T * T + 2 * nb])
+        return tuple(p.to(q.dtype).to(q.device) for p, q in zip(parts, (eps, sig, k, r0)))
+
+    def _positions(self, x: torch.Tensor) -> torch.Tensor:
+        if x.shape != (self.n, 3):
+            raise ValueError(f"positions must have shape ({self.n}, 3)")
+        return x.detach().to(self.device, torch.float32).contiguous()
+
+    # ------------------------------------------------------------- public
+
+    def forces(self, x: torch.Tensor, params: Params) -> tuple[torch.Tensor, torch.Tensor]:
+        """Forces [n, 3] (binary32) and total potential energy (FP64 scalar)."""
+        return _FrameForces.apply(self, x, *params.tensors())
+
+    def trajectory(self, x0: torch.Tensor, v0: torch.Tensor, params: Params, steps: int,
+                   checkpoint_every: int = 20) -> tuple[torch.Tensor, torch.Tensor]:
+        """Velocity Verlet (NVE, the workload's dt) from (x0, v0); returns the
+        final positions and velocities, differentiable w.r.t. x0, v0, params."""
+        if steps < 1 or checkpoint_every < 1:
+            raise ValueError("steps and checkpoint_every must be positive")
+        return _VerletTrajectory.apply(self, int(steps), int(checkpoint_every), x0, v0,
+                                       *params.tensors())
+
+
+def force_matching_loss(model: FrameModel, frames: list[torch.Tensor],
+                        targets: list[torch.Tensor], params: Params) -> torch.Tensor:
+    """mean over frames, atoms and components of (F(x; params) - F_target)^2."""
+    total = None
+    for x, ft in zip(frames, targets):
+        f, _ = model.forces(x, params)
+        term = ((f.double() - ft.double()) ** 2).sum()
+        total = term if total is None else total + term
+    return total / (len(frames) * model.n * 3)
+
+
+class _FrameForces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, model: FrameModel, x, eps, sigma, k, r0):
+        model._set_params(eps, sigma, k, r0)
+        pos = model._positions(x)
+        f, e, token = model._forward_raw(pos)
+        ctx.model = model
+        ctx.token = token
+        ctx.x_dtype = x.dtype
+        ctx.save_for_backward(pos, eps, sigma, k, r0)
+        energy = torch.tensor(float(np.sum(e)), dtype=torch.float64, device=model.device)
+        return f, energy
+
+    @staticmethod
+    def backward(ctx, g_forces, g_energy):
+        model: FrameModel = ctx.model
+        pos, eps, sigma, k, r0 = ctx.saved_tensors
+        model._set_params(eps, sigma, k, r0)
+        if model._loaded != ctx.token:
+            model._forward_raw(pos, reuse_list=False)
+            model._loaded = ctx.token
+        gF = (torch.zeros_like(pos) if g_forces is None
+              else g_forces.detach().to(model.device, torch.float32).contiguous())
+        gE = 0.0 if g_energy is None else float(g_energy)
+        gx, gp = model._backward_raw(gF, gE)
+        g_eps, g_sig, g_k, g_r0 = model._split(gp, (eps, sigma, k, r0))
+        return None, gx.to(ctx.x_dtype), g_eps, g_sig, g_k, g_r0
+
+
+class _VerletTrajectory(torch.autograd.Function):
+    """x_{n+1} = x_n + dt v_n + dt^2/(2m) F_n, v_{n+1} = v_n + dt/(2m) (F_n + F_{n+1}).
+
+    Reverse sweep for L(x_T, v_T), a = dt/(2m), carry = adjoint of F_n from
+    the later half-kick:
+        Fbar_n  = carry + a vbar_n
+        xbar_n += J(x_n)^T Fbar_n,   thetabar += (dF_n/dtheta)^T Fbar_n
+        vhbar   = vbar_n + dt xbar_n
+        xbar_{n-1} = xbar_n, vbar_{n-1} = vhbar, carry = a vhbar
+    and at n = 0: xbar_0 += J(x_0)^T carry. Positions of one segment are
+    recomputed from its checkpoint before its reverse pass."""
+
+    @staticmethod
+    def forward(ctx, model: FrameModel, steps, every, x0, v0, eps, sigma, k, r0):
+        model._set_params(eps, sigma, k, r0)
+        dt = float(model.workload.dt)
+        a = (0.5 * dt) / model.masses
+        x = model._positions(x0)
+        v = v0.detach().to(model.device, torch.float32).contiguous()
+        checkpoints = [(x.clone(), v.clone())]
+        f, _, _ = model._forward_raw(x)
+        for s in range(steps):
+            v = v + a * f
+            x = (x + dt * v).contiguous()
+            f, _, _ = model._forward_raw(x)
+            v = v + a * f
+            if (s + 1) % every == 0 and s + 1 < steps:
+                checkpoints.append((x.clone(), v.clone()))
+        ctx.model = model
+        ctx.steps, ctx.every = steps, every
+        ctx.checkpoints = checkpoints
+        ctx.dtypes = (x0.dtype, v0.dtype)
+        ctx.save_for_backward(eps, sigma, k, r0)
+        return x.to(x0.dtype), v.to(v0.dtype)
+
+    @staticmethod
+    def backward(ctx, gx_T, gv_T):
+        model: FrameModel = ctx.model
+        eps, sigma, k, r0 = ctx.saved_tensors
+        model._set_params(eps, sigma, k, r0)
+        dt = float(model.workload.dt)
+        a = (0.5 * dt) / model.masses
+        steps, every = ctx.steps, ctx.every
+        dev = model.device
+        xbar = (torch.zeros((model.n, 3), dtype=torch.float32, device=dev) if gx_T is None
+                else gx_T.detach().to(dev, torch.float32).clone())
+        vbar = (torch.zeros((model.n, 3), dtype=torch.float32, device=dev) if gv_T is None
+                else gv_T.detach().to(dev, torch.float32).clone())
+        carry = torch.zeros_like(xbar)
+        n_p = 2 * model.n_types * model.n_types + 2 * model.n_bond_types
+        pbar = torch.zeros(max(n_p, 1), dtype=torch.float64, device=dev)
+        for seg in range(len(ctx.checkpoints) - 1, -1, -1):
+            start = seg * every
+            end = min(start + every, steps)
+            # Recompute positions x_start..x_end of this segment.
+            x, v = ctx.checkpoints[seg]
+            xs = [x]
+            f, _, _ = model._forward_raw(x)
+            for _ in range(start, end):
+                v = v + a * f
+                x = (x + dt * v).contiguous()
+                f, _, _ = model._forward_raw(x)
+                v = v + a * f
+                xs.append(x)
+            # Reverse over steps end-1 .. start (frames end .. start+1).
+            for n in range(end, start, -1):
+                xn = xs[n - start]
+                if n != end:
+                    model._forward_raw(xn)
+                fbar = (carry + a * vbar).contiguous()
+                gx, gp = model._backward_raw(fbar, 0.0)
+                xbar = xbar + gx
+                pbar = pbar + gp
+                vbar = vbar + dt * xbar
+                carry = a * vbar
+            if seg == 0:
+                model._forward_raw(xs[0])
+                gx, gp = model._backward_raw(carry.contiguous(), 0.0)
+                xbar = xbar + gx
+                pbar = pbar + gp
+        g_eps, g_sig, g_k, g_r0 = model._split(pbar, (eps, sigma, k, r0))
+        return (None, None, None, xbar.to(ctx.dtypes[0]), vbar.to(ctx.dtypes[1]),
+                g_eps, g_sig, g_k, g_r0)

@@ -96,7 +96,25 @@ SIGNATURES = {
     "beads_engine_read_cells": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p, _u32p, C.c_uint64]),
     "beads_engine_profile_advance": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32,
                                                C.POINTER(PhaseTimes)]),
+    "beads_engine_set_lj_params": (C.c_int, [C.c_void_p, _f32p, _f32p]),
+    "beads_engine_set_bond_params": (C.c_int, [C.c_void_p, _f32p, _f32p]),
+    "beads_engine_frame_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_double), C.c_int32]),
+    "beads_engine_frame_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                              C.c_void_p]),
+    "beads_engine_frame_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]),
     "beads_engine_destroy": (None, [C.c_void_p]),
+    "beads_batch_create": (C.c_int, [C.POINTER(Spec), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "beads_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double)]),
+    "beads_batch_set_block": (C.c_int, [C.c_void_p, C.c_int32]),
+    "beads_batch_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32]),
+    "beads_batch_profile": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    "beads_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "beads_batch_replica_stats": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Stats)]),
+    "beads_batch_read_state": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(StateOut)]),
+    "beads_batch_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -265,3 +283,114 @@ class Engine:
         _check(self.lib.beads_engine_profile_advance(self.handle, steps, 1 if per_phase else 0,
                                                      C.byref(t)))
         return t
+
+    # ---- frame mode (device pointers; see beads_engine_frame_forces) ----
+
+    def set_lj_params(self, epsilon: np.ndarray, sigma: np.ndarray) -> None:
+        e = np.ascontiguousarray(epsilon, dtype=np.float32)
+        s = np.ascontiguousarray(sigma, dtype=np.float32)
+        _check(self.lib.beads_engine_set_lj_params(self.handle, _ptr(e, C.c_float),
+                                                   _ptr(s, C.c_float)))
+
+    def set_bond_params(self, k: np.ndarray, r0: np.ndarray) -> None:
+        kk = np.ascontiguousarray(k, dtype=np.float32)
+        rr = np.ascontiguousarray(r0, dtype=np.float32)
+        _check(self.lib.beads_engine_set_bond_params(self.handle, _ptr(kk, C.c_float),
+                                                     _ptr(rr, C.c_float)))
+
+    def frame_forces(self, positions_ptr: int, forces_ptr: int,
+                     reuse_list: bool = False) -> np.ndarray:
+        """Forces at device positions; returns [pair, bond, angle, dihedral] energies."""
+        e = np.zeros(4, np.float64)
+        _check(self.lib.beads_engine_frame_forces(self.handle, C.c_void_p(positions_ptr),
+                                                  C.c_void_p(forces_ptr),
+                                                  _ptr(e, C.c_double), 1 if reuse_list else 0))
+        return e
+
+    def frame_backward(self, grad_forces_ptr: int, grad_energy: float, grad_positions_ptr: int,
+                       grad_params_ptr: int) -> None:
+        _check(self.lib.beads_engine_frame_backward(
+            self.handle, C.c_void_p(grad_forces_ptr), float(grad_energy),
+            C.c_void_p(grad_positions_ptr), C.c_void_p(grad_params_ptr)))
+
+    def frame_timing(self) -> tuple[float, float]:
+        f, b = C.c_double(), C.c_double()
+        _check(self.lib.beads_engine_frame_timing(self.handle, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+
+def _state_buffers(n: int):
+    out = {"positions": np.zeros((n, 3), np.float32), "velocities": np.zeros((n, 3), np.float32),
+           "forces": np.zeros((n, 3), np.float32), "images": np.zeros((n, 3), np.int32),
+           "tags": np.zeros(n, np.uint32), "types": np.zeros(n, np.int32),
+           "molecule_ids": np.zeros(n, np.uint32), "masses": np.zeros(n, np.float32)}
+    so = StateOut(_ptr(out["positions"], C.c_float), _ptr(out["velocities"], C.c_float),
+                  _ptr(out["forces"], C.c_float), _ptr(out["images"], C.c_int32),
+                  _ptr(out["tags"], C.c_uint32), _ptr(out["types"], C.c_int32),
+                  _ptr(out["molecule_ids"], C.c_uint32), _ptr(out["masses"], C.c_float))
+    return out, so
+
+
+class Batch:
+    """Many pair-only systems advanced together by one persistent launch."""
+
+    def __init__(self, workloads: list[Workload]):
+        self.lib = library()
+        self.workloads = list(workloads)
+        specs = (Spec * len(self.workloads))()
+        self._keep = []
+        for k, w in enumerate(self.workloads):
+            s, keep = make_spec(w)
+            specs[k] = s
+            self._keep.append(keep)
+        self._specs = specs
+        handle = C.c_void_p()
+        _check(self.lib.beads_batch_create(specs, len(self.workloads), C.byref(handle)))
+        self.handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.beads_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, trial_steps: int = 20) -> tuple[int, list[float]]:
+        block = C.c_int32()
+        secs = (C.c_double * 3)()
+        _check(self.lib.beads_batch_autotune(self.handle, trial_steps, C.byref(block), secs))
+        return block.value, list(secs)
+
+    def set_block(self, block: int) -> None:
+        _check(self.lib.beads_batch_set_block(self.handle, block))
+
+    def run(self, steps: int, observables_on_last: bool = False) -> None:
+        _check(self.lib.beads_batch_run(self.handle, steps, 1 if observables_on_last else 0))
+
+    def profile(self, steps: int) -> float:
+        t = C.c_double()
+        _check(self.lib.beads_batch_profile(self.handle, steps, C.byref(t)))
+        return t.value
+
+    def synchronize(self) -> None:
+        _check(self.lib.beads_batch_synchronize(self.handle))
+
+    def stats(self, r: int) -> Stats:
+        s = Stats()
+        _check(self.lib.beads_batch_replica_stats(self.handle, r, C.byref(s)))
+        return s
+
+    def state(self, r: int) -> dict[str, np.ndarray]:
+        out, so = _state_buffers(self.workloads[r].n)
+        _check(self.lib.beads_batch_read_state(self.handle, r, C.byref(so)))
+        return out

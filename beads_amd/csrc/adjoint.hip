@@ -1,0 +1,246 @@
+// Adjoint of the force evaluation (no reference counterpart: the reference has
+// no gradients). Given g = dL/dF for every atom and gE = dL/dE, one pass over
+// the stored full neighbour list and the per-atom bond roles produces
+//   * dL/dtheta for the LJ epsilon/sigma of every type pair and the harmonic
+//     bond k/r0 of every bond type, accumulated in FP64, and
+//   * dL/dx = J^T g - gE F, J = dF/dx (the Hessian-vector product), in FP64
+//     per atom, rounded to binary32 on store.
+// The same pass serves force matching (g = dL/dF) and the reverse sweep of
+// velocity-Verlet trajectory backprop (g = adjoint of the force).
+//
+// LJ pair, d = x_i - x_j, u = 1/r^2, sr6 = (sigma^2 u)^3, sr12 = sr6^2:
+//   F_i = sum_j s d,                 s  = 24 eps u (2 sr12 - sr6)  (pair_lj.cu:56-66)
+//   ds/d(r^2) = -24 eps u^2 (14 sr12 - 4 sr6)
+//   ds/deps   = s / eps,  ds/dsigma = 24 eps u / sigma (24 sr12 - 6 sr6)
+//   dL/dx_i  += s (g_i - g_j) + 2 ds/d(r^2) d (d . (g_i - g_j))   (symmetric tables)
+//   dL/dtheta += (g_i . d) ds/dtheta        (each direction of the full list)
+//   E = 1/2 sum_i sum_j [4 eps (sr12 - sr6) - shift],
+//   shift = 4 eps (sc12 - sc6), sc6 = (sigma^2 / rc^2)^3 when the energy is shifted.
+// Harmonic bond from the atom's side, d = x_self - x_partner, f = phi(r) d,
+// phi = -k (r - r0) / r, E = k/2 (r - r0)^2 (bond_harmonic.cu:41-79):
+//   dL/dx_self += phi (g_self - g_p) + (phi'/r) d (d . (g_self - g_p)), phi' = -k r0 / r^2
+//   dL/dk  += g_self . (-(r - r0) / r) d,   dL/dr0 += g_self . (k / r) d
+// Every bond is seen from both atoms, so each side adds half of dE/dk, dE/dr0.
+#include "adjoint.hpp"
+
+#include "device_common.hpp"
+
+namespace beads_amd {
+namespace {
+
+using namespace dev;
+
+template <bool CHECK>
+__global__ __launch_bounds__(kBlock) void frame_load_kernel(FrameLoadParams p) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  float4 x = p.pos[i];
+  x.x = p.xyz[3 * i];
+  x.y = p.xyz[3 * i + 1];
+  x.z = p.xyz[3 * i + 2];
+  float c[3];
+  {
+#pragma clang fp contract(off)
+    c[0] = floorf((x.x - p.box.lo[0]) * p.box.invL[0]);
+    c[1] = floorf((x.y - p.box.lo[1]) * p.box.invL[1]);
+    c[2] = floorf((x.z - p.box.lo[2]) * p.box.invL[2]);
+    x.x -= c[0] * p.box.L[0];
+    x.y -= c[1] * p.box.L[1];
+    x.z -= c[2] * p.box.L[2];
+  }
+  p.pos[i] = x;
+  const int4 im = make_int4(static_cast<int>(c[0]), static_cast<int>(c[1]),
+                            static_cast<int>(c[2]), 0);
+  p.image[i] = im;
+  if (CHECK) {
+#pragma clang fp contract(off)
+    const float4 r = p.ref_pos[i];
+    const int4 ri = p.ref_image[i];
+    const float dx = (x.x - r.x) + static_cast<float>(im.x - ri.x) * p.box.L[0];
+    const float dy = (x.y - r.y) + static_cast<float>(im.y - ri.y) * p.box.L[1];
+    const float dz = (x.z - r.z) + static_cast<float>(im.z - ri.z) * p.box.L[2];
+    if (dx * dx + dy * dy + dz * dz > p.threshold_sq) {
+      __hip_atomic_store(&p.ctrl->rebuild, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void frame_store_kernel(uint32_t n, const float4* force,
+                                                             float* xyz) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 f = force[i];
+  xyz[3 * i] = f.x;
+  xyz[3 * i + 1] = f.y;
+  xyz[3 * i + 2] = f.z;
+}
+
+__global__ __launch_bounds__(kBlock) void pack_xyz_kernel(uint32_t n, const float* xyz,
+                                                          float4* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+}
+
+// TREG > 0: the atom's row of the type table (at most TREG types) is
+// accumulated in registers and flushed once per atom; TREG == 0: every
+// interaction adds into LDS directly (any type count up to kMaxLdsTypes).
+template <int TREG>
+__global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
+  __shared__ double acc[2 * kMaxLdsTypes * kMaxLdsTypes + 2 * kMaxBondTypesAdjoint];
+  const int T = p.n_types;
+  const int n_pair = T * T;
+  const int n_acc = 2 * n_pair + 2 * p.n_bond_types;
+  for (int k = threadIdx.x; k < n_acc; k += kBlock) acc[k] = 0.0;
+  __syncthreads();
+  const uint32_t n = p.n;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const double gE = p.grad_energy;
+  if (i < n) {
+    const float4 xi = p.pos[i];
+    const int ti = __float_as_int(xi.w);
+    const float4 gi = p.g[i];
+    const double gix = gi.x, giy = gi.y, giz = gi.z;
+    const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+    const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+    const double inv_rc2 = 1.0 / p.rc2_f64;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double re[TREG > 0 ? TREG : 1], rs[TREG > 0 ? TREG : 1];
+#pragma unroll
+    for (int t = 0; t < (TREG > 0 ? TREG : 1); ++t) re[t] = rs[t] = 0.0;
+    const uint32_t cnt = p.nbr_count[i];
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t j = p.nbr[nbr_chunk_base(k / kNbrChunk, i, n) + (k & (kNbrChunk - 1))];
+      const float4 xj = p.pos[j];
+      const float dx = mic_diff(xi.x, xj.x, L0, h0);
+      const float dy = mic_diff(xi.y, xj.y, L1, h1);
+      const float dz = mic_diff(xi.z, xj.z, L2, h2);
+      if (r2_strict(dx, dy, dz) >= p.rc2) continue;
+      const int tj = __float_as_int(xj.w);
+      const double eps = p.eps[ti * T + tj];
+      const double sig = p.sigma[ti * T + tj];
+      const double ddx = dx, ddy = dy, ddz = dz;
+      const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      const double u = 1.0 / r2;
+      const double sig2 = sig * sig;
+      const double sr2 = sig2 * u;
+      const double sr6 = sr2 * sr2 * sr2;
+      const double sr12 = sr6 * sr6;
+      const double s = 24.0 * eps * u * (2.0 * sr12 - sr6);
+      const double sp = -24.0 * eps * u * u * (14.0 * sr12 - 4.0 * sr6);
+      const float4 gj = p.g[j];
+      const double ex = gix - gj.x, ey = giy - gj.y, ez = giz - gj.z;
+      const double dd = ddx * ex + ddy * ey + ddz * ez;
+      gx += s * ex + 2.0 * sp * ddx * dd;
+      gy += s * ey + 2.0 * sp * ddy * dd;
+      gz += s * ez + 2.0 * sp * ddz * dd;
+      fx += s * ddx;
+      fy += s * ddy;
+      fz += s * ddz;
+      const double w = gix * ddx + giy * ddy + giz * ddz;
+      // dF/deps and dF/dsigma terms, plus half of dE/deps and dE/dsigma.
+      double de = w * 24.0 * u * (2.0 * sr12 - sr6);
+      double ds = w * 24.0 * eps * u / sig * (24.0 * sr12 - 6.0 * sr6);
+      if (gE != 0.0) {
+        double e_eps = 4.0 * (sr12 - sr6);
+        double e_sig = 4.0 * eps * (12.0 * sr12 - 6.0 * sr6) / sig;
+        if (p.lj_shift) {
+          const double sc2 = sig2 * inv_rc2;
+          const double sc6 = sc2 * sc2 * sc2;
+          const double sc12 = sc6 * sc6;
+          e_eps -= 4.0 * (sc12 - sc6);
+          e_sig -= 4.0 * eps * (12.0 * sc12 - 6.0 * sc6) / sig;
+        }
+        de += 0.5 * gE * e_eps;
+        ds += 0.5 * gE * e_sig;
+      }
+      if constexpr (TREG > 0) {
+#pragma unroll
+        for (int t = 0; t < TREG; ++t) {
+          if (tj == t) {
+            re[t] += de;
+            rs[t] += ds;
+          }
+        }
+      } else {
+        atomicAdd(&acc[ti * T + tj], de);
+        atomicAdd(&acc[n_pair + ti * T + tj], ds);
+      }
+    }
+    if constexpr (TREG > 0) {
+#pragma unroll
+      for (int t = 0; t < TREG; ++t) {
+        if (t < T) {
+          if (re[t] != 0.0) atomicAdd(&acc[ti * T + t], re[t]);
+          if (rs[t] != 0.0) atomicAdd(&acc[n_pair + ti * T + t], rs[t]);
+        }
+      }
+    }
+    if (p.bcap > 0) {
+      const uint32_t nbond = p.topo_count[i] & 255u;
+      const float3 x3 = f3(xi);
+      for (uint32_t k = 0; k < nbond; ++k) {
+        const uint2 r = p.bond[static_cast<size_t>(k) * n + i];
+        const uint32_t t = r.y & 0x7fffffffu;
+        const double kk = p.bond_k[t], r0 = p.bond_r0[t];
+        const float3 d = mic3(x3, f3(p.pos[r.x]), p.box);
+        const double ddx = d.x, ddy = d.y, ddz = d.z;
+        const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        if (rr == 0.0) continue;
+        const double phi = -kk * (rr - r0) / rr;
+        const double dphi = -kk * r0 / (rr * rr);
+        const float4 gp = p.g[r.x];
+        const double ex = gix - gp.x, ey = giy - gp.y, ez = giz - gp.z;
+        const double dd = ddx * ex + ddy * ey + ddz * ez;
+        gx += phi * ex + dphi / rr * ddx * dd;
+        gy += phi * ey + dphi / rr * ddy * dd;
+        gz += phi * ez + dphi / rr * ddz * dd;
+        fx += phi * ddx;
+        fy += phi * ddy;
+        fz += phi * ddz;
+        const double w = gix * ddx + giy * ddy + giz * ddz;
+        const double stretch = rr - r0;
+        atomicAdd(&acc[2 * n_pair + t], w * (-stretch / rr) + gE * 0.25 * stretch * stretch);
+        atomicAdd(&acc[2 * n_pair + p.n_bond_types + t], w * (kk / rr) - gE * 0.5 * kk * stretch);
+      }
+    }
+    p.grad_pos[3 * i] = static_cast<float>(gx - gE * fx);
+    p.grad_pos[3 * i + 1] = static_cast<float>(gy - gE * fy);
+    p.grad_pos[3 * i + 2] = static_cast<float>(gz - gE * fz);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n_acc; k += kBlock) {
+    if (acc[k] != 0.0) atomicAdd(&p.grad_params[k], acc[k]);
+  }
+}
+
+}  // namespace
+
+void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream) {
+  if (check) {
+    frame_load_kernel<true><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  } else {
+    frame_load_kernel<false><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  }
+}
+
+void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream) {
+  frame_store_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, force, xyz);
+}
+
+void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream) {
+  pack_xyz_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, xyz, out);
+}
+
+void launch_fm_backward(const AdjointParams& p, hipStream_t stream) {
+  if (p.n_types <= 1) {
+    fm_backward_kernel<1><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  } else if (p.n_types <= 4) {
+    fm_backward_kernel<4><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  } else {
+    fm_backward_kernel<0><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  }
+}
+
+}  // namespace beads_amd

@@ -1,0 +1,57 @@
+// Adjoint (force matching / trajectory backprop) kernels. See adjoint.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace beads_amd {
+
+constexpr int kMaxBondTypesAdjoint = 64;
+
+struct AdjointParams {
+  uint32_t n;
+  const float4* pos;        // positions, .w = type bits
+  const float4* g;          // dL/dF per atom
+  const uint32_t* nbr_count;
+  const uint32_t* nbr;      // chunked full list
+  BoxView box;
+  float rc2;                // binary32 cutoff decision, as in the forward
+  double rc2_f64;           // FP64 rc^2 for the energy-shift derivative
+  int32_t lj_shift;
+  int32_t n_types;
+  const double* eps;        // [T*T], symmetric
+  const double* sigma;      // [T*T], symmetric
+  const uint32_t* topo_count;
+  const uint2* bond;
+  uint32_t bcap;
+  int32_t n_bond_types;
+  const double* bond_k;
+  const double* bond_r0;
+  double grad_energy;       // dL/dE for the total potential energy
+  float* grad_pos;          // [n][3]
+  double* grad_params;      // eps[T*T], sigma[T*T], k[nb], r0[nb]; zeroed by the caller
+};
+
+// Caller positions [n][3] (unwrapped allowed) -> wrapped pos (type kept in .w)
+// and image counts. With check set, an atom farther than the half skin from its
+// position at the last build raises ctrl->rebuild (integrate_kernel's test).
+struct FrameLoadParams {
+  uint32_t n;
+  const float* xyz;
+  float4* pos;
+  int4* image;
+  const float4* ref_pos;
+  const int4* ref_image;
+  BoxView box;
+  float threshold_sq;
+  Ctrl* ctrl;
+};
+
+void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream);
+void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream);
+void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream);
+void launch_fm_backward(const AdjointParams& p, hipStream_t stream);
+
+}  // namespace beads_amd

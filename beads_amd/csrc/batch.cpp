@@ -1,0 +1,319 @@
+// Host side of the batched-replica mode: validates and packs many LJ systems
+// into one set of device arrays, picks the workgroup size with a small
+// autotuner, and advances every replica with one persistent launch that is
+// captured once into a hipGraph and replayed.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "engine.hpp"
+
+namespace beads_amd {
+
+Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
+  require(specs != nullptr && count > 0, "batch needs at least one replica spec.");
+  std::vector<NeighborPlan> plans(count);
+  for (uint32_t r = 0; r < count; ++r) {
+    const beads_spec_t& s = specs[r];
+    validate_spec(s);
+    require(s.n_bonds == 0 && s.n_angles == 0 && s.n_dihedrals == 0,
+            "batched replicas support pair-only (LJ) systems; replica " + std::to_string(r) +
+                " has listed terms.");
+    require(s.n_types <= kMaxLdsTypes, "batched replicas support at most 16 particle types.");
+    plans[r] = make_neighbor_plan(make_box_view(s.box_bound), s.lj_cutoff, s.cutoff_buffer,
+                                  s.n_particles, s.max_neighbors, s.rebuild_check_every,
+                                  s.sort_every_rebuild);
+  }
+  // Dispatch order: largest replicas first, so the long-running workgroups
+  // start in the first wave of the launch.
+  order_.resize(count);
+  std::iota(order_.begin(), order_.end(), 0u);
+  std::stable_sort(order_.begin(), order_.end(), [&](uint32_t a, uint32_t b) {
+    return specs[a].n_particles > specs[b].n_particles;
+  });
+  position_of_.resize(count);
+  for (uint32_t k = 0; k < count; ++k) position_of_[order_[k]] = k;
+
+  desc_host_.resize(count);
+  uint64_t atoms = 0, cells = 0, nbr = 0, lj = 0;
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t r = order_[k];
+    const beads_spec_t& s = specs[r];
+    const NeighborPlan& p = plans[r];
+    const UnitSystem units = unit_system_from(s.units);
+    ReplicaDesc d{};
+    d.n = s.n_particles;
+    d.atom0 = static_cast<uint32_t>(atoms);
+    d.cell0 = static_cast<uint32_t>(cells);
+    d.max_neighbors = p.max_neighbors;
+    d.nbr0 = nbr;
+    d.box = make_box_view(s.box_bound);
+    d.grid.nx = p.nx;
+    d.grid.ny = p.ny;
+    d.grid.nz = p.nz;
+    d.grid.n_cells = p.n_cells;
+    for (int a = 0; a < 3; ++a) d.grid.inv_cell[a] = p.inv_cell[a];
+    d.grid.path = p.path;
+    d.rs2 = p.search_cutoff_sq;
+    d.rc2 = p.force_cutoff * p.force_cutoff;
+    const float half = 0.5f * p.cutoff_buffer;
+    d.thr2 = half * half;
+    d.dt = s.dt;
+    d.cutoff_buffer = p.cutoff_buffer;
+    d.berendsen = s.berendsen ? 1 : 0;
+    d.n_types = s.n_types;
+    d.lj0 = static_cast<uint32_t>(lj);
+    d.target_thermal = units.kb * static_cast<double>(s.berendsen_temperature);
+    d.dt_over_tau = s.berendsen ? static_cast<double>(s.dt) / s.berendsen_tau : 0.0;
+    d.dof = std::max(0.0, 3.0 * s.n_particles - 3.0);
+    d.check_every = p.check_every;
+    d.sort_every = p.sort_every;
+    desc_host_[k] = d;
+    atoms += s.n_particles;
+    cells += static_cast<uint64_t>(p.n_cells) + 1;
+    nbr += static_cast<uint64_t>(p.max_neighbors) * s.n_particles;
+    lj += static_cast<uint64_t>(s.n_types) * s.n_types;
+  }
+  require(atoms <= 0xffffffffull && cells <= 0xffffffffull,
+          "batch exceeds 2^32 atoms or cells.", BEADS_E_OVERFLOW);
+  total_atoms_ = atoms;
+
+  // Host images of the initial state, kept to restore after autotuning.
+  init_pos_.resize(atoms);
+  init_vel_.resize(atoms);
+  init_img_.resize(atoms);
+  init_tag_.resize(atoms);
+  init_cell_atoms_.resize(atoms);
+  std::vector<float4> lj_all;
+  lj_all.reserve(lj);
+  for (uint32_t k = 0; k < count; ++k) {
+    const beads_spec_t& s = specs[order_[k]];
+    const uint32_t a0 = desc_host_[k].atom0;
+    for (uint32_t i = 0; i < s.n_particles; ++i) {
+      int32_t t = s.types[i] - 1;
+      float tw;
+      std::memcpy(&tw, &t, sizeof(float));
+      init_pos_[a0 + i] = make_float4(s.positions[3 * i], s.positions[3 * i + 1],
+                                      s.positions[3 * i + 2], tw);
+      init_vel_[a0 + i] = make_float4(s.velocities[3 * i], s.velocities[3 * i + 1],
+                                      s.velocities[3 * i + 2], s.masses[i]);
+      init_img_[a0 + i] = make_int4(s.images[3 * i], s.images[3 * i + 1], s.images[3 * i + 2], 0);
+      init_tag_[a0 + i] = s.tags[i];
+      init_cell_atoms_[a0 + i] = i;
+    }
+    const std::vector<float4> t = pack_lj_table(s);
+    lj_all.insert(lj_all.end(), t.begin(), t.end());
+  }
+
+  BEADS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  desc_.upload(desc_host_.data(), count, stream_);
+  lj_.upload(lj_all.data(), lj_all.size(), stream_);
+  const size_t n = atoms;
+  force_.resize(n);
+  ref_pos_.resize(n);
+  ref_image_.resize(n);
+  cell_of_.resize(n);
+  nbr_count_.resize(n);
+  nbr_.resize(nbr);
+  cell_count_.resize(cells);
+  cell_start_.resize(cells);
+  cell_cursor_.resize(cells);
+  s_pos_.resize(n);
+  s_vel_.resize(n);
+  s_image_.resize(n);
+  s_tag_.resize(n);
+  s_cell_of_.resize(n);
+  state_.resize(count);
+  reset_state();
+
+  ReplicaArrays& A = params_.arrays;
+  A.pos = pos_.data();
+  A.vel = vel_.data();
+  A.force = force_.data();
+  A.image = image_.data();
+  A.tag = tag_.data();
+  A.ref_pos = ref_pos_.data();
+  A.ref_image = ref_image_.data();
+  A.cell_of = cell_of_.data();
+  A.cell_atoms = cell_atoms_.data();
+  A.nbr_count = nbr_count_.data();
+  A.nbr = nbr_.data();
+  A.cell_count = cell_count_.data();
+  A.cell_start = cell_start_.data();
+  A.cell_cursor = cell_cursor_.data();
+  A.s_pos = s_pos_.data();
+  A.s_vel = s_vel_.data();
+  A.s_image = s_image_.data();
+  A.s_tag = s_tag_.data();
+  A.s_cell_of = s_cell_of_.data();
+  params_.desc = desc_.data();
+  params_.state = state_.data();
+  params_.lj = lj_.data();
+  BEADS_HIP(hipStreamSynchronize(stream_));
+}
+
+Batch::~Batch() {
+  for (auto& g : graphs_) {
+    (void)hipGraphExecDestroy(g.exec);
+  }
+  if (stream_ != nullptr) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Batch::reset_state() {
+  const size_t n = total_atoms_;
+  pos_.upload(init_pos_.data(), n, stream_);
+  vel_.upload(init_vel_.data(), n, stream_);
+  image_.upload(init_img_.data(), n, stream_);
+  tag_.upload(init_tag_.data(), n, stream_);
+  cell_atoms_.upload(init_cell_atoms_.data(), n, stream_);
+  force_.zero(stream_);
+  nbr_count_.zero(stream_);
+  cell_count_.zero(stream_);
+  cell_start_.zero(stream_);
+  cell_of_.zero(stream_);
+  std::vector<ReplicaState> st(count_);
+  for (auto& s : st) {
+    std::memset(&s, 0, sizeof(s));
+    s.lambda = 1.0;
+  }
+  state_.upload(st.data(), count_, stream_);
+}
+
+hipGraphExec_t Batch::graph_for(int block, uint64_t steps, bool obs) {
+  for (const auto& g : graphs_) {
+    if (g.block == block && g.steps == steps && g.obs == obs) return g.exec;
+  }
+  hipGraph_t graph;
+  BEADS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  launch_replicas(params_, count_, block, steps, obs, stream_);
+  BEADS_HIP(hipStreamEndCapture(stream_, &graph));
+  hipGraphExec_t exec;
+  BEADS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  BEADS_HIP(hipGraphDestroy(graph));
+  graphs_.push_back({block, steps, obs, exec});
+  return exec;
+}
+
+void Batch::run(uint64_t steps, bool observables_on_last) {
+  // steps == 0 still launches: the first launch performs every replica's
+  // initial build and step-0 forces.
+  BEADS_HIP(hipGraphLaunch(graph_for(block_, steps, observables_on_last), stream_));
+  steps_done_ += steps;
+}
+
+void Batch::synchronize() { BEADS_HIP(hipStreamSynchronize(stream_)); }
+
+double Batch::profile(uint64_t steps) {
+  hipEvent_t a, b;
+  BEADS_HIP(hipEventCreate(&a));
+  BEADS_HIP(hipEventCreate(&b));
+  BEADS_HIP(hipEventRecord(a, stream_));
+  run(steps, false);
+  BEADS_HIP(hipEventRecord(b, stream_));
+  BEADS_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  BEADS_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms * 1e-3;
+}
+
+// Autotuner: time each candidate workgroup size on the real batch from the
+// initial state, keep the fastest, then restore the initial state.
+int Batch::autotune(uint32_t trial_steps, double* seconds_out) {
+  const int candidates[] = {256, 512, 1024};
+  double best = 0.0;
+  int best_block = block_;
+  for (int k = 0; k < 3; ++k) {
+    reset_state();
+    block_ = candidates[k];
+    run(1, false);  // initial build + forces, outside the timing
+    const double t = profile(trial_steps);
+    if (seconds_out != nullptr) seconds_out[k] = t;
+    if (k == 0 || t < best) {
+      best = t;
+      best_block = candidates[k];
+    }
+  }
+  block_ = best_block;
+  reset_state();
+  steps_done_ = 0;
+  synchronize();
+  return block_;
+}
+
+void Batch::replica_stats(uint32_t r, beads_engine_stats_t* out) {
+  require(r < count_, "replica index out of range.", BEADS_E_INDEX);
+  const uint32_t k = position_of_[r];
+  ReplicaState st{};
+  BEADS_HIP(hipMemcpyAsync(&st, state_.data() + k, sizeof(st), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  const ReplicaDesc& d = desc_host_[k];
+  std::memset(out, 0, sizeof(*out));
+  out->step = st.step;
+  out->build_count = st.build_count;
+  out->dangerous_rebuild_count = st.dangerous;
+  out->reorder_count = st.reorder_count;
+  out->overflow = static_cast<int32_t>(st.overflow);
+  out->path = d.grid.path;
+  out->nx = d.grid.nx;
+  out->ny = d.grid.ny;
+  out->nz = d.grid.nz;
+  out->max_neighbors = d.max_neighbors;
+  out->ke = st.ke;
+  out->lambda = st.lambda;
+  out->pe[0] = st.pe;
+  out->virial[0] = st.virial;
+  out->temperature = d.dof > 0 ? 2.0 * st.ke / d.dof : 0.0;  // in kB units
+}
+
+void Batch::read_state(uint32_t r, const beads_state_out_t& o) {
+  require(r < count_, "replica index out of range.", BEADS_E_INDEX);
+  const ReplicaDesc& d = desc_host_[position_of_[r]];
+  const size_t n = d.n, a0 = d.atom0;
+  std::vector<float4> pos(n), vel(n), force(n);
+  std::vector<int4> img(n);
+  BEADS_HIP(hipMemcpyAsync(pos.data(), pos_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(vel.data(), vel_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(force.data(), force_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(img.data(), image_.data() + a0, n * sizeof(int4), hipMemcpyDeviceToHost, stream_));
+  if (o.tags) {
+    BEADS_HIP(hipMemcpyAsync(o.tags, tag_.data() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  }
+  synchronize();
+  for (size_t i = 0; i < n; ++i) {
+    if (o.positions) {
+      o.positions[3 * i] = pos[i].x;
+      o.positions[3 * i + 1] = pos[i].y;
+      o.positions[3 * i + 2] = pos[i].z;
+    }
+    if (o.types) {
+      int32_t t;
+      std::memcpy(&t, &pos[i].w, sizeof(int32_t));
+      o.types[i] = t + 1;
+    }
+    if (o.velocities) {
+      o.velocities[3 * i] = vel[i].x;
+      o.velocities[3 * i + 1] = vel[i].y;
+      o.velocities[3 * i + 2] = vel[i].z;
+    }
+    if (o.masses) o.masses[i] = vel[i].w;
+    if (o.forces) {
+      o.forces[3 * i] = force[i].x;
+      o.forces[3 * i + 1] = force[i].y;
+      o.forces[3 * i + 2] = force[i].z;
+    }
+    if (o.images) {
+      o.images[3 * i] = img[i].x;
+      o.images[3 * i + 1] = img[i].y;
+      o.images[3 * i + 2] = img[i].z;
+    }
+  }
+}
+
+}  // namespace beads_amd

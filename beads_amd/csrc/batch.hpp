@@ -1,0 +1,60 @@
+// Batched replicas (host side). See batch.cpp.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "replicas.hpp"
+
+namespace beads_amd {
+
+class Batch {
+ public:
+  Batch(const beads_spec_t* specs, uint32_t count);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  int autotune(uint32_t trial_steps, double* seconds_out);
+  void run(uint64_t steps, bool observables_on_last);
+  void synchronize();
+  double profile(uint64_t steps);
+  void replica_stats(uint32_t r, beads_engine_stats_t* out);
+  void read_state(uint32_t r, const beads_state_out_t& out);
+  uint32_t count() const { return count_; }
+  uint64_t total_atoms() const { return total_atoms_; }
+  int block() const { return block_; }
+  void set_block(int b) { block_ = b; }
+
+ private:
+  struct CachedGraph {
+    int block;
+    uint64_t steps;
+    bool obs;
+    hipGraphExec_t exec;
+  };
+  void reset_state();
+  hipGraphExec_t graph_for(int block, uint64_t steps, bool obs);
+
+  uint32_t count_ = 0;
+  uint64_t total_atoms_ = 0;
+  uint64_t steps_done_ = 0;
+  int block_ = 512;
+  std::vector<uint32_t> order_, position_of_;
+  std::vector<ReplicaDesc> desc_host_;
+  std::vector<float4> init_pos_, init_vel_;
+  std::vector<int4> init_img_;
+  std::vector<uint32_t> init_tag_, init_cell_atoms_;
+  std::vector<CachedGraph> graphs_;
+  hipStream_t stream_ = nullptr;
+
+  DeviceBuffer<ReplicaDesc> desc_;
+  DeviceBuffer<ReplicaState> state_;
+  DeviceBuffer<float4> lj_, pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
+  DeviceBuffer<int4> image_, ref_image_, s_image_;
+  DeviceBuffer<uint32_t> tag_, s_tag_, cell_of_, s_cell_of_, cell_atoms_, nbr_count_, nbr_;
+  DeviceBuffer<uint32_t> cell_count_, cell_start_, cell_cursor_;
+  ReplicaParams params_{};
+};
+
+}  // namespace beads_amd

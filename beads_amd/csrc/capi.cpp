@@ -2,16 +2,22 @@
 // catches all exceptions and converts them to a status plus a thread-local
 // message (the reference lets pybind11 translate C++ exceptions instead,
 // native/src/bindings/module.cpp:36-55).
+#include <cmath>
 #include <exception>
 #include <new>
 #include <string>
 
 #include "beads_amd.h"
 #include "common.hpp"
+#include "batch.hpp"
 #include "engine.hpp"
 
 struct beads_engine {
   beads_amd::Engine* impl;
+};
+
+struct beads_batch {
+  beads_amd::Batch* impl;
 };
 
 namespace {
@@ -52,6 +58,13 @@ beads_amd::Engine& engine_of(beads_engine_t* e) {
     beads_amd::fail(BEADS_E_VALUE, "engine handle is null");
   }
   return *e->impl;
+}
+
+beads_amd::Batch& batch_of(beads_batch_t* b) {
+  if (b == nullptr || b->impl == nullptr) {
+    beads_amd::fail(BEADS_E_VALUE, "batch handle is null");
+  }
+  return *b->impl;
 }
 
 }  // namespace
@@ -143,10 +156,112 @@ beads_status_t beads_engine_profile_advance(beads_engine_t* e, uint64_t n, int32
   });
 }
 
+beads_status_t beads_engine_set_lj_params(beads_engine_t* e, const float* epsilon,
+                                          const float* sigma) {
+  return guarded([&] {
+    beads_amd::require(epsilon && sigma, "lj parameter arrays are null", BEADS_E_TYPE);
+    engine_of(e).set_lj_params(epsilon, sigma);
+  });
+}
+
+beads_status_t beads_engine_set_bond_params(beads_engine_t* e, const float* k, const float* r0) {
+  return guarded([&] {
+    beads_amd::Engine& eng = engine_of(e);
+    beads_amd::require(eng.n_bond_types() == 0 || (k && r0), "bond parameter arrays are null",
+                       BEADS_E_TYPE);
+    eng.set_bond_params(k, r0);
+  });
+}
+
+beads_status_t beads_engine_frame_forces(beads_engine_t* e, const float* positions, float* forces,
+                                         double* energies, int32_t reuse_list) {
+  return guarded([&] {
+    beads_amd::require(positions && forces, "frame arrays are null", BEADS_E_TYPE);
+    engine_of(e).frame_forces(positions, forces, energies, reuse_list != 0);
+  });
+}
+
+beads_status_t beads_engine_frame_backward(beads_engine_t* e, const float* grad_forces,
+                                           double grad_energy, float* grad_positions,
+                                           double* grad_params) {
+  return guarded([&] {
+    beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",
+                       BEADS_E_TYPE);
+    beads_amd::require(std::isfinite(grad_energy), "grad_energy must be finite");
+    engine_of(e).frame_backward(grad_forces, grad_energy, grad_positions, grad_params);
+  });
+}
+
+beads_status_t beads_engine_frame_timing(beads_engine_t* e, double* forward_s,
+                                         double* backward_s) {
+  return guarded([&] { engine_of(e).frame_timing(forward_s, backward_s); });
+}
+
 void beads_engine_destroy(beads_engine_t* e) {
   if (e != nullptr) {
     delete e->impl;
     delete e;
+  }
+}
+
+beads_status_t beads_batch_create(const beads_spec_t* specs, uint32_t count, beads_batch_t** out) {
+  return guarded([&] {
+    beads_amd::require(out != nullptr, "batch output is null");
+    *out = nullptr;
+    auto* impl = new beads_amd::Batch(specs, count);
+    *out = new beads_batch{impl};
+  });
+}
+
+beads_status_t beads_batch_autotune(beads_batch_t* b, uint32_t trial_steps, int32_t* chosen,
+                                    double* seconds) {
+  return guarded([&] {
+    const int block = batch_of(b).autotune(trial_steps, seconds);
+    if (chosen != nullptr) *chosen = block;
+  });
+}
+
+beads_status_t beads_batch_set_block(beads_batch_t* b, int32_t block) {
+  return guarded([&] {
+    beads_amd::require(block == 256 || block == 512 || block == 1024,
+                       "batch workgroup size must be 256, 512 or 1024.");
+    batch_of(b).set_block(block);
+  });
+}
+
+beads_status_t beads_batch_run(beads_batch_t* b, uint64_t n, int32_t obs_last) {
+  return guarded([&] { batch_of(b).run(n, obs_last != 0); });
+}
+
+beads_status_t beads_batch_profile(beads_batch_t* b, uint64_t n, double* seconds) {
+  return guarded([&] {
+    beads_amd::require(seconds != nullptr, "profile output is null");
+    *seconds = batch_of(b).profile(n);
+  });
+}
+
+beads_status_t beads_batch_synchronize(beads_batch_t* b) {
+  return guarded([&] { batch_of(b).synchronize(); });
+}
+
+beads_status_t beads_batch_replica_stats(beads_batch_t* b, uint32_t r, beads_engine_stats_t* out) {
+  return guarded([&] {
+    beads_amd::require(out != nullptr, "stats output is null");
+    batch_of(b).replica_stats(r, out);
+  });
+}
+
+beads_status_t beads_batch_read_state(beads_batch_t* b, uint32_t r, const beads_state_out_t* out) {
+  return guarded([&] {
+    beads_amd::require(out != nullptr, "state output is null");
+    batch_of(b).read_state(r, *out);
+  });
+}
+
+void beads_batch_destroy(beads_batch_t* b) {
+  if (b != nullptr) {
+    delete b->impl;
+    delete b;
   }
 }
 

@@ -20,6 +20,12 @@
 namespace beads_amd {
 
 constexpr int kMaxLdsTypes = 16;  // LJ type tables up to 16x16 live in LDS
+constexpr uint32_t kNbrChunk = 16;  // neighbour entries per stored chunk (64 bytes)
+
+// Entries allocated per atom for a capacity of max_neighbors.
+inline uint32_t nbr_padded_capacity(uint32_t max_neighbors) {
+  return (max_neighbors + kNbrChunk - 1) / kNbrChunk * kNbrChunk;
+}
 
 struct BoxView {
   float lo[3];

@@ -175,5 +175,125 @@ __device__ __forceinline__ double lj_pair(float dx, float dy, float dz, float4 c
   return static_cast<double>(c.y) * inv * (2.0 * sr12 - sr6);
 }
 
+// ------------------------------------------------------- listed terms ------
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) {
+  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ __forceinline__ float3 add3(float3 a, float3 b) {
+  return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+__device__ __forceinline__ float3 scale3(float3 a, float s) {
+  return make_float3(a.x * s, a.y * s, a.z * s);
+}
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Minimum-image displacement p - q (wrapped coordinates), see mic_diff.
+__device__ __forceinline__ float3 mic3(float3 p, float3 q, const BoxView& b) {
+  return make_float3(mic_diff(p.x, q.x, b.L[0], b.hL[0]), mic_diff(p.y, q.y, b.L[1], b.hL[1]),
+                     mic_diff(p.z, q.z, b.L[2], b.hL[2]));
+}
+
+// Harmonic bond U = k/2 (r - r0)^2 (bond_harmonic.cu:41-79) on the displacement
+// d = x_self - x_partner; returns the force on self and the term's energy and
+// virial d.f.
+__device__ __forceinline__ float3 bond_force(float3 d, float2 kc, float& energy, float& virial) {
+  const float r2 = dot3(d, d);
+  const float rr = sqrtf(r2);
+  const float stretch = rr - kc.y;
+  energy = 0.5f * kc.x * stretch * stretch;
+  if (rr == 0.f) {
+    virial = 0.f;
+    return make_float3(0.f, 0.f, 0.f);
+  }
+  const float s = -kc.x * stretch / rr;
+  virial = s * r2;
+  return scale3(d, s);
+}
+
+// Harmonic angle U = k/2 (theta - theta0)^2 (angle_harmonic.cu:61-158) for one
+// role. Same mathematics as the reference, evaluated in a binary32-stable form:
+// theta = atan2(|a x b|, a.b) and sin(theta) = |a x b| / (|a||b|) instead of
+// acos(c) and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
+// instead of differences of nearly parallel vectors. The reference's floor
+// (sin theta replaced by 0.001 at or below 0.001) is kept.
+__device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 xk, float2 kc,
+                                                   int role, const BoxView& box, float& energy,
+                                                   float& virial) {
+  const float3 a = mic3(xi, xj, box);
+  const float3 b = mic3(xk, xj, box);
+  const float ra2 = dot3(a, a), rb2 = dot3(b, b);
+  if (ra2 == 0.f || rb2 == 0.f) {
+    const float d = 0.f - kc.y;
+    energy = 0.5f * kc.x * d * d;
+    virial = 0.f;
+    return make_float3(0.f, 0.f, 0.f);
+  }
+  const float ra = sqrtf(ra2), rb = sqrtf(rb2);
+  const float3 nrm = cross3(a, b);
+  const float nn = sqrtf(dot3(nrm, nrm));
+  const float theta = atan2f(nn, dot3(a, b));
+  const float dth = theta - kc.y;
+  energy = 0.5f * kc.x * dth * dth;
+  const float sn_true = nn / (ra * rb);
+  const float sn = sn_true <= 0.001f ? 0.001f : sn_true;
+  const float A = -kc.x * dth / sn;
+  const float3 fi = scale3(cross3(a, nrm), A / (ra2 * ra * rb));
+  const float3 fk = scale3(cross3(b, nrm), -A / (rb2 * rb * ra));
+  virial = dot3(a, fi) + dot3(b, fk);
+  if (role == 0) return fi;
+  if (role == 2) return fk;
+  return scale3(add3(fi, fk), -1.f);
+}
+
+// Harmonic dihedral k[1 + d cos(n phi)] (dihedral_harmonic.cu:83-206), one role.
+// phi = atan2(|v1| v0.c1, c0.c1) carries the reference's sign convention
+// (negative when v0.c1 < 0) without the ill-conditioned acos near 0 and pi.
+__device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, float3 xk, float3 xl,
+                                                      float4 kc, int role, const BoxView& box,
+                                                      float& energy, float& virial) {
+  const float3 v0 = mic3(xi, xj, box);
+  const float3 v1 = mic3(xk, xj, box);
+  const float3 v2 = mic3(xk, xl, box);
+  const float3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
+  const float n1 = dot3(v1, v1), nc0 = dot3(c0, c0), nc1 = dot3(c1, c1);
+  const int nn = static_cast<int>(kc.z);
+  if (n1 == 0.f || nc0 == 0.f || nc1 == 0.f) {
+    energy = nn == 0 ? kc.x * (1.f + kc.y) : kc.x;
+    virial = 0.f;
+    return make_float3(0.f, 0.f, 0.f);
+  }
+  const float nb = sqrtf(n1);
+  const float phi = atan2f(nb * dot3(v0, c1), dot3(c0, c1));
+  const float arg = kc.z * phi;
+  float sn, cs;
+  sincosf(arg, &sn, &cs);
+  energy = kc.x * (1.f + kc.y * cs);
+  const float dE = -kc.x * kc.y * kc.z * sn;
+  const float3 fi = scale3(c0, -dE * nb / nc0);
+  const float3 fl = scale3(c1, dE * nb / nc1);
+  const float inv_n1 = 1.f / n1;
+  const float3 s = sub3(scale3(fi, dot3(v0, v1) * inv_n1), scale3(fl, dot3(v2, v1) * inv_n1));
+  const float3 fj = sub3(s, fi);
+  const float3 fk = scale3(add3(s, fl), -1.f);
+  virial = dot3(v0, fi) + dot3(v1, fk) + dot3(sub3(v1, v2), fl);
+  if (role == 0) return fi;
+  if (role == 1) return fj;
+  if (role == 2) return fk;
+  return fl;
+}
+
+// Neighbour storage: each atom's list is cut into chunks of kNbrChunk entries;
+// chunk c of atom i is kNbrChunk consecutive words at ((c * N) + i) * kNbrChunk.
+// A build appends into an LDS staging row and writes whole 64-byte chunks; the
+// force pass reads a chunk as four 16-byte loads.
+__device__ __forceinline__ size_t nbr_chunk_base(uint32_t c, uint32_t i, uint32_t n) {
+  return (static_cast<size_t>(c) * n + i) * kNbrChunk;
+}
+
 }  // namespace dev
 }  // namespace beads_amd

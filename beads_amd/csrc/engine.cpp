@@ -1,6 +1,8 @@
 // Host engine implementation. See engine.hpp.
 #include "engine.hpp"
 
+#include "adjoint.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -250,7 +252,33 @@ HostTopology build_host_topology(const beads_spec_t& s) {
   return h;
 }
 
-namespace {
+// pair_lj.cu:171-192 coefficient packing in binary32:
+// {sigma^2, 24 eps, 4 eps, shift} per ordered type pair.
+std::vector<float4> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
+                                  float cutoff, bool shift_energy) {
+  const int T = n_types;
+  std::vector<float4> lj(static_cast<size_t>(T) * T);
+  const float rc = cutoff;
+  for (int a = 0; a < T; ++a) {
+    for (int b = 0; b < T; ++b) {
+      const float eps = epsilon[a * T + b], sig = sigma[a * T + b];
+      const float sigma2 = sig * sig;
+      const float eps4 = 4.f * eps;
+      float shift = 0.f;
+      if (shift_energy) {
+        const float sr2 = sigma2 / (rc * rc);
+        const float sr6 = sr2 * sr2 * sr2;
+        shift = eps4 * (sr6 * sr6 - sr6);
+      }
+      lj[static_cast<size_t>(a) * T + b] = make_float4(sigma2, 6.f * eps4, eps4, shift);
+    }
+  }
+  return lj;
+}
+
+std::vector<float4> pack_lj_table(const beads_spec_t& s) {
+  return pack_lj_table(s.n_types, s.lj_epsilon, s.lj_sigma, s.lj_cutoff, s.lj_shift != 0);
+}
 
 // Shell validation of the typed spec (native_spec_validation.hpp:23-716,
 // restated for the fields this engine consumes).
@@ -359,8 +387,6 @@ void validate_spec(const beads_spec_t& s) {
           "output.log.prefix is required for echo=log/both.");
 }
 
-}  // namespace
-
 Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
   validate_spec(spec);
   if (spec.thermo_path) thermo_path_ = spec.thermo_path;
@@ -429,7 +455,7 @@ void Engine::upload_state() {
   ref_image_.resize(n);
   nbr_count_.resize(n);
   nbr_count_.zero(stream_);
-  nbr_.resize(static_cast<size_t>(plan_.max_neighbors) * n);
+  nbr_.resize(static_cast<size_t>(nbr_padded_capacity(plan_.max_neighbors)) * n);
   cell_of_.resize(n);
   cell_atoms_.resize(n);
   cell_count_.resize(plan_.n_cells);
@@ -472,28 +498,16 @@ void Engine::upload_state() {
     new_of_old_.resize(n);
   }
 
-  // pair_lj.cu:171-192 coefficient packing in binary32.
-  const int T = s.n_types;
-  std::vector<float4> lj(static_cast<size_t>(T) * T);
-  const float rc = s.lj_cutoff;
-  for (int a = 0; a < T; ++a) {
-    for (int b = 0; b < T; ++b) {
-      const float eps = s.lj_epsilon[a * T + b], sig = s.lj_sigma[a * T + b];
-      const float sigma2 = sig * sig;
-      const float eps4 = 4.f * eps;
-      float shift = 0.f;
-      if (s.lj_shift) {
-        const float sr2 = sigma2 / (rc * rc);
-        const float sr6 = sr2 * sr2 * sr2;
-        shift = eps4 * (sr6 * sr6 - sr6);
-      }
-      lj[static_cast<size_t>(a) * T + b] = make_float4(sigma2, 6.f * eps4, eps4, shift);
-    }
-  }
-  lj_.upload(lj.data(), lj.size(), stream_);
-  std::vector<float2> bc(std::max(1, s.n_bond_types)), ac(std::max(1, s.n_angle_types));
+  n_types_ = s.n_types;
+  n_bond_types_ = s.n_bond_types;
+  lj_shift_ = s.lj_shift != 0;
+  lj_eps_.assign(s.lj_epsilon, s.lj_epsilon + static_cast<size_t>(n_types_) * n_types_);
+  lj_sig_.assign(s.lj_sigma, s.lj_sigma + static_cast<size_t>(n_types_) * n_types_);
+  bond_k_.assign(s.bond_k, s.bond_k + n_bond_types_);
+  bond_r0_.assign(s.bond_r0, s.bond_r0 + n_bond_types_);
+  upload_parameters();
+  std::vector<float2> ac(std::max(1, s.n_angle_types));
   std::vector<float4> dc(std::max(1, s.n_dihedral_types));
-  for (int t = 0; t < s.n_bond_types; ++t) bc[t] = make_float2(s.bond_k[t], s.bond_r0[t]);
   for (int t = 0; t < s.n_angle_types; ++t) {
     ac[t] = make_float2(s.angle_k[t], s.angle_theta0_deg[t] * static_cast<float>(kPi) / 180.f);
   }
@@ -501,7 +515,6 @@ void Engine::upload_state() {
     dc[t] = make_float4(s.dihedral_k[t], static_cast<float>(s.dihedral_d[t]),
                         static_cast<float>(s.dihedral_n[t]), 0.f);
   }
-  bond_coeff_.upload(bc.data(), bc.size(), stream_);
   angle_coeff_.upload(ac.data(), ac.size(), stream_);
   dih_coeff_.upload(dc.data(), dc.size(), stream_);
 
@@ -673,6 +686,7 @@ void Engine::rebuild_sequence(bool forced) {
 }
 
 void Engine::initialize(bool observables) {
+  require(!frame_mode_, "an engine used in frame mode cannot run dynamics.", BEADS_E_RUNTIME);
   rebuild_sequence(true);
   launch_force(force_params_, false, observables, stream_);
   BEADS_HIP(hipGetLastError());
@@ -803,9 +817,20 @@ void Engine::read_state(const beads_state_out_t& o) {
 void Engine::read_neighbors(uint32_t* counts, uint32_t* list, uint64_t capacity) {
   require(capacity >= static_cast<uint64_t>(plan_.max_neighbors) * n_,
           "neighbor list capacity must be max_neighbors * n_particles.");
+  // The device keeps 16-entry chunks per atom; hand out the reference's
+  // slot-major layout (entry k of atom i at k * N + i, neighbor_list.cuh:12-19).
+  const size_t n = n_;
+  const uint32_t padded = nbr_padded_capacity(plan_.max_neighbors);
+  std::vector<uint32_t> chunked(static_cast<size_t>(padded) * n);
   nbr_count_.download(counts, n_, stream_);
-  nbr_.download(list, static_cast<size_t>(plan_.max_neighbors) * n_, stream_);
+  nbr_.download(chunked.data(), chunked.size(), stream_);
   synchronize();
+  for (size_t i = 0; i < n; ++i) {
+    for (uint32_t k = 0; k < counts[i] && k < plan_.max_neighbors; ++k) {
+      list[static_cast<size_t>(k) * n + i] =
+          chunked[((k / kNbrChunk) * n + i) * kNbrChunk + (k % kNbrChunk)];
+    }
+  }
 }
 
 void Engine::read_cells(uint32_t* cell_of, uint32_t* cell_atoms, uint32_t* cell_start,
@@ -871,6 +896,159 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
   for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
+}
+
+// -------------------------------------------------------------- frame mode --
+
+// Binary32 tables as the forward uses them ({sigma^2, 24 eps, 4 eps, shift},
+// {k, r0}) plus FP64 copies of the same binary32 values for the adjoint.
+void Engine::upload_parameters() {
+  const std::vector<float4> lj =
+      pack_lj_table(n_types_, lj_eps_.data(), lj_sig_.data(), plan_.force_cutoff, lj_shift_);
+  lj_.upload(lj.data(), lj.size(), stream_);
+  std::vector<float2> bc(std::max(1, n_bond_types_), make_float2(0.f, 0.f));
+  for (int t = 0; t < n_bond_types_; ++t) bc[t] = make_float2(bond_k_[t], bond_r0_[t]);
+  bond_coeff_.upload(bc.data(), bc.size(), stream_);
+  const std::vector<double> e(lj_eps_.begin(), lj_eps_.end()), s(lj_sig_.begin(), lj_sig_.end());
+  std::vector<double> k(std::max(1, n_bond_types_), 0.0), r0(std::max(1, n_bond_types_), 0.0);
+  for (int t = 0; t < n_bond_types_; ++t) {
+    k[t] = bond_k_[t];
+    r0[t] = bond_r0_[t];
+  }
+  eps_d_.upload(e.data(), e.size(), stream_);
+  sig_d_.upload(s.data(), s.size(), stream_);
+  bond_k_d_.upload(k.data(), k.size(), stream_);
+  bond_r0_d_.upload(r0.data(), r0.size(), stream_);
+  BEADS_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_lj_params(const float* eps, const float* sigma) {
+  const size_t tt = static_cast<size_t>(n_types_) * n_types_;
+  for (size_t q = 0; q < tt; ++q) {
+    require(std::isfinite(eps[q]) && eps[q] >= 0.f, "lj epsilon must be finite and >= 0.");
+    require(std::isfinite(sigma[q]) && sigma[q] > 0.f, "lj sigma must be finite and > 0.");
+  }
+  lj_eps_.assign(eps, eps + tt);
+  lj_sig_.assign(sigma, sigma + tt);
+  upload_parameters();
+}
+
+void Engine::set_bond_params(const float* k, const float* r0) {
+  for (int t = 0; t < n_bond_types_; ++t) {
+    require(std::isfinite(k[t]) && k[t] >= 0.f && std::isfinite(r0[t]) && r0[t] >= 0.f,
+            "bond k and r0 must be finite and non-negative.");
+  }
+  bond_k_.assign(k, k + n_bond_types_);
+  bond_r0_.assign(r0, r0 + n_bond_types_);
+  upload_parameters();
+}
+
+// One force evaluation at caller-supplied positions ([n][3] binary32 in slot
+// order, device memory): wrap into the box, neighbour rebuild without
+// reordering (forced, or with reuse_list only when some atom moved more than
+// half the skin since the last build, the integrator's test), forces +
+// energies. Frame mode and dynamics are exclusive.
+void Engine::frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
+                          bool reuse_list) {
+  require(!initialized_, "frame mode is not available on an engine that runs dynamics.",
+          BEADS_E_RUNTIME);
+  if (!frame_mode_) {
+    frame_mode_ = true;
+    control_.path = 0;  // never reorder: slots stay the caller's order
+  }
+  hipEvent_t t0, t1;
+  BEADS_HIP(hipEventCreate(&t0));
+  BEADS_HIP(hipEventCreate(&t1));
+  BEADS_HIP(hipEventRecord(t0, stream_));
+  const bool check = reuse_list && frame_loaded_ && plan_.cutoff_buffer > 0.f;
+  FrameLoadParams lp{};
+  lp.n = n_;
+  lp.xyz = positions_dev;
+  lp.pos = pos_.data();
+  lp.image = image_.data();
+  lp.ref_pos = ref_pos_.data();
+  lp.ref_image = ref_image_.data();
+  lp.box = box_;
+  lp.threshold_sq = integrate_params_.threshold_sq;
+  lp.ctrl = ctrl_.data();
+  launch_frame_load(lp, check, stream_);
+  rebuild_sequence(!check);
+  launch_force(force_params_, false, true, stream_);
+  launch_frame_store(n_, force_.data(), forces_dev, stream_);
+  BEADS_HIP(hipEventRecord(t1, stream_));
+  BEADS_HIP(hipGetLastError());
+  const Ctrl c = fetch_ctrl();
+  BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  check_overflow(c);
+  frame_loaded_ = true;
+  if (energies_out != nullptr) {
+    for (int q = 0; q < 4; ++q) energies_out[q] = c.pe[q];
+  }
+}
+
+// Adjoint of the last frame_forces call: grad_forces [n][3] binary32 (device),
+// grad_energy scales dE/d(.), outputs dL/dx [n][3] binary32 and the FP64
+// parameter gradients [eps T*T | sigma T*T | k nb | r0 nb] (device).
+void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
+                            float* grad_positions_dev, double* grad_params_dev) {
+  require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
+  require(acap_ == 0 && dcap_ == 0,
+          "the adjoint differentiates LJ pairs and harmonic bonds; systems with angle or "
+          "dihedral terms are not supported.",
+          BEADS_E_VALUE);
+  require(n_types_ <= kMaxLdsTypes && n_bond_types_ <= kMaxBondTypesAdjoint,
+          "too many types for the adjoint kernels.", BEADS_E_VALUE);
+  for (int a = 0; a < n_types_; ++a) {
+    for (int b = 0; b < a; ++b) {
+      require(lj_eps_[a * n_types_ + b] == lj_eps_[b * n_types_ + a] &&
+                  lj_sig_[a * n_types_ + b] == lj_sig_[b * n_types_ + a],
+              "the adjoint requires symmetric LJ tables.", BEADS_E_VALUE);
+    }
+  }
+  const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
+  grad_force4_.resize(n_);
+  hipEvent_t t0, t1;
+  BEADS_HIP(hipEventCreate(&t0));
+  BEADS_HIP(hipEventCreate(&t1));
+  BEADS_HIP(hipEventRecord(t0, stream_));
+  BEADS_HIP(hipMemsetAsync(grad_params_dev, 0, n_params * sizeof(double), stream_));
+  launch_pack_xyz(n_, grad_forces_dev, grad_force4_.data(), stream_);
+  AdjointParams p{};
+  p.n = n_;
+  p.pos = pos_.data();
+  p.g = grad_force4_.data();
+  p.nbr_count = nbr_count_.data();
+  p.nbr = nbr_.data();
+  p.box = box_;
+  p.rc2 = plan_.force_cutoff * plan_.force_cutoff;
+  p.rc2_f64 = static_cast<double>(plan_.force_cutoff) * plan_.force_cutoff;
+  p.lj_shift = lj_shift_ ? 1 : 0;
+  p.n_types = n_types_;
+  p.eps = eps_d_.data();
+  p.sigma = sig_d_.data();
+  p.topo_count = topo_count_.data();
+  p.bond = bond_.data();
+  p.bcap = bcap_;
+  p.n_bond_types = n_bond_types_;
+  p.bond_k = bond_k_d_.data();
+  p.bond_r0 = bond_r0_d_.data();
+  p.grad_energy = grad_energy;
+  p.grad_pos = grad_positions_dev;
+  p.grad_params = grad_params_dev;
+  launch_fm_backward(p, stream_);
+  BEADS_HIP(hipEventRecord(t1, stream_));
+  BEADS_HIP(hipGetLastError());
+  BEADS_HIP(hipEventSynchronize(t1));
+  BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+}
+
+void Engine::frame_timing(double* forward_s, double* backward_s) {
+  if (forward_s != nullptr) *forward_s = last_forward_ms_ * 1e-3;
+  if (backward_s != nullptr) *backward_s = last_backward_ms_ * 1e-3;
 }
 
 // ----------------------------------------------------------------- outputs --

@@ -59,6 +59,14 @@ struct HostTopology {
 
 HostTopology build_host_topology(const beads_spec_t& spec);
 
+// Shell validation of one spec (throws Error with the matching status).
+void validate_spec(const beads_spec_t& spec);
+
+// Packed LJ coefficients {sigma^2, 24 eps, 4 eps, shift} for all type pairs.
+std::vector<float4> pack_lj_table(const beads_spec_t& spec);
+std::vector<float4> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
+                                  float cutoff, bool shift_energy);
+
 // Bond-graph exclusion pairs (tag_a < tag_b) within the given distance
 // (exclusions.cu:514-598).
 std::vector<std::pair<uint32_t, uint32_t>> bond_graph_exclusions(uint32_t n, uint32_t n_bonds,
@@ -83,7 +91,21 @@ class Engine {
                   uint32_t* cell_count, uint64_t capacity);
   void profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times_t* out);
 
+  // Frame mode (force matching, trajectory backprop): positions come from the
+  // caller in slot order, so particles are never reordered.
+  void set_lj_params(const float* eps, const float* sigma);
+  void set_bond_params(const float* k, const float* r0);
+  void frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
+                    bool reuse_list);
+  void frame_backward(const float* grad_forces_dev, double grad_energy,
+                      float* grad_positions_dev, double* grad_params_dev);
+  void frame_timing(double* forward_s, double* backward_s);
+  uint32_t n_particles() const { return n_; }
+  int32_t n_types() const { return n_types_; }
+  int32_t n_bond_types() const { return n_bond_types_; }
+
  private:
+  void upload_parameters();
   void upload_state();
   void set_up_params();
   void rebuild_sequence(bool forced);
@@ -134,6 +156,16 @@ class Engine {
   PinnedBuffer<Ctrl> ctrl_host_;
   uint32_t bcap_ = 0, acap_ = 0, dcap_ = 0, ecap_ = 0;
   bool has_topology_ = false;
+
+  // Parameters as given (binary32) and their FP64 copies for the adjoint.
+  int32_t n_types_ = 0, n_bond_types_ = 0;
+  bool lj_shift_ = false;
+  std::vector<float> lj_eps_, lj_sig_, bond_k_, bond_r0_;
+  DeviceBuffer<double> eps_d_, sig_d_, bond_k_d_, bond_r0_d_;
+  DeviceBuffer<float4> grad_force4_;
+  bool frame_mode_ = false;   // positions supplied per frame; never reordered
+  bool frame_loaded_ = false;
+  float last_forward_ms_ = 0.f, last_backward_ms_ = 0.f;
 
   // kernel parameter blocks
   ForceParams force_params_{};

@@ -1,0 +1,82 @@
+// Batched replicas: device descriptors and the persistent-launch entry point.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace beads_amd {
+
+// One replica's geometry, parameters and offsets into the batch's arrays.
+struct ReplicaDesc {
+  uint32_t n;             // atoms
+  uint32_t atom0;         // first atom in the concatenated per-atom arrays
+  uint32_t cell0;         // first cell entry (each replica owns n_cells + 1)
+  uint32_t max_neighbors;
+  uint64_t nbr0;          // first neighbour-index entry (max_neighbors * n, slot-major)
+  BoxView box;
+  CellGridView grid;
+  float rs2, rc2, thr2, dt;
+  float cutoff_buffer;
+  int32_t berendsen;
+  int32_t n_types;
+  uint32_t lj0;           // first LJ table entry (n_types^2 float4)
+  double target_thermal;  // kB * T0
+  double dt_over_tau;
+  double dof;
+  uint64_t check_every;
+  uint64_t sort_every;
+};
+
+// Per-replica run state carried across launches.
+struct ReplicaState {
+  uint64_t step;
+  uint64_t build_count;
+  uint64_t reorder_count;
+  uint64_t dangerous;
+  uint32_t checked;
+  uint32_t overflow;
+  uint32_t initialized;
+  uint32_t pad;
+  double lambda;  // Berendsen factor still to be applied to the stored velocities
+  double ke;      // kinetic energy after the last second half-kick
+  double pe;      // pair energy (last observable step)
+  double virial;
+};
+
+struct ReplicaArrays {
+  float4* pos;
+  float4* vel;
+  float4* force;
+  int4* image;
+  uint32_t* tag;
+  float4* ref_pos;
+  int4* ref_image;
+  uint32_t* cell_of;
+  uint32_t* cell_atoms;
+  uint32_t* nbr_count;
+  uint32_t* nbr;
+  uint32_t* cell_count;
+  uint32_t* cell_start;
+  uint32_t* cell_cursor;
+  float4* s_pos;
+  float4* s_vel;
+  int4* s_image;
+  uint32_t* s_tag;
+  uint32_t* s_cell_of;
+};
+
+struct ReplicaParams {
+  const ReplicaDesc* desc;
+  ReplicaState* state;
+  const float4* lj;
+  ReplicaArrays arrays;
+};
+
+// One workgroup per replica, all n_steps inside the launch.
+void launch_replicas(const ReplicaParams& p, uint32_t count, int block, uint64_t n_steps,
+                     bool observables_on_last, hipStream_t stream);
+
+}  // namespace beads_amd

@@ -248,6 +248,58 @@ def polymer_melt(n_chains: int = 20000, beads_per_chain: int = 50, density: floa
         sort_every_rebuild=10, max_neighbors=512, steps=steps, seed=seed)
 
 
+def polymer_solvent(n_chains: int = 500, beads_per_chain: int = 100, n_solvent: int = 50_000,
+                    density: float = 8.5, bond_length: float = 0.47, mass: float = 72.0,
+                    temperature: float = 300.0, seed: int = 21, jitter: float = 0.02,
+                    shift: bool = True) -> Workload:
+    """Four-type polymer/solvent system in nm_kjmol for force matching: chain
+    beads alternate types 1/2, solvent beads are types 3/4 at random; harmonic
+    bonds only (k 1250, r0 0.47), exclusions to distance 2, LJ r_c 1.2 shifted,
+    eps_ab ~ U[2,4] kJ/mol and sigma_ab ~ U[0.43,0.51] nm symmetric, dt 0.01 ps,
+    no thermostat. Chains are consecutive runs of the serpentine lattice path
+    separated by seeded gaps; solvent takes seeded random free sites; every bead
+    is jittered by up to +-jitter nm."""
+    n_chain_beads = n_chains * beads_per_chain
+    n = n_chain_beads + n_solvent
+    L = (n / density) ** (1.0 / 3.0)
+    side = int(math.floor(L / bond_length))
+    while side ** 3 < n:
+        side += 1
+    spacing = L / side
+    path = _serpentine(side)
+    rng = np.random.default_rng(seed)
+    free = side ** 3 - n_chain_beads
+    gaps = rng.multinomial(free, np.full(n_chains + 1, 1.0 / (n_chains + 1)))
+    starts = np.concatenate([[gaps[0]], gaps[0] + np.cumsum(gaps[1:n_chains] + beads_per_chain)])
+    chain_sites = (starts[:, None] + np.arange(beads_per_chain)[None, :]).reshape(-1)
+    used = np.zeros(side ** 3, bool)
+    used[chain_sites] = True
+    solvent_sites = np.sort(rng.choice(np.flatnonzero(~used), size=n_solvent, replace=False))
+    site = np.concatenate([chain_sites, solvent_sites])
+    pos = (path[site].astype(np.float64) + 0.5) * spacing
+    pos += rng.uniform(-jitter, jitter, pos.shape)
+    lo, hi = np.zeros(3), np.full(3, L)
+    masses = np.full(n, mass, np.float32)
+    types = np.concatenate([np.tile(np.array([1, 2], np.int32), n_chain_beads // 2 + 1)[:n_chain_beads],
+                            rng.integers(3, 5, n_solvent).astype(np.int32)])
+    eps = rng.uniform(2.0, 4.0, (4, 4))
+    sig = rng.uniform(0.43, 0.51, (4, 4))
+    eps = np.triu(eps) + np.triu(eps, 1).T
+    sig = np.triu(sig) + np.triu(sig, 1).T
+    tags = np.arange(1, n_chain_beads + 1, dtype=np.uint32).reshape(n_chains, beads_per_chain)
+    b = np.stack([tags[:, :-1], tags[:, 1:], np.ones_like(tags[:, 1:])], axis=-1).reshape(-1, 3)
+    mol = np.concatenate([np.repeat(np.arange(1, n_chains + 1, dtype=np.uint32), beads_per_chain),
+                          np.arange(n_chains + 1, n_chains + 1 + n_solvent, dtype=np.uint32)])
+    return Workload(
+        name=f"polymer_solvent_{n}", units="nm_kjmol", positions=_wrap_into(pos, lo, hi),
+        velocities=maxwell_boltzmann(rng, masses, temperature, KB["nm_kjmol"]), masses=masses,
+        types=types, box_bound=np.stack([lo, hi]).astype(np.float32), lj_epsilon=eps,
+        lj_sigma=sig, cutoff=1.2, shift=shift, molecule_ids=mol, bonds=b,
+        bond_k=np.array([1250.0]), bond_r0=np.array([0.47]), exclusion_distance=2, dt=0.01,
+        cutoff_buffer=0.3, rebuild_check_every=1, sort_every_rebuild=10, max_neighbors=512,
+        steps=200, seed=seed, temperature=temperature)
+
+
 def lj_mixture(n: int = 4_000_000, n_types: int = 4, density: float = 8.5, seed: int = 11,
                steps: int = 200, temperature: float = 300.0, mass: float = 72.0) -> Workload:
     """Multi-type LJ mixture in nm_kjmol: types uniform at random, eps_ab ~ U[1,4]

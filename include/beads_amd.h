@@ -208,7 +208,72 @@ typedef struct beads_phase_times {
 } beads_phase_times_t;
 beads_status_t beads_engine_profile_advance(beads_engine_t* engine, uint64_t n_steps,
                                             int32_t per_phase, beads_phase_times_t* out);
+
+/* ---- frame mode: forces and their adjoint at caller-supplied positions ----
+ * No reference counterpart: the reference computes forces only inside its
+ * dynamics loop (simulation_run.cpp:212-235) and has no gradients. Frame mode
+ * serves force matching and trajectory backprop. Positions are given in slot
+ * order (the order of the spec's arrays) and are never reordered; an engine
+ * used in frame mode cannot also run dynamics. All [n][3] arrays are binary32
+ * device memory; every call synchronises the engine's stream before returning.
+ *
+ * Parameter updates take host arrays: epsilon/sigma [T][T] (symmetric for the
+ * adjoint), bond k/r0 [n_bond_types]. */
+beads_status_t beads_engine_set_lj_params(beads_engine_t* engine, const float* epsilon,
+                                          const float* sigma);
+beads_status_t beads_engine_set_bond_params(beads_engine_t* engine, const float* k,
+                                            const float* r0);
+/* Wrap positions into the box, update the neighbour list, compute forces.
+ * reuse_list = 0 rebuilds the list at every call; reuse_list = 1 keeps the
+ * previous frame's list while every atom stays within half the skin of its
+ * position at the last build (the dynamics loop's test,
+ * neighbor_system.cpp:107-146). energies (host, may be NULL): pair, bond,
+ * angle, dihedral. */
+beads_status_t beads_engine_frame_forces(beads_engine_t* engine, const float* positions,
+                                         float* forces, double* energies, int32_t reuse_list);
+/* Adjoint of the last frame_forces call for L with dL/dF = grad_forces and
+ * dL/dE = grad_energy (E = total potential energy). Writes dL/dx [n][3] and
+ * FP64 parameter gradients (device) laid out as
+ * [epsilon T*T | sigma T*T | bond k nb | bond r0 nb]. LJ pairs and harmonic
+ * bonds only: systems with angle or dihedral terms are rejected. */
+beads_status_t beads_engine_frame_backward(beads_engine_t* engine, const float* grad_forces,
+                                           double grad_energy, float* grad_positions,
+                                           double* grad_params);
+/* Device time of the last frame_forces / frame_backward (HIP events). */
+beads_status_t beads_engine_frame_timing(beads_engine_t* engine, double* forward_s,
+                                         double* backward_s);
 void beads_engine_destroy(beads_engine_t* engine);
+
+/* ---- batched replicas ----
+ * No reference counterpart: the reference runs one system per process
+ * (simulation_run.cpp). Many independent pair-only (LJ) systems, each a full
+ * beads_spec_t (outputs ignored), advance together in ONE persistent launch:
+ * one workgroup per replica runs that replica's whole step loop (integrate,
+ * rebuild with stable sort, full-list LJ + kick + Berendsen) per launch. The
+ * launch is captured once into a hipGraph and replayed. */
+typedef struct beads_batch beads_batch_t;
+
+beads_status_t beads_batch_create(const beads_spec_t* specs, uint32_t count,
+                                  beads_batch_t** out);
+/* Times each candidate workgroup size (256, 512, 1024) for trial_steps from the
+ * initial state, keeps the fastest and restores the initial state. seconds may
+ * be NULL or hold 3 doubles. */
+beads_status_t beads_batch_autotune(beads_batch_t* batch, uint32_t trial_steps,
+                                    int32_t* chosen_block, double* seconds);
+beads_status_t beads_batch_set_block(beads_batch_t* batch, int32_t block);
+/* The first call also performs every replica's initial build and forces. */
+beads_status_t beads_batch_run(beads_batch_t* batch, uint64_t n_steps,
+                               int32_t observables_on_last);
+beads_status_t beads_batch_profile(beads_batch_t* batch, uint64_t n_steps, double* seconds);
+beads_status_t beads_batch_synchronize(beads_batch_t* batch);
+/* stats of replica r (index in the order given to create): build/reorder
+ * counts, kinetic energy, pending lambda, pair energy and virial in pe[0] and
+ * virial[0] (last observable step). */
+beads_status_t beads_batch_replica_stats(beads_batch_t* batch, uint32_t r,
+                                         beads_engine_stats_t* out);
+beads_status_t beads_batch_read_state(beads_batch_t* batch, uint32_t r,
+                                      const beads_state_out_t* out);
+void beads_batch_destroy(beads_batch_t* batch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,293 @@
+"""Adjoint kernel and trajectory backprop on the GPU, against the FP64 oracle.
+
+Gradients of the force-matching loss, of the potential energy and of a loss
+on the end state of a velocity-Verlet trajectory are compared with central
+finite differences of the same quantity evaluated by the FP64 oracle
+(``oracle.forces_f64``). Parameter steps are h = 1e-4 |theta| rounded to the
+binary32 grid (the engine's parameters are binary32); the difference quotient
+divides by the exact FP64 distance between the two binary32 values.
+
+Tolerance (stated here and applied per entry): |g - g_fd| <= 1e-4 *
+max(|g_fd|, 1e-2 * max|g_fd|) for the force-matching and energy gradients;
+trajectory gradients carry the binary32 forward trajectory's rounding and are
+held to 1e-3 with the same denominator.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from beads_amd import workloads as W
+from beads_amd.adjoint import FrameModel, Params, force_matching_loss
+from beads_amd.capi import Engine
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-4
+TRAJ_RTOL = 1e-3
+FORCE_RTOL = 1e-5
+
+
+def small_system(seed=3, shift=True):
+    return W.polymer_solvent(n_chains=12, beads_per_chain=20, n_solvent=900, seed=seed,
+                             jitter=0.03, shift=shift)
+
+
+def _rel_err(g, ref):
+    g = np.asarray(g, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    scale = max(np.max(np.abs(ref)), 1e-300)
+    return float(np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1e-2 * scale)))
+
+
+def _force_error(f_gpu, f_ref):
+    scale = np.sqrt(np.mean(f_ref ** 2))
+    return float(np.max(np.abs(f_gpu - f_ref) / np.maximum(np.abs(f_ref), scale)))
+
+
+def _frames(w, count, seed, amp=0.03):
+    rng = np.random.default_rng(seed)
+    lo = w.box_bound[0].astype(np.float64)
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    out = []
+    for _ in range(count):
+        p = w.positions.astype(np.float64) + rng.uniform(-amp, amp, (w.n, 3))
+        p = lo + np.mod(p - lo, L)
+        out.append(p.astype(np.float32))
+    return out
+
+
+def _with_params(w, eps, sig, k, r0):
+    return w.with_(lj_epsilon=np.asarray(eps, np.float32), lj_sigma=np.asarray(sig, np.float32),
+                   bond_k=np.asarray(k, np.float32), bond_r0=np.asarray(r0, np.float32))
+
+
+def _param_vector(w):
+    return [w.lj_epsilon.astype(np.float32).copy(), w.lj_sigma.astype(np.float32).copy(),
+            w.bond_k.astype(np.float32).copy(), w.bond_r0.astype(np.float32).copy()]
+
+
+def _fd_param(loss_fn, params, which, idx):
+    """Central difference of loss_fn(params) in one parameter (symmetric pair for
+    the [T, T] tables), h = 1e-4 |theta| on the binary32 grid."""
+    base = params[which]
+    theta = float(base[idx])
+    h = 1e-4 * abs(theta)
+    plus = [p.copy() for p in params]
+    minus = [p.copy() for p in params]
+    tp, tm = np.float32(theta + h), np.float32(theta - h)
+    for arr, val in ((plus[which], tp), (minus[which], tm)):
+        arr[idx] = val
+        if arr.ndim == 2:
+            arr[idx[::-1]] = val
+    return (loss_fn(plus) - loss_fn(minus)) / (float(tp) - float(tm))
+
+
+def _param_indices(T):
+    return [(0, (a, b)) for a in range(T) for b in range(a, T)] + \
+           [(1, (a, b)) for a in range(T) for b in range(a, T)] + [(2, 0), (3, 0)]
+
+
+def _gpu_param_grad(grads, which, idx):
+    g = grads[which]
+    if g.ndim == 2:
+        a, b = idx
+        return float(g[a, b] + g[b, a]) if a != b else float(g[a, b])
+    return float(g[idx])
+
+
+def test_frame_forces_match_dynamics_engine_and_oracle():
+    w = small_system()
+    with FrameModel(w) as m:
+        x = torch.tensor(w.positions, device="cuda")
+        p = m.initial_params(requires_grad=False)
+        f, e = m.forces(x, p)
+        f = f.cpu().numpy()
+    with Engine(w) as eng:
+        eng.initialize(observables=True)
+        st = eng.state()
+        stats = eng.stats()
+    np.testing.assert_array_equal(st["tags"], w.tags)
+    np.testing.assert_array_equal(f, st["forces"])
+    assert float(e) == pytest.approx(stats.pe[0] + stats.pe[1], rel=1e-12)
+    f_ref, obs = O.forces_f64(w, w.positions.astype(np.float64))
+    assert _force_error(f.astype(np.float64), f_ref) <= FORCE_RTOL
+    assert float(e) == pytest.approx(obs["pair_pe"] + obs["bond_pe"], rel=1e-5)
+
+
+def test_force_matching_parameter_gradients_vs_fd():
+    w = small_system()
+    frames = _frames(w, 3, seed=9)
+    # Targets: FP64 forces at "true" parameters, 4 % away from the model's.
+    rng = np.random.default_rng(4)
+    true = _param_vector(w)
+    true = [np.asarray(t * rng.uniform(0.96, 1.04, t.shape), np.float32) for t in true]
+    true[0] = np.triu(true[0]) + np.triu(true[0], 1).T
+    true[1] = np.triu(true[1]) + np.triu(true[1], 1).T
+    wt = _with_params(w, *true)
+    targets = [O.forces_f64(wt, x.astype(np.float64))[0] for x in frames]
+
+    def loss64(params):
+        wp = _with_params(w, *params)
+        tot = 0.0
+        for x, ft in zip(frames, targets):
+            f = O.forces_f64(wp, x.astype(np.float64))[0]
+            tot += np.sum((f - ft) ** 2)
+        return tot / (len(frames) * w.n * 3)
+
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        xs = [torch.tensor(x, device="cuda") for x in frames]
+        ts = [torch.tensor(t, device="cuda") for t in targets]
+        loss = force_matching_loss(m, xs, ts, p)
+        loss.backward()
+        grads = [t.grad.cpu().numpy().astype(np.float64) for t in p.tensors()]
+        fwd_s, bwd_s = m.engine.frame_timing()
+    assert fwd_s > 0 and bwd_s > 0
+    params = _param_vector(w)
+    g_gpu, g_fd = [], []
+    for which, idx in _param_indices(w.n_types):
+        g_gpu.append(_gpu_param_grad(grads, which, idx))
+        g_fd.append(_fd_param(loss64, params, which, idx))
+    assert _rel_err(g_gpu, g_fd) <= GRAD_RTOL, (g_gpu, g_fd)
+
+
+def test_force_matching_position_gradient_vs_fd():
+    w = small_system(seed=5)
+    x0 = _frames(w, 1, seed=2)[0]
+    target = O.forces_f64(w, x0.astype(np.float64))[0] * 0.9
+    rng = np.random.default_rng(1)
+    # bonded beads, solvent beads and a few at random
+    atoms = [0, 1, 19, 20, 250, 700, int(w.n - 1)] + list(rng.integers(0, w.n, 5))
+    coords = [(a, int(c)) for a in atoms for c in (0, 1, 2)]
+
+    def loss64(pos64):
+        f = O.forces_f64(w, pos64)[0]
+        return np.sum((f - target) ** 2) / (w.n * 3)
+
+    with FrameModel(w) as m:
+        p = m.initial_params(requires_grad=False)
+        x = torch.tensor(x0, device="cuda", requires_grad=True)
+        loss = force_matching_loss(m, [x], [torch.tensor(target, device="cuda")], p)
+        loss.backward()
+        gx = x.grad.cpu().numpy().astype(np.float64)
+    base = x0.astype(np.float64)
+    g_fd = []
+    for a, c in coords:
+        h = 1e-5
+        pp, pm = base.copy(), base.copy()
+        pp[a, c] += h
+        pm[a, c] -= h
+        g_fd.append((loss64(pp) - loss64(pm)) / (2 * h))
+    g_gpu = [gx[a, c] for a, c in coords]
+    # The device evaluates J^T g from binary32 coordinates: hold it to the
+    # float32 displacement rounding (|dx| ulp / |dx| ~ 1e-6 relative per pair).
+    assert _rel_err(g_gpu, g_fd) <= GRAD_RTOL, (g_gpu, g_fd)
+
+
+@pytest.mark.parametrize("shift", [True, False])
+def test_energy_gradients(shift):
+    w = small_system(seed=7, shift=shift)
+    x0 = _frames(w, 1, seed=3)[0]
+
+    def energy64(params):
+        o = O.forces_f64(_with_params(w, *params), x0.astype(np.float64))[1]
+        return o["pair_pe"] + o["bond_pe"]
+
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        x = torch.tensor(x0, device="cuda", requires_grad=True)
+        f, e = m.forces(x, p)
+        e.backward()
+        grads = [t.grad.cpu().numpy().astype(np.float64) for t in p.tensors()]
+        gx = x.grad.cpu().numpy()
+        f = f.detach().cpu().numpy()
+    # dE/dx = -F: the adjoint's FP64 force against the forward kernel's binary32
+    # force, held to the force tolerance.
+    assert _force_error(-gx.astype(np.float64), f.astype(np.float64)) <= FORCE_RTOL
+    params = _param_vector(w)
+    g_gpu = [_gpu_param_grad(grads, which, idx) for which, idx in _param_indices(w.n_types)]
+    g_fd = [_fd_param(energy64, params, which, idx) for which, idx in _param_indices(w.n_types)]
+    assert _rel_err(g_gpu, g_fd) <= GRAD_RTOL, (g_gpu, g_fd)
+
+
+def _oracle_trajectory(w, params, x0, v0, steps):
+    wp = _with_params(w, *params)
+    dt = float(w.dt)
+    a = 0.5 * dt / w.masses.astype(np.float64)[:, None]
+    lo = w.box_bound[0].astype(np.float64)
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+
+    def force(x):
+        return O.forces_f64(wp, lo + np.mod(x - lo, L))[0]
+
+    x, v = x0.astype(np.float64).copy(), v0.astype(np.float64).copy()
+    f = force(x)
+    for _ in range(steps):
+        v = v + a * f
+        x = x + dt * v
+        f = force(x)
+        v = v + a * f
+    return x, v
+
+
+def test_trajectory_backprop_vs_fd():
+    w = small_system(seed=11)
+    steps, every = 30, 7
+    rng = np.random.default_rng(8)
+    cx = rng.normal(size=(w.n, 3))
+    cv = rng.normal(size=(w.n, 3)) * 0.1
+
+    def loss64(params, x0=w.positions, v0=w.velocities):
+        x, v = _oracle_trajectory(w, params, x0, v0, steps)
+        return float(np.sum(cx * x) + np.sum(cv * v))
+
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        x0 = torch.tensor(w.positions, device="cuda", requires_grad=True)
+        v0 = torch.tensor(w.velocities, device="cuda", requires_grad=True)
+        xT, vT = m.trajectory(x0, v0, p, steps=steps, checkpoint_every=every)
+        loss = (torch.tensor(cx, device="cuda") * xT.double()).sum() + \
+               (torch.tensor(cv, device="cuda") * vT.double()).sum()
+        loss.backward()
+        grads = [t.grad.cpu().numpy().astype(np.float64) for t in p.tensors()]
+        gx0 = x0.grad.cpu().numpy().astype(np.float64)
+        gv0 = v0.grad.cpu().numpy().astype(np.float64)
+    params = _param_vector(w)
+    idx = [(0, (0, 0)), (0, (0, 2)), (1, (1, 1)), (1, (2, 3)), (2, 0), (3, 0)]
+    g_gpu = [_gpu_param_grad(grads, which, i) for which, i in idx]
+    g_fd = [_fd_param(loss64, params, which, i) for which, i in idx]
+    assert _rel_err(g_gpu, g_fd) <= TRAJ_RTOL, (g_gpu, g_fd)
+    # initial-state gradients for a few coordinates
+    base_x = w.positions.astype(np.float64)
+    base_v = w.velocities.astype(np.float64)
+    fd_x, fd_v, gpu_x, gpu_v = [], [], [], []
+    for a, c in [(0, 0), (5, 1), (300, 2), (900, 0)]:
+        h = 1e-5
+        pp, pm = base_x.copy(), base_x.copy()
+        pp[a, c] += h
+        pm[a, c] -= h
+        fd_x.append((loss64(params, pp, base_v) - loss64(params, pm, base_v)) / (2 * h))
+        gpu_x.append(gx0[a, c])
+        vp, vm = base_v.copy(), base_v.copy()
+        vp[a, c] += h
+        vm[a, c] -= h
+        fd_v.append((loss64(params, base_x, vp) - loss64(params, base_x, vm)) / (2 * h))
+        gpu_v.append(gv0[a, c])
+    assert _rel_err(gpu_x, fd_x) <= TRAJ_RTOL, (gpu_x, fd_x)
+    assert _rel_err(gpu_v, fd_v) <= TRAJ_RTOL, (gpu_v, fd_v)
+
+
+def test_frame_mode_rejects_angles_and_dynamics_mixing():
+    w = W.polymer_melt(n_chains=4, beads_per_chain=10, density=8.5)
+    with pytest.raises(ValueError):
+        FrameModel(w)
+    ws = small_system()
+    with FrameModel(ws) as m:
+        x = torch.tensor(ws.positions, device="cuda")
+        m.forces(x, m.initial_params(requires_grad=False))
+        with pytest.raises(Exception):
+            m.engine.initialize()

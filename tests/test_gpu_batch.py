@@ -1,0 +1,87 @@
+"""Batched replicas on the GPU: one persistent launch advances many LJ systems.
+
+Each replica must match the single-system engine (same decisions and FP64
+pair arithmetic, so identical initial forces) and the FP64 oracle to the same
+force bar as the single path.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from beads_amd import workloads as W
+from beads_amd.capi import Batch, Engine
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5
+
+
+def replicas():
+    sizes = [300, 1800, 4000, 700, 2600, 520]
+    dens = [0.9, 0.7, 0.8442, 0.65, 0.85, 0.6]
+    temps = [1.0, 1.3, 0.9, 1.5, 0.8, 1.1]
+    return [W.lj_box(n, d, t, seed=40 + k, berendsen=True, tau=0.5)
+            for k, (n, d, t) in enumerate(zip(sizes, dens, temps))]
+
+
+def _force_error(f_gpu, f_ref):
+    scale = np.sqrt(np.mean(f_ref ** 2))
+    return float(np.max(np.abs(f_gpu - f_ref) / np.maximum(np.abs(f_ref), scale)))
+
+
+def test_initial_forces_match_single_engine_and_oracle():
+    ws = replicas()
+    with Batch(ws) as b:
+        b.run(0)
+        for r, w in enumerate(ws):
+            st = b.state(r)
+            with Engine(w) as e:
+                e.initialize(observables=False)
+                single = e.state()
+            np.testing.assert_array_equal(st["tags"], single["tags"])
+            np.testing.assert_array_equal(st["forces"], single["forces"])
+            f_ref, _ = O.forces(w, positions=st["positions"], tags=st["tags"], types=st["types"])
+            assert _force_error(st["forces"].astype(np.float64), f_ref) <= FORCE_RTOL
+
+
+def test_replicas_follow_single_engine_trajectories():
+    ws = replicas()
+    steps = 60
+    with Batch(ws) as b:
+        b.run(steps, observables_on_last=True)
+        for r, w in enumerate(ws):
+            st = b.state(r)
+            bs = b.stats(r)
+            assert bs.step == steps and bs.overflow == 0
+            with Engine(w) as e:
+                e.initialize(observables=False)
+                e.advance(steps, observables_on_last=True)
+                single = e.state()
+                ss = e.stats()
+            assert bs.build_count == ss.build_count and bs.reorder_count == ss.reorder_count
+            order_b = np.argsort(st["tags"])
+            order_s = np.argsort(single["tags"])
+            L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+            dx = st["positions"][order_b].astype(np.float64) - single["positions"][order_s]
+            dx -= np.round(dx / L) * L
+            assert np.max(np.abs(dx)) < 1e-4, r
+            assert bs.pe[0] == pytest.approx(ss.pe[0], rel=1e-5)
+            assert bs.ke == pytest.approx(ss.ke, rel=1e-5)
+            f_ref, _ = O.forces(w, positions=st["positions"], tags=st["tags"], types=st["types"])
+            assert _force_error(st["forces"].astype(np.float64), f_ref) <= FORCE_RTOL
+
+
+def test_autotuner_picks_a_candidate_and_restores_state():
+    ws = replicas()
+    with Batch(ws) as b:
+        block, secs = b.autotune(trial_steps=10)
+        assert block in (256, 512, 1024)
+        assert all(s > 0 for s in secs)
+        b.run(0)
+        st = b.state(2)
+        np.testing.assert_array_equal(np.sort(st["tags"]), np.arange(1, ws[2].n + 1))
+        # after the autotuner the batch restarts from the initial state
+        assert b.stats(2).step == 0
