@@ -6,11 +6,15 @@ build_info and the ``io`` helpers. Runs execute in the gfx950 engine
 (libbeads_amd.so) through the C ABI declared in include/beads_amd.h.
 """
 
-from . import io
-from ._build_config import build_info
-from .forcefield import Dynamics, ForceField
-from .simulation import Simulation
-from .system import System
+from ._runtime import share_hip_runtime
+
+share_hip_runtime()  # before anything loads libbeads_amd.so
+
+from . import io  # noqa: E402
+from ._build_config import build_info  # noqa: E402
+from .forcefield import Dynamics, ForceField  # noqa: E402
+from .simulation import Simulation  # noqa: E402
+from .system import System  # noqa: E402
 
 __version__ = "0.0.1"
 

@@ -43,6 +43,7 @@ class Spec(C.Structure):
         ("berendsen_tau", C.c_float),
         ("cutoff_buffer", C.c_float), ("rebuild_check_every", C.c_uint64),
         ("sort_every_rebuild", C.c_uint64), ("max_neighbors", C.c_uint32),
+        ("half_list", C.c_int32),
         ("thermo_path", C.c_char_p), ("thermo_every", C.c_uint64),
         ("trajectory_path", C.c_char_p), ("trajectory_every", C.c_uint64),
         ("trajectory_fields", C.c_uint32), ("final_state_path", C.c_char_p),
@@ -200,6 +201,7 @@ def make_spec(w: Workload) -> tuple[Spec, list]:
     s.rebuild_check_every = w.rebuild_check_every
     s.sort_every_rebuild = w.sort_every_rebuild
     s.max_neighbors = w.max_neighbors
+    s.half_list = 1 if w.half_list else 0
     s.log_echo = 3
     s.runsteps = w.steps
     return s, keep

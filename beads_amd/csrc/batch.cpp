@@ -22,6 +22,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
             "batched replicas support pair-only (LJ) systems; replica " + std::to_string(r) +
                 " has listed terms.");
     require(s.n_types <= kMaxLdsTypes, "batched replicas support at most 16 particle types.");
+    require(s.half_list == 0, "batched replicas use the full neighbour list.");
     plans[r] = make_neighbor_plan(make_box_view(s.box_bound), s.lj_cutoff, s.cutoff_buffer,
                                   s.n_particles, s.max_neighbors, s.rebuild_check_every,
                                   s.sort_every_rebuild);

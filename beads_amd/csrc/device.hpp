@@ -5,8 +5,12 @@
 //   vel    float4[N]  vx, vy, vz, w = mass
 //   force  float4[N]  fx, fy, fz, w unused
 //   image  int4[N]    ix, iy, iz, unused
-//   nbr_count u32[N], nbr u32[max_neighbors * N] slot-major
-//     (neighbor_list.cuh:12-19: entry k of atom i at k * N + i)
+//   nbr_count u32[N], nbr u32[ceil(max_neighbors / 16) * 16 * N] in 64-byte
+//     chunks: entries 16c..16c+15 of atom i at ((c * N) + i) * 16 (the
+//     reference's slot-major k * N + i, neighbor_list.cuh:12-19, is what
+//     beads_engine_read_neighbors returns)
+//   pair_acc double[3 N] (half list only): partner-side LJ sums, zeroed by
+//     the force pass that consumes them
 //   per-slot listed-term roles (rebuilt on every reorder):
 //     topo_count u32[N] = bonds | angles << 8 | dihedrals << 16 | exclusions << 24
 //     bond_rec  uint2[bcap * N]  {partner slot, type | first-role << 31}
@@ -60,6 +64,7 @@ struct alignas(256) Ctrl {
   double ke;                // kinetic energy of the velocities as stored
   double pe[4];             // pair, bond, angle, dihedral
   double virial[4];
+  double half_pair[2];      // half list: pair energy and virial summed by the pair pass
 };
 
 struct NeighborControl {
@@ -102,6 +107,8 @@ struct ForceParams {
   Ctrl* ctrl;
   uint32_t* cell_count;   // zeroed after a rebuild consumed it
   uint32_t n_cells;
+  int32_t half;           // stored list holds each pair once (j > i)
+  double* pair_acc;       // [3 N] LJ sums of the half-list pair pass
 };
 
 struct IntegrateParams {
@@ -138,6 +145,7 @@ struct BuildParams {
   uint32_t* nbr_count;
   uint32_t* nbr;
   Ctrl* ctrl;
+  int32_t half;  // keep only partners with a higher slot
 };
 
 struct ReorderParams {

@@ -375,6 +375,7 @@ void validate_spec(const beads_spec_t& s) {
   }
   require(!(s.cutoff_buffer == 0.f && s.rebuild_check_every != 1),
           "neighbor cutoff_buffer=0 requires rebuild_check_every=1");
+  require(s.half_list == 0 || s.half_list == 1, "neighbor half_list must be 0 or 1.");
   // neighbor_list.cu:13-28: slot-major indices are 32-bit.
   if (static_cast<uint64_t>(s.n_particles) * s.max_neighbors > 0xffffffffull) {
     fail(BEADS_E_OVERFLOW, "NeighborList n_particles * max_neighbors exceeds index_t range.");
@@ -400,6 +401,7 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
   for (int a = 0; a < 3; ++a) spec_box_hi_[a] = spec.box_bound[3 + a];
   n_ = spec.n_particles;
   thermostat_ = spec.berendsen != 0;
+  half_list_ = spec.half_list != 0;
   has_bond_style_ = spec.n_bond_types > 0;
   has_angle_style_ = spec.n_angle_types > 0;
   has_dihedral_style_ = spec.n_dihedral_types > 0;
@@ -519,6 +521,10 @@ void Engine::upload_state() {
   dih_coeff_.upload(dc.data(), dc.size(), stream_);
 
   partials_.resize(9 * static_cast<size_t>(grid_for(n_)));
+  if (half_list_) {
+    pair_acc_.resize(3 * n);
+    pair_acc_.zero(stream_);
+  }
   Ctrl c{};
   c.lambda = 1.0;
   ctrl_.upload(&c, 1, stream_);
@@ -565,6 +571,8 @@ void Engine::set_up_params() {
   f.ctrl = ctrl_.data();
   f.cell_count = cell_count_.data();
   f.n_cells = plan_.n_cells;
+  f.half = half_list_ ? 1 : 0;
+  f.pair_acc = pair_acc_.data();
 
   IntegrateParams& in = integrate_params_;
   in.n = n_;
@@ -608,6 +616,7 @@ void Engine::set_up_params() {
   b.nbr_count = nbr_count_.data();
   b.nbr = nbr_.data();
   b.ctrl = ctrl_.data();
+  b.half = half_list_ ? 1 : 0;
 
   BinParams& bp = bin_params_;
   bp.n = n_;
@@ -994,6 +1003,8 @@ void Engine::frame_forces(const float* positions_dev, float* forces_dev, double*
 void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
                             float* grad_positions_dev, double* grad_params_dev) {
   require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
+  require(!half_list_, "the adjoint reads the full neighbour list; create the engine with "
+          "half_list = 0.", BEADS_E_VALUE);
   require(acap_ == 0 && dcap_ == 0,
           "the adjoint differentiates LJ pairs and harmonic bonds; systems with angle or "
           "dihedral terms are not supported.",

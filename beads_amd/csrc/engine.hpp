@@ -130,6 +130,7 @@ class Engine {
   NeighborPlan plan_;
   uint32_t n_ = 0;
   bool thermostat_ = false;
+  bool half_list_ = false;
   bool has_bond_style_ = false, has_angle_style_ = false, has_dihedral_style_ = false;
   uint64_t step_ = 0;
   bool initialized_ = false;
@@ -152,6 +153,7 @@ class Engine {
   DeviceBuffer<float2> bond_coeff_, angle_coeff_;
   DeviceBuffer<float4> dih_coeff_;
   DeviceBuffer<double> partials_;
+  DeviceBuffer<double> pair_acc_;  // half list: partner-side LJ sums [3 N]
   DeviceBuffer<Ctrl> ctrl_;
   PinnedBuffer<Ctrl> ctrl_host_;
   uint32_t bcap_ = 0, acap_ = 0, dcap_ = 0, ecap_ = 0;

@@ -222,10 +222,11 @@ __global__ __launch_bounds__(kBlock) void reorder_commit_kernel(ReorderParams p)
 // time so their position loads overlap. Appends are staged per work-item in
 // LDS and written as whole 64-byte chunks. EXCL: 0 = no exclusions, 1 = up to
 // 8 partners held in registers, 2 = any count read from memory. Also captures
-// the positions the displacement check compares against.
+// the positions the displacement check compares against. HALF keeps only
+// partners with a higher slot, so every pair is stored once.
 constexpr uint32_t kStageStride = kNbrChunk + 4;  // 80-byte rows: 16-byte aligned
 
-template <int EXCL>
+template <int EXCL, bool HALF>
 __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
   if (p.ctrl->rebuild == 0) {
     return;
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
       const float dy = mic(xi.y - xj.y, L1, h1);
       const float dz = mic(xi.z - xj.z, L2, h2);
       const float r2 = r2_strict(dx, dy, dz);
-      if (r2 > p.rs2 || j == i) {
+      if (r2 > p.rs2 || (HALF ? j <= i : j == i)) {
         return;
       }
       if (EXCL == 1) {
@@ -347,28 +348,123 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 
 // ------------------------------------------------------------------ force --
 
-// Fused force evaluation: LJ over the stored full list (pair_lj.cu:21-167),
-// listed terms gathered per atom (each atom evaluates every term it belongs to
-// and keeps its own role's force: no atomics, deterministic), then optionally
-// the post-force half-kick (velocity_verlet.cu:97-111) and the kinetic-energy
-// reduction for the Berendsen factor (berendsen_thermostat.cu:109-163), with
-// the factor computed in FP64 by the last workgroup.
-// The neighbour loop reads one 16-byte quad of indices and issues the four
-// position gathers together before any arithmetic.
+// LJ over one atom's stored list (pair_lj.cu:21-167). The loop reads one
+// 16-byte quad of indices and issues the four position gathers together
+// before any arithmetic. The decision r^2 < r_c^2 is binary32 on the
+// exact-intermediate minimum image; the pair value is FP64. REACT adds the
+// reaction -s d of every pair to pair_acc[j] (half list).
 // TYPES: 0 = single type (coefficients in registers), 1 = table in LDS,
 // 2 = table in global memory.
-template <bool KICK, bool OBS, int TYPES>
-__global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
-  __shared__ float4 lj_lds[TYPES == 1 ? kMaxLdsTypes * kMaxLdsTypes : 1];
-  __shared__ double red_lds[4];
+template <bool OBS, int TYPES, bool REACT>
+__device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4 xi4,
+                                        const float4* lj_lds, float4 c0, double& fxd,
+                                        double& fyd, double& fzd, double& e_pair,
+                                        double& w_pair) {
   const int T = p.n_types;
+  const int ti = __float_as_int(xi4.w);
+  const uint32_t n = p.n;
+  const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+  const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+  const uint32_t cnt = p.nbr_count[i];
+  for (uint32_t base = 0; base < cnt; base += 4) {
+    const uint4 q = *reinterpret_cast<const uint4*>(
+        p.nbr + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1)));
+    const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
+    float4 xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xj[u] = p.pos[base + u < cnt ? jj[u] : i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
+      const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
+      const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
+      const float r2 = r2_strict(dx, dy, dz);
+      if (base + u < cnt && r2 < p.rc2) {
+        float4 c;
+        if (TYPES == 0) {
+          c = c0;
+        } else if (TYPES == 1) {
+          c = lj_lds[ti * T + __float_as_int(xj[u].w)];
+        } else {
+          c = p.lj[ti * T + __float_as_int(xj[u].w)];
+        }
+        double rr, u_pair;
+        const double s = lj_pair(dx, dy, dz, c, rr, u_pair);
+        fxd += s * dx;
+        fyd += s * dy;
+        fzd += s * dz;
+        if (REACT) {
+          double* acc = p.pair_acc + 3 * static_cast<size_t>(jj[u]);
+          unsafeAtomicAdd(acc, -s * dx);
+          unsafeAtomicAdd(acc + 1, -s * dy);
+          unsafeAtomicAdd(acc + 2, -s * dz);
+        }
+        if (OBS) {
+          e_pair += u_pair;
+          w_pair += s * rr;
+        }
+      }
+    }
+  }
+}
+
+template <int TYPES>
+__device__ __forceinline__ void load_lj_table(const ForceParams& p, float4* lj_lds) {
   if (TYPES == 1) {
-    for (int k = threadIdx.x; k < T * T; k += kBlock) {
+    for (int k = threadIdx.x; k < p.n_types * p.n_types; k += kBlock) {
       lj_lds[k] = p.lj[k];
     }
     __syncthreads();
   }
+}
+
+// Half-list pair pass (no reference counterpart; the reference keeps a full
+// list): every stored pair (i, j > i) is evaluated once, the force on i is
+// summed in registers and the reaction on j added to pair_acc with FP64
+// atomics; i's own sum is added last. Pair energy and virial are summed per
+// workgroup and added to ctrl->half_pair. The force pass that follows adds the
+// listed terms, kicks, reduces and clears pair_acc.
+template <bool OBS, int TYPES>
+__global__ __launch_bounds__(kBlock) void half_pair_kernel(ForceParams p) {
+  __shared__ float4 lj_lds[TYPES == 1 ? kMaxLdsTypes * kMaxLdsTypes : 1];
+  __shared__ double red_lds[4];
+  load_lj_table<TYPES>(p, lj_lds);
   const float4 c0 = TYPES == 0 ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double e_pair = 0.0, w_pair = 0.0;
+  if (i < p.n) {
+    double fxd = 0.0, fyd = 0.0, fzd = 0.0;
+    lj_list<OBS, TYPES, true>(p, i, p.pos[i], lj_lds, c0, fxd, fyd, fzd, e_pair, w_pair);
+    double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
+    unsafeAtomicAdd(acc, fxd);
+    unsafeAtomicAdd(acc + 1, fyd);
+    unsafeAtomicAdd(acc + 2, fzd);
+  }
+  if (OBS) {
+    const double e = block_sum<kBlock>(e_pair, red_lds);
+    const double w = block_sum<kBlock>(w_pair, red_lds);
+    if (threadIdx.x == 0) {
+      unsafeAtomicAdd(&p.ctrl->half_pair[0], e);
+      unsafeAtomicAdd(&p.ctrl->half_pair[1], w);
+    }
+  }
+}
+
+// Fused force evaluation: LJ over the stored full list (pair_lj.cu:21-167) or
+// the half-list pair pass's sums (HALF), listed terms gathered per atom (each
+// atom evaluates every term it belongs to and keeps its own role's force: no
+// atomics, deterministic), then optionally the post-force half-kick
+// (velocity_verlet.cu:97-111) and the kinetic-energy reduction for the
+// Berendsen factor (berendsen_thermostat.cu:109-163), with the factor computed
+// in FP64 by the last workgroup.
+template <bool KICK, bool OBS, int TYPES, bool HALF>
+__global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
+  __shared__ float4 lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
+  __shared__ double red_lds[4];
+  if (!HALF) load_lj_table<TYPES>(p, lj_lds);
+  const float4 c0 = TYPES == 0 && !HALF ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const uint32_t n = p.n;
   double ke = 0.0;
@@ -376,46 +472,17 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
   if (i < n) {
     const float4 xi4 = p.pos[i];
-    const int ti = __float_as_int(xi4.w);
-    const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
-    const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     double fxd = 0.0, fyd = 0.0, fzd = 0.0;
-    const uint32_t cnt = p.nbr_count[i];
-    for (uint32_t base = 0; base < cnt; base += 4) {
-      const uint4 q = *reinterpret_cast<const uint4*>(
-          p.nbr + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1)));
-      const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
-      float4 xj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = p.pos[base + u < cnt ? jj[u] : i];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
-        const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
-        const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
-        const float r2 = r2_strict(dx, dy, dz);
-        if (base + u < cnt && r2 < p.rc2) {
-          float4 c;
-          if (TYPES == 0) {
-            c = c0;
-          } else if (TYPES == 1) {
-            c = lj_lds[ti * T + __float_as_int(xj[u].w)];
-          } else {
-            c = p.lj[ti * T + __float_as_int(xj[u].w)];
-          }
-          double rr, u_pair;
-          const double s = lj_pair(dx, dy, dz, c, rr, u_pair);
-          fxd += s * dx;
-          fyd += s * dy;
-          fzd += s * dz;
-          if (OBS) {
-            e_pair += u_pair;
-            w_pair += s * rr;
-          }
-        }
-      }
+    if (HALF) {
+      double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
+      fxd = acc[0];
+      fyd = acc[1];
+      fzd = acc[2];
+      acc[0] = 0.0;
+      acc[1] = 0.0;
+      acc[2] = 0.0;
+    } else {
+      lj_list<OBS, TYPES, false>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, e_pair, w_pair);
     }
     float fx = static_cast<float>(fxd);
     float fy = static_cast<float>(fyd);
@@ -526,9 +593,9 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
         ctrl->lambda = lambda;
         if (!KICK) ctrl->ke = acc;
       } else if (q <= 4) {
-        ctrl->pe[q - 1] = acc;
+        ctrl->pe[q - 1] = (HALF && q == 1) ? ctrl->half_pair[0] : acc;
       } else {
-        ctrl->virial[q - 5] = acc;
+        ctrl->virial[q - 5] = (HALF && q == 5) ? ctrl->half_pair[1] : acc;
       }
     }
   }
@@ -536,6 +603,10 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
     p.ctrl->rebuild = 0;
     p.ctrl->reorder = 0;
     p.ctrl->arrive_force = 0;
+    if (HALF) {
+      p.ctrl->half_pair[0] = 0.0;
+      p.ctrl->half_pair[1] = 0.0;
+    }
   }
 }
 
@@ -666,22 +737,43 @@ void launch_reorder(const ReorderParams& p, hipStream_t stream) {
 
 void launch_build(const BuildParams& p, hipStream_t stream) {
   const unsigned g = grid_for(p.n);
-  if (p.ecap == 0) {
-    build_kernel<0><<<g, kBlock, 0, stream>>>(p);
+  if (p.half) {
+    if (p.ecap == 0) {
+      build_kernel<0, true><<<g, kBlock, 0, stream>>>(p);
+    } else if (p.ecap <= 8) {
+      build_kernel<1, true><<<g, kBlock, 0, stream>>>(p);
+    } else {
+      build_kernel<2, true><<<g, kBlock, 0, stream>>>(p);
+    }
+  } else if (p.ecap == 0) {
+    build_kernel<0, false><<<g, kBlock, 0, stream>>>(p);
   } else if (p.ecap <= 8) {
-    build_kernel<1><<<g, kBlock, 0, stream>>>(p);
+    build_kernel<1, false><<<g, kBlock, 0, stream>>>(p);
   } else {
-    build_kernel<2><<<g, kBlock, 0, stream>>>(p);
+    build_kernel<2, false><<<g, kBlock, 0, stream>>>(p);
   }
 }
 
 void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream) {
   const unsigned g = grid_for(p.n);
   const int types = p.n_types == 1 ? 0 : (p.n_types <= kMaxLdsTypes ? 1 : 2);
-#define BEADS_FORCE_CASE(K, O, T)                                       \
-  if (kick == K && obs == O && types == T) {                            \
-    force_kernel<K, O, T><<<g, kBlock, 0, stream>>>(p);                 \
-    return;                                                             \
+  const bool half = p.half != 0;
+  if (half) {
+#define BEADS_HALF_CASE(O, T)                                           \
+    if (obs == O && types == T) half_pair_kernel<O, T><<<g, kBlock, 0, stream>>>(p);
+    BEADS_HALF_CASE(false, 0)
+    BEADS_HALF_CASE(false, 1)
+    BEADS_HALF_CASE(false, 2)
+    BEADS_HALF_CASE(true, 0)
+    BEADS_HALF_CASE(true, 1)
+    BEADS_HALF_CASE(true, 2)
+#undef BEADS_HALF_CASE
+  }
+#define BEADS_FORCE_CASE(K, O, T)                                        \
+  if (kick == K && obs == O && (half || types == T)) {                   \
+    if (half) force_kernel<K, O, 0, true><<<g, kBlock, 0, stream>>>(p);  \
+    else force_kernel<K, O, T, false><<<g, kBlock, 0, stream>>>(p);      \
+    return;                                                              \
   }
   BEADS_FORCE_CASE(true, false, 0)
   BEADS_FORCE_CASE(true, false, 1)

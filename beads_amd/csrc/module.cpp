@@ -380,8 +380,16 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
 }
 
 void parse_neighbor(const py::dict& nb, ParsedSpec& out) {
-  require_only_keys(nb, {"cutoff_buffer", "rebuild_check_every", "sort_every_rebuild", "max_neighbors"},
+  require_only_keys(nb, {"cutoff_buffer", "rebuild_check_every", "sort_every_rebuild", "max_neighbors",
+                         "half_list"},
                     "neighbor");
+  // Optional, no reference counterpart: store each LJ pair once.
+  out.c.half_list = 0;
+  if (nb.contains("half_list")) {
+    const py::handle h = nb["half_list"];
+    if (!py::isinstance<py::bool_>(h)) type_error("neighbor.half_list must be a bool.");
+    out.c.half_list = h.cast<bool>() ? 1 : 0;
+  }
   const double buf = require_real(require_item(nb, "cutoff_buffer", "neighbor"), "neighbor.cutoff_buffer");
   if (buf < 0.0) value_error("neighbor cutoff_buffer must be finite and non-negative.");
   out.c.cutoff_buffer = static_cast<float>(buf);

@@ -27,8 +27,13 @@ class NeighborSettings:
     rebuild_check_every: int = 1
     sort_every_rebuild: int = 10
     max_neighbors: int = 512
+    # No reference counterpart: store each LJ pair once (j > i) and add the
+    # reaction with atomics instead of keeping the reference's full list.
+    half_list: bool = False
 
     def __post_init__(self) -> None:
+        if not isinstance(self.half_list, bool):
+            raise TypeError("neighbor half_list must be a bool")
         buf = nonnegative_real("neighbor cutoff_buffer", self.cutoff_buffer)
         check = positive_uint64("neighbor rebuild_check_every", self.rebuild_check_every)
         sort = positive_uint64("neighbor sort_every_rebuild", self.sort_every_rebuild)
@@ -97,6 +102,9 @@ def _log_settings(echo: object, prefix: object | None) -> dict:
 
 def load_native():
     """The compiled engine binding; fails loudly when it is not built."""
+    from ._runtime import share_hip_runtime
+
+    share_hip_runtime()
     try:
         return import_module("beads_amd._native")
     except ModuleNotFoundError as exc:
@@ -145,10 +153,11 @@ class Simulation:
         return self
 
     def set_neighbor(self, *, cutoff_buffer: object = 0.3, rebuild_check_every: object = 1,
-                     sort_every_rebuild: object = 10, max_neighbors: object = 512) -> "Simulation":
+                     sort_every_rebuild: object = 10, max_neighbors: object = 512,
+                     half_list: bool = False) -> "Simulation":
         self._require_not_executed()
         self._neighbor = NeighborSettings(cutoff_buffer, rebuild_check_every,
-                                          sort_every_rebuild, max_neighbors)
+                                          sort_every_rebuild, max_neighbors, half_list)
         return self
 
     def set_thermo(self, *, every: object, prefix: object) -> "Simulation":
@@ -231,6 +240,7 @@ class Simulation:
                 "rebuild_check_every": self._neighbor.rebuild_check_every,
                 "sort_every_rebuild": self._neighbor.sort_every_rebuild,
                 "max_neighbors": self._neighbor.max_neighbors,
+                **({"half_list": True} if self._neighbor.half_list else {}),
             },
             "output": output,
             "runsteps": self._runsteps,

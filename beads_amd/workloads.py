@@ -56,6 +56,7 @@ class Workload:
     rebuild_check_every: int = 1
     sort_every_rebuild: int = 10
     max_neighbors: int = 512
+    half_list: bool = False
     steps: int = 100
     seed: int = 0
 
@@ -139,7 +140,7 @@ class Workload:
         sim.set_neighbor(cutoff_buffer=float(self.cutoff_buffer),
                          rebuild_check_every=int(self.rebuild_check_every),
                          sort_every_rebuild=int(self.sort_every_rebuild),
-                         max_neighbors=int(self.max_neighbors))
+                         max_neighbors=int(self.max_neighbors), half_list=bool(self.half_list))
         sim.set_runsteps(int(self.steps))
         return sim
 

@@ -16,6 +16,14 @@ roofline: the fused force kernel (the dominant kernel) against HBM peak.
   duration measured with HIP events on the engine's stream.
 cpu_baseline: the FP64 OpenMP host oracle on a bounded sample of the same
   workload (same system, fewer steps), on this host's cores.
+
+batched: 512 reduced-unit LJ replicas (N log-uniform in [500, 20000]) advanced
+  by one persistent hipGraph-captured launch (workgroup size from the
+  autotuner) against the same replicas run one after another on the plain
+  single-system path; both timed by the host clock around synchronised runs.
+adjoint: the 100,000-bead polymer/solvent force-matching workload (64 seeded
+  frames, MSE to seeded target forces) forward vs forward+backward, plus a
+  200-step velocity-Verlet trajectory backprop with checkpoints every 20 steps.
 """
 
 from __future__ import annotations
@@ -61,13 +69,141 @@ def cpu_baseline(w, sample_steps: int) -> dict:
                       f"{st.wall_seconds:.1f} s step loop"}
 
 
+def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
+    """Batched replicas vs serial plain runs of the same replicas."""
+    from beads_amd import workloads as W
+    from beads_amd.capi import Batch, Engine
+
+    ws = W.replica_set(count)
+    atoms = sum(w.n for w in ws)
+    print(f"bench: batched leg, {count} replicas, {atoms} atoms", file=sys.stderr, flush=True)
+    with Batch(ws) as b:
+        block, trial = b.autotune(trial_steps=20)
+        b.run(0)
+        b.synchronize()
+        t0 = time.perf_counter()
+        dev = b.profile(steps)
+        b.synchronize()
+        wall = time.perf_counter() - t0
+    batched = atoms * steps / wall
+    # Serial: the plain path, one replica at a time, each warmed up first.
+    serial_wall = 0.0
+    for w in ws:
+        with Engine(w) as e:
+            e.initialize(observables=False)
+            e.profile(5)
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.profile(serial_steps)
+            e.synchronize()
+            serial_wall += time.perf_counter() - t0
+    serial = atoms * serial_steps / serial_wall
+    return {"workload": f"{count} LJ replicas, N log-uniform [500, 20000], {atoms} atoms, "
+                        f"reduced units, Berendsen",
+            "steps": steps, "block": block, "autotune_s": [round(x, 5) for x in trial],
+            "batched_atom_steps_per_s": batched, "batched_device_s": dev,
+            "batched_replica_steps_per_s": count * steps / wall,
+            "serial_atom_steps_per_s": serial, "serial_steps_per_replica": serial_steps,
+            "serial_replica_steps_per_s": count * serial_steps / serial_wall,
+            "speedup": batched / serial}
+
+
+def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
+    """Force-matching gradient and trajectory backprop cost vs the forward pass."""
+    import numpy as np
+    import torch
+
+    from beads_amd import workloads as W
+    from beads_amd.adjoint import FrameModel, force_matching_loss
+
+    w = W.polymer_solvent()
+    print(f"bench: adjoint leg, {w.n} beads, {frames} frames", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(123)
+    lo = w.box_bound[0].astype(np.float64)
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    xs = []
+    for _ in range(frames):
+        p = w.positions.astype(np.float64) + rng.uniform(-0.03, 0.03, (w.n, 3))
+        xs.append(torch.tensor((lo + np.mod(p - lo, L)).astype(np.float32), device="cuda"))
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        # Seeded targets: forces at parameters 4 % away from the model's.
+        with torch.no_grad():
+            true = [t.detach() * torch.tensor(rng.uniform(0.96, 1.04, tuple(t.shape)),
+                                               dtype=torch.float32, device="cuda")
+                    for t in p.tensors()]
+            true[0] = torch.triu(true[0]) + torch.triu(true[0], 1).T
+            true[1] = torch.triu(true[1]) + torch.triu(true[1], 1).T
+            from beads_amd.adjoint import Params
+            tp = Params(*true)
+            targets = [m.forces(x, tp)[0].clone() for x in xs]
+        # warm-up
+        force_matching_loss(m, xs[:2], targets[:2], p).backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            force_matching_loss(m, xs, targets, p)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        fwd_dev = bwd_dev = 0.0
+        calls = 0
+        for x in xs[:8]:
+            f, _ = m.forces(x, p)
+            fd, _ = m.engine.frame_timing()
+            (f.double() ** 2).sum().backward()
+            _, bd = m.engine.frame_timing()
+            fwd_dev += fd
+            bwd_dev += bd
+            calls += 1
+        for t in p.tensors():
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = force_matching_loss(m, xs, targets, p)
+        loss.backward()
+        torch.cuda.synchronize()
+        t_fb = time.perf_counter() - t0
+        grad_norm = float(sum((t.grad.double() ** 2).sum() for t in p.tensors()) ** 0.5)
+        # trajectory backprop
+        x0 = torch.tensor(w.positions, device="cuda", requires_grad=True)
+        v0 = torch.tensor(w.velocities, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xT, vT = m.trajectory(x0, v0, p, steps=traj_steps, checkpoint_every=every)
+        torch.cuda.synchronize()
+        t_traj_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ((xT.double() ** 2).sum() * 1e-3 + (vT.double() ** 2).sum()).backward()
+        torch.cuda.synchronize()
+        t_traj_b = time.perf_counter() - t0
+    return {"workload": f"polymer/solvent {w.n} beads (500 x 100 chains + 50,000 solvent), "
+                        f"4 types, 22 parameters, nm_kjmol",
+            "frames": frames, "loss": float(loss), "grad_norm": grad_norm,
+            "forward_s": t_fwd, "forward_backward_s": t_fb,
+            "gradient_over_forward": t_fb / t_fwd,
+            "backward_over_forward": (t_fb - t_fwd) / t_fwd,
+            "kernel_backward_over_forward": bwd_dev / fwd_dev,
+            "frame_forward_device_us": fwd_dev / calls * 1e6,
+            "frame_backward_device_us": bwd_dev / calls * 1e6,
+            "trajectory": {"steps": traj_steps, "checkpoint_every": every,
+                           "forward_s": t_traj_f, "backward_s": t_traj_b,
+                           "backward_over_forward": t_traj_b / t_traj_f}}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="melt1m")
-    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--cpu-sample-steps", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--serial-steps", type=int, default=100)
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--adjoint-frames", type=int, default=64)
+    ap.add_argument("--traj-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-phase-profile", action="store_true")
     args = ap.parse_args()
@@ -114,12 +250,20 @@ def main() -> None:
     builds = int(s1.build_count - s0.build_count)
     eng.close()
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(args.batch_steps, args.serial_steps, args.replicas)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(args.adjoint_frames, args.traj_steps, 20)
+
     cpu = None
     if not args.no_cpu_baseline:
+        print("bench: CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w, args.cpu_sample_steps)
 
     line = {
-        "metric": "atom-steps/s at 1M beads",
+        "metric": "atom-steps/s at 1M beads; batched vs serial replicas/s; adjoint/forward cost",
         "value": value,
         "unit": "atom-steps/s",
         "n_gpus": 1,
@@ -136,6 +280,8 @@ def main() -> None:
                    "mean_neighbors": round(s1.mean_neighbors, 2), "phases": phases},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(line))
 

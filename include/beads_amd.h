@@ -113,6 +113,9 @@ typedef struct beads_spec {
   uint64_t rebuild_check_every;
   uint64_t sort_every_rebuild;
   uint32_t max_neighbors;
+  int32_t half_list;             /* 0: full list (the reference's); 1: each pair stored once,
+                                    j > i, reactions added with FP64 atomics (no reference
+                                    counterpart; LJ only, listed terms unchanged) */
 
   /* outputs (NULL path = disabled) */
   const char* thermo_path;

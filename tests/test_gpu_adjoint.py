@@ -70,12 +70,12 @@ def _param_vector(w):
             w.bond_k.astype(np.float32).copy(), w.bond_r0.astype(np.float32).copy()]
 
 
-def _fd_param(loss_fn, params, which, idx):
+def _fd_param(loss_fn, params, which, idx, h_rel=1e-4):
     """Central difference of loss_fn(params) in one parameter (symmetric pair for
-    the [T, T] tables), h = 1e-4 |theta| on the binary32 grid."""
+    the [T, T] tables), h = h_rel |theta| on the binary32 grid."""
     base = params[which]
     theta = float(base[idx])
-    h = 1e-4 * abs(theta)
+    h = h_rel * abs(theta)
     plus = [p.copy() for p in params]
     minus = [p.copy() for p in params]
     tp, tm = np.float32(theta + h), np.float32(theta - h)
@@ -215,14 +215,22 @@ def test_energy_gradients(shift):
 
 
 def _oracle_trajectory(w, params, x0, v0, steps):
+    """FP64 velocity Verlet with oracle forces; also returns, per force
+    evaluation, the set of pairs closer than r_c (periodic KD-tree)."""
+    from scipy.spatial import cKDTree
+
     wp = _with_params(w, *params)
     dt = float(w.dt)
     a = 0.5 * dt / w.masses.astype(np.float64)[:, None]
     lo = w.box_bound[0].astype(np.float64)
     L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    inside = []
 
     def force(x):
-        return O.forces_f64(wp, lo + np.mod(x - lo, L))[0]
+        xw = lo + np.mod(x - lo, L)
+        tree = cKDTree(np.minimum(xw - lo, np.nextafter(L, 0)), boxsize=L)
+        inside.append(tree.query_pairs(float(w.cutoff), output_type="ndarray"))
+        return O.forces_f64(wp, xw)[0]
 
     x, v = x0.astype(np.float64).copy(), v0.astype(np.float64).copy()
     f = force(x)
@@ -231,7 +239,34 @@ def _oracle_trajectory(w, params, x0, v0, steps):
         x = x + dt * v
         f = force(x)
         v = v + a * f
-    return x, v
+    return x, v, inside
+
+
+def _same_pairs(a, b):
+    if len(a) != len(b):
+        return False
+    for p, q in zip(a, b):
+        if p.shape != q.shape:
+            return False
+        kp = np.sort(p[:, 0].astype(np.int64) * 1_000_003 + p[:, 1])
+        kq = np.sort(q[:, 0].astype(np.int64) * 1_000_003 + q[:, 1])
+        if not np.array_equal(kp, kq):
+            return False
+    return True
+
+
+def _smooth_central_difference(run, steps_h):
+    """run(h) -> (loss, inside-pair sets) at the perturbed point h (h may be
+    negative). The cut-off makes the force jump at r_c, so a difference
+    quotient whose two runs put some pair on opposite sides of r_c at some
+    force evaluation measures a jump that the derivative does not have. Take
+    the largest step whose two runs see identical pair sets everywhere."""
+    for h in steps_h:
+        lp, sp = run(h)
+        lm, sm = run(-h)
+        if _same_pairs(sp, sm):
+            return (lp - lm) / (2 * h)
+    raise AssertionError("no finite-difference step keeps the cut-off crossings unchanged")
 
 
 def test_trajectory_backprop_vs_fd():
@@ -241,9 +276,9 @@ def test_trajectory_backprop_vs_fd():
     cx = rng.normal(size=(w.n, 3))
     cv = rng.normal(size=(w.n, 3)) * 0.1
 
-    def loss64(params, x0=w.positions, v0=w.velocities):
-        x, v = _oracle_trajectory(w, params, x0, v0, steps)
-        return float(np.sum(cx * x) + np.sum(cv * v))
+    def loss_pairs(params, x0=w.positions, v0=w.velocities):
+        x, v, inside = _oracle_trajectory(w, params, x0, v0, steps)
+        return float(np.sum(cx * x) + np.sum(cv * v)), inside
 
     with FrameModel(w) as m:
         p = m.initial_params()
@@ -259,23 +294,50 @@ def test_trajectory_backprop_vs_fd():
     params = _param_vector(w)
     idx = [(0, (0, 0)), (0, (0, 2)), (1, (1, 1)), (1, (2, 3)), (2, 0), (3, 0)]
     g_gpu = [_gpu_param_grad(grads, which, i) for which, i in idx]
-    g_fd = [_fd_param(loss64, params, which, i) for which, i in idx]
+    g_fd = []
+    for which, i in idx:
+        theta = float(params[which][i])
+
+        def run(h, which=which, i=i, theta=theta):
+            q = [t.copy() for t in params]
+            val = np.float32(theta + h)
+            q[which][i] = val
+            if q[which].ndim == 2:
+                q[which][i[::-1]] = val
+            loss, inside = loss_pairs(q)
+            return loss, inside
+
+        # steps on the binary32 grid: the quotient divides by the exact distance
+        for h_rel in (1e-4, 2e-5, 4e-6, 1e-6, 2.5e-7):
+            h = h_rel * abs(theta)
+            tp, tm = np.float32(theta + h), np.float32(theta - h)
+            lp, sp = run(float(tp) - theta)
+            lm, sm = run(float(tm) - theta)
+            if _same_pairs(sp, sm):
+                g_fd.append((lp - lm) / (float(tp) - float(tm)))
+                break
+        else:
+            raise AssertionError(f"no smooth finite-difference step for parameter {which} {i}")
     assert _rel_err(g_gpu, g_fd) <= TRAJ_RTOL, (g_gpu, g_fd)
     # initial-state gradients for a few coordinates
     base_x = w.positions.astype(np.float64)
     base_v = w.velocities.astype(np.float64)
     fd_x, fd_v, gpu_x, gpu_v = [], [], [], []
     for a, c in [(0, 0), (5, 1), (300, 2), (900, 0)]:
-        h = 1e-5
-        pp, pm = base_x.copy(), base_x.copy()
-        pp[a, c] += h
-        pm[a, c] -= h
-        fd_x.append((loss64(params, pp, base_v) - loss64(params, pm, base_v)) / (2 * h))
+        def run_x(h, a=a, c=c):
+            q = base_x.copy()
+            q[a, c] += h
+            return loss_pairs(params, q, base_v)
+
+        def run_v(h, a=a, c=c):
+            q = base_v.copy()
+            q[a, c] += h
+            return loss_pairs(params, base_x, q)
+
+        hs = (1e-5, 2e-6, 4e-7, 1e-7)
+        fd_x.append(_smooth_central_difference(run_x, hs))
+        fd_v.append(_smooth_central_difference(run_v, hs))
         gpu_x.append(gx0[a, c])
-        vp, vm = base_v.copy(), base_v.copy()
-        vp[a, c] += h
-        vm[a, c] -= h
-        fd_v.append((loss64(params, base_x, vp) - loss64(params, base_x, vm)) / (2 * h))
         gpu_v.append(gv0[a, c])
     assert _rel_err(gpu_x, fd_x) <= TRAJ_RTOL, (gpu_x, fd_x)
     assert _rel_err(gpu_v, fd_v) <= TRAJ_RTOL, (gpu_v, fd_v)
