@@ -64,10 +64,12 @@ class FrameModel:
         self.masses = torch.as_tensor(workload.masses, dtype=torch.float32,
                                       device=self.device).reshape(-1, 1)
         self._params_key: bytes | None = None
+        self._params_ident: tuple | None = None
         self._loaded = 0   # token of the frame whose neighbour list the engine holds
         self._tokens = 0
         self.forward_calls = 0
         self.backward_calls = 0
+        self.reload_calls = 0
 
     def close(self) -> None:
         self.engine.close()
@@ -90,15 +92,21 @@ class FrameModel:
         return Params(t(w.lj_epsilon), t(w.lj_sigma), t(w.bond_k), t(w.bond_r0))
 
     def _set_params(self, eps, sigma, k, r0) -> None:
+        # Same tensors, unmodified since the last upload: nothing to do (no
+        # device-to-host copy).
+        ident = tuple((p.data_ptr(), p._version, str(p.device), tuple(p.shape))
+                      for p in (eps, sigma, k, r0))
+        if ident == self._params_ident:
+            return
         arrs = [np.ascontiguousarray(p.detach().to("cpu", torch.float32).numpy())
                 for p in (eps, sigma, k, r0)]
         key = b"".join(a.tobytes() for a in arrs)
-        if key == self._params_key:
-            return
-        self.engine.set_lj_params(arrs[0], arrs[1])
-        if self.n_bond_types:
-            self.engine.set_bond_params(arrs[2], arrs[3])
-        self._params_key = key
+        if key != self._params_key:
+            self.engine.set_lj_params(arrs[0], arrs[1])
+            if self.n_bond_types:
+                self.engine.set_bond_params(arrs[2], arrs[3])
+            self._params_key = key
+        self._params_ident = ident
 
     # ------------------------------------------------------------ raw calls
 
@@ -110,6 +118,15 @@ class FrameModel:
         self._loaded = self._tokens
         self.forward_calls += 1
         return f, e, self._tokens
+
+    def _load_raw(self, x: torch.Tensor, token: int) -> None:
+        """Make frame x (evaluated earlier under token) current for a backward."""
+        if self._loaded == token:
+            return
+        torch.cuda.current_stream(self.device).synchronize()
+        self.engine.frame_load(x.data_ptr(), reuse_list=True)
+        self._loaded = token
+        self.reload_calls += 1
 
     def _backward_raw(self, g_forces: torch.Tensor, g_energy: float):
         n_p = 2 * self.n_types * self.n_types + 2 * self.n_bond_types
@@ -177,9 +194,7 @@ class _FrameForces(torch.autograd.Function):
         model: FrameModel = ctx.model
         pos, eps, sigma, k, r0 = ctx.saved_tensors
         model._set_params(eps, sigma, k, r0)
-        if model._loaded != ctx.token:
-            model._forward_raw(pos, reuse_list=False)
-            model._loaded = ctx.token
+        model._load_raw(pos, ctx.token)
         gF = (torch.zeros_like(pos) if g_forces is None
               else g_forces.detach().to(model.device, torch.float32).contiguous())
         gE = 0.0 if g_energy is None else float(g_energy)
@@ -244,19 +259,18 @@ class _VerletTrajectory(torch.autograd.Function):
             end = min(start + every, steps)
             # Recompute positions x_start..x_end of this segment.
             x, v = ctx.checkpoints[seg]
-            xs = [x]
-            f, _, _ = model._forward_raw(x)
+            f, _, tok = model._forward_raw(x)
+            xs, toks = [x], [tok]
             for _ in range(start, end):
                 v = v + a * f
                 x = (x + dt * v).contiguous()
-                f, _, _ = model._forward_raw(x)
+                f, _, tok = model._forward_raw(x)
                 v = v + a * f
                 xs.append(x)
+                toks.append(tok)
             # Reverse over steps end-1 .. start (frames end .. start+1).
             for n in range(end, start, -1):
-                xn = xs[n - start]
-                if n != end:
-                    model._forward_raw(xn)
+                model._load_raw(xs[n - start], toks[n - start])
                 fbar = (carry + a * vbar).contiguous()
                 gx, gp = model._backward_raw(fbar, 0.0)
                 xbar = xbar + gx
@@ -264,7 +278,7 @@ class _VerletTrajectory(torch.autograd.Function):
                 vbar = vbar + dt * xbar
                 carry = a * vbar
             if seg == 0:
-                model._forward_raw(xs[0])
+                model._load_raw(xs[0], toks[0])
                 gx, gp = model._backward_raw(carry.contiguous(), 0.0)
                 xbar = xbar + gx
                 pbar = pbar + gp

@@ -101,6 +101,7 @@ SIGNATURES = {
     "beads_engine_set_bond_params": (C.c_int, [C.c_void_p, _f32p, _f32p]),
     "beads_engine_frame_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.POINTER(C.c_double), C.c_int32]),
+    "beads_engine_frame_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "beads_engine_frame_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                               C.c_void_p]),
     "beads_engine_frame_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
@@ -308,6 +309,10 @@ class Engine:
                                                   C.c_void_p(forces_ptr),
                                                   _ptr(e, C.c_double), 1 if reuse_list else 0))
         return e
+
+    def frame_load(self, positions_ptr: int, reuse_list: bool = True) -> None:
+        _check(self.lib.beads_engine_frame_load(self.handle, C.c_void_p(positions_ptr),
+                                                1 if reuse_list else 0))
 
     def frame_backward(self, grad_forces_ptr: int, grad_energy: float, grad_positions_ptr: int,
                        grad_params_ptr: int) -> None:

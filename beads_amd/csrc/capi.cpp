@@ -181,6 +181,14 @@ beads_status_t beads_engine_frame_forces(beads_engine_t* e, const float* positio
   });
 }
 
+beads_status_t beads_engine_frame_load(beads_engine_t* e, const float* positions,
+                                       int32_t reuse_list) {
+  return guarded([&] {
+    beads_amd::require(positions != nullptr, "frame positions are null", BEADS_E_TYPE);
+    engine_of(e).frame_load(positions, reuse_list != 0);
+  });
+}
+
 beads_status_t beads_engine_frame_backward(beads_engine_t* e, const float* grad_forces,
                                            double grad_energy, float* grad_positions,
                                            double* grad_params) {

@@ -426,6 +426,9 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
 }
 
 Engine::~Engine() {
+  for (auto& e : frame_events_) {
+    if (e != nullptr) (void)hipEventDestroy(e);
+  }
   if (stream_ != nullptr) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
@@ -957,18 +960,18 @@ void Engine::set_bond_params(const float* k, const float* r0) {
 // reordering (forced, or with reuse_list only when some atom moved more than
 // half the skin since the last build, the integrator's test), forces +
 // energies. Frame mode and dynamics are exclusive.
-void Engine::frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
-                          bool reuse_list) {
+void Engine::enter_frame_mode() {
   require(!initialized_, "frame mode is not available on an engine that runs dynamics.",
           BEADS_E_RUNTIME);
   if (!frame_mode_) {
+    for (auto& e : frame_events_) BEADS_HIP(hipEventCreate(&e));
     frame_mode_ = true;
     control_.path = 0;  // never reorder: slots stay the caller's order
   }
-  hipEvent_t t0, t1;
-  BEADS_HIP(hipEventCreate(&t0));
-  BEADS_HIP(hipEventCreate(&t1));
-  BEADS_HIP(hipEventRecord(t0, stream_));
+}
+
+void Engine::load_frame(const float* positions_dev, bool reuse_list) {
+  enter_frame_mode();
   const bool check = reuse_list && frame_loaded_ && plan_.cutoff_buffer > 0.f;
   FrameLoadParams lp{};
   lp.n = n_;
@@ -982,19 +985,37 @@ void Engine::frame_forces(const float* positions_dev, float* forces_dev, double*
   lp.ctrl = ctrl_.data();
   launch_frame_load(lp, check, stream_);
   rebuild_sequence(!check);
+}
+
+void Engine::frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
+                          bool reuse_list) {
+  enter_frame_mode();
+  frame_loaded_ = false;
+  BEADS_HIP(hipEventRecord(frame_events_[0], stream_));
+  load_frame(positions_dev, reuse_list);
   launch_force(force_params_, false, true, stream_);
   launch_frame_store(n_, force_.data(), forces_dev, stream_);
-  BEADS_HIP(hipEventRecord(t1, stream_));
+  BEADS_HIP(hipEventRecord(frame_events_[1], stream_));
   BEADS_HIP(hipGetLastError());
   const Ctrl c = fetch_ctrl();
-  BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
+  BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, frame_events_[0], frame_events_[1]));
   check_overflow(c);
   frame_loaded_ = true;
   if (energies_out != nullptr) {
     for (int q = 0; q < 4; ++q) energies_out[q] = c.pe[q];
   }
+}
+
+// Positions and neighbour list of a frame without evaluating forces: what
+// frame_backward needs when another frame was evaluated in between.
+void Engine::frame_load(const float* positions_dev, bool reuse_list) {
+  frame_loaded_ = false;
+  load_frame(positions_dev, reuse_list);
+  // The force pass normally clears the rebuild flags; no force pass runs here.
+  BEADS_HIP(hipMemsetAsync(ctrl_.data(), 0, 2 * sizeof(uint32_t), stream_));
+  BEADS_HIP(hipGetLastError());
+  check_overflow(fetch_ctrl());
+  frame_loaded_ = true;
 }
 
 // Adjoint of the last frame_forces call: grad_forces [n][3] binary32 (device),
@@ -1020,10 +1041,7 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   }
   const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
   grad_force4_.resize(n_);
-  hipEvent_t t0, t1;
-  BEADS_HIP(hipEventCreate(&t0));
-  BEADS_HIP(hipEventCreate(&t1));
-  BEADS_HIP(hipEventRecord(t0, stream_));
+  BEADS_HIP(hipEventRecord(frame_events_[2], stream_));
   BEADS_HIP(hipMemsetAsync(grad_params_dev, 0, n_params * sizeof(double), stream_));
   launch_pack_xyz(n_, grad_forces_dev, grad_force4_.data(), stream_);
   AdjointParams p{};
@@ -1049,12 +1067,10 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   p.grad_pos = grad_positions_dev;
   p.grad_params = grad_params_dev;
   launch_fm_backward(p, stream_);
-  BEADS_HIP(hipEventRecord(t1, stream_));
+  BEADS_HIP(hipEventRecord(frame_events_[3], stream_));
   BEADS_HIP(hipGetLastError());
-  BEADS_HIP(hipEventSynchronize(t1));
-  BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
+  BEADS_HIP(hipEventSynchronize(frame_events_[3]));
+  BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, frame_events_[2], frame_events_[3]));
 }
 
 void Engine::frame_timing(double* forward_s, double* backward_s) {

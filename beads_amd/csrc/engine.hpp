@@ -97,6 +97,7 @@ class Engine {
   void set_bond_params(const float* k, const float* r0);
   void frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
                     bool reuse_list);
+  void frame_load(const float* positions_dev, bool reuse_list);
   void frame_backward(const float* grad_forces_dev, double grad_energy,
                       float* grad_positions_dev, double* grad_params_dev);
   void frame_timing(double* forward_s, double* backward_s);
@@ -106,6 +107,8 @@ class Engine {
 
  private:
   void upload_parameters();
+  void enter_frame_mode();
+  void load_frame(const float* positions_dev, bool reuse_list);
   void upload_state();
   void set_up_params();
   void rebuild_sequence(bool forced);
@@ -168,6 +171,7 @@ class Engine {
   bool frame_mode_ = false;   // positions supplied per frame; never reordered
   bool frame_loaded_ = false;
   float last_forward_ms_ = 0.f, last_backward_ms_ = 0.f;
+  hipEvent_t frame_events_[4] = {nullptr, nullptr, nullptr, nullptr};
 
   // kernel parameter blocks
   ForceParams force_params_{};

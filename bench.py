@@ -190,6 +190,33 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
                            "backward_over_forward": t_traj_b / t_traj_f}}
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per force-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration), or None
+    when no summary covers this workload."""
+    path = ROOT / "profiles" / f"force_kernel_{workload}_hbm_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration(mib: int) -> None:
+    """A plain copy of known size: each launch reads and writes mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": n * 4, "copy_bytes_written_per_launch": n * 4,
+                      "launches": 5}))
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,21 +229,35 @@ def main() -> None:
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--adjoint-frames", type=int, default=64)
     ap.add_argument("--traj-steps", type=int, default=200)
+    ap.add_argument("--half-list", action="store_true",
+                    help="store each LJ pair once (reaction by FP64 atomics)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-phase-profile", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         raise SystemExit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from beads_amd.capi import Engine
 
     w, description = make_workload(args.workload)
+    if args.half_list:
+        w = w.with_(half_list=True)
+        description += ", half neighbour list"
     print(f"bench: {description}; building device state", file=sys.stderr, flush=True)
-    eng = Engine(w)
+    t_setup = time.perf_counter()
+    eng = Engine(w)  # host topology + host-to-device copies of the whole state
+    eng.synchronize()
+    setup_s = time.perf_counter() - t_setup
     eng.initialize(observables=False)
     eng.profile(args.warmup)
     eng.synchronize()
@@ -241,8 +282,10 @@ def main() -> None:
         per_atom = force_bytes_per_atom(w, k_mean)
         achieved = per_atom * w.n / force_avg / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.workload),
                     "kernel": "force_kernel", "bytes_per_atom": round(per_atom, 1),
+                    "algorithmic_bytes_per_launch": round(per_atom * w.n),
                     "mean_launch_us": round(force_avg * 1e6, 2)}
         phases = {"integrate_us": round(pt.integrate_s / psteps * 1e6, 2),
                   "rebuild_us": round(pt.rebuild_s / psteps * 1e6, 2),
@@ -277,7 +320,9 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": description, "n_beads": w.n, "steps": args.steps,
                    "neighbor_builds_in_timed_steps": builds,
-                   "mean_neighbors": round(s1.mean_neighbors, 2), "phases": phases},
+                   "mean_neighbors": round(s1.mean_neighbors, 2), "phases": phases,
+                   "setup_s": round(setup_s, 3),
+                   "host_inclusive_atom_steps_per_s": w.n * args.steps / (wall + setup_s)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

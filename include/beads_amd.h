@@ -234,7 +234,12 @@ beads_status_t beads_engine_set_bond_params(beads_engine_t* engine, const float*
  * angle, dihedral. */
 beads_status_t beads_engine_frame_forces(beads_engine_t* engine, const float* positions,
                                          float* forces, double* energies, int32_t reuse_list);
-/* Adjoint of the last frame_forces call for L with dL/dF = grad_forces and
+/* Load positions and update the neighbour list exactly as frame_forces does,
+ * without evaluating forces: re-establishes a frame for frame_backward after
+ * other frames were evaluated. */
+beads_status_t beads_engine_frame_load(beads_engine_t* engine, const float* positions,
+                                       int32_t reuse_list);
+/* Adjoint of the last frame_forces (or frame_load) call for L with dL/dF = grad_forces and
  * dL/dE = grad_energy (E = total potential energy). Writes dL/dx [n][3] and
  * FP64 parameter gradients (device) laid out as
  * [epsilon T*T | sigma T*T | bond k nb | bond r0 nb]. LJ pairs and harmonic

@@ -139,6 +139,11 @@ def test_native_spec_dict_layout():
                                 "sort_every_rebuild": 10, "max_neighbors": 512}
     assert spec["output"]["trajectory"]["fields"] == ["tag", "position", "force"]
     assert spec["output"]["log"] == {"echo": "screen"}
+    # The half-list option (no reference counterpart) only appears when chosen.
+    sim.set_neighbor(half_list=True)
+    assert sim._native_spec()["neighbor"]["half_list"] is True
+    with pytest.raises(TypeError):
+        sim.set_neighbor(half_list=1)
 
 
 def test_default_exclusion_policy():
