@@ -697,10 +697,14 @@ void Engine::rebuild_sequence(bool forced) {
   launch_build(build_params_, stream_);
 }
 
+void Engine::force_pass(bool kick, bool observables) {
+  launch_force(force_params_, kick, observables, stream_);
+}
+
 void Engine::initialize(bool observables) {
   require(!frame_mode_, "an engine used in frame mode cannot run dynamics.", BEADS_E_RUNTIME);
   rebuild_sequence(true);
-  launch_force(force_params_, false, observables, stream_);
+  force_pass(false, observables);
   BEADS_HIP(hipGetLastError());
   ke_from_scale_ = false;
   initialized_ = true;
@@ -717,7 +721,7 @@ void Engine::step_once(bool observables) {
   if (every || check) {
     rebuild_sequence(every);
   }
-  launch_force(force_params_, true, observables, stream_);
+  force_pass(true, observables);
   ke_from_scale_ = false;
   step_ = physical;
 }
@@ -884,7 +888,7 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
     BEADS_HIP(hipEventRecord(ev[4 * k + 1], stream_));
     if (every || check) rebuild_sequence(every);
     BEADS_HIP(hipEventRecord(ev[4 * k + 2], stream_));
-    launch_force(force_params_, true, false, stream_);
+    force_pass(true, false);
     BEADS_HIP(hipEventRecord(ev[4 * k + 3], stream_));
     step_ = physical;
   }
@@ -972,7 +976,10 @@ void Engine::enter_frame_mode() {
 
 void Engine::load_frame(const float* positions_dev, bool reuse_list) {
   enter_frame_mode();
-  const bool check = reuse_list && frame_loaded_ && plan_.cutoff_buffer > 0.f;
+  // The list may be kept only if one was built in frame mode and the last
+  // call completed (so its reference positions are the list's).
+  const bool check = reuse_list && frame_list_valid_ && plan_.cutoff_buffer > 0.f;
+  frame_list_valid_ = false;
   FrameLoadParams lp{};
   lp.n = n_;
   lp.xyz = positions_dev;
@@ -993,7 +1000,7 @@ void Engine::frame_forces(const float* positions_dev, float* forces_dev, double*
   frame_loaded_ = false;
   BEADS_HIP(hipEventRecord(frame_events_[0], stream_));
   load_frame(positions_dev, reuse_list);
-  launch_force(force_params_, false, true, stream_);
+  force_pass(false, true);
   launch_frame_store(n_, force_.data(), forces_dev, stream_);
   BEADS_HIP(hipEventRecord(frame_events_[1], stream_));
   BEADS_HIP(hipGetLastError());
@@ -1001,6 +1008,7 @@ void Engine::frame_forces(const float* positions_dev, float* forces_dev, double*
   BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, frame_events_[0], frame_events_[1]));
   check_overflow(c);
   frame_loaded_ = true;
+  frame_list_valid_ = true;
   if (energies_out != nullptr) {
     for (int q = 0; q < 4; ++q) energies_out[q] = c.pe[q];
   }
@@ -1016,6 +1024,7 @@ void Engine::frame_load(const float* positions_dev, bool reuse_list) {
   BEADS_HIP(hipGetLastError());
   check_overflow(fetch_ctrl());
   frame_loaded_ = true;
+  frame_list_valid_ = true;
 }
 
 // Adjoint of the last frame_forces call: grad_forces [n][3] binary32 (device),

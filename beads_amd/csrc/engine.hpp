@@ -112,6 +112,7 @@ class Engine {
   void upload_state();
   void set_up_params();
   void rebuild_sequence(bool forced);
+  void force_pass(bool kick, bool observables);
   void step_once(bool observables);
   void materialize_velocities();
   Ctrl fetch_ctrl();
@@ -157,6 +158,7 @@ class Engine {
   DeviceBuffer<float4> dih_coeff_;
   DeviceBuffer<double> partials_;
   DeviceBuffer<double> pair_acc_;  // half list: partner-side LJ sums [3 N]
+
   DeviceBuffer<Ctrl> ctrl_;
   PinnedBuffer<Ctrl> ctrl_host_;
   uint32_t bcap_ = 0, acap_ = 0, dcap_ = 0, ecap_ = 0;
@@ -170,6 +172,7 @@ class Engine {
   DeviceBuffer<float4> grad_force4_;
   bool frame_mode_ = false;   // positions supplied per frame; never reordered
   bool frame_loaded_ = false;
+  bool frame_list_valid_ = false;  // the stored list and its reference positions are current
   float last_forward_ms_ = 0.f, last_backward_ms_ = 0.f;
   hipEvent_t frame_events_[4] = {nullptr, nullptr, nullptr, nullptr};
 

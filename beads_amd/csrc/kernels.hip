@@ -459,111 +459,110 @@ __global__ __launch_bounds__(kBlock) void half_pair_kernel(ForceParams p) {
 // (velocity_verlet.cu:97-111) and the kinetic-energy reduction for the
 // Berendsen factor (berendsen_thermostat.cu:109-163), with the factor computed
 // in FP64 by the last workgroup.
-template <bool KICK, bool OBS, int TYPES, bool HALF>
-__global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
-  __shared__ float4 lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
-  __shared__ double red_lds[4];
-  if (!HALF) load_lj_table<TYPES>(p, lj_lds);
-  const float4 c0 = TYPES == 0 && !HALF ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const uint32_t n = p.n;
+// Per-thread sums of one force pass.
+struct StepSums {
   double ke = 0.0;
   double e_pair = 0.0, w_pair = 0.0;
   double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
-  if (i < n) {
-    const float4 xi4 = p.pos[i];
-    double fxd = 0.0, fyd = 0.0, fzd = 0.0;
-    if (HALF) {
-      double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
-      fxd = acc[0];
-      fyd = acc[1];
-      fzd = acc[2];
-      acc[0] = 0.0;
-      acc[1] = 0.0;
-      acc[2] = 0.0;
-    } else {
-      lj_list<OBS, TYPES, false>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, e_pair, w_pair);
-    }
-    float fx = static_cast<float>(fxd);
-    float fy = static_cast<float>(fyd);
-    float fz = static_cast<float>(fzd);
-    if (p.topo.has_listed) {
-      const TopologyView& tp = p.topo;
-      const uint32_t cw = tp.count[i];
-      const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
-      const float3 xi = f3(xi4);
-      for (uint32_t k = 0; k < nbond; ++k) {
-        const uint2 r = tp.bond[static_cast<size_t>(k) * n + i];
-        float e, w;
-        const float3 f = bond_force(mic3(xi, f3(p.pos[r.x]), p.box),
-                                    tp.bond_coeff[r.y & 0x7fffffffu], e, w);
-        fx += f.x;
-        fy += f.y;
-        fz += f.z;
-        if (OBS && (r.y >> 31) != 0) {
-          e_bond += e;
-          w_bond += w;
-        }
-      }
-      for (uint32_t k = 0; k < nang; ++k) {
-        const uint4 r = tp.angle[static_cast<size_t>(k) * n + i];
-        const int role = static_cast<int>(r.w);
-        const float3 a = f3(p.pos[r.x]);
-        const float3 b = f3(p.pos[r.y]);
-        const float3 pi = role == 0 ? xi : a;
-        const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
-        const float3 pk = role == 2 ? xi : b;
-        float e, w;
-        const float3 f = angle_role_force(pi, pj, pk, tp.angle_coeff[r.z], role, p.box, e, w);
-        fx += f.x;
-        fy += f.y;
-        fz += f.z;
-        if (OBS && role == 0) {
-          e_angle += e;
-          w_angle += w;
-        }
-      }
-      for (uint32_t k = 0; k < ndih; ++k) {
-        const uint4 r = tp.dih[static_cast<size_t>(k) * n + i];
-        const int role = static_cast<int>(r.w >> 30);
-        const float3 a = f3(p.pos[r.x]);
-        const float3 b = f3(p.pos[r.y]);
-        const float3 c = f3(p.pos[r.z]);
-        float3 q0, q1, q2, q3;
-        if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
-        else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
-        else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
-        else { q0 = a; q1 = b; q2 = c; q3 = xi; }
-        float e, w;
-        const float3 f = dihedral_role_force(q0, q1, q2, q3, tp.dih_coeff[r.w & 0x3fffffffu],
-                                             role, p.box, e, w);
-        fx += f.x;
-        fy += f.y;
-        fz += f.z;
-        if (OBS && role == 0) {
-          e_dih += e;
-          w_dih += w;
-        }
+};
+
+// Everything after the pair sum for atom i: listed terms gathered per atom
+// (each atom evaluates every term it belongs to and keeps its own role's
+// force: no atomics, deterministic), the force store, optionally the
+// post-force half-kick (velocity_verlet.cu:97-111), and the atom's kinetic
+// energy.
+template <bool KICK, bool OBS>
+__device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, float4 xi4,
+                                            double fxd, double fyd, double fzd, StepSums& sums) {
+  const uint32_t n = p.n;
+  float fx = static_cast<float>(fxd);
+  float fy = static_cast<float>(fyd);
+  float fz = static_cast<float>(fzd);
+  if (p.topo.has_listed) {
+    const TopologyView& tp = p.topo;
+    const uint32_t cw = tp.count[i];
+    const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
+    const float3 xi = f3(xi4);
+    for (uint32_t k = 0; k < nbond; ++k) {
+      const uint2 r = tp.bond[static_cast<size_t>(k) * n + i];
+      float e, w;
+      const float3 f = bond_force(mic3(xi, f3(p.pos[r.x]), p.box),
+                                  tp.bond_coeff[r.y & 0x7fffffffu], e, w);
+      fx += f.x;
+      fy += f.y;
+      fz += f.z;
+      if (OBS && (r.y >> 31) != 0) {
+        sums.e_bond += e;
+        sums.w_bond += w;
       }
     }
-    p.force[i] = make_float4(fx, fy, fz, 0.f);
-    float4 v = p.vel[i];
-    if (KICK) {
-      const float h = 0.5f * p.dt / v.w;
-      v.x += h * fx;
-      v.y += h * fy;
-      v.z += h * fz;
-      p.vel[i] = v;
+    for (uint32_t k = 0; k < nang; ++k) {
+      const uint4 r = tp.angle[static_cast<size_t>(k) * n + i];
+      const int role = static_cast<int>(r.w);
+      const float3 a = f3(p.pos[r.x]);
+      const float3 b = f3(p.pos[r.y]);
+      const float3 pi = role == 0 ? xi : a;
+      const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
+      const float3 pk = role == 2 ? xi : b;
+      float e, w;
+      const float3 f = angle_role_force(pi, pj, pk, tp.angle_coeff[r.z], role, p.box, e, w);
+      fx += f.x;
+      fy += f.y;
+      fz += f.z;
+      if (OBS && role == 0) {
+        sums.e_angle += e;
+        sums.w_angle += w;
+      }
     }
-    const double vx = v.x, vy = v.y, vz = v.z;
-    ke = 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
+    for (uint32_t k = 0; k < ndih; ++k) {
+      const uint4 r = tp.dih[static_cast<size_t>(k) * n + i];
+      const int role = static_cast<int>(r.w >> 30);
+      const float3 a = f3(p.pos[r.x]);
+      const float3 b = f3(p.pos[r.y]);
+      const float3 c = f3(p.pos[r.z]);
+      float3 q0, q1, q2, q3;
+      if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
+      else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
+      else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
+      else { q0 = a; q1 = b; q2 = c; q3 = xi; }
+      float e, w;
+      const float3 f = dihedral_role_force(q0, q1, q2, q3, tp.dih_coeff[r.w & 0x3fffffffu],
+                                           role, p.box, e, w);
+      fx += f.x;
+      fy += f.y;
+      fz += f.z;
+      if (OBS && role == 0) {
+        sums.e_dih += e;
+        sums.w_dih += w;
+      }
+    }
   }
+  p.force[i] = make_float4(fx, fy, fz, 0.f);
+  float4 v = p.vel[i];
+  if (KICK) {
+    const float h = 0.5f * p.dt / v.w;
+    v.x += h * fx;
+    v.y += h * fy;
+    v.z += h * fz;
+    p.vel[i] = v;
+  }
+  const double vx = v.x, vy = v.y, vz = v.z;
+  sums.ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
+}
+
+// Workgroup sums into partials, then the last workgroup reduces them in FP64,
+// computes the Berendsen factor (berendsen_thermostat.cu:109-163) and clears
+// the per-step flags.
+template <bool KICK, bool OBS, bool HALF>
+__device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums& sums,
+                                            double* red_lds) {
   const uint32_t nbk = gridDim.x;
   double* part = p.partials;
-  double s = block_sum<kBlock>(ke, red_lds);
+  double s = block_sum<kBlock>(sums.ke, red_lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
   if (OBS) {
-    const double vals[8] = {0.5 * e_pair, e_bond, e_angle, e_dih, 0.5 * w_pair, w_bond, w_angle, w_dih};
+    const double vals[8] = {0.5 * sums.e_pair, sums.e_bond, sums.e_angle, sums.e_dih,
+                            0.5 * sums.w_pair, sums.w_bond, sums.w_angle, sums.w_dih};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       s = block_sum<kBlock>(vals[q], red_lds);
@@ -608,6 +607,33 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
       p.ctrl->half_pair[1] = 0.0;
     }
   }
+}
+
+template <bool KICK, bool OBS, int TYPES, bool HALF>
+__global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
+  __shared__ float4 lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
+  __shared__ double red_lds[4];
+  if (!HALF) load_lj_table<TYPES>(p, lj_lds);
+  const float4 c0 = TYPES == 0 && !HALF ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  StepSums sums;
+  if (i < p.n) {
+    const float4 xi4 = p.pos[i];
+    double fxd = 0.0, fyd = 0.0, fzd = 0.0;
+    if (HALF) {
+      double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
+      fxd = acc[0];
+      fyd = acc[1];
+      fzd = acc[2];
+      acc[0] = 0.0;
+      acc[1] = 0.0;
+      acc[2] = 0.0;
+    } else {
+      lj_list<OBS, TYPES, false>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
+    }
+    finish_atom<KICK, OBS>(p, i, xi4, fxd, fyd, fzd, sums);
+  }
+  finish_step<KICK, OBS, HALF>(p, sums, red_lds);
 }
 
 // Apply the pending Berendsen factor and reduce the kinetic energy of the
@@ -660,6 +686,9 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
   const float4 f = p.force[i];
   float4 x = p.pos[i];
   if (THERMO) {
+    // Rounded on its own, as the reference's separate rescale kernel and
+    // scale_kernel (output steps) round it: never fused into the kick.
+#pragma clang fp contract(off)
     const float lam = static_cast<float>(p.ctrl->lambda);
     v.x *= lam;
     v.y *= lam;
@@ -707,6 +736,7 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
     }
   }
 }
+
 
 }  // namespace
 

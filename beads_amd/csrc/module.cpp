@@ -332,8 +332,8 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
   out.c.dihedral_n = out.dih_n.data();
 
   // native_spec_parser.cpp:730-753 default exclusion policy.
-  const py::object dist = ff.contains("bonded_exclusion_distance") ? ff["bonded_exclusion_distance"]
-                                                                   : py::none();
+  py::object dist = py::none();
+  if (ff.contains("bonded_exclusion_distance")) dist = ff["bonded_exclusion_distance"];
   std::string policy = ff.contains("bonded_exclusion_policy")
                            ? require_str(ff["bonded_exclusion_policy"], "bonded_exclusion_policy")
                            : "auto";

@@ -238,3 +238,23 @@ def test_trajectory_reader_and_lammps_dump(tmp_path):
     lines = out.read_text().splitlines()
     assert lines[0] == "ITEM: TIMESTEP" and lines[8].startswith("ITEM: ATOMS id x y z ix iy iz")
     assert lines[9].split()[0] == "1"  # sorted by tag
+
+
+def test_native_parser_accepts_a_full_spec(tmp_path):
+    """Every field the builders emit parses in _native (explicit exclusion
+    distance, thermostat, all outputs); without a device the run itself stops
+    at the first HIP call, with RuntimeError, never a parse error."""
+    w = W.polymer_melt(n_chains=40, beads_per_chain=50, seed=3).with_(steps=5)
+    sim = w.to_simulation()
+    sim.set_thermo(every=1, prefix=str(tmp_path / "run"))
+    sim.set_trajectory(every=5, prefix=str(tmp_path / "run"),
+                       fields=["tag", "type", "position", "image", "velocity", "force"])
+    sim.save_final_state(prefix=str(tmp_path / "run"))
+    sim.set_logging(echo="none")
+    sim.set_neighbor(half_list=True)
+    from beads_amd import _native
+
+    try:
+        _native.execute_simulation(sim._native_spec())
+    except RuntimeError as exc:
+        assert "HIP" in str(exc) or "device" in str(exc), exc

@@ -1,0 +1,127 @@
+"""The drop-in boundary end to end on the GPU: ``Simulation.execute()`` through
+``_native.execute_simulation`` writes the thermo CSV, the BEADS trajectory and
+the state snapshot, and every file agrees with the engine handle run on the
+same workload (same kernels, same step sequence) and with the FP64 oracle.
+
+Bars: thermo values are the reference's binary32 values written with 9
+significant digits, so they must equal the engine's FP64 statistics to the
+binary32 rounding (2e-7 relative); the step-0 trajectory frame is a binary32
+copy of the device state and must equal it bit for bit, later frames and the
+snapshot within a few binary32 ulps (see below); step-0 forces meet the 1e-5
+force bar against the oracle.
+"""
+
+from __future__ import annotations
+
+import csv
+import ctypes
+
+import numpy as np
+import pytest
+
+import beads_amd as beads
+import oracle as O
+from beads_amd import capi
+from beads_amd.io import read_beads_trajectory
+from gpu_cases import cases
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5
+
+
+def _force_error(f_gpu, f_ref):
+    scale = np.sqrt(np.mean(f_ref ** 2))
+    return float(np.max(np.abs(f_gpu - f_ref) / np.maximum(np.abs(f_ref), scale)))
+
+
+def _rows(path):
+    with open(path, newline="") as fh:
+        return list(csv.DictReader(fh))
+
+
+def test_execute_writes_thermo_trajectory_and_snapshot(tmp_path):
+    w = cases()["melt"].with_(steps=50)
+    sim = w.to_simulation()
+    sim.set_thermo(every=10, prefix=str(tmp_path / "run"))
+    sim.set_trajectory(every=25, prefix=str(tmp_path / "run"),
+                       fields=["tag", "type", "position", "image", "velocity", "force"])
+    sim.save_final_state(prefix=str(tmp_path / "run"))
+    sim.set_logging(echo="log", prefix=str(tmp_path / "run"))
+    sim.execute()
+
+    thermo = _rows(tmp_path / "run.thermo.csv")
+    assert [int(r["step"]) for r in thermo] == [0, 10, 20, 30, 40, 50]
+    assert list(thermo[0]) == ["step", "pe", "pair_pe", "bond_pe", "angle_pe", "dihedral_pe",
+                               "ke", "temp", "press"]
+
+    # The same run through the engine handle.
+    with capi.Engine(w) as e:
+        e.initialize(observables=True)
+        s0 = e.stats()
+        st0 = e.state()
+        e.advance(50, observables_on_last=True)
+        s50 = e.stats()
+        st50 = e.state()
+    # Thermo rows come after the step's Berendsen rescale (simulation_run.cpp:
+    # 212-235); the handle's stats report the stored velocities with the
+    # factor still pending, so step 50 scales the kinetic energy by lambda^2.
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    volume = float(np.prod(L))
+    for row, s, lam in ((thermo[0], s0, 1.0), (thermo[-1], s50, s50.lambda_)):
+        assert float(row["pair_pe"]) == pytest.approx(s.pe[0], rel=2e-7)
+        assert float(row["bond_pe"]) == pytest.approx(s.pe[1], rel=2e-7)
+        assert float(row["pe"]) == pytest.approx(sum(s.pe), rel=2e-7)
+        ke = s.ke * lam * lam
+        assert float(row["ke"]) == pytest.approx(ke, rel=1e-6)
+        assert float(row["temp"]) == pytest.approx(s.temperature * lam * lam, rel=1e-6)
+        press = 16.605390666 * (2.0 * ke + sum(s.virial)) / (3.0 * volume)
+        assert float(row["press"]) == pytest.approx(press, rel=1e-5, abs=1e-4)
+
+    traj = read_beads_trajectory(tmp_path / "run.trajectory.beadsbin")
+    assert traj["steps"].tolist() == [0, 25, 50]
+    rec0, rec2 = traj["records"][0], traj["records"][-1]
+    o0, oe = np.argsort(rec0["tag"]), np.argsort(st0["tags"])
+    np.testing.assert_array_equal(rec0["position"][o0], st0["positions"][oe])
+    np.testing.assert_array_equal(rec0["force"][o0], st0["forces"][oe])
+    f_ref, _ = O.forces(w, positions=st0["positions"], tags=st0["tags"], types=st0["types"])
+    assert _force_error(st0["forces"].astype(np.float64), f_ref) <= FORCE_RTOL
+    # Output steps run the force kernel with observables (energy terms kept),
+    # which may round a force by an ulp differently from the variant without
+    # them, exactly as the reference's two kernel variants may; later frames
+    # are therefore held to a few binary32 ulps, not bit for bit.
+    o2, oe2 = np.argsort(rec2["tag"]), np.argsort(st50["tags"])
+    np.testing.assert_allclose(rec2["position"][o2], st50["positions"][oe2], rtol=0, atol=1e-5)
+    np.testing.assert_array_equal(rec2["image"][o2], st50["images"][oe2])
+
+    snap = beads.System.from_state(tmp_path / "run.state.beadsbin")
+    tags = np.asarray(snap.tags)
+    os_, oe3 = np.argsort(tags), np.argsort(st50["tags"])
+    np.testing.assert_array_equal(np.asarray(snap.positions)[os_], rec2["position"][o2])
+    np.testing.assert_allclose(np.asarray(snap.positions)[os_], st50["positions"][oe3],
+                               rtol=0, atol=1e-5)
+    # The snapshot carries the rescaled velocities (the factor is applied at
+    # the end of the step); the handle still holds it pending.
+    vel = st50["velocities"][oe3].astype(np.float64) * s50.lambda_
+    np.testing.assert_allclose(np.asarray(snap.velocities)[os_], vel, rtol=1e-5,
+                               atol=1e-5 * float(np.abs(vel).max()))
+    np.testing.assert_array_equal(np.asarray(snap.images)[os_], st50["images"][oe3])
+    log = (tmp_path / "run.log").read_text()
+    assert "[BEADS]" in log
+
+
+def test_interrupt_callback_stops_the_run():
+    w = cases()["lj_regular"].with_(steps=100000)
+    spec, keep = capi.make_spec(w)
+    calls = []
+
+    @ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+    def stop(_ctx):
+        calls.append(1)
+        return 1 if len(calls) >= 2 else 0
+
+    spec.interrupt = ctypes.cast(stop, ctypes.c_void_p).value
+    lib = capi.library()
+    status = lib.beads_execute_simulation(ctypes.byref(spec))
+    assert status == 6  # BEADS_E_INTERRUPTED
+    assert len(calls) == 2
