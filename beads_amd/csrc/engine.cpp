@@ -404,6 +404,12 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
   thermostat_ = spec.berendsen != 0;
   half_list_ = spec.half_list != 0;
   {
+    // Diagnostics switch: BEADS_AMD_GRAPHS=0 launches every step's kernels
+    // one by one instead of replaying captured step graphs.
+    const char* env = std::getenv("BEADS_AMD_GRAPHS");
+    use_graphs_ = !(env != nullptr && env[0] == '0');
+  }
+  {
     // Diagnostics switch: BEADS_AMD_PRUNE=0 keeps every force pass on the
     // stored list.
     const char* env = std::getenv("BEADS_AMD_PRUNE");
@@ -433,6 +439,10 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
 }
 
 Engine::~Engine() {
+  if (step_graph_exec_ != nullptr) {
+    if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+    (void)hipGraphExecDestroy(step_graph_exec_);
+  }
   for (auto& e : frame_events_) {
     if (e != nullptr) (void)hipEventDestroy(e);
   }
@@ -752,10 +762,47 @@ void Engine::step_once(bool observables) {
   step_ = physical;
 }
 
+// Steps per captured graph: whole check periods, at least 16 steps, so a
+// replay always starts on a step whose check pattern is the captured one.
+uint64_t Engine::graph_steps() const {
+  const uint64_t period = plan_.cutoff_buffer == 0.f ? 1 : plan_.check_every;
+  return period * ((16 + period - 1) / period);
+}
+
+// One hipGraph of graph_steps() consecutive steps without observables,
+// captured from the same launch sequence step_once issues (no memsets, no
+// host synchronisation inside), instantiated once and replayed.
+void Engine::replay_step_graph() {
+  const uint64_t g = graph_steps();
+  if (step_graph_exec_ == nullptr) {
+    const uint64_t saved = step_;
+    BEADS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (uint64_t k = 0; k < g; ++k) step_once(false);
+    hipGraph_t graph = nullptr;
+    BEADS_HIP(hipStreamEndCapture(stream_, &graph));
+    BEADS_HIP(hipGraphInstantiate(&step_graph_exec_, graph, nullptr, nullptr, 0));
+    BEADS_HIP(hipGraphDestroy(graph));
+    step_ = saved;  // capture recorded the launches without running them
+  }
+  BEADS_HIP(hipGraphLaunch(step_graph_exec_, stream_));
+  step_ += g;
+  ke_from_scale_ = false;
+}
+
 void Engine::advance(uint64_t n_steps, bool observables_on_last) {
   require(initialized_, "engine must be initialized before advancing.", BEADS_E_RUNTIME);
-  for (uint64_t k = 0; k < n_steps; ++k) {
-    step_once(observables_on_last && k + 1 == n_steps);
+  const uint64_t g = graph_steps();
+  uint64_t left = n_steps;
+  while (left > 0) {
+    // Whole graphs while they fit and leave the observable step to step_once.
+    const uint64_t reserve = observables_on_last ? 1 : 0;
+    if (use_graphs_ && step_ % g == 0 && left >= g + reserve) {
+      replay_step_graph();
+      left -= g;
+      continue;
+    }
+    step_once(observables_on_last && left == 1);
+    --left;
   }
   BEADS_HIP(hipGetLastError());
 }
@@ -902,11 +949,8 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
     for (auto& e : ev) BEADS_HIP(hipEventCreate(&e));
   }
   BEADS_HIP(hipEventRecord(t0, stream_));
-  for (uint64_t k = 0; k < n_steps; ++k) {
-    if (!per_phase) {
-      step_once(false);
-      continue;
-    }
+  if (!per_phase) advance(n_steps, false);
+  for (uint64_t k = 0; k < n_steps && per_phase; ++k) {
     const uint64_t physical = step_ + 1;
     const bool every = plan_.cutoff_buffer == 0.f;
     const bool check = !every && physical % plan_.check_every == 0;
@@ -1359,10 +1403,35 @@ void Engine::run() {
   if (output_due(0)) {
     stage_outputs(0);
   }
-  for (uint64_t s = 0; s < runsteps; ++s) {
+  const uint64_t g = graph_steps();
+  // A multiple of `every` in (from, to]?
+  auto hits = [](uint64_t every, uint64_t from, uint64_t to) {
+    return every > 0 && to / every > from / every;
+  };
+  for (uint64_t s = 0; s < runsteps;) {
+    // Stretches of g steps with no thermo row (which needs observables) and
+    // no trajectory frame before their last step replay a captured graph;
+    // the interrupt callback then runs once per stretch.
+    const bool thermo_inside =
+        !thermo_path_.empty() && hits(spec_.thermo_every, step_, step_ + g);
+    const bool traj_inside =
+        !trajectory_path_.empty() && hits(spec_.trajectory_every, step_, step_ + g - 1);
+    if (use_graphs_ && step_ % g == 0 && s + g <= runsteps && !thermo_inside && !traj_inside) {
+      poll_interrupt();
+      const uint64_t before = step_;
+      replay_step_graph();
+      s += g;
+      if (output_due(step_)) {
+        stage_outputs(step_);
+      } else if (hits(kOverflowPollSteps, before, step_)) {
+        check_overflow(fetch_ctrl());
+      }
+      continue;
+    }
     if (s % interrupt_interval_ == 0) {
       poll_interrupt();
     }
+    ++s;
     const uint64_t physical = step_ + 1;
     const bool thermo_due = !thermo_path_.empty() && physical % spec_.thermo_every == 0;
     step_once(thermo_due);

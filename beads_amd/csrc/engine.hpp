@@ -114,6 +114,8 @@ class Engine {
   void rebuild_sequence(bool forced);
   void force_pass(bool kick, bool observables);
   void step_once(bool observables);
+  uint64_t graph_steps() const;
+  void replay_step_graph();
   void materialize_velocities();
   Ctrl fetch_ctrl();
   void check_overflow(const Ctrl& c);
@@ -136,6 +138,8 @@ class Engine {
   bool thermostat_ = false;
   bool half_list_ = false;
   bool prune_ = false;  // force passes read the pruned inner list
+  bool use_graphs_ = true;
+  hipGraphExec_t step_graph_exec_ = nullptr;
   bool has_bond_style_ = false, has_angle_style_ = false, has_dihedral_style_ = false;
   uint64_t step_ = 0;
   bool initialized_ = false;

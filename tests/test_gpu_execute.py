@@ -125,3 +125,24 @@ def test_interrupt_callback_stops_the_run():
     status = lib.beads_execute_simulation(ctypes.byref(spec))
     assert status == 6  # BEADS_E_INTERRUPTED
     assert len(calls) == 2
+
+
+def test_execute_with_step_graphs_writes_the_same_files(tmp_path, monkeypatch):
+    """The run loop replays captured step graphs between output steps; the
+    files must be byte-identical to a run that launches every step eagerly."""
+    w = cases()["lj_regular"].with_(steps=100)
+    blobs = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("BEADS_AMD_GRAPHS", graphs)
+        sim = w.to_simulation()
+        prefix = str(tmp_path / f"g{graphs}")
+        sim.set_thermo(every=32, prefix=prefix)
+        sim.set_trajectory(every=48, prefix=prefix, fields=["tag", "position", "velocity"])
+        sim.save_final_state(prefix=prefix)
+        sim.set_logging(echo="none")
+        sim.execute()
+        blobs[graphs] = [(tmp_path / f"g{graphs}.{ext}").read_bytes()
+                         for ext in ("thermo.csv", "trajectory.beadsbin", "state.beadsbin")]
+    rows = blobs["1"][0].decode().splitlines()
+    assert [r.split(",")[0] for r in rows[1:]] == ["0", "32", "64", "96"]
+    assert blobs["1"] == blobs["0"]
