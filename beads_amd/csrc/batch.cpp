@@ -87,7 +87,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   init_img_.resize(atoms);
   init_tag_.resize(atoms);
   init_cell_atoms_.resize(atoms);
-  std::vector<float4> lj_all;
+  std::vector<LjCoeff> lj_all;
   lj_all.reserve(lj);
   for (uint32_t k = 0; k < count; ++k) {
     const beads_spec_t& s = specs[order_[k]];
@@ -104,7 +104,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
       init_tag_[a0 + i] = s.tags[i];
       init_cell_atoms_[a0 + i] = i;
     }
-    const std::vector<float4> t = pack_lj_table(s);
+    const std::vector<LjCoeff> t = pack_lj_table(s);
     lj_all.insert(lj_all.end(), t.begin(), t.end());
   }
 

@@ -50,7 +50,8 @@ class Batch {
 
   DeviceBuffer<ReplicaDesc> desc_;
   DeviceBuffer<ReplicaState> state_;
-  DeviceBuffer<float4> lj_, pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
+  DeviceBuffer<LjCoeff> lj_;
+  DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
   DeviceBuffer<int4> image_, ref_image_, s_image_;
   DeviceBuffer<uint32_t> tag_, s_tag_, cell_of_, s_cell_of_, cell_atoms_, nbr_count_, nbr_;
   DeviceBuffer<uint32_t> cell_count_, cell_start_, cell_cursor_;

@@ -31,6 +31,14 @@ inline uint32_t nbr_padded_capacity(uint32_t max_neighbors) {
   return (max_neighbors + kNbrChunk - 1) / kNbrChunk * kNbrChunk;
 }
 
+// LJ coefficients of one ordered type pair, formed in FP64 from the binary32
+// epsilon, sigma and cutoff (pair_lj.cu:171-192 packs the same quantities in
+// binary32): sigma^2, 24 eps, 4 eps and the energy shift
+// 4 eps ((sigma/rc)^12 - (sigma/rc)^6) (0 when unshifted).
+struct LjCoeff {
+  double sigma2, eps24, eps4, shift;
+};
+
 struct BoxView {
   float lo[3];
   float L[3];
@@ -97,7 +105,7 @@ struct ForceParams {
   BoxView box;
   float rc2;
   int32_t n_types;
-  const float4* lj;  // [T*T] sigma^2, 24 eps, 4 eps, shift
+  const LjCoeff* lj;  // [T*T]
   TopologyView topo;
   float dt;
   int32_t berendsen;

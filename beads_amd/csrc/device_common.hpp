@@ -161,18 +161,18 @@ __device__ __forceinline__ CellNeighborhood neighborhood(uint32_t c, const CellG
 // the non-packed FP32 rate on CDNA4; binary32 would amplify the rounding of r^2
 // six-fold through (sigma^2/r^2)^6). Returns the force scale s with
 // f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift.
-__device__ __forceinline__ double lj_pair(float dx, float dy, float dz, float4 c, double& rr,
-                                          double& energy) {
+__device__ __forceinline__ double lj_pair(float dx, float dy, float dz, const LjCoeff& c,
+                                          double& rr, double& energy) {
   const double ddx = dx, ddy = dy, ddz = dz;
   rr = ddx * ddx + ddy * ddy + ddz * ddz;
   double inv = __builtin_amdgcn_rcp(rr);
   inv = inv * (2.0 - rr * inv);
   inv = inv * (2.0 - rr * inv);
-  const double sr2 = static_cast<double>(c.x) * inv;
+  const double sr2 = c.sigma2 * inv;
   const double sr6 = sr2 * sr2 * sr2;
   const double sr12 = sr6 * sr6;
-  energy = static_cast<double>(c.z) * (sr12 - sr6) - static_cast<double>(c.w);
-  return static_cast<double>(c.y) * inv * (2.0 * sr12 - sr6);
+  energy = c.eps4 * (sr12 - sr6) - c.shift;
+  return c.eps24 * inv * (2.0 * sr12 - sr6);
 }
 
 // ------------------------------------------------------- listed terms ------
@@ -198,56 +198,81 @@ __device__ __forceinline__ float3 mic3(float3 p, float3 q, const BoxView& b) {
                      mic_diff(p.z, q.z, b.L[2], b.hL[2]));
 }
 
+// FP64 vectors for the listed terms. Their displacements come exact (or
+// rounded once) from the binary32 positions through mic3; everything after
+// that is FP64, because near-collinear triples make the angle and dihedral
+// algebra ill-conditioned in binary32 (|v0 x v1| ~ 1e-5 |v0||v1| occurs in a
+// million-bead melt and costs binary32 three significant digits there).
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 dbl(float3 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float3 flt(d3 v) {
+  return make_float3(static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z));
+}
+__device__ __forceinline__ d3 dsub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 dadd(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 dscale(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
 // Harmonic bond U = k/2 (r - r0)^2 (bond_harmonic.cu:41-79) on the displacement
 // d = x_self - x_partner; returns the force on self and the term's energy and
 // virial d.f.
-__device__ __forceinline__ float3 bond_force(float3 d, float2 kc, float& energy, float& virial) {
-  const float r2 = dot3(d, d);
-  const float rr = sqrtf(r2);
-  const float stretch = rr - kc.y;
-  energy = 0.5f * kc.x * stretch * stretch;
-  if (rr == 0.f) {
-    virial = 0.f;
+__device__ __forceinline__ float3 bond_force(float3 df, float2 kc, double& energy,
+                                             double& virial) {
+  const d3 d = dbl(df);
+  const double k = kc.x, r0 = kc.y;
+  const double r2 = ddot(d, d);
+  const double rr = sqrt(r2);
+  const double stretch = rr - r0;
+  energy = 0.5 * k * stretch * stretch;
+  if (rr == 0.0) {
+    virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const float s = -kc.x * stretch / rr;
+  const double s = -k * stretch / rr;
   virial = s * r2;
-  return scale3(d, s);
+  return flt(dscale(d, s));
 }
 
 // Harmonic angle U = k/2 (theta - theta0)^2 (angle_harmonic.cu:61-158) for one
-// role. Same mathematics as the reference, evaluated in a binary32-stable form:
-// theta = atan2(|a x b|, a.b) and sin(theta) = |a x b| / (|a||b|) instead of
-// acos(c) and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
+// role. Same mathematics as the reference, in a stable form: theta =
+// atan2(|a x b|, a.b) and sin(theta) = |a x b| / (|a||b|) instead of acos(c)
+// and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
 // instead of differences of nearly parallel vectors. The reference's floor
 // (sin theta replaced by 0.001 at or below 0.001) is kept.
 __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 xk, float2 kc,
-                                                   int role, const BoxView& box, float& energy,
-                                                   float& virial) {
-  const float3 a = mic3(xi, xj, box);
-  const float3 b = mic3(xk, xj, box);
-  const float ra2 = dot3(a, a), rb2 = dot3(b, b);
-  if (ra2 == 0.f || rb2 == 0.f) {
-    const float d = 0.f - kc.y;
-    energy = 0.5f * kc.x * d * d;
-    virial = 0.f;
+                                                   int role, const BoxView& box, double& energy,
+                                                   double& virial) {
+  const d3 a = dbl(mic3(xi, xj, box));
+  const d3 b = dbl(mic3(xk, xj, box));
+  const double k = kc.x, theta0 = kc.y;
+  const double ra2 = ddot(a, a), rb2 = ddot(b, b);
+  if (ra2 == 0.0 || rb2 == 0.0) {
+    energy = 0.5 * k * theta0 * theta0;
+    virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const float ra = sqrtf(ra2), rb = sqrtf(rb2);
-  const float3 nrm = cross3(a, b);
-  const float nn = sqrtf(dot3(nrm, nrm));
-  const float theta = atan2f(nn, dot3(a, b));
-  const float dth = theta - kc.y;
-  energy = 0.5f * kc.x * dth * dth;
-  const float sn_true = nn / (ra * rb);
-  const float sn = sn_true <= 0.001f ? 0.001f : sn_true;
-  const float A = -kc.x * dth / sn;
-  const float3 fi = scale3(cross3(a, nrm), A / (ra2 * ra * rb));
-  const float3 fk = scale3(cross3(b, nrm), -A / (rb2 * rb * ra));
-  virial = dot3(a, fi) + dot3(b, fk);
-  if (role == 0) return fi;
-  if (role == 2) return fk;
-  return scale3(add3(fi, fk), -1.f);
+  const double ra = sqrt(ra2), rb = sqrt(rb2);
+  const d3 nrm = dcross(a, b);
+  const double nn = sqrt(ddot(nrm, nrm));
+  // The FP64 arguments are well conditioned; a binary32 atan2 of them is
+  // accurate to a few 1e-8 relative and keeps the pass's register budget.
+  const double theta = atan2f(static_cast<float>(nn), static_cast<float>(ddot(a, b)));
+  const double dth = theta - theta0;
+  energy = 0.5 * k * dth * dth;
+  const double sn_true = nn / (ra * rb);
+  const double sn = sn_true <= 0.001 ? 0.001 : sn_true;
+  const double A = -k * dth / sn;
+  const d3 fi = dscale(dcross(a, nrm), A / (ra2 * ra * rb));
+  const d3 fk = dscale(dcross(b, nrm), -A / (rb2 * rb * ra));
+  virial = ddot(a, fi) + ddot(b, fk);
+  if (role == 0) return flt(fi);
+  if (role == 2) return flt(fk);
+  return flt(dscale(dadd(fi, fk), -1.0));
 }
 
 // Harmonic dihedral k[1 + d cos(n phi)] (dihedral_harmonic.cu:83-206), one role.
@@ -255,36 +280,45 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
 // (negative when v0.c1 < 0) without the ill-conditioned acos near 0 and pi.
 __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, float3 xk, float3 xl,
                                                       float4 kc, int role, const BoxView& box,
-                                                      float& energy, float& virial) {
-  const float3 v0 = mic3(xi, xj, box);
-  const float3 v1 = mic3(xk, xj, box);
-  const float3 v2 = mic3(xk, xl, box);
-  const float3 c0 = cross3(v0, v1), c1 = cross3(v1, v2);
-  const float n1 = dot3(v1, v1), nc0 = dot3(c0, c0), nc1 = dot3(c1, c1);
-  const int nn = static_cast<int>(kc.z);
-  if (n1 == 0.f || nc0 == 0.f || nc1 == 0.f) {
-    energy = nn == 0 ? kc.x * (1.f + kc.y) : kc.x;
-    virial = 0.f;
+                                                      double& energy, double& virial) {
+  const d3 v0 = dbl(mic3(xi, xj, box));
+  const d3 v1 = dbl(mic3(xk, xj, box));
+  const d3 v2 = dbl(mic3(xk, xl, box));
+  const d3 c0 = dcross(v0, v1), c1 = dcross(v1, v2);
+  const double n1 = ddot(v1, v1), nc0 = ddot(c0, c0), nc1 = ddot(c1, c1);
+  const double k = kc.x, dsign = kc.y, nmul = kc.z;
+  if (n1 == 0.0 || nc0 == 0.0 || nc1 == 0.0) {
+    energy = nmul == 0.0 ? k * (1.0 + dsign) : k;
+    virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const float nb = sqrtf(n1);
-  const float phi = atan2f(nb * dot3(v0, c1), dot3(c0, c1));
-  const float arg = kc.z * phi;
-  float sn, cs;
-  sincosf(arg, &sn, &cs);
-  energy = kc.x * (1.f + kc.y * cs);
-  const float dE = -kc.x * kc.y * kc.z * sn;
-  const float3 fi = scale3(c0, -dE * nb / nc0);
-  const float3 fl = scale3(c1, dE * nb / nc1);
-  const float inv_n1 = 1.f / n1;
-  const float3 s = sub3(scale3(fi, dot3(v0, v1) * inv_n1), scale3(fl, dot3(v2, v1) * inv_n1));
-  const float3 fj = sub3(s, fi);
-  const float3 fk = scale3(add3(s, fl), -1.f);
-  virial = dot3(v0, fi) + dot3(v1, fk) + dot3(sub3(v1, v2), fl);
-  if (role == 0) return fi;
-  if (role == 1) return fj;
-  if (role == 2) return fk;
-  return fl;
+  const double nb = sqrt(n1);
+  // cos(phi) and sin(phi) from the cross products (sin carries the sign of
+  // v0.c1), then cos(n phi), sin(n phi) by the angle-addition recurrence:
+  // the reference's k[1 + d cos(n phi)] and its derivative without a
+  // transcendental call.
+  const double inv_c = 1.0 / sqrt(nc0 * nc1);
+  const double c1phi = ddot(c0, c1) * inv_c;
+  const double s1phi = nb * ddot(v0, c1) * inv_c;
+  double cs = 1.0, sn = 0.0;
+  for (int m = 0; m < static_cast<int>(nmul); ++m) {
+    const double c2 = cs * c1phi - sn * s1phi;
+    sn = sn * c1phi + cs * s1phi;
+    cs = c2;
+  }
+  energy = k * (1.0 + dsign * cs);
+  const double dE = -k * dsign * nmul * sn;
+  const d3 fi = dscale(c0, -dE * nb / nc0);
+  const d3 fl = dscale(c1, dE * nb / nc1);
+  const double inv_n1 = 1.0 / n1;
+  const d3 sv = dsub(dscale(fi, ddot(v0, v1) * inv_n1), dscale(fl, ddot(v2, v1) * inv_n1));
+  const d3 fj = dsub(sv, fi);
+  const d3 fk = dscale(dadd(sv, fl), -1.0);
+  virial = ddot(v0, fi) + ddot(v1, fk) + ddot(dsub(v1, v2), fl);
+  if (role == 0) return flt(fi);
+  if (role == 1) return flt(fj);
+  if (role == 2) return flt(fk);
+  return flt(fl);
 }
 
 // Neighbour storage: each atom's list is cut into chunks of kNbrChunk entries;

@@ -255,29 +255,31 @@ HostTopology build_host_topology(const beads_spec_t& s) {
 
 // pair_lj.cu:171-192 coefficient packing in binary32:
 // {sigma^2, 24 eps, 4 eps, shift} per ordered type pair.
-std::vector<float4> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                  float cutoff, bool shift_energy) {
+std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
+                                   float cutoff, bool shift_energy) {
   const int T = n_types;
-  std::vector<float4> lj(static_cast<size_t>(T) * T);
-  const float rc = cutoff;
+  std::vector<LjCoeff> lj(static_cast<size_t>(T) * T);
+  const double rc = cutoff;
   for (int a = 0; a < T; ++a) {
     for (int b = 0; b < T; ++b) {
-      const float eps = epsilon[a * T + b], sig = sigma[a * T + b];
-      const float sigma2 = sig * sig;
-      const float eps4 = 4.f * eps;
-      float shift = 0.f;
+      const double eps = epsilon[a * T + b], sig = sigma[a * T + b];
+      LjCoeff c;
+      c.sigma2 = sig * sig;
+      c.eps4 = 4.0 * eps;
+      c.eps24 = 24.0 * eps;
+      c.shift = 0.0;
       if (shift_energy) {
-        const float sr2 = sigma2 / (rc * rc);
-        const float sr6 = sr2 * sr2 * sr2;
-        shift = eps4 * (sr6 * sr6 - sr6);
+        const double sr2 = c.sigma2 / (rc * rc);
+        const double sr6 = sr2 * sr2 * sr2;
+        c.shift = c.eps4 * (sr6 * sr6 - sr6);
       }
-      lj[static_cast<size_t>(a) * T + b] = make_float4(sigma2, 6.f * eps4, eps4, shift);
+      lj[static_cast<size_t>(a) * T + b] = c;
     }
   }
   return lj;
 }
 
-std::vector<float4> pack_lj_table(const beads_spec_t& s) {
+std::vector<LjCoeff> pack_lj_table(const beads_spec_t& s) {
   return pack_lj_table(s.n_types, s.lj_epsilon, s.lj_sigma, s.lj_cutoff, s.lj_shift != 0);
 }
 
@@ -990,7 +992,7 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
 // Binary32 tables as the forward uses them ({sigma^2, 24 eps, 4 eps, shift},
 // {k, r0}) plus FP64 copies of the same binary32 values for the adjoint.
 void Engine::upload_parameters() {
-  const std::vector<float4> lj =
+  const std::vector<LjCoeff> lj =
       pack_lj_table(n_types_, lj_eps_.data(), lj_sig_.data(), plan_.force_cutoff, lj_shift_);
   lj_.upload(lj.data(), lj.size(), stream_);
   std::vector<float2> bc(std::max(1, n_bond_types_), make_float2(0.f, 0.f));

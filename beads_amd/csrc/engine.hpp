@@ -62,10 +62,11 @@ HostTopology build_host_topology(const beads_spec_t& spec);
 // Shell validation of one spec (throws Error with the matching status).
 void validate_spec(const beads_spec_t& spec);
 
-// Packed LJ coefficients {sigma^2, 24 eps, 4 eps, shift} for all type pairs.
-std::vector<float4> pack_lj_table(const beads_spec_t& spec);
-std::vector<float4> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                  float cutoff, bool shift_energy);
+// FP64 LJ coefficients {sigma^2, 24 eps, 4 eps, shift} for all ordered type
+// pairs, from the binary32 parameters.
+std::vector<LjCoeff> pack_lj_table(const beads_spec_t& spec);
+std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
+                                   float cutoff, bool shift_energy);
 
 // Bond-graph exclusion pairs (tag_a < tag_b) within the given distance
 // (exclusions.cu:514-598).
@@ -158,7 +159,7 @@ class Engine {
   DeviceBuffer<uint32_t> topo_count_, s_topo_count_, excl_, s_excl_;
   DeviceBuffer<uint2> bond_, s_bond_;
   DeviceBuffer<uint4> angle_, s_angle_, dih_, s_dih_;
-  DeviceBuffer<float4> lj_;
+  DeviceBuffer<LjCoeff> lj_;
   DeviceBuffer<float2> bond_coeff_, angle_coeff_;
   DeviceBuffer<float4> dih_coeff_;
   DeviceBuffer<double> partials_;

@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 // all three, so the force sums are identical.
 template <bool OBS, int TYPES, bool REACT, int LIST = 0>
 __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4 xi4,
-                                        const float4* lj_lds, float4 c0, double& fxd,
+                                        const LjCoeff* lj_lds, const LjCoeff& c0, double& fxd,
                                         double& fyd, double& fzd, double& e_pair,
                                         double& w_pair) {
   const int T = p.n_types;
@@ -402,7 +402,7 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
         ++kept;
       }
       if (base + u < cnt && r2 < p.rc2) {
-        float4 c;
+        LjCoeff c;
         if (TYPES == 0) {
           c = c0;
         } else if (TYPES == 1) {
@@ -442,7 +442,7 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
 }
 
 template <int TYPES>
-__device__ __forceinline__ void load_lj_table(const ForceParams& p, float4* lj_lds) {
+__device__ __forceinline__ void load_lj_table(const ForceParams& p, LjCoeff* lj_lds) {
   if (TYPES == 1) {
     for (int k = threadIdx.x; k < p.n_types * p.n_types; k += kBlock) {
       lj_lds[k] = p.lj[k];
@@ -459,10 +459,10 @@ __device__ __forceinline__ void load_lj_table(const ForceParams& p, float4* lj_l
 // listed terms, kicks, reduces and clears pair_acc.
 template <bool OBS, int TYPES>
 __global__ __launch_bounds__(kBlock) void half_pair_kernel(ForceParams p) {
-  __shared__ float4 lj_lds[TYPES == 1 ? kMaxLdsTypes * kMaxLdsTypes : 1];
+  __shared__ LjCoeff lj_lds[TYPES == 1 ? kMaxLdsTypes * kMaxLdsTypes : 1];
   __shared__ double red_lds[4];
   load_lj_table<TYPES>(p, lj_lds);
-  const float4 c0 = TYPES == 0 ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const LjCoeff c0 = TYPES == 0 ? p.lj[0] : LjCoeff{0.0, 0.0, 0.0, 0.0};
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double e_pair = 0.0, w_pair = 0.0;
   if (i < p.n) {
@@ -516,7 +516,7 @@ __device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, fl
     const float3 xi = f3(xi4);
     for (uint32_t k = 0; k < nbond; ++k) {
       const uint2 r = tp.bond[static_cast<size_t>(k) * n + i];
-      float e, w;
+      double e, w;
       const float3 f = bond_force(mic3(xi, f3(p.pos[r.x]), p.box),
                                   tp.bond_coeff[r.y & 0x7fffffffu], e, w);
       fx += f.x;
@@ -535,7 +535,7 @@ __device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, fl
       const float3 pi = role == 0 ? xi : a;
       const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
       const float3 pk = role == 2 ? xi : b;
-      float e, w;
+      double e, w;
       const float3 f = angle_role_force(pi, pj, pk, tp.angle_coeff[r.z], role, p.box, e, w);
       fx += f.x;
       fy += f.y;
@@ -556,7 +556,7 @@ __device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, fl
       else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
       else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
       else { q0 = a; q1 = b; q2 = c; q3 = xi; }
-      float e, w;
+      double e, w;
       const float3 f = dihedral_role_force(q0, q1, q2, q3, tp.dih_coeff[r.w & 0x3fffffffu],
                                            role, p.box, e, w);
       fx += f.x;
@@ -644,10 +644,10 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
 
 template <bool KICK, bool OBS, int TYPES, bool HALF, bool PRUNE>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
-  __shared__ float4 lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
+  __shared__ LjCoeff lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
   __shared__ double red_lds[4];
   if (!HALF) load_lj_table<TYPES>(p, lj_lds);
-  const float4 c0 = TYPES == 0 && !HALF ? p.lj[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const LjCoeff c0 = TYPES == 0 && !HALF ? p.lj[0] : LjCoeff{0.0, 0.0, 0.0, 0.0};
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   StepSums sums;
   if (i < p.n) {

@@ -26,7 +26,7 @@ template <int BLOCK>
 struct ReplicaShared {
   double red[BLOCK / 64];
   uint32_t scan[BLOCK];
-  float4 lj[kMaxLdsTypes * kMaxLdsTypes];
+  LjCoeff lj[kMaxLdsTypes * kMaxLdsTypes];
   double lambda;
   uint32_t flag;
 };
@@ -192,7 +192,7 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
 // kinetic energy, pair energy and virial.
 template <int BLOCK>
 __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool kick, bool single,
-                              const float4* lj, double& ke, double& pe, double& vir) {
+                              const LjCoeff* lj, double& ke, double& pe, double& vir) {
   const uint32_t n = d.n;
   const float4* pos = A.pos + d.atom0;
   float4* vel = A.vel + d.atom0;
@@ -202,7 +202,7 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
   const float L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
   const float h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
   const int T = d.n_types;
-  const float4 c0 = lj[0];
+  const LjCoeff c0 = lj[0];
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     const float4 xi = pos[i];
     const int ti = __float_as_int(xi.w);
@@ -215,7 +215,7 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
       const float dy = mic_diff(xi.y, xj.y, L1, h1);
       const float dz = mic_diff(xi.z, xj.z, L2, h2);
       if (r2_strict(dx, dy, dz) < d.rc2) {
-        const float4 c = single ? c0 : lj[ti * T + __float_as_int(xj.w)];
+        const LjCoeff c = single ? c0 : lj[ti * T + __float_as_int(xj.w)];
         double rr, u;
         const double s = lj_pair(dx, dy, dz, c, rr, u);
         fx += s * dx;

@@ -22,7 +22,7 @@ struct ReplicaDesc {
   float cutoff_buffer;
   int32_t berendsen;
   int32_t n_types;
-  uint32_t lj0;           // first LJ table entry (n_types^2 float4)
+  uint32_t lj0;           // first LJ table entry (n_types^2 entries)
   double target_thermal;  // kB * T0
   double dt_over_tau;
   double dof;
@@ -71,7 +71,7 @@ struct ReplicaArrays {
 struct ReplicaParams {
   const ReplicaDesc* desc;
   ReplicaState* state;
-  const float4* lj;
+  const LjCoeff* lj;
   ReplicaArrays arrays;
 };
 

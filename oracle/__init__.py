@@ -217,8 +217,17 @@ def neighbor_lists(w, positions=None, tags=None, types=None, path: int = -1,
 
 
 def forces(w, positions=None, tags=None, types=None, counts=None, lst=None):
-    """FP64 forces [n][3] and observables. counts/lst are atom-major if given."""
+    """FP64 forces [n][3] and observables. counts/lst are atom-major if given;
+    otherwise the lists are built as the engine builds them, with the
+    workload's skin (r_s = r_c + cutoff_buffer): the force pass keeps pairs
+    with r^2 < r_c^2 on the exact-intermediate minimum image, which can admit
+    a pair at r_c that a list built with r_s = r_c (rounding its minimum image
+    at the scale of the box) would have dropped."""
     lib = library()
+    if counts is None:
+        counts, lst, ovf = neighbor_lists(w, positions, tags, types)
+        if ovf:
+            raise OracleError("oracle neighbour list overflow; raise max_neighbors")
     sys = OracleSystem(w, positions, tags, types)
     f = np.zeros((w.n, 3), np.float64)
     obs = Observables()
