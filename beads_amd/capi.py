@@ -60,7 +60,7 @@ class Stats(C.Structure):
         ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("max_neighbors", C.c_uint32),
         ("mean_neighbors", C.c_double), ("ke", C.c_double), ("lambda_", C.c_double),
         ("pe", C.c_double * 4), ("virial", C.c_double * 4), ("temperature", C.c_double),
-        ("pressure", C.c_double), ("prune_count", C.c_uint64),
+        ("pressure", C.c_double),
     ]
 
 

@@ -66,14 +66,13 @@ struct alignas(256) Ctrl {
   uint32_t arrive_force;    // last-block arrival counters
   uint32_t arrive_scale;
   uint32_t arrive_build;
-  uint32_t prune;           // some atom left the pruned list's validity radius
+  uint32_t pad0;
   double lambda;            // Berendsen factor still to be applied to vel
   double ke_pre;            // kinetic energy after the post-force half-kick
   double ke;                // kinetic energy of the velocities as stored
   double pe[4];             // pair, bond, angle, dihedral
   double virial[4];
   double half_pair[2];      // half list: pair energy and virial summed by the pair pass
-  uint64_t prune_count;     // force passes that rewrote the pruned inner list
 };
 
 struct NeighborControl {
@@ -118,14 +117,6 @@ struct ForceParams {
   uint32_t n_cells;
   int32_t half;           // stored list holds each pair once (j > i)
   double* pair_acc;       // [3 N] LJ sums of the half-list pair pass
-  // Pruned inner list (full list, dynamics): pairs closer than r_in at the
-  // last prune; rewritten by the force pass of every rebuild or prune step.
-  int32_t prune;
-  float rin2;
-  const int4* image;
-  uint32_t* inner_count;
-  uint32_t* inner;
-  float4* prune_ref;      // unwrapped positions at the last prune
 };
 
 struct IntegrateParams {
@@ -140,8 +131,6 @@ struct IntegrateParams {
   float dt;
   float threshold_sq;
   Ctrl* ctrl;
-  const float4* prune_ref;  // unwrapped positions at the last prune
-  float prune_sq;           // (half the prune margin)^2
 };
 
 struct BuildParams {

@@ -5,7 +5,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <iostream>
@@ -411,12 +410,6 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
     const char* env = std::getenv("BEADS_AMD_GRAPHS");
     use_graphs_ = !(env != nullptr && env[0] == '0');
   }
-  {
-    // Diagnostics switch: BEADS_AMD_PRUNE=0 keeps every force pass on the
-    // stored list.
-    const char* env = std::getenv("BEADS_AMD_PRUNE");
-    prune_ = !half_list_ && spec.cutoff_buffer > 0.f && !(env != nullptr && env[0] == '0');
-  }
   has_bond_style_ = spec.n_bond_types > 0;
   has_angle_style_ = spec.n_angle_types > 0;
   has_dihedral_style_ = spec.n_dihedral_types > 0;
@@ -547,13 +540,6 @@ void Engine::upload_state() {
     pair_acc_.resize(3 * n);
     pair_acc_.zero(stream_);
   }
-  if (prune_) {
-    inner_count_.resize(n);
-    inner_count_.zero(stream_);
-    inner_.resize(static_cast<size_t>(nbr_padded_capacity(plan_.max_neighbors)) * n);
-    prune_ref_.resize(n);
-    prune_ref_.zero(stream_);
-  }
   Ctrl c{};
   c.lambda = 1.0;
   ctrl_.upload(&c, 1, stream_);
@@ -602,16 +588,6 @@ void Engine::set_up_params() {
   f.n_cells = plan_.n_cells;
   f.half = half_list_ ? 1 : 0;
   f.pair_acc = pair_acc_.data();
-  // Pruned inner list: margin a third of the skin, so an atom may move a
-  // sixth of the skin between prunes (checked every step in integrate).
-  const float margin = plan_.cutoff_buffer / 3.f;
-  const float rin = plan_.force_cutoff + margin;
-  f.prune = prune_ ? 1 : 0;
-  f.rin2 = rin * rin;
-  f.image = image_.data();
-  f.inner_count = inner_count_.data();
-  f.inner = inner_.data();
-  f.prune_ref = prune_ref_.data();
 
   IntegrateParams& in = integrate_params_;
   in.n = n_;
@@ -626,8 +602,6 @@ void Engine::set_up_params() {
   const float half_buffer = 0.5f * plan_.cutoff_buffer;
   in.threshold_sq = half_buffer * half_buffer;
   in.ctrl = ctrl_.data();
-  in.prune_ref = prune_ref_.data();
-  in.prune_sq = 0.98f * (0.5f * margin) * (0.5f * margin);
 
   CellGridView grid{};
   grid.nx = plan_.nx;
@@ -755,7 +729,7 @@ void Engine::step_once(bool observables) {
   const uint64_t physical = step_ + 1;
   const bool every = plan_.cutoff_buffer == 0.f;
   const bool check = !every && physical % plan_.check_every == 0;
-  launch_integrate(integrate_params_, thermostat_, check, prune_, stream_);
+  launch_integrate(integrate_params_, thermostat_, check, false, stream_);
   if (every || check) {
     rebuild_sequence(every);
   }
@@ -853,7 +827,6 @@ void Engine::stats(beads_engine_stats_t* out) {
   // Kinetic energy of the velocities as stored (before any pending factor).
   out->ke = ke_from_scale_ ? c.ke : c.ke_pre;
   out->lambda = c.lambda;
-  out->prune_count = c.prune_count;
   for (int q = 0; q < 4; ++q) {
     out->pe[q] = c.pe[q];
     out->virial[q] = c.virial[q];
@@ -957,7 +930,7 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
     const bool every = plan_.cutoff_buffer == 0.f;
     const bool check = !every && physical % plan_.check_every == 0;
     BEADS_HIP(hipEventRecord(ev[4 * k], stream_));
-    launch_integrate(integrate_params_, thermostat_, check, prune_, stream_);
+    launch_integrate(integrate_params_, thermostat_, check, false, stream_);
     BEADS_HIP(hipEventRecord(ev[4 * k + 1], stream_));
     if (every || check) rebuild_sequence(every);
     BEADS_HIP(hipEventRecord(ev[4 * k + 2], stream_));
@@ -1042,9 +1015,6 @@ void Engine::enter_frame_mode() {
           BEADS_E_RUNTIME);
   if (!frame_mode_) {
     for (auto& e : frame_events_) BEADS_HIP(hipEventCreate(&e));
-    // The adjoint pass reads the stored list: no pruning in frame mode.
-    prune_ = false;
-    force_params_.prune = 0;
     frame_mode_ = true;
     control_.path = 0;  // never reorder: slots stay the caller's order
   }

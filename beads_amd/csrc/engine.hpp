@@ -138,7 +138,6 @@ class Engine {
   uint32_t n_ = 0;
   bool thermostat_ = false;
   bool half_list_ = false;
-  bool prune_ = false;  // force passes read the pruned inner list
   bool use_graphs_ = true;
   hipGraphExec_t step_graph_exec_ = nullptr;
   bool has_bond_style_ = false, has_angle_style_ = false, has_dihedral_style_ = false;
@@ -164,8 +163,6 @@ class Engine {
   DeviceBuffer<float4> dih_coeff_;
   DeviceBuffer<double> partials_;
   DeviceBuffer<double> pair_acc_;  // half list: partner-side LJ sums [3 N]
-  DeviceBuffer<uint32_t> inner_count_, inner_;  // pruned inner list (chunked like nbr_)
-  DeviceBuffer<float4> prune_ref_;              // unwrapped positions at the last prune
 
   DeviceBuffer<Ctrl> ctrl_;
   PinnedBuffer<Ctrl> ctrl_host_;

@@ -355,11 +355,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 // reaction -s d of every pair to pair_acc[j] (half list).
 // TYPES: 0 = single type (coefficients in registers), 1 = table in LDS,
 // 2 = table in global memory.
-// LIST: 0 = the stored list; 1 = the pruned inner list; 2 = the stored list,
-// also rewriting the inner list (pairs with r^2 < r_in^2) and the atom's
-// prune reference. The pairs inside r_c, and their order, are the same in
-// all three, so the force sums are identical.
-template <bool OBS, int TYPES, bool REACT, int LIST = 0>
+template <bool OBS, int TYPES, bool REACT>
 __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4 xi4,
                                         const LjCoeff* lj_lds, const LjCoeff& c0, double& fxd,
                                         double& fyd, double& fzd, double& e_pair,
@@ -369,13 +365,10 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
   const uint32_t n = p.n;
   const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
   const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
-  const uint32_t* list = LIST == 1 ? p.inner : p.nbr;
-  const uint32_t cnt = LIST == 1 ? p.inner_count[i] : p.nbr_count[i];
-  uint32_t kept = 0;
-  uint4 stash = make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t cnt = p.nbr_count[i];
   for (uint32_t base = 0; base < cnt; base += 4) {
     const uint4 q = *reinterpret_cast<const uint4*>(
-        list + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1)));
+        p.nbr + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1)));
     const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
     float4 xj[4];
 #pragma unroll
@@ -388,19 +381,6 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
       const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
       const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
       const float r2 = r2_strict(dx, dy, dz);
-      if (LIST == 2 && base + u < cnt && r2 < p.rin2) {
-        // Append to the inner list four entries (one 16-byte store) at a time.
-        const uint32_t slot = kept & 3u;
-        stash.x = slot == 0 ? jj[u] : stash.x;
-        stash.y = slot == 1 ? jj[u] : stash.y;
-        stash.z = slot == 2 ? jj[u] : stash.z;
-        stash.w = slot == 3 ? jj[u] : stash.w;
-        if (slot == 3) {
-          *reinterpret_cast<uint4*>(p.inner + nbr_chunk_base(kept / kNbrChunk, i, n) +
-                                    (kept & (kNbrChunk - 4))) = stash;
-        }
-        ++kept;
-      }
       if (base + u < cnt && r2 < p.rc2) {
         LjCoeff c;
         if (TYPES == 0) {
@@ -427,17 +407,6 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
         }
       }
     }
-  }
-  if (LIST == 2) {
-    if ((kept & 3u) != 0) {
-      *reinterpret_cast<uint4*>(p.inner + nbr_chunk_base(kept / kNbrChunk, i, n) +
-                                (kept & (kNbrChunk - 4))) = stash;
-    }
-    p.inner_count[i] = kept;
-    const int4 im = p.image[i];
-    p.prune_ref[i] = make_float4(xi4.x + static_cast<float>(im.x) * L0,
-                                 xi4.y + static_cast<float>(im.y) * L1,
-                                 xi4.z + static_cast<float>(im.z) * L2, 0.f);
   }
 }
 
@@ -630,10 +599,8 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
     }
   }
   if (threadIdx.x == 0) {
-    if (p.prune != 0 && (p.ctrl->rebuild != 0 || p.ctrl->prune != 0)) p.ctrl->prune_count += 1;
     p.ctrl->rebuild = 0;
     p.ctrl->reorder = 0;
-    p.ctrl->prune = 0;
     p.ctrl->arrive_force = 0;
     if (HALF) {
       p.ctrl->half_pair[0] = 0.0;
@@ -642,7 +609,7 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
   }
 }
 
-template <bool KICK, bool OBS, int TYPES, bool HALF, bool PRUNE>
+template <bool KICK, bool OBS, int TYPES, bool HALF>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   __shared__ LjCoeff lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
   __shared__ double red_lds[4];
@@ -661,16 +628,6 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
       acc[0] = 0.0;
       acc[1] = 0.0;
       acc[2] = 0.0;
-    } else if (PRUNE) {
-      // Rewrite the inner list on rebuild steps and when an atom has left
-      // its validity radius (integrate_kernel's test); read it otherwise.
-      if (p.ctrl->rebuild != 0 || p.ctrl->prune != 0) {
-        lj_list<OBS, TYPES, false, 2>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair,
-                                      sums.w_pair);
-      } else {
-        lj_list<OBS, TYPES, false, 1>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair,
-                                      sums.w_pair);
-      }
     } else {
       lj_list<OBS, TYPES, false>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
     }
@@ -719,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void scale_kernel(ScaleParams p) {
 // Pending Berendsen factor, first half-kick, drift and wrap
 // (velocity_verlet.cu:59-95; box_geometry_view.cuh:92-102), then the
 // displacement test against the last build (rebuild_tracker.cu:55-94).
-template <bool THERMO, bool CHECK, bool PRUNE>
+template <bool THERMO, bool CHECK>
 __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= p.n) {
@@ -758,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
   p.vel[i] = v;
   const bool wrapped = c[0] != 0.f || c[1] != 0.f || c[2] != 0.f;
   int4 im = make_int4(0, 0, 0, 0);
-  if (wrapped || CHECK || PRUNE) {
+  if (wrapped || CHECK) {
     im = p.image[i];
   }
   if (wrapped) {
@@ -776,18 +733,6 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
     const float dz = (x.z - r.z) + static_cast<float>(im.z - ri.z) * p.box.L[2];
     if (dx * dx + dy * dy + dz * dz > p.threshold_sq) {
       __hip_atomic_store(&p.ctrl->rebuild, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  if (PRUNE) {
-    // Displacement since the last prune, every step: an atom that moved more
-    // than half the prune margin may bring a pair outside the inner list
-    // within the cut-off, so the next force pass prunes again.
-    const float4 r = p.prune_ref[i];
-    const float dx = x.x + static_cast<float>(im.x) * p.box.L[0] - r.x;
-    const float dy = x.y + static_cast<float>(im.y) * p.box.L[1] - r.y;
-    const float dz = x.z + static_cast<float>(im.z) * p.box.L[2] - r.z;
-    if (dx * dx + dy * dy + dz * dz > p.prune_sq) {
-      __hip_atomic_store(&p.ctrl->prune, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -854,13 +799,11 @@ void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream)
     BEADS_HALF_CASE(true, 2)
 #undef BEADS_HALF_CASE
   }
-  const bool prune = p.prune != 0;
-#define BEADS_FORCE_CASE(K, O, T)                                               \
-  if (kick == K && obs == O && (half || types == T)) {                          \
-    if (half) force_kernel<K, O, 0, true, false><<<g, kBlock, 0, stream>>>(p);  \
-    else if (prune) force_kernel<K, O, T, false, true><<<g, kBlock, 0, stream>>>(p); \
-    else force_kernel<K, O, T, false, false><<<g, kBlock, 0, stream>>>(p);      \
-    return;                                                                     \
+#define BEADS_FORCE_CASE(K, O, T)                                        \
+  if (kick == K && obs == O && (half || types == T)) {                   \
+    if (half) force_kernel<K, O, 0, true><<<g, kBlock, 0, stream>>>(p);  \
+    else force_kernel<K, O, T, false><<<g, kBlock, 0, stream>>>(p);      \
+    return;                                                              \
   }
   BEADS_FORCE_CASE(true, false, 0)
   BEADS_FORCE_CASE(true, false, 1)
@@ -877,23 +820,16 @@ void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream)
 #undef BEADS_FORCE_CASE
 }
 
-void launch_integrate(const IntegrateParams& p, bool thermostat, bool check, bool prune,
-                      hipStream_t stream) {
+void launch_integrate(const IntegrateParams& p, bool thermostat, bool check,
+                      bool /*force_rebuild*/, hipStream_t stream) {
   const unsigned g = grid_for(p.n);
-#define BEADS_INTEGRATE_CASE(TH, CH, PR)                                  \
-  if (thermostat == TH && check == CH && prune == PR) {                   \
-    integrate_kernel<TH, CH, PR><<<g, kBlock, 0, stream>>>(p);            \
-    return;                                                               \
+  if (thermostat) {
+    if (check) integrate_kernel<true, true><<<g, kBlock, 0, stream>>>(p);
+    else integrate_kernel<true, false><<<g, kBlock, 0, stream>>>(p);
+  } else {
+    if (check) integrate_kernel<false, true><<<g, kBlock, 0, stream>>>(p);
+    else integrate_kernel<false, false><<<g, kBlock, 0, stream>>>(p);
   }
-  BEADS_INTEGRATE_CASE(true, true, true)
-  BEADS_INTEGRATE_CASE(true, true, false)
-  BEADS_INTEGRATE_CASE(true, false, true)
-  BEADS_INTEGRATE_CASE(true, false, false)
-  BEADS_INTEGRATE_CASE(false, true, true)
-  BEADS_INTEGRATE_CASE(false, true, false)
-  BEADS_INTEGRATE_CASE(false, false, true)
-  BEADS_INTEGRATE_CASE(false, false, false)
-#undef BEADS_INTEGRATE_CASE
 }
 
 void launch_scale(const ScaleParams& p, bool thermostat, hipStream_t stream) {

@@ -54,8 +54,8 @@ void launch_build(const BuildParams& p, hipStream_t stream);
 
 // One step's work.
 void launch_force(const ForceParams& p, bool kick, bool observables, hipStream_t stream);
-void launch_integrate(const IntegrateParams& p, bool thermostat, bool check, bool prune,
-                      hipStream_t stream);
+void launch_integrate(const IntegrateParams& p, bool thermostat, bool check,
+                      bool force_rebuild, hipStream_t stream);
 void launch_scale(const ScaleParams& p, bool thermostat, hipStream_t stream);
 
 }  // namespace beads_amd

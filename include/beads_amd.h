@@ -169,8 +169,6 @@ typedef struct beads_engine_stats {
   double virial[4];
   double temperature;
   double pressure;
-  uint64_t prune_count;          /* force passes that rewrote the pruned inner list (no
-                                    reference counterpart; 0 when pruning is off) */
 } beads_engine_stats_t;
 
 /* Host-side views filled by beads_engine_read_state (each pointer may be NULL). */

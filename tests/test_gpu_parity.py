@@ -156,29 +156,6 @@ def test_nve_energy_conservation():
     assert per_step < 1e-6, (drifts, e0)
 
 
-@pytest.mark.parametrize("name", ["lj_regular", "melt", "mixture"])
-def test_pruned_inner_list_matches_stored_list_bit_for_bit(name, monkeypatch):
-    """Force passes between rebuilds read a pruned inner list (pairs within
-    r_c + skin/3 at the last prune, re-pruned when an atom has moved a sixth of
-    the skin). It holds every pair that can come within r_c before the next
-    prune, in the stored list's order, so trajectories are bit-identical to
-    force passes over the whole stored list."""
-    w = CASES[name]
-    out = {}
-    for prune in ("1", "0"):
-        monkeypatch.setenv("BEADS_AMD_PRUNE", prune)
-        with _engine(w) as e:
-            e.advance(150, observables_on_last=True)
-            out[prune] = (e.state(), e.stats())
-    (a, sa), (b, sb) = out["1"], out["0"]
-    assert sa.prune_count > sa.build_count >= 1 and sb.prune_count == 0
-    np.testing.assert_array_equal(a["tags"], b["tags"])
-    np.testing.assert_array_equal(a["positions"], b["positions"])
-    np.testing.assert_array_equal(a["velocities"], b["velocities"])
-    np.testing.assert_array_equal(a["forces"], b["forces"])
-    assert sa.pe[0] == sb.pe[0] and sa.build_count == sb.build_count
-
-
 @pytest.mark.parametrize("name", ["lj_regular", "melt"])
 def test_step_graph_replays_match_eager_launches(name, monkeypatch):
     """Stretches of 16+ steps replay one captured hipGraph; with eager reads and
