@@ -63,10 +63,7 @@ struct alignas(256) Ctrl {
   uint64_t dangerous;
   uint64_t reorder_count;
   uint64_t neighbor_total;  // sum of neighbour counts of the current list
-  uint32_t arrive_force;    // last-block arrival counters
-  uint32_t arrive_scale;
-  uint32_t arrive_build;
-  uint32_t pad0;
+  uint32_t pad0[4];
   double lambda;            // Berendsen factor still to be applied to vel
   double ke_pre;            // kinetic energy after the post-force half-kick
   double ke;                // kinetic energy of the velocities as stored

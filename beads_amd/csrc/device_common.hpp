@@ -71,26 +71,6 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return v;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Last-workgroup election after each workgroup has published its partials.
-__device__ __forceinline__ bool arrive_last(uint32_t* counter) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t prev = atomicAdd(counter, 1u);
-    last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-  }
-  return last;
-}
-
 // cell_list_build.cu:110-123: binary32 cell index, clamped.
 __device__ __forceinline__ uint32_t axis_cell(float x, float lo, float inv, uint32_t count) {
   const float s = (x - lo) * inv;

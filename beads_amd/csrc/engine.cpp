@@ -535,7 +535,8 @@ void Engine::upload_state() {
   angle_coeff_.upload(ac.data(), ac.size(), stream_);
   dih_coeff_.upload(dc.data(), dc.size(), stream_);
 
-  partials_.resize(9 * static_cast<size_t>(grid_for(n_)));
+  // Per-workgroup partial sums of the force pass (the widest grid) and of scale.
+  partials_.resize(9 * static_cast<size_t>(std::max(force_grid(n_), grid_for(n_))));
   if (half_list_) {
     pair_acc_.resize(3 * n);
     pair_acc_.zero(stream_);

@@ -1,11 +1,15 @@
 // gfx950 kernels of the per-step MD path.
 //
-// Design (CDNA4): one work-item per atom, 256-wide workgroups (4 wave64s),
-// float4 SoA streams, FP64 for LJ pair values and global reductions. Kernels
-// that gather neighbour positions map workgroups onto atoms XCD by XCD so each
-// XCD's L2 holds one contiguous (spatially sorted) slab of positions. Every
-// kernel of the neighbour-rebuild sequence reads the device control block first
-// and exits when no rebuild is due, so a step never waits on the host.
+// Design (CDNA4): 256-wide workgroups (4 wave64s), one work-item per atom
+// except in the force pass (a team of kTeam lanes per atom), float4 SoA
+// streams, FP64 for LJ pair values and global reductions. Kernels that gather
+// neighbour positions map workgroups onto atoms XCD by XCD so each XCD's L2
+// holds one contiguous (spatially sorted) slab of positions. Every kernel of
+// the neighbour-rebuild sequence reads the device control block first and
+// exits when no rebuild is due, so a step never waits on the host. No kernel
+// fences at device scope (on gfx950 that is an L2 write-back and invalidate
+// of the XCD): cross-workgroup totals are formed by a one-workgroup kernel
+// launched after the workgroups that wrote the partials.
 //
 // Reference counterparts (paths under native/src):
 //   control/bin/scan/scatter/sort  simulation/neighbor/cell_list_build.cu:110-219,347-414,
@@ -72,41 +76,71 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(BinParams p) {
 }
 
 // One workgroup: exclusive scan of cell counts into cell_start[0..n_cells],
-// copy to the scatter cursor and clear the counts for the next rebuild.
+// copy to the scatter cursor and clear the counts for the next rebuild. The
+// counts pass through LDS in tiles of kScanTile cells: coalesced loads (all
+// of a thread's loads in flight), a scan of kScanPer consecutive cells per
+// thread plus a Hillis-Steele scan of the thread sums, coalesced stores. Rows
+// are padded by one word per kScanPer so a thread's row starts in its own
+// bank.
 constexpr int kScanBlock = 1024;
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanTile = kScanBlock * kScanPer;
+
+__device__ __forceinline__ uint32_t scan_slot(uint32_t k) { return k + k / kScanPer; }
 
 __global__ __launch_bounds__(kScanBlock) void cell_scan_kernel(CellScanParams p) {
   if (p.ctrl->rebuild == 0) {
     return;
   }
+  __shared__ uint32_t tile[kScanTile + kScanBlock];
   __shared__ uint32_t sums[kScanBlock];
   const uint32_t t = threadIdx.x;
-  const uint32_t chunk = (p.n_cells + kScanBlock - 1) / kScanBlock;
-  const uint32_t begin = min(p.n_cells, t * chunk);
-  const uint32_t end = min(p.n_cells, begin + chunk);
-  uint32_t local = 0;
-  for (uint32_t c = begin; c < end; ++c) {
-    local += p.cell_count[c];
-  }
-  sums[t] = local;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over the 1024 chunk sums.
-  for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t v = t >= off ? sums[t - off] : 0u;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < p.n_cells; base += kScanTile) {
+    const uint32_t len = min(kScanTile, p.n_cells - base);
+#pragma unroll
+    for (uint32_t m = 0; m < kScanPer; ++m) {
+      const uint32_t k = m * kScanBlock + t;
+      tile[scan_slot(k)] = k < len ? p.cell_count[base + k] : 0u;
+    }
     __syncthreads();
-    sums[t] += v;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t m = 0; m < kScanPer; ++m) {
+      local += tile[scan_slot(t * kScanPer + m)];
+    }
+    sums[t] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
+      const uint32_t v = t >= off ? sums[t - off] : 0u;
+      __syncthreads();
+      sums[t] += v;
+      __syncthreads();
+    }
+    uint32_t run = carry + sums[t] - local;
+#pragma unroll
+    for (uint32_t m = 0; m < kScanPer; ++m) {
+      const uint32_t s = scan_slot(t * kScanPer + m);
+      const uint32_t cnt = tile[s];
+      tile[s] = run;
+      run += cnt;
+    }
+    carry += sums[kScanBlock - 1];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t m = 0; m < kScanPer; ++m) {
+      const uint32_t k = m * kScanBlock + t;
+      if (k < len) {
+        const uint32_t start = tile[scan_slot(k)];
+        p.cell_start[base + k] = start;
+        p.cell_cursor[base + k] = start;
+        p.cell_count[base + k] = 0;
+      }
+    }
     __syncthreads();
   }
-  uint32_t run = sums[t] - local;
-  for (uint32_t c = begin; c < end; ++c) {
-    const uint32_t cnt = p.cell_count[c];
-    p.cell_start[c] = run;
-    p.cell_cursor[c] = run;
-    p.cell_count[c] = 0;
-    run += cnt;
-  }
-  if (t == kScanBlock - 1) {
-    p.cell_start[p.n_cells] = sums[t];
+  if (t == 0) {
+    p.cell_start[p.n_cells] = carry;
   }
 }
 
@@ -124,25 +158,45 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(CellScanParams p) 
 
 // Order each cell's members by slot: together with the counting sort this is a
 // stable sort by (cell, slot), deterministic unlike the reference's atomic
-// scatter (cell_list_build.cu:168-181).
+// scatter (cell_list_build.cu:168-181). One wave per cell: for up to 64
+// members lane l holds member l, whose rank is the number of members with a
+// smaller slot (slots are distinct), and stores it at start + rank. A larger
+// cell is insertion-sorted by lane 0.
+constexpr uint32_t kSortCellsPerBlock = kBlock / 64;
+
 __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
   if (p.ctrl->rebuild == 0) {
     return;
   }
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = blockIdx.x * kSortCellsPerBlock + threadIdx.x / 64;
   if (c >= p.n_cells) {
-    return;
+    return;  // uniform over the wave
   }
   const uint32_t b = p.cell_start[c], e = p.cell_start[c + 1];
+  const uint32_t cnt = e - b;
   uint32_t* a = p.cell_atoms;
-  for (uint32_t k = b + 1; k < e; ++k) {
-    const uint32_t v = a[k];
-    uint32_t m = k;
-    while (m > b && a[m - 1] > v) {
-      a[m] = a[m - 1];
-      --m;
+  if (cnt <= 64) {
+    const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
+    uint32_t rank = 0;
+    for (uint32_t m = 0; m < cnt; ++m) {
+      const uint32_t w = static_cast<uint32_t>(
+          __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(m)));
+      rank += w < v ? 1u : 0u;
     }
-    a[m] = v;
+    if (lane < cnt) {
+      a[b + rank] = v;
+    }
+  } else if (lane == 0) {
+    for (uint32_t k = b + 1; k < e; ++k) {
+      const uint32_t v = a[k];
+      uint32_t m = k;
+      while (m > b && a[m - 1] > v) {
+        a[m] = a[m - 1];
+        --m;
+      }
+      a[m] = v;
+    }
   }
 }
 
@@ -348,8 +402,8 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 
 // ------------------------------------------------------------------ force --
 
-// LJ over one atom's stored list (pair_lj.cu:21-167). The loop reads one
-// 16-byte quad of indices and issues the four position gathers together
+// LJ over one atom's stored list (pair_lj.cu:21-167). The loop reads two
+// 16-byte quads of indices and issues the eight position gathers together
 // before any arithmetic. The decision r^2 < r_c^2 is binary32 on the
 // exact-intermediate minimum image; the pair value is FP64. REACT adds the
 // reaction -s d of every pair to pair_acc[j] (half list).
@@ -366,17 +420,23 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
   const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
   const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
   const uint32_t cnt = p.nbr_count[i];
-  for (uint32_t base = 0; base < cnt; base += 4) {
-    const uint4 q = *reinterpret_cast<const uint4*>(
-        p.nbr + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1)));
-    const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
-    float4 xj[4];
+  // Eight slots per iteration: both index quads of a half chunk, then eight
+  // gathers in flight before any arithmetic. Slots at or past cnt gather
+  // atom i itself (never used), so every load stays inside pos and inside the
+  // atom's allocated chunks (kNbrChunk is a multiple of 8).
+  static_assert(kNbrChunk % 8 == 0, "neighbour chunks hold whole 8-slot groups");
+  for (uint32_t base = 0; base < cnt; base += 8) {
+    const uint32_t* cp = p.nbr + nbr_chunk_base(base / kNbrChunk, i, n) + (base & (kNbrChunk - 1));
+    const uint4 qa = *reinterpret_cast<const uint4*>(cp);
+    const uint4 qb = *reinterpret_cast<const uint4*>(cp + 4);
+    const uint32_t jj[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+    float4 xj[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       xj[u] = p.pos[base + u < cnt ? jj[u] : i];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
       const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
       const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
@@ -401,6 +461,63 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
           unsafeAtomicAdd(acc + 1, -s * dy);
           unsafeAtomicAdd(acc + 2, -s * dz);
         }
+        if (OBS) {
+          e_pair += u_pair;
+          w_pair += s * rr;
+        }
+      }
+    }
+  }
+}
+
+// LJ over atom i's stored list, shared by the kTeam lanes of its team: lane t
+// takes slots base + u*kTeam + t, u < kTeamSlots, and keeps FP64 partial sums
+// (pair_lj.cu:21-167; same decision and pair value as lj_list). Slots at or
+// past cnt neither load an index nor count; they gather atom i itself.
+template <bool OBS, int TYPES>
+__device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32_t t, float4 xi4,
+                                        const LjCoeff* lj_lds, const LjCoeff& c0, double& fxd,
+                                        double& fyd, double& fzd, double& e_pair,
+                                        double& w_pair) {
+  static_assert(kNbrChunk % kTeam == 0, "a team's slots of one round share a chunk");
+  const int T = p.n_types;
+  const int ti = __float_as_int(xi4.w);
+  const uint32_t n = p.n;
+  const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+  const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+  const uint32_t cnt = p.nbr_count[i];
+  for (uint32_t base = 0; base < cnt; base += kTeam * kTeamSlots) {
+    uint32_t jj[kTeamSlots];
+#pragma unroll
+    for (int u = 0; u < kTeamSlots; ++u) {
+      const uint32_t s = base + u * kTeam + t;
+      jj[u] = s < cnt ? p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))] : i;
+    }
+    float4 xj[kTeamSlots];
+#pragma unroll
+    for (int u = 0; u < kTeamSlots; ++u) {
+      xj[u] = p.pos[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kTeamSlots; ++u) {
+      const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
+      const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
+      const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
+      const float r2 = r2_strict(dx, dy, dz);
+      if (base + u * kTeam + t < cnt && r2 < p.rc2) {
+        LjCoeff c;
+        if (TYPES == 0) {
+          c = c0;
+        } else if (TYPES == 1) {
+          c = lj_lds[ti * T + __float_as_int(xj[u].w)];
+        } else {
+          c = p.lj[ti * T + __float_as_int(xj[u].w)];
+        }
+        double rr, u_pair;
+        const double s = lj_pair(dx, dy, dz, c, rr, u_pair);
+        fxd += s * dx;
+        fyd += s * dy;
+        fzd += s * dz;
         if (OBS) {
           e_pair += u_pair;
           w_pair += s * rr;
@@ -452,13 +569,6 @@ __global__ __launch_bounds__(kBlock) void half_pair_kernel(ForceParams p) {
   }
 }
 
-// Fused force evaluation: LJ over the stored full list (pair_lj.cu:21-167) or
-// the half-list pair pass's sums (HALF), listed terms gathered per atom (each
-// atom evaluates every term it belongs to and keeps its own role's force: no
-// atomics, deterministic), then optionally the post-force half-kick
-// (velocity_verlet.cu:97-111) and the kinetic-energy reduction for the
-// Berendsen factor (berendsen_thermostat.cu:109-163), with the factor computed
-// in FP64 by the last workgroup.
 // Per-thread sums of one force pass.
 struct StepSums {
   double ke = 0.0;
@@ -466,77 +576,118 @@ struct StepSums {
   double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
 };
 
-// Everything after the pair sum for atom i: listed terms gathered per atom
-// (each atom evaluates every term it belongs to and keeps its own role's
-// force: no atomics, deterministic), the force store, optionally the
-// post-force half-kick (velocity_verlet.cu:97-111), and the atom's kinetic
-// energy.
-template <bool KICK, bool OBS>
-__device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, float4 xi4,
-                                            double fxd, double fyd, double fzd, StepSums& sums) {
-  const uint32_t n = p.n;
-  float fx = static_cast<float>(fxd);
-  float fy = static_cast<float>(fyd);
-  float fz = static_cast<float>(fzd);
-  if (p.topo.has_listed) {
-    const TopologyView& tp = p.topo;
-    const uint32_t cw = tp.count[i];
-    const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
-    const float3 xi = f3(xi4);
-    for (uint32_t k = 0; k < nbond; ++k) {
-      const uint2 r = tp.bond[static_cast<size_t>(k) * n + i];
-      double e, w;
-      const float3 f = bond_force(mic3(xi, f3(p.pos[r.x]), p.box),
-                                  tp.bond_coeff[r.y & 0x7fffffffu], e, w);
-      fx += f.x;
-      fy += f.y;
-      fz += f.z;
-      if (OBS && (r.y >> 31) != 0) {
-        sums.e_bond += e;
-        sums.w_bond += w;
-      }
-    }
-    for (uint32_t k = 0; k < nang; ++k) {
-      const uint4 r = tp.angle[static_cast<size_t>(k) * n + i];
-      const int role = static_cast<int>(r.w);
-      const float3 a = f3(p.pos[r.x]);
-      const float3 b = f3(p.pos[r.y]);
-      const float3 pi = role == 0 ? xi : a;
-      const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
-      const float3 pk = role == 2 ? xi : b;
-      double e, w;
-      const float3 f = angle_role_force(pi, pj, pk, tp.angle_coeff[r.z], role, p.box, e, w);
-      fx += f.x;
-      fy += f.y;
-      fz += f.z;
-      if (OBS && role == 0) {
-        sums.e_angle += e;
-        sums.w_angle += w;
-      }
-    }
-    for (uint32_t k = 0; k < ndih; ++k) {
-      const uint4 r = tp.dih[static_cast<size_t>(k) * n + i];
-      const int role = static_cast<int>(r.w >> 30);
-      const float3 a = f3(p.pos[r.x]);
-      const float3 b = f3(p.pos[r.y]);
-      const float3 c = f3(p.pos[r.z]);
-      float3 q0, q1, q2, q3;
-      if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
-      else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
-      else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
-      else { q0 = a; q1 = b; q2 = c; q3 = xi; }
-      double e, w;
-      const float3 f = dihedral_role_force(q0, q1, q2, q3, tp.dih_coeff[r.w & 0x3fffffffu],
-                                           role, p.box, e, w);
-      fx += f.x;
-      fy += f.y;
-      fz += f.z;
-      if (OBS && role == 0) {
-        sums.e_dih += e;
-        sums.w_dih += w;
-      }
-    }
+// Listed terms, atom-centric: atom i evaluates every term it belongs to and
+// keeps its own role's force (no atomics, deterministic; the reference adds
+// each term's forces with float atomics, bond_harmonic.cu, angle_harmonic.cu,
+// dihedral_harmonic.cu). A term's energy and virial are counted once, by role
+// 0 (bonds: the flag bit of the coefficient word).
+template <bool OBS>
+__device__ __forceinline__ float3 bond_term(const ForceParams& p, uint32_t i, float3 xi,
+                                            uint32_t k, StepSums& sums) {
+  const uint2 r = p.topo.bond[static_cast<size_t>(k) * p.n + i];
+  double e, w;
+  const float3 f =
+      bond_force(mic3(xi, f3(p.pos[r.x]), p.box), p.topo.bond_coeff[r.y & 0x7fffffffu], e, w);
+  if (OBS && (r.y >> 31) != 0) {
+    sums.e_bond += e;
+    sums.w_bond += w;
   }
+  return f;
+}
+
+template <bool OBS>
+__device__ __forceinline__ float3 angle_term(const ForceParams& p, uint32_t i, float3 xi,
+                                             uint32_t k, StepSums& sums) {
+  const uint4 r = p.topo.angle[static_cast<size_t>(k) * p.n + i];
+  const int role = static_cast<int>(r.w);
+  const float3 a = f3(p.pos[r.x]);
+  const float3 b = f3(p.pos[r.y]);
+  const float3 pi = role == 0 ? xi : a;
+  const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
+  const float3 pk = role == 2 ? xi : b;
+  double e, w;
+  const float3 f = angle_role_force(pi, pj, pk, p.topo.angle_coeff[r.z], role, p.box, e, w);
+  if (OBS && role == 0) {
+    sums.e_angle += e;
+    sums.w_angle += w;
+  }
+  return f;
+}
+
+template <bool OBS>
+__device__ __forceinline__ float3 dihedral_term(const ForceParams& p, uint32_t i, float3 xi,
+                                                uint32_t k, StepSums& sums) {
+  const uint4 r = p.topo.dih[static_cast<size_t>(k) * p.n + i];
+  const int role = static_cast<int>(r.w >> 30);
+  const float3 a = f3(p.pos[r.x]);
+  const float3 b = f3(p.pos[r.y]);
+  const float3 c = f3(p.pos[r.z]);
+  float3 q0, q1, q2, q3;
+  if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
+  else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
+  else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
+  else { q0 = a; q1 = b; q2 = c; q3 = xi; }
+  double e, w;
+  const float3 f = dihedral_role_force(q0, q1, q2, q3, p.topo.dih_coeff[r.w & 0x3fffffffu], role,
+                                       p.box, e, w);
+  if (OBS && role == 0) {
+    sums.e_dih += e;
+    sums.w_dih += w;
+  }
+  return f;
+}
+
+// The listed terms of atom i shared over its team: lane t takes terms t,
+// t + kTeam, ... of each kind and adds their forces to its FP64 partial sums.
+template <bool OBS>
+__device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, uint32_t t,
+                                            float4 xi4, double& fxd, double& fyd, double& fzd,
+                                            StepSums& sums) {
+  if (!p.topo.has_listed) {
+    return;
+  }
+  const uint32_t cw = p.topo.count[i];
+  const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
+  const float3 xi = f3(xi4);
+  for (uint32_t k = t; k < nbond; k += kTeam) {
+    const float3 f = bond_term<OBS>(p, i, xi, k, sums);
+    fxd += f.x;
+    fyd += f.y;
+    fzd += f.z;
+  }
+  for (uint32_t k = t; k < nang; k += kTeam) {
+    const float3 f = angle_term<OBS>(p, i, xi, k, sums);
+    fxd += f.x;
+    fyd += f.y;
+    fzd += f.z;
+  }
+  for (uint32_t k = t; k < ndih; k += kTeam) {
+    const float3 f = dihedral_term<OBS>(p, i, xi, k, sums);
+    fxd += f.x;
+    fyd += f.y;
+    fzd += f.z;
+  }
+}
+
+// Sum of a value over the kTeam lanes of a team (xor butterfly: every lane
+// ends with the same bits, since a + b == b + a exactly).
+__device__ __forceinline__ double team_sum(double v) {
+#pragma unroll
+  for (int m = kTeam / 2; m > 0; m >>= 1) {
+    v += __shfl_xor(v, m, kTeam);
+  }
+  return v;
+}
+
+// Atom i's total force (rounded once from FP64), the force store, optionally
+// the post-force half-kick (velocity_verlet.cu:97-111), and the atom's
+// kinetic energy. Run by one lane of the team.
+template <bool KICK>
+__device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, double fxd,
+                                            double fyd, double fzd, StepSums& sums) {
+  const float fx = static_cast<float>(fxd);
+  const float fy = static_cast<float>(fyd);
+  const float fz = static_cast<float>(fzd);
   p.force[i] = make_float4(fx, fy, fz, 0.f);
   float4 v = p.vel[i];
   if (KICK) {
@@ -550,12 +701,11 @@ __device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, fl
   sums.ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
 }
 
-// Workgroup sums into partials, then the last workgroup reduces them in FP64,
-// computes the Berendsen factor (berendsen_thermostat.cu:109-163) and clears
-// the per-step flags.
-template <bool KICK, bool OBS, bool HALF>
-__device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums& sums,
-                                            double* red_lds) {
+// Workgroup sums of one force pass: thread 0 writes partials[q * nbk + block]
+// (q = 0 kinetic energy; with OBS 1..4 energies, 5..8 virials).
+template <bool OBS>
+__device__ __forceinline__ void store_step_partials(const ForceParams& p, const StepSums& sums,
+                                                    double* red_lds) {
   const uint32_t nbk = gridDim.x;
   double* part = p.partials;
   double s = block_sum<kBlock>(sums.ke, red_lds);
@@ -569,16 +719,40 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
       if (threadIdx.x == 0) part[(q + 1) * static_cast<size_t>(nbk) + blockIdx.x] = s;
     }
   }
-  if (!arrive_last(&p.ctrl->arrive_force)) {
-    return;
+}
+
+// One workgroup after the force pass: FP64 totals of the nbk workgroups'
+// partials, the Berendsen factor (berendsen_thermostat.cu:109-163), and the
+// per-step flags cleared. A kernel of its own, so the force pass publishes
+// its partials at the kernel boundary instead of with a device-scope fence
+// (an L2 write-back and invalidate on gfx950) in every workgroup.
+constexpr int kReduceBlock = 1024;
+
+// This thread's share of v[0..n) for a kReduceBlock-wide sum: elements
+// threadIdx.x + m*kReduceBlock, four independent loads in flight.
+__device__ __forceinline__ double strided_sum(const double* v, uint32_t n) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  uint32_t b = threadIdx.x;
+  for (; b + 3 * kReduceBlock < n; b += 4 * kReduceBlock) {
+    a0 += v[b];
+    a1 += v[b + kReduceBlock];
+    a2 += v[b + 2 * kReduceBlock];
+    a3 += v[b + 3 * kReduceBlock];
   }
+  for (; b < n; b += kReduceBlock) {
+    a0 += v[b];
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
+template <bool KICK, bool OBS, bool HALF>
+__global__ __launch_bounds__(kReduceBlock) void step_reduce_kernel(ForceParams p, uint32_t nbk) {
+  __shared__ double red_lds[kReduceBlock / 64];
+  const double* part = p.partials;
   const int nq = OBS ? 9 : 1;
   for (int q = 0; q < nq; ++q) {
-    double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < nbk; b += kBlock) {
-      acc += load_coherent(part + q * static_cast<size_t>(nbk) + b);
-    }
-    acc = block_sum<kBlock>(acc, red_lds);
+    const double acc = block_sum<kReduceBlock>(strided_sum(part + q * static_cast<size_t>(nbk), nbk),
+                                               red_lds);
     if (threadIdx.x == 0) {
       Ctrl* ctrl = p.ctrl;
       if (q == 0) {
@@ -601,7 +775,6 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
   if (threadIdx.x == 0) {
     p.ctrl->rebuild = 0;
     p.ctrl->reorder = 0;
-    p.ctrl->arrive_force = 0;
     if (HALF) {
       p.ctrl->half_pair[0] = 0.0;
       p.ctrl->half_pair[1] = 0.0;
@@ -609,31 +782,52 @@ __device__ __forceinline__ void finish_step(const ForceParams& p, const StepSums
   }
 }
 
+// Fused force pass: LJ over the stored full list (pair_lj.cu:21-167), or the
+// half-list pair pass's sums (HALF), plus the listed terms, then optionally
+// the post-force half-kick (velocity_verlet.cu:97-111) and the kinetic-energy
+// reduction for the Berendsen factor (berendsen_thermostat.cu:109-163) as
+// per-workgroup partials, which step_reduce_kernel totals in FP64.
+//
+// Each atom has a team of kTeam adjacent lanes. Lane t reads list slots
+// base + u*kTeam + t (u < kTeamSlots), so one gather instruction fetches
+// kTeam consecutive entries of each of the wave's 64/kTeam lists: mostly
+// consecutive slots of one cell, a few cache lines per atom instead of one
+// line per lane. The team's FP64 partial sums meet in a shuffle butterfly and
+// the atom's total is rounded to binary32 once.
 template <bool KICK, bool OBS, int TYPES, bool HALF>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   __shared__ LjCoeff lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
   __shared__ double red_lds[4];
   if (!HALF) load_lj_table<TYPES>(p, lj_lds);
   const LjCoeff c0 = TYPES == 0 && !HALF ? p.lj[0] : LjCoeff{0.0, 0.0, 0.0, 0.0};
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const uint32_t t = threadIdx.x & (kTeam - 1);
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kForceAtoms + threadIdx.x / kTeam;
   StepSums sums;
-  if (i < p.n) {
+  if (i < p.n) {  // uniform over a team
     const float4 xi4 = p.pos[i];
     double fxd = 0.0, fyd = 0.0, fzd = 0.0;
     if (HALF) {
-      double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
-      fxd = acc[0];
-      fyd = acc[1];
-      fzd = acc[2];
-      acc[0] = 0.0;
-      acc[1] = 0.0;
-      acc[2] = 0.0;
+      if (t == 0) {
+        double* acc = p.pair_acc + 3 * static_cast<size_t>(i);
+        fxd = acc[0];
+        fyd = acc[1];
+        fzd = acc[2];
+        acc[0] = 0.0;
+        acc[1] = 0.0;
+        acc[2] = 0.0;
+      }
     } else {
-      lj_list<OBS, TYPES, false>(p, i, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
+      lj_team<OBS, TYPES>(p, i, t, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
     }
-    finish_atom<KICK, OBS>(p, i, xi4, fxd, fyd, fzd, sums);
+    listed_team<OBS>(p, i, t, xi4, fxd, fyd, fzd, sums);
+    fxd = team_sum(fxd);
+    fyd = team_sum(fyd);
+    fzd = team_sum(fzd);
+    if (t == 0) {
+      finish_atom<KICK>(p, i, fxd, fyd, fzd, sums);
+    }
   }
-  finish_step<KICK, OBS, HALF>(p, sums, red_lds);
+  store_step_partials<OBS>(p, sums, red_lds);
 }
 
 // Apply the pending Berendsen factor and reduce the kinetic energy of the
@@ -658,18 +852,15 @@ __global__ __launch_bounds__(kBlock) void scale_kernel(ScaleParams p) {
   }
   const double s = block_sum<kBlock>(ke, red_lds);
   if (threadIdx.x == 0) p.partials[blockIdx.x] = s;
-  if (!arrive_last(&p.ctrl->arrive_scale)) {
-    return;
-  }
-  double acc = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-    acc += load_coherent(p.partials + b);
-  }
-  acc = block_sum<kBlock>(acc, red_lds);
+}
+
+// One workgroup after scale_kernel: FP64 total of its nbk partials.
+__global__ __launch_bounds__(kReduceBlock) void scale_reduce_kernel(ScaleParams p, uint32_t nbk) {
+  __shared__ double red_lds[kReduceBlock / 64];
+  const double acc = block_sum<kReduceBlock>(strided_sum(p.partials, nbk), red_lds);
   if (threadIdx.x == 0) {
     p.ctrl->ke = acc;
     p.ctrl->lambda = 1.0;
-    p.ctrl->arrive_scale = 0;
   }
 }
 
@@ -757,7 +948,7 @@ void launch_cell_scatter(const CellScanParams& p, hipStream_t stream) {
 }
 
 void launch_cell_sort(const CellScanParams& p, hipStream_t stream) {
-  cell_sort_kernel<<<grid_for(p.n_cells), kBlock, 0, stream>>>(p);
+  cell_sort_kernel<<<grid_for(p.n_cells, kSortCellsPerBlock), kBlock, 0, stream>>>(p);
 }
 
 void launch_reorder(const ReorderParams& p, hipStream_t stream) {
@@ -799,11 +990,17 @@ void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream)
     BEADS_HALF_CASE(true, 2)
 #undef BEADS_HALF_CASE
   }
-#define BEADS_FORCE_CASE(K, O, T)                                        \
-  if (kick == K && obs == O && (half || types == T)) {                   \
-    if (half) force_kernel<K, O, 0, true><<<g, kBlock, 0, stream>>>(p);  \
-    else force_kernel<K, O, T, false><<<g, kBlock, 0, stream>>>(p);      \
-    return;                                                              \
+  const unsigned gf = force_grid(p.n);
+#define BEADS_FORCE_CASE(K, O, T)                                             \
+  if (kick == K && obs == O && (half || types == T)) {                        \
+    if (half) {                                                               \
+      force_kernel<K, O, 0, true><<<gf, kBlock, 0, stream>>>(p);              \
+      step_reduce_kernel<K, O, true><<<1, kReduceBlock, 0, stream>>>(p, gf);  \
+    } else {                                                                  \
+      force_kernel<K, O, T, false><<<gf, kBlock, 0, stream>>>(p);             \
+      step_reduce_kernel<K, O, false><<<1, kReduceBlock, 0, stream>>>(p, gf); \
+    }                                                                         \
+    return;                                                                   \
   }
   BEADS_FORCE_CASE(true, false, 0)
   BEADS_FORCE_CASE(true, false, 1)
@@ -833,8 +1030,10 @@ void launch_integrate(const IntegrateParams& p, bool thermostat, bool check,
 }
 
 void launch_scale(const ScaleParams& p, bool thermostat, hipStream_t stream) {
-  if (thermostat) scale_kernel<true><<<grid_for(p.n), kBlock, 0, stream>>>(p);
-  else scale_kernel<false><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+  const unsigned g = grid_for(p.n);
+  if (thermostat) scale_kernel<true><<<g, kBlock, 0, stream>>>(p);
+  else scale_kernel<false><<<g, kBlock, 0, stream>>>(p);
+  scale_reduce_kernel<<<1, kReduceBlock, 0, stream>>>(p, g);
 }
 
 }  // namespace beads_amd

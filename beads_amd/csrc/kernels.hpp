@@ -42,6 +42,15 @@ inline unsigned grid_for(uint32_t n, int block = kBlock) {
   return static_cast<unsigned>((static_cast<uint64_t>(n) + block - 1) / block);
 }
 
+// The force pass gives every atom a team of kTeam adjacent lanes; each lane
+// takes kTeamSlots neighbour slots per round. A workgroup covers kForceAtoms
+// atoms.
+constexpr int kTeam = 4;
+constexpr int kTeamSlots = 8;
+constexpr int kForceAtoms = kBlock / kTeam;
+
+inline unsigned force_grid(uint32_t n) { return grid_for(n, kForceAtoms); }
+
 // Neighbour-rebuild sequence; every kernel is predicated on ctrl->rebuild so
 // the host never waits for the displacement check.
 void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, hipStream_t stream);
