@@ -140,13 +140,14 @@ __device__ __forceinline__ CellNeighborhood neighborhood(uint32_t c, const CellG
 // LJ pair value in FP64 from an exact binary32 displacement (FP64 VALU runs at
 // the non-packed FP32 rate on CDNA4; binary32 would amplify the rounding of r^2
 // six-fold through (sigma^2/r^2)^6). Returns the force scale s with
-// f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift.
+// f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift. 1/r^2 is the hardware
+// FP64 reciprocal (measured on gfx950: relative error <= 4.6e-8 over
+// [0.25, 4)) refined by one Newton step (measured <= 2.3e-15).
 __device__ __forceinline__ double lj_pair(float dx, float dy, float dz, const LjCoeff& c,
                                           double& rr, double& energy) {
   const double ddx = dx, ddy = dy, ddz = dz;
   rr = ddx * ddx + ddy * ddy + ddz * ddz;
   double inv = __builtin_amdgcn_rcp(rr);
-  inv = inv * (2.0 - rr * inv);
   inv = inv * (2.0 - rr * inv);
   const double sr2 = c.sigma2 * inv;
   const double sr6 = sr2 * sr2 * sr2;

@@ -270,37 +270,44 @@ __global__ __launch_bounds__(kBlock) void reorder_commit_kernel(ReorderParams p)
   }
 }
 
+constexpr int kBuildTeam = 4;
+constexpr int kBuildSlots = 4;
+constexpr uint32_t kBuildAtoms = kBlock / kBuildTeam;
+
 // Full-list build with the append rule of full_neighbor_path_common.cuh:49-94:
 // skip self, skip excluded, keep r2 <= r_s^2 (reference minimum image and
-// operation order), overflow past max_neighbors. Candidates come four at a
-// time so their position loads overlap. Appends are staged per work-item in
-// LDS and written as whole 64-byte chunks. EXCL: 0 = no exclusions, 1 = up to
-// 8 partners held in registers, 2 = any count read from memory. Also captures
-// the positions the displacement check compares against. HALF keeps only
-// partners with a higher slot, so every pair is stored once.
-constexpr uint32_t kStageStride = kNbrChunk + 4;  // 80-byte rows: 16-byte aligned
-
+// operation order), overflow past max_neighbors. Each atom has a team of
+// kBuildTeam adjacent lanes; lane t takes candidates k + u*kBuildTeam + t
+// (u < kBuildSlots) of every candidate run, so the team's id and position
+// loads are consecutive entries. Accepted candidates are appended in
+// candidate order (the order of a one-lane scan) through a ballot of the
+// team's lanes and written straight into the atom's chunks. EXCL: 0 = no
+// exclusions, 4 or 8 = up to that many partners held in registers (every
+// candidate within r_s is compared with all of them), -1 = any count read
+// from memory. Also captures the positions the displacement check compares
+// against. HALF keeps only partners with a higher slot, so every pair is
+// stored once.
 template <int EXCL, bool HALF>
 __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
   if (p.ctrl->rebuild == 0) {
     return;
   }
   __shared__ double red_lds[4];
-  __shared__ __align__(16) uint32_t stage[kBlock * kStageStride];
-  uint32_t* row = stage + threadIdx.x * kStageStride;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  uint32_t count = 0;
-  bool overflowed = false;
-  if (i < p.n) {
+  const uint32_t t = threadIdx.x & (kBuildTeam - 1);
+  const uint32_t team_lane0 = (threadIdx.x & 63u) & ~static_cast<uint32_t>(kBuildTeam - 1);
+  const uint32_t below = (1u << t) - 1u;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBuildAtoms + threadIdx.x / kBuildTeam;
+  uint32_t count = 0;  // the same in every lane of a team
+  if (i < p.n) {       // uniform over a team
     const float4 xi = p.pos[i];
     const uint32_t n = p.n;
-    uint32_t ex[8];
+    uint32_t ex[EXCL > 0 ? EXCL : 1];
     uint32_t ne = 0;
     if (EXCL != 0) {
       ne = p.topo_count[i] >> 24;
-      if (EXCL == 1) {
+      if (EXCL > 0) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < EXCL; ++k) {
           ex[k] = k < static_cast<int>(ne) ? p.excl[static_cast<size_t>(k) * n + i] : 0xffffffffu;
         }
       }
@@ -308,56 +315,56 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
     const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     const uint32_t maxn = p.max_neighbors;
-    auto flush = [&](uint32_t chunk) {
-      const uint4* src = reinterpret_cast<const uint4*>(row);
-      uint4* dst = reinterpret_cast<uint4*>(p.nbr + nbr_chunk_base(chunk, i, n));
-      dst[0] = src[0];
-      dst[1] = src[1];
-      dst[2] = src[2];
-      dst[3] = src[3];
-    };
-    auto test = [&](uint32_t j, float4 xj) {
+    auto accept = [&](uint32_t j, float4 xj) -> bool {
       const float dx = mic(xi.x - xj.x, L0, h0);
       const float dy = mic(xi.y - xj.y, L1, h1);
       const float dz = mic(xi.z - xj.z, L2, h2);
       const float r2 = r2_strict(dx, dy, dz);
       if (r2 > p.rs2 || (HALF ? j <= i : j == i)) {
-        return;
+        return false;
       }
-      if (EXCL == 1) {
+      if (EXCL > 0) {
         bool hit = false;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < EXCL; ++k) {
           hit |= ex[k] == j;
         }
-        if (hit) return;
-      } else if (EXCL == 2) {
+        return !hit;
+      } else if (EXCL < 0) {
         for (uint32_t k = 0; k < ne; ++k) {
-          if (p.excl[static_cast<size_t>(k) * n + i] == j) return;
+          if (p.excl[static_cast<size_t>(k) * n + i] == j) return false;
         }
       }
-      if (count < maxn) {
-        row[count & (kNbrChunk - 1)] = j;
-        if ((count & (kNbrChunk - 1)) == kNbrChunk - 1) flush(count / kNbrChunk);
-        ++count;
-      } else {
-        overflowed = true;
-      }
+      return true;
     };
-    // Candidates [b, e) of a list given by ids(k); four position loads in flight.
+    // Candidates [b, e) of a list given by ids(k), kBuildTeam * kBuildSlots
+    // per round; slots past e load atom i itself and test nothing.
     auto scan_range = [&](uint32_t b, uint32_t e, auto ids) {
-      uint32_t k = b;
-      for (; k + 4 <= e; k += 4) {
-        const uint32_t j0 = ids(k), j1 = ids(k + 1), j2 = ids(k + 2), j3 = ids(k + 3);
-        const float4 x0 = p.pos[j0], x1 = p.pos[j1], x2 = p.pos[j2], x3 = p.pos[j3];
-        test(j0, x0);
-        test(j1, x1);
-        test(j2, x2);
-        test(j3, x3);
-      }
-      for (; k < e; ++k) {
-        const uint32_t j = ids(k);
-        test(j, p.pos[j]);
+      for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
+        uint32_t j[kBuildSlots];
+#pragma unroll
+        for (int u = 0; u < kBuildSlots; ++u) {
+          const uint32_t s = k + u * kBuildTeam + t;
+          j[u] = s < e ? ids(s) : i;
+        }
+        float4 x[kBuildSlots];
+#pragma unroll
+        for (int u = 0; u < kBuildSlots; ++u) {
+          x[u] = p.pos[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kBuildSlots; ++u) {
+          const bool ok = k + u * kBuildTeam + t < e && accept(j[u], x[u]);
+          const uint32_t bits =
+              static_cast<uint32_t>(__ballot(ok) >> team_lane0) & ((1u << kBuildTeam) - 1u);
+          if (ok) {
+            const uint32_t slot = count + __builtin_popcount(bits & below);
+            if (slot < maxn) {
+              p.nbr[nbr_chunk_base(slot / kNbrChunk, i, n) + (slot & (kNbrChunk - 1))] = j[u];
+            }
+          }
+          count += __builtin_popcount(bits);
+        }
       }
     };
     if (p.grid.path == 0) {
@@ -383,19 +390,20 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
       }
     }
-    if ((count & (kNbrChunk - 1)) != 0) {
-      flush(count / kNbrChunk);  // partial last chunk; entries past count are never read
+    if (t == 0) {
+      p.nbr_count[i] = min(count, maxn);
+      p.ref_pos[i] = xi;
+      p.ref_image[i] = p.image[i];
     }
-    p.nbr_count[i] = count;
-    p.ref_pos[i] = xi;
-    p.ref_image[i] = p.image[i];
   }
-  const double total = block_sum<kBlock>(static_cast<double>(count), red_lds);
+  const bool counted = t == 0 && i < p.n;
+  const uint32_t stored = counted ? min(count, p.max_neighbors) : 0u;
+  const double total = block_sum<kBlock>(static_cast<double>(stored), red_lds);
   if (threadIdx.x == 0) {
     atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctrl->neighbor_total),
               static_cast<unsigned long long>(total));
   }
-  if (overflowed) {
+  if (counted && count > p.max_neighbors) {
     atomicOr(&p.ctrl->overflow, 1u);
   }
 }
@@ -957,22 +965,21 @@ void launch_reorder(const ReorderParams& p, hipStream_t stream) {
 }
 
 void launch_build(const BuildParams& p, hipStream_t stream) {
-  const unsigned g = grid_for(p.n);
-  if (p.half) {
-    if (p.ecap == 0) {
-      build_kernel<0, true><<<g, kBlock, 0, stream>>>(p);
-    } else if (p.ecap <= 8) {
-      build_kernel<1, true><<<g, kBlock, 0, stream>>>(p);
-    } else {
-      build_kernel<2, true><<<g, kBlock, 0, stream>>>(p);
-    }
-  } else if (p.ecap == 0) {
-    build_kernel<0, false><<<g, kBlock, 0, stream>>>(p);
-  } else if (p.ecap <= 8) {
-    build_kernel<1, false><<<g, kBlock, 0, stream>>>(p);
-  } else {
-    build_kernel<2, false><<<g, kBlock, 0, stream>>>(p);
-  }
+  const unsigned g = grid_for(p.n, kBuildAtoms);
+  // Exclusion registers: the fewest of 4 / 8 that hold every slot's
+  // exclusions (ecap is the largest count), else read from memory.
+  const int ex = p.ecap == 0 ? 0 : (p.ecap <= 4 ? 4 : (p.ecap <= 8 ? 8 : -1));
+#define BEADS_BUILD_CASE(E, H) \
+  if (ex == E && (p.half != 0) == H) build_kernel<E, H><<<g, kBlock, 0, stream>>>(p);
+  BEADS_BUILD_CASE(0, false)
+  BEADS_BUILD_CASE(4, false)
+  BEADS_BUILD_CASE(8, false)
+  BEADS_BUILD_CASE(-1, false)
+  BEADS_BUILD_CASE(0, true)
+  BEADS_BUILD_CASE(4, true)
+  BEADS_BUILD_CASE(8, true)
+  BEADS_BUILD_CASE(-1, true)
+#undef BEADS_BUILD_CASE
 }
 
 void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream) {
