@@ -82,9 +82,16 @@ __global__ __launch_bounds__(kBlock) void pack_xyz_kernel(uint32_t n, const floa
   out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
 }
 
+// Lanes per atom and list slots per lane per round of the backward pass (the
+// force pass's team layout: lane t takes slots base + u*kAdjTeam + t).
+constexpr int kAdjTeam = 4;
+constexpr int kAdjSlots = 4;
+constexpr uint32_t kAdjAtoms = kBlock / kAdjTeam;
+
 // TREG > 0: the atom's row of the type table (at most TREG types) is
-// accumulated in registers and flushed once per atom; TREG == 0: every
+// accumulated in registers and flushed once per lane; TREG == 0: every
 // interaction adds into LDS directly (any type count up to kMaxLdsTypes).
+// Each atom's team sums its partial dL/dx in FP64 before the store.
 template <int TREG>
 __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
   __shared__ double acc[2 * kMaxLdsTypes * kMaxLdsTypes + 2 * kMaxBondTypesAdjoint];
@@ -94,120 +101,143 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
   for (int k = threadIdx.x; k < n_acc; k += kBlock) acc[k] = 0.0;
   __syncthreads();
   const uint32_t n = p.n;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const uint32_t t = threadIdx.x & (kAdjTeam - 1);
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kAdjAtoms + threadIdx.x / kAdjTeam;
   const double gE = p.grad_energy;
-  if (i < n) {
+  if (i < n) {  // uniform over a team
     const float4 xi = p.pos[i];
     const int ti = __float_as_int(xi.w);
     const float4 gi = p.g[i];
     const double gix = gi.x, giy = gi.y, giz = gi.z;
     const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
     const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
-    const double inv_rc2 = 1.0 / p.rc2_f64;
+    const double inv_rc2 = rcp_f64(p.rc2_f64);
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double re[TREG > 0 ? TREG : 1], rs[TREG > 0 ? TREG : 1];
 #pragma unroll
-    for (int t = 0; t < (TREG > 0 ? TREG : 1); ++t) re[t] = rs[t] = 0.0;
+    for (int q = 0; q < (TREG > 0 ? TREG : 1); ++q) re[q] = rs[q] = 0.0;
     const uint32_t cnt = p.nbr_count[i];
-    for (uint32_t k = 0; k < cnt; ++k) {
-      const uint32_t j = p.nbr[nbr_chunk_base(k / kNbrChunk, i, n) + (k & (kNbrChunk - 1))];
-      const float4 xj = p.pos[j];
-      const float dx = mic_diff(xi.x, xj.x, L0, h0);
-      const float dy = mic_diff(xi.y, xj.y, L1, h1);
-      const float dz = mic_diff(xi.z, xj.z, L2, h2);
-      if (r2_strict(dx, dy, dz) >= p.rc2) continue;
-      const int tj = __float_as_int(xj.w);
-      const double eps = p.eps[ti * T + tj];
-      const double sig = p.sigma[ti * T + tj];
-      const double ddx = dx, ddy = dy, ddz = dz;
-      const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      const double u = 1.0 / r2;
-      const double sig2 = sig * sig;
-      const double sr2 = sig2 * u;
-      const double sr6 = sr2 * sr2 * sr2;
-      const double sr12 = sr6 * sr6;
-      const double s = 24.0 * eps * u * (2.0 * sr12 - sr6);
-      const double sp = -24.0 * eps * u * u * (14.0 * sr12 - 4.0 * sr6);
-      const float4 gj = p.g[j];
-      const double ex = gix - gj.x, ey = giy - gj.y, ez = giz - gj.z;
-      const double dd = ddx * ex + ddy * ey + ddz * ez;
-      gx += s * ex + 2.0 * sp * ddx * dd;
-      gy += s * ey + 2.0 * sp * ddy * dd;
-      gz += s * ez + 2.0 * sp * ddz * dd;
-      fx += s * ddx;
-      fy += s * ddy;
-      fz += s * ddz;
-      const double w = gix * ddx + giy * ddy + giz * ddz;
-      // dF/deps and dF/dsigma terms, plus half of dE/deps and dE/dsigma.
-      double de = w * 24.0 * u * (2.0 * sr12 - sr6);
-      double ds = w * 24.0 * eps * u / sig * (24.0 * sr12 - 6.0 * sr6);
-      if (gE != 0.0) {
-        double e_eps = 4.0 * (sr12 - sr6);
-        double e_sig = 4.0 * eps * (12.0 * sr12 - 6.0 * sr6) / sig;
-        if (p.lj_shift) {
-          const double sc2 = sig2 * inv_rc2;
-          const double sc6 = sc2 * sc2 * sc2;
-          const double sc12 = sc6 * sc6;
-          e_eps -= 4.0 * (sc12 - sc6);
-          e_sig -= 4.0 * eps * (12.0 * sc12 - 6.0 * sc6) / sig;
-        }
-        de += 0.5 * gE * e_eps;
-        ds += 0.5 * gE * e_sig;
-      }
-      if constexpr (TREG > 0) {
+    for (uint32_t base = 0; base < cnt; base += kAdjTeam * kAdjSlots) {
+      uint32_t jj[kAdjSlots];
 #pragma unroll
-        for (int t = 0; t < TREG; ++t) {
-          if (tj == t) {
-            re[t] += de;
-            rs[t] += ds;
+      for (int u = 0; u < kAdjSlots; ++u) {
+        const uint32_t s = base + u * kAdjTeam + t;
+        jj[u] = s < cnt ? p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))] : i;
+      }
+      float4 xj[kAdjSlots], gj[kAdjSlots];
+#pragma unroll
+      for (int u = 0; u < kAdjSlots; ++u) {
+        xj[u] = p.pos[jj[u]];
+        gj[u] = p.g[jj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kAdjSlots; ++u) {
+        const float dx = mic_diff(xi.x, xj[u].x, L0, h0);
+        const float dy = mic_diff(xi.y, xj[u].y, L1, h1);
+        const float dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        if (base + u * kAdjTeam + t >= cnt || r2_strict(dx, dy, dz) >= p.rc2) continue;
+        const int tj = __float_as_int(xj[u].w);
+        const double eps = p.eps[ti * T + tj];
+        const double sig = p.sigma[ti * T + tj];
+        const double inv_sig = rcp_f64(sig);
+        const double ddx = dx, ddy = dy, ddz = dz;
+        const double u2 = rcp_f64(ddx * ddx + ddy * ddy + ddz * ddz);
+        const double sig2 = sig * sig;
+        const double sr2 = sig2 * u2;
+        const double sr6 = sr2 * sr2 * sr2;
+        const double sr12 = sr6 * sr6;
+        const double s = 24.0 * eps * u2 * (2.0 * sr12 - sr6);
+        const double sp = -24.0 * eps * u2 * u2 * (14.0 * sr12 - 4.0 * sr6);
+        const double ex = gix - gj[u].x, ey = giy - gj[u].y, ez = giz - gj[u].z;
+        const double dd = ddx * ex + ddy * ey + ddz * ez;
+        gx += s * ex + 2.0 * sp * ddx * dd;
+        gy += s * ey + 2.0 * sp * ddy * dd;
+        gz += s * ez + 2.0 * sp * ddz * dd;
+        fx += s * ddx;
+        fy += s * ddy;
+        fz += s * ddz;
+        const double w = gix * ddx + giy * ddy + giz * ddz;
+        // dF/deps and dF/dsigma terms, plus half of dE/deps and dE/dsigma.
+        double de = w * 24.0 * u2 * (2.0 * sr12 - sr6);
+        double ds = w * 24.0 * eps * u2 * inv_sig * (24.0 * sr12 - 6.0 * sr6);
+        if (gE != 0.0) {
+          double e_eps = 4.0 * (sr12 - sr6);
+          double e_sig = 4.0 * eps * (12.0 * sr12 - 6.0 * sr6) * inv_sig;
+          if (p.lj_shift) {
+            const double sc2 = sig2 * inv_rc2;
+            const double sc6 = sc2 * sc2 * sc2;
+            const double sc12 = sc6 * sc6;
+            e_eps -= 4.0 * (sc12 - sc6);
+            e_sig -= 4.0 * eps * (12.0 * sc12 - 6.0 * sc6) * inv_sig;
           }
+          de += 0.5 * gE * e_eps;
+          ds += 0.5 * gE * e_sig;
         }
-      } else {
-        atomicAdd(&acc[ti * T + tj], de);
-        atomicAdd(&acc[n_pair + ti * T + tj], ds);
+        if constexpr (TREG > 0) {
+#pragma unroll
+          for (int q = 0; q < TREG; ++q) {
+            if (tj == q) {
+              re[q] += de;
+              rs[q] += ds;
+            }
+          }
+        } else {
+          atomicAdd(&acc[ti * T + tj], de);
+          atomicAdd(&acc[n_pair + ti * T + tj], ds);
+        }
       }
     }
     if constexpr (TREG > 0) {
 #pragma unroll
-      for (int t = 0; t < TREG; ++t) {
-        if (t < T) {
-          if (re[t] != 0.0) atomicAdd(&acc[ti * T + t], re[t]);
-          if (rs[t] != 0.0) atomicAdd(&acc[n_pair + ti * T + t], rs[t]);
+      for (int q = 0; q < TREG; ++q) {
+        if (q < T) {
+          if (re[q] != 0.0) atomicAdd(&acc[ti * T + q], re[q]);
+          if (rs[q] != 0.0) atomicAdd(&acc[n_pair + ti * T + q], rs[q]);
         }
       }
     }
     if (p.bcap > 0) {
       const uint32_t nbond = p.topo_count[i] & 255u;
       const float3 x3 = f3(xi);
-      for (uint32_t k = 0; k < nbond; ++k) {
+      for (uint32_t k = t; k < nbond; k += kAdjTeam) {
         const uint2 r = p.bond[static_cast<size_t>(k) * n + i];
-        const uint32_t t = r.y & 0x7fffffffu;
-        const double kk = p.bond_k[t], r0 = p.bond_r0[t];
+        const uint32_t bt = r.y & 0x7fffffffu;
+        const double kk = p.bond_k[bt], r0 = p.bond_r0[bt];
         const float3 d = mic3(x3, f3(p.pos[r.x]), p.box);
         const double ddx = d.x, ddy = d.y, ddz = d.z;
         const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
         if (rr == 0.0) continue;
-        const double phi = -kk * (rr - r0) / rr;
-        const double dphi = -kk * r0 / (rr * rr);
+        const double inv_r = rcp_f64(rr);
+        const double phi = -kk * (rr - r0) * inv_r;
+        const double dphi_r = -kk * r0 * inv_r * inv_r * inv_r;  // phi' / r
         const float4 gp = p.g[r.x];
         const double ex = gix - gp.x, ey = giy - gp.y, ez = giz - gp.z;
         const double dd = ddx * ex + ddy * ey + ddz * ez;
-        gx += phi * ex + dphi / rr * ddx * dd;
-        gy += phi * ey + dphi / rr * ddy * dd;
-        gz += phi * ez + dphi / rr * ddz * dd;
+        gx += phi * ex + dphi_r * ddx * dd;
+        gy += phi * ey + dphi_r * ddy * dd;
+        gz += phi * ez + dphi_r * ddz * dd;
         fx += phi * ddx;
         fy += phi * ddy;
         fz += phi * ddz;
         const double w = gix * ddx + giy * ddy + giz * ddz;
         const double stretch = rr - r0;
-        atomicAdd(&acc[2 * n_pair + t], w * (-stretch / rr) + gE * 0.25 * stretch * stretch);
-        atomicAdd(&acc[2 * n_pair + p.n_bond_types + t], w * (kk / rr) - gE * 0.5 * kk * stretch);
+        atomicAdd(&acc[2 * n_pair + bt], w * (-stretch * inv_r) + gE * 0.25 * stretch * stretch);
+        atomicAdd(&acc[2 * n_pair + p.n_bond_types + bt],
+                  w * (kk * inv_r) - gE * 0.5 * kk * stretch);
       }
     }
-    p.grad_pos[3 * i] = static_cast<float>(gx - gE * fx);
-    p.grad_pos[3 * i + 1] = static_cast<float>(gy - gE * fy);
-    p.grad_pos[3 * i + 2] = static_cast<float>(gz - gE * fz);
+    gx = team_sum<kAdjTeam>(gx);
+    gy = team_sum<kAdjTeam>(gy);
+    gz = team_sum<kAdjTeam>(gz);
+    fx = team_sum<kAdjTeam>(fx);
+    fy = team_sum<kAdjTeam>(fy);
+    fz = team_sum<kAdjTeam>(fz);
+    if (t == 0) {
+      p.grad_pos[3 * i] = static_cast<float>(gx - gE * fx);
+      p.grad_pos[3 * i + 1] = static_cast<float>(gy - gE * fy);
+      p.grad_pos[3 * i + 2] = static_cast<float>(gz - gE * fz);
+    }
   }
   __syncthreads();
   for (int k = threadIdx.x; k < n_acc; k += kBlock) {
@@ -234,12 +264,13 @@ void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stre
 }
 
 void launch_fm_backward(const AdjointParams& p, hipStream_t stream) {
+  const unsigned g = grid_for(p.n, kAdjAtoms);
   if (p.n_types <= 1) {
-    fm_backward_kernel<1><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+    fm_backward_kernel<1><<<g, kBlock, 0, stream>>>(p);
   } else if (p.n_types <= 4) {
-    fm_backward_kernel<4><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+    fm_backward_kernel<4><<<g, kBlock, 0, stream>>>(p);
   } else {
-    fm_backward_kernel<0><<<grid_for(p.n), kBlock, 0, stream>>>(p);
+    fm_backward_kernel<0><<<g, kBlock, 0, stream>>>(p);
   }
 }
 

@@ -52,6 +52,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sum of a value over an aligned team of TEAM adjacent lanes (xor butterfly:
+// every lane ends with the same bits, since a + b == b + a exactly).
+template <int TEAM>
+__device__ __forceinline__ double team_sum(double v) {
+#pragma unroll
+  for (int m = TEAM / 2; m > 0; m >>= 1) {
+    v += __shfl_xor(v, m, TEAM);
+  }
+  return v;
+}
+
+// 1/x in FP64: the hardware reciprocal (relative error <= 4.6e-8 measured on
+// gfx950) refined by one Newton step (<= 2.3e-15).
+__device__ __forceinline__ double rcp_f64(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return r * (2.0 - x * r);
+}
+
 // Sum over a BLOCK-wide workgroup (BLOCK a multiple of 64, at most 1024);
 // result valid in thread 0. lds needs BLOCK / 64 doubles.
 template <int BLOCK>
@@ -140,15 +158,12 @@ __device__ __forceinline__ CellNeighborhood neighborhood(uint32_t c, const CellG
 // LJ pair value in FP64 from an exact binary32 displacement (FP64 VALU runs at
 // the non-packed FP32 rate on CDNA4; binary32 would amplify the rounding of r^2
 // six-fold through (sigma^2/r^2)^6). Returns the force scale s with
-// f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift. 1/r^2 is the hardware
-// FP64 reciprocal (measured on gfx950: relative error <= 4.6e-8 over
-// [0.25, 4)) refined by one Newton step (measured <= 2.3e-15).
+// f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift.
 __device__ __forceinline__ double lj_pair(float dx, float dy, float dz, const LjCoeff& c,
                                           double& rr, double& energy) {
   const double ddx = dx, ddy = dy, ddz = dz;
   rr = ddx * ddx + ddy * ddy + ddz * ddz;
-  double inv = __builtin_amdgcn_rcp(rr);
-  inv = inv * (2.0 - rr * inv);
+  const double inv = rcp_f64(rr);
   const double sr2 = c.sigma2 * inv;
   const double sr6 = sr2 * sr2 * sr2;
   const double sr12 = sr6 * sr6;

@@ -677,16 +677,6 @@ __device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, ui
   }
 }
 
-// Sum of a value over the kTeam lanes of a team (xor butterfly: every lane
-// ends with the same bits, since a + b == b + a exactly).
-__device__ __forceinline__ double team_sum(double v) {
-#pragma unroll
-  for (int m = kTeam / 2; m > 0; m >>= 1) {
-    v += __shfl_xor(v, m, kTeam);
-  }
-  return v;
-}
-
 // Atom i's total force (rounded once from FP64), the force store, optionally
 // the post-force half-kick (velocity_verlet.cu:97-111), and the atom's
 // kinetic energy. Run by one lane of the team.
@@ -828,9 +818,9 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
       lj_team<OBS, TYPES>(p, i, t, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
     }
     listed_team<OBS>(p, i, t, xi4, fxd, fyd, fzd, sums);
-    fxd = team_sum(fxd);
-    fyd = team_sum(fyd);
-    fzd = team_sum(fzd);
+    fxd = team_sum<kTeam>(fxd);
+    fyd = team_sum<kTeam>(fyd);
+    fzd = team_sum<kTeam>(fzd);
     if (t == 0) {
       finish_atom<KICK>(p, i, fxd, fyd, fzd, sums);
     }
