@@ -22,6 +22,10 @@ namespace {
 
 using namespace dev;
 
+// Force-pass team layout inside a replica's workgroup.
+constexpr int kReplicaTeam = 4;
+constexpr int kReplicaSlots = 4;
+
 template <int BLOCK>
 struct ReplicaShared {
   double red[BLOCK / 64];
@@ -203,27 +207,51 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
   const float h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
   const int T = d.n_types;
   const LjCoeff c0 = lj[0];
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+  // A team of kReplicaTeam adjacent lanes per atom: lane t takes list slots
+  // base + u*kReplicaTeam + t, kReplicaSlots of them in flight per round; the
+  // team's FP64 partial forces meet in a butterfly and lane 0 stores.
+  const uint32_t t = threadIdx.x & (kReplicaTeam - 1);
+  for (uint32_t i = threadIdx.x / kReplicaTeam; i < n; i += BLOCK / kReplicaTeam) {
     const float4 xi = pos[i];
     const int ti = __float_as_int(xi.w);
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
     const uint32_t cnt = nbr_count[i];
-    for (uint32_t k = 0; k < cnt; ++k) {
-      const uint32_t j = nbr[static_cast<size_t>(k) * n + i];
-      const float4 xj = pos[j];
-      const float dx = mic_diff(xi.x, xj.x, L0, h0);
-      const float dy = mic_diff(xi.y, xj.y, L1, h1);
-      const float dz = mic_diff(xi.z, xj.z, L2, h2);
-      if (r2_strict(dx, dy, dz) < d.rc2) {
-        const LjCoeff c = single ? c0 : lj[ti * T + __float_as_int(xj.w)];
-        double rr, u;
-        const double s = lj_pair(dx, dy, dz, c, rr, u);
-        fx += s * dx;
-        fy += s * dy;
-        fz += s * dz;
-        e += u;
-        w += s * rr;
+    for (uint32_t base = 0; base < cnt; base += kReplicaTeam * kReplicaSlots) {
+      uint32_t jj[kReplicaSlots];
+#pragma unroll
+      for (int u = 0; u < kReplicaSlots; ++u) {
+        const uint32_t s = base + u * kReplicaTeam + t;
+        jj[u] = s < cnt ? nbr[static_cast<size_t>(s) * n + i] : i;
       }
+      float4 xj[kReplicaSlots];
+#pragma unroll
+      for (int u = 0; u < kReplicaSlots; ++u) {
+        xj[u] = pos[jj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kReplicaSlots; ++u) {
+        const float dx = mic_diff(xi.x, xj[u].x, L0, h0);
+        const float dy = mic_diff(xi.y, xj[u].y, L1, h1);
+        const float dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        if (base + u * kReplicaTeam + t < cnt && r2_strict(dx, dy, dz) < d.rc2) {
+          const LjCoeff c = single ? c0 : lj[ti * T + __float_as_int(xj[u].w)];
+          double rr, uu;
+          const double s = lj_pair(dx, dy, dz, c, rr, uu);
+          fx += s * dx;
+          fy += s * dy;
+          fz += s * dz;
+          e += uu;
+          w += s * rr;
+        }
+      }
+    }
+    fx = team_sum<kReplicaTeam>(fx);
+    fy = team_sum<kReplicaTeam>(fy);
+    fz = team_sum<kReplicaTeam>(fz);
+    pe += 0.5 * e;
+    vir += 0.5 * w;
+    if (t != 0) {
+      continue;
     }
     const float ffx = static_cast<float>(fx), ffy = static_cast<float>(fy),
                 ffz = static_cast<float>(fz);
@@ -238,8 +266,6 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
     }
     const double vx = v.x, vy = v.y, vz = v.z;
     ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
-    pe += 0.5 * e;
-    vir += 0.5 * w;
   }
 }
 
