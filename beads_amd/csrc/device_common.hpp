@@ -63,6 +63,31 @@ __device__ __forceinline__ double team_sum(double v) {
   return v;
 }
 
+// counter[key] += 1 for every active lane of the wave with one atomic per
+// distinct key: the lanes sharing a key add their count through the first of
+// them. Returns the counter's value before this lane's increment (the lanes of
+// a key take consecutive values in lane order).
+__device__ __forceinline__ uint32_t wave_increment(uint32_t* counter, uint32_t key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t pending = __ballot(1);
+  uint32_t result = 0;
+  while (pending != 0) {
+    const int first = __builtin_ctzll(pending);
+    const uint32_t lead = static_cast<uint32_t>(__shfl(static_cast<int>(key), first));
+    const uint64_t group = __ballot(key == lead) & pending;
+    if ((group >> lane) & 1ull) {
+      const uint32_t rank = static_cast<uint32_t>(__builtin_popcountll(group & ((1ull << lane) - 1ull)));
+      uint32_t base = 0;
+      if (rank == 0) {
+        base = atomicAdd(&counter[lead], static_cast<uint32_t>(__builtin_popcountll(group)));
+      }
+      result = static_cast<uint32_t>(__shfl(static_cast<int>(base), first)) + rank;
+    }
+    pending &= ~group;
+  }
+  return result;
+}
+
 // 1/x in FP64: the hardware reciprocal (relative error <= 4.6e-8 measured on
 // gfx950) refined by one Newton step (<= 2.3e-15).
 __device__ __forceinline__ double rcp_f64(double x) {

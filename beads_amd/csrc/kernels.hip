@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace beads_amd {
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(BinParams p) {
   }
   const uint32_t c = cell_of_position(p.pos[i], p.box, p.grid);
   p.cell_of[i] = c;
-  atomicAdd(&p.cell_count[c], 1u);
+  wave_increment(p.cell_count, c);  // one atomic per distinct cell of the wave
 }
 
 // One workgroup: exclusive scan of cell counts into cell_start[0..n_cells],
@@ -152,7 +154,8 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(CellScanParams p) 
   if (i >= p.n) {
     return;
   }
-  const uint32_t slot = atomicAdd(&p.cell_cursor[p.cell_of[i]], 1u);
+  // One atomic per distinct cell of the wave; cell_sort orders the members.
+  const uint32_t slot = wave_increment(p.cell_cursor, p.cell_of[i]);
   p.cell_atoms[slot] = i;
 }
 
@@ -315,10 +318,20 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
     const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     const uint32_t maxn = p.max_neighbors;
-    auto accept = [&](uint32_t j, float4 xj) -> bool {
-      const float dx = mic(xi.x - xj.x, L0, h0);
-      const float dy = mic(xi.y - xj.y, L1, h1);
-      const float dz = mic(xi.z - xj.z, L2, h2);
+    // sh: the image shift of the candidate's cell, with mic(xi - xj) equal to
+    // (xi - xj) + sh on every axis (cells of a regular grid with at least 5
+    // cells per axis); generic: mic() per candidate.
+    auto accept = [&](uint32_t j, float4 xj, auto generic, float3 sh) -> bool {
+      float dx, dy, dz;
+      if (decltype(generic)::value) {
+        dx = mic(xi.x - xj.x, L0, h0);
+        dy = mic(xi.y - xj.y, L1, h1);
+        dz = mic(xi.z - xj.z, L2, h2);
+      } else {
+        dx = (xi.x - xj.x) + sh.x;
+        dy = (xi.y - xj.y) + sh.y;
+        dz = (xi.z - xj.z) + sh.z;
+      }
       const float r2 = r2_strict(dx, dy, dz);
       if (r2 > p.rs2 || (HALF ? j <= i : j == i)) {
         return false;
@@ -339,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     };
     // Candidates [b, e) of a list given by ids(k), kBuildTeam * kBuildSlots
     // per round; slots past e load atom i itself and test nothing.
-    auto scan_range = [&](uint32_t b, uint32_t e, auto ids) {
+    auto scan_range = [&](uint32_t b, uint32_t e, auto ids, auto generic, float3 sh) {
       for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
         uint32_t j[kBuildSlots];
 #pragma unroll
@@ -354,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
-          const bool ok = k + u * kBuildTeam + t < e && accept(j[u], x[u]);
+          const bool ok = k + u * kBuildTeam + t < e && accept(j[u], x[u], generic, sh);
           const uint32_t bits =
               static_cast<uint32_t>(__ballot(ok) >> team_lane0) & ((1u << kBuildTeam) - 1u);
           if (ok) {
@@ -367,27 +380,65 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
       }
     };
+    const float3 no_shift = make_float3(0.f, 0.f, 0.f);
     if (p.grid.path == 0) {
-      scan_range(0, n, [](uint32_t k) { return k; });
+      scan_range(0, n, [](uint32_t k) { return k; }, std::true_type{}, no_shift);
     } else {
-      const CellNeighborhood h = neighborhood(p.cell_of[i], p.grid);
+      const uint32_t c = p.cell_of[i];
+      const CellGridView& g = p.grid;
+      const uint32_t cx = c % g.nx, cyz = c / g.nx, cy = cyz % g.ny, cz = cyz / g.ny;
       const uint32_t* ca = p.cell_atoms;
-      // Fully unrolled so the per-axis cell arrays stay in registers.
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (a >= h.nz) break;
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          if (b >= h.ny) break;
-          const uint32_t rowc = p.grid.nx * (h.ys[b] + p.grid.ny * h.zs[a]);
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            if (d >= h.nx) break;
-            const uint32_t nc = h.xs[d] + rowc;
-            scan_range(p.cell_start[nc], p.cell_start[nc + 1],
-                       [ca](uint32_t k) { return ca[k]; });
+      const auto ids = [ca](uint32_t k) { return ca[k]; };
+      // Neighbour k (0, 1, 2) along an axis, in the order of unique_axis, and
+      // its image shift: with at least 5 cells per axis a candidate in a
+      // wrapped neighbour cell lies more than L/2 away (mic adds +L or -L),
+      // and in any other neighbour cell less than L/2 away (mic adds 0).
+      auto axis = [](uint32_t cc, uint32_t count, int k, float L, float& shift) -> uint32_t {
+        shift = 0.f;
+        if (count <= 1) return 0u;
+        if (count == 2) return k == 0 ? (cc == 0 ? 1u : 0u) : cc;
+        if (k == 0) {
+          if (cc == 0) {
+            shift = L;
+            return count - 1;
+          }
+          return cc - 1;
+        }
+        if (k == 2) {
+          if (cc + 1 == count) {
+            shift = -L;
+            return 0u;
+          }
+          return cc + 1;
+        }
+        return cc;
+      };
+      const int mx = g.nx < 3 ? static_cast<int>(g.nx) : 3;
+      const int my = g.ny < 3 ? static_cast<int>(g.ny) : 3;
+      const int mz = g.nz < 3 ? static_cast<int>(g.nz) : 3;
+      auto cells = [&](auto generic) {
+#pragma unroll 1
+        for (int a = 0; a < mz; ++a) {
+          float sz;
+          const uint32_t zc = axis(cz, g.nz, a, L2, sz);
+#pragma unroll 1
+          for (int b = 0; b < my; ++b) {
+            float sy;
+            const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
+#pragma unroll 1
+            for (int d = 0; d < mx; ++d) {
+              float sx;
+              const uint32_t nc = axis(cx, g.nx, d, L0, sx) + rowc;
+              scan_range(p.cell_start[nc], p.cell_start[nc + 1], ids, generic,
+                         make_float3(sx, sy, sz));
+            }
           }
         }
+      };
+      if (g.path == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {
+        cells(std::false_type{});
+      } else {
+        cells(std::true_type{});
       }
     }
     if (t == 0) {
