@@ -44,6 +44,9 @@ def cases() -> dict[str, W.Workload]:
     return {
         "fcc_direct": jittered_fcc(),
         "lj_regular": W.lj_box(4000, 0.8442, 1.0, seed=3),
+        # regular path with 4 cells per axis: the build takes the minimum
+        # image per candidate (image shifts per cell need at least 5)
+        "lj_regular_4cells": W.lj_box(1200, 0.8442, 1.0, seed=6),
         "lj_small_dim": elongated_lj(),
         "melt": W.polymer_melt(n_chains=80, beads_per_chain=50, seed=3),
         "mixture": W.lj_mixture(n=30000, seed=4),
