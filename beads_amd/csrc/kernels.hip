@@ -461,11 +461,11 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 
 // ------------------------------------------------------------------ force --
 
-// LJ over one atom's stored list (pair_lj.cu:21-167). The loop reads two
-// 16-byte quads of indices and issues the eight position gathers together
-// before any arithmetic. The decision r^2 < r_c^2 is binary32 on the
-// exact-intermediate minimum image; the pair value is FP64. REACT adds the
-// reaction -s d of every pair to pair_acc[j] (half list).
+// LJ over one atom's stored list by one lane (pair_lj.cu:21-167), used by the
+// half-list pair pass. The loop reads two 16-byte quads of indices and issues
+// the eight position gathers together before any arithmetic. The decision
+// r^2 < r_c^2 is binary32 on the exact-intermediate minimum image; the pair
+// value is FP64. REACT adds the reaction -s d of every pair to pair_acc[j].
 // TYPES: 0 = single type (coefficients in registers), 1 = table in LDS,
 // 2 = table in global memory.
 template <bool OBS, int TYPES, bool REACT>

@@ -142,6 +142,26 @@ def test_rebuilds_reorders_and_long_run_stay_consistent():
     np.testing.assert_array_equal(st["masses"][order], w.masses)
 
 
+@pytest.mark.parametrize("n", [1200, 4000])  # 4 and 6 cells per axis
+def test_neighbor_overflow_caps_lists_in_scan_order(n):
+    """max_neighbors below the real counts: the overflow flag is set and every
+    stored list holds the first max_neighbors accepted candidates in the
+    builder's scan order, the same entries as the oracle's capped lists."""
+    w = W.lj_box(n, 0.8, 1.0, seed=4).with_(max_neighbors=8)
+    with Engine(w) as e:
+        e.initialize(observables=False)
+        stats = e.stats()
+        st = e.state()
+        counts, lst = e.neighbors()
+    assert stats.overflow == 1
+    rc, rl, ovf = O.neighbor_lists(w, positions=st["positions"], tags=st["tags"],
+                                   types=st["types"], max_neighbors=8)
+    assert ovf
+    np.testing.assert_array_equal(counts, rc)
+    for i in range(w.n):
+        assert np.array_equal(np.sort(lst[: counts[i], i]), np.sort(rl[i, : rc[i]])), i
+
+
 def test_nve_energy_conservation():
     w = W.lj_fluid_fcc(cells=8, steps=1000)  # 2048 beads, 4^3 cells
     with _engine(w) as e:
