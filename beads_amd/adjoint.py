@@ -16,9 +16,10 @@ gradients. This module puts the engine's frame-mode entry points
   steps from its stored (x, v) checkpoint.
 
 Forces are binary32 as in the dynamics path; the adjoint kernel evaluates in
-FP64 from the binary32 coordinates. Every engine call synchronises: torch's
-current stream is drained before the engine reads a tensor, and the engine's
-stream is drained before a result is handed back.
+FP64 from the binary32 coordinates. Engine calls are enqueued on torch's
+current stream (the engine's stream-ordered frame entry points), so tensors
+flow between torch and the engine without host waits; energies come back as
+device tensors. A neighbour-list overflow surfaces at ``FrameModel.check()``.
 """
 
 from __future__ import annotations
@@ -67,6 +68,7 @@ class FrameModel:
         self._params_ident: tuple | None = None
         self._loaded = 0   # token of the frame whose neighbour list the engine holds
         self._tokens = 0
+        self._stream = None  # the torch stream the engine enqueues on
         self.forward_calls = 0
         self.backward_calls = 0
         self.reload_calls = 0
@@ -110,10 +112,22 @@ class FrameModel:
 
     # ------------------------------------------------------------ raw calls
 
+    def _on_current_stream(self) -> None:
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._stream:
+            self.engine.set_stream(s)
+            self._stream = s
+
+    def check(self) -> None:
+        """Wait for the enqueued frame work; raises on a neighbour-list overflow."""
+        self.engine.synchronize()
+
     def _forward_raw(self, x: torch.Tensor, reuse_list: bool = True):
-        torch.cuda.current_stream(self.device).synchronize()
+        self._on_current_stream()
         f = torch.empty_like(x)
-        e = self.engine.frame_forces(x.data_ptr(), f.data_ptr(), reuse_list=reuse_list)
+        e = torch.empty(4, dtype=torch.float64, device=self.device)
+        self.engine.frame_forces_async(x.data_ptr(), f.data_ptr(), e.data_ptr(),
+                                       reuse_list=reuse_list)
         self._tokens += 1
         self._loaded = self._tokens
         self.forward_calls += 1
@@ -123,17 +137,18 @@ class FrameModel:
         """Make frame x (evaluated earlier under token) current for a backward."""
         if self._loaded == token:
             return
-        torch.cuda.current_stream(self.device).synchronize()
-        self.engine.frame_load(x.data_ptr(), reuse_list=True)
+        self._on_current_stream()
+        self.engine.frame_load_async(x.data_ptr(), reuse_list=True)
         self._loaded = token
         self.reload_calls += 1
 
     def _backward_raw(self, g_forces: torch.Tensor, g_energy: float):
+        self._on_current_stream()
         n_p = 2 * self.n_types * self.n_types + 2 * self.n_bond_types
         gx = torch.empty((self.n, 3), dtype=torch.float32, device=self.device)
         gp = torch.empty(max(n_p, 1), dtype=torch.float64, device=self.device)
-        torch.cuda.current_stream(self.device).synchronize()
-        self.engine.frame_backward(g_forces.data_ptr(), g_energy, gx.data_ptr(), gp.data_ptr())
+        self.engine.frame_backward_async(g_forces.data_ptr(), g_energy, gx.data_ptr(),
+                                         gp.data_ptr())
         self.backward_calls += 1
         return gx, gp
 
@@ -182,12 +197,14 @@ class _FrameForces(torch.autograd.Function):
         model._set_params(eps, sigma, k, r0)
         pos = model._positions(x)
         f, e, token = model._forward_raw(pos)
+        # An output that the loss does not use arrives as None, not as a zero
+        # tensor whose value the backward would read back to the host.
+        ctx.set_materialize_grads(False)
         ctx.model = model
         ctx.token = token
         ctx.x_dtype = x.dtype
         ctx.save_for_backward(pos, eps, sigma, k, r0)
-        energy = torch.tensor(float(np.sum(e)), dtype=torch.float64, device=model.device)
-        return f, energy
+        return f, e.sum()
 
     @staticmethod
     def backward(ctx, g_forces, g_energy):

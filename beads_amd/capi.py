@@ -106,6 +106,12 @@ SIGNATURES = {
                                               C.c_void_p]),
     "beads_engine_frame_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double)]),
+    "beads_engine_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "beads_engine_frame_forces_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_int32]),
+    "beads_engine_frame_load_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "beads_engine_frame_backward_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double,
+                                                    C.c_void_p, C.c_void_p]),
     "beads_engine_destroy": (None, [C.c_void_p]),
     "beads_batch_create": (C.c_int, [C.POINTER(Spec), C.c_uint32, C.POINTER(C.c_void_p)]),
     "beads_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32),
@@ -317,6 +323,29 @@ class Engine:
     def frame_backward(self, grad_forces_ptr: int, grad_energy: float, grad_positions_ptr: int,
                        grad_params_ptr: int) -> None:
         _check(self.lib.beads_engine_frame_backward(
+            self.handle, C.c_void_p(grad_forces_ptr), float(grad_energy),
+            C.c_void_p(grad_positions_ptr), C.c_void_p(grad_params_ptr)))
+
+    # Stream-ordered frame mode: the calls below enqueue on the stream given to
+    # set_stream (a hipStream_t as an int; 0 = HIP's null stream) and return at
+    # once; synchronize() waits and reports a list overflow.
+    def set_stream(self, stream_handle: int) -> None:
+        _check(self.lib.beads_engine_set_stream(self.handle, C.c_void_p(stream_handle or None)))
+
+    def frame_forces_async(self, positions_ptr: int, forces_ptr: int, energies_ptr: int,
+                           reuse_list: bool = True) -> None:
+        """energies_ptr: device memory for [pair, bond, angle, dihedral] (FP64)."""
+        _check(self.lib.beads_engine_frame_forces_async(
+            self.handle, C.c_void_p(positions_ptr), C.c_void_p(forces_ptr),
+            C.c_void_p(energies_ptr or None), 1 if reuse_list else 0))
+
+    def frame_load_async(self, positions_ptr: int, reuse_list: bool = True) -> None:
+        _check(self.lib.beads_engine_frame_load_async(self.handle, C.c_void_p(positions_ptr),
+                                                      1 if reuse_list else 0))
+
+    def frame_backward_async(self, grad_forces_ptr: int, grad_energy: float,
+                             grad_positions_ptr: int, grad_params_ptr: int) -> None:
+        _check(self.lib.beads_engine_frame_backward_async(
             self.handle, C.c_void_p(grad_forces_ptr), float(grad_energy),
             C.c_void_p(grad_positions_ptr), C.c_void_p(grad_params_ptr)))
 

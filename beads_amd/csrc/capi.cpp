@@ -205,6 +205,38 @@ beads_status_t beads_engine_frame_timing(beads_engine_t* e, double* forward_s,
   return guarded([&] { engine_of(e).frame_timing(forward_s, backward_s); });
 }
 
+beads_status_t beads_engine_set_stream(beads_engine_t* e, void* stream) {
+  return guarded([&] { engine_of(e).set_stream(static_cast<hipStream_t>(stream)); });
+}
+
+beads_status_t beads_engine_frame_forces_async(beads_engine_t* e, const float* positions,
+                                               float* forces, double* energies,
+                                               int32_t reuse_list) {
+  return guarded([&] {
+    beads_amd::require(positions && forces, "frame arrays are null", BEADS_E_TYPE);
+    engine_of(e).frame_forces(positions, forces, energies, reuse_list != 0, true);
+  });
+}
+
+beads_status_t beads_engine_frame_load_async(beads_engine_t* e, const float* positions,
+                                             int32_t reuse_list) {
+  return guarded([&] {
+    beads_amd::require(positions != nullptr, "frame positions are null", BEADS_E_TYPE);
+    engine_of(e).frame_load(positions, reuse_list != 0, true);
+  });
+}
+
+beads_status_t beads_engine_frame_backward_async(beads_engine_t* e, const float* grad_forces,
+                                                 double grad_energy, float* grad_positions,
+                                                 double* grad_params) {
+  return guarded([&] {
+    beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",
+                       BEADS_E_TYPE);
+    beads_amd::require(std::isfinite(grad_energy), "grad_energy must be finite");
+    engine_of(e).frame_backward(grad_forces, grad_energy, grad_positions, grad_params, true);
+  });
+}
+
 void beads_engine_destroy(beads_engine_t* e) {
   if (e != nullptr) {
     delete e->impl;

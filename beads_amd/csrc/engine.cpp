@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <iomanip>
 #include <iostream>
@@ -417,7 +418,8 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
                              spec.rebuild_check_every, spec.sort_every_rebuild);
   interrupt_interval_ = std::max<uint64_t>(1, 100000 / std::max<uint32_t>(1, n_));
 
-  BEADS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  BEADS_HIP(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking));
+  stream_ = own_stream_;
   upload_state();
   set_up_params();
   // The spec's arrays belong to the caller; drop them.
@@ -438,12 +440,13 @@ Engine::~Engine() {
     if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
     (void)hipGraphExecDestroy(step_graph_exec_);
   }
+  if (stream_ != own_stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : frame_events_) {
     if (e != nullptr) (void)hipEventDestroy(e);
   }
-  if (stream_ != nullptr) {
-    (void)hipStreamSynchronize(stream_);
-    (void)hipStreamDestroy(stream_);
+  if (own_stream_ != nullptr) {
+    (void)hipStreamSynchronize(own_stream_);
+    (void)hipStreamDestroy(own_stream_);
   }
 }
 
@@ -789,7 +792,12 @@ void Engine::materialize_velocities() {
   ke_from_scale_ = true;
 }
 
-void Engine::synchronize() { BEADS_HIP(hipStreamSynchronize(stream_)); }
+// Waits for the enqueued work; in frame mode also reports a neighbour-list
+// overflow that async frame calls left unchecked.
+void Engine::synchronize() {
+  BEADS_HIP(hipStreamSynchronize(stream_));
+  if (frame_mode_) check_overflow(fetch_ctrl());
+}
 
 Ctrl Engine::fetch_ctrl() {
   BEADS_HIP(hipMemcpyAsync(ctrl_host_.data(), ctrl_.data(), sizeof(Ctrl), hipMemcpyDeviceToHost,
@@ -1042,7 +1050,7 @@ void Engine::load_frame(const float* positions_dev, bool reuse_list) {
 }
 
 void Engine::frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
-                          bool reuse_list) {
+                          bool reuse_list, bool async) {
   enter_frame_mode();
   frame_loaded_ = false;
   BEADS_HIP(hipEventRecord(frame_events_[0], stream_));
@@ -1051,25 +1059,46 @@ void Engine::frame_forces(const float* positions_dev, float* forces_dev, double*
   launch_frame_store(n_, force_.data(), forces_dev, stream_);
   BEADS_HIP(hipEventRecord(frame_events_[1], stream_));
   BEADS_HIP(hipGetLastError());
-  const Ctrl c = fetch_ctrl();
-  BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, frame_events_[0], frame_events_[1]));
-  check_overflow(c);
+  if (async) {
+    if (energies_out != nullptr) {
+      const char* pe = reinterpret_cast<const char*>(ctrl_.data()) + offsetof(Ctrl, pe);
+      BEADS_HIP(hipMemcpyAsync(energies_out, pe, 4 * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream_));
+    }
+    forward_time_pending_ = true;
+  } else {
+    const Ctrl c = fetch_ctrl();
+    BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, frame_events_[0], frame_events_[1]));
+    forward_time_pending_ = false;
+    check_overflow(c);
+    if (energies_out != nullptr) {
+      for (int q = 0; q < 4; ++q) energies_out[q] = c.pe[q];
+    }
+  }
   frame_loaded_ = true;
   frame_list_valid_ = true;
-  if (energies_out != nullptr) {
-    for (int q = 0; q < 4; ++q) energies_out[q] = c.pe[q];
-  }
+}
+
+// Later frame calls enqueue on the caller's stream, taken as given: NULL is
+// HIP's null stream (torch's default stream), not the engine's own. What is
+// already enqueued on the current stream completes first, so the switch keeps
+// every ordering.
+void Engine::set_stream(hipStream_t stream) {
+  enter_frame_mode();
+  if (stream == stream_) return;
+  BEADS_HIP(hipStreamSynchronize(stream_));
+  stream_ = stream;
 }
 
 // Positions and neighbour list of a frame without evaluating forces: what
 // frame_backward needs when another frame was evaluated in between.
-void Engine::frame_load(const float* positions_dev, bool reuse_list) {
+void Engine::frame_load(const float* positions_dev, bool reuse_list, bool async) {
   frame_loaded_ = false;
   load_frame(positions_dev, reuse_list);
   // The force pass normally clears the rebuild flags; no force pass runs here.
   BEADS_HIP(hipMemsetAsync(ctrl_.data(), 0, 2 * sizeof(uint32_t), stream_));
   BEADS_HIP(hipGetLastError());
-  check_overflow(fetch_ctrl());
+  if (!async) check_overflow(fetch_ctrl());
   frame_loaded_ = true;
   frame_list_valid_ = true;
 }
@@ -1078,7 +1107,7 @@ void Engine::frame_load(const float* positions_dev, bool reuse_list) {
 // grad_energy scales dE/d(.), outputs dL/dx [n][3] binary32 and the FP64
 // parameter gradients [eps T*T | sigma T*T | k nb | r0 nb] (device).
 void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
-                            float* grad_positions_dev, double* grad_params_dev) {
+                            float* grad_positions_dev, double* grad_params_dev, bool async) {
   require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
   require(!half_list_, "the adjoint reads the full neighbour list; create the engine with "
           "half_list = 0.", BEADS_E_VALUE);
@@ -1125,11 +1154,28 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   launch_fm_backward(p, stream_);
   BEADS_HIP(hipEventRecord(frame_events_[3], stream_));
   BEADS_HIP(hipGetLastError());
-  BEADS_HIP(hipEventSynchronize(frame_events_[3]));
-  BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, frame_events_[2], frame_events_[3]));
+  if (async) {
+    backward_time_pending_ = true;
+  } else {
+    BEADS_HIP(hipEventSynchronize(frame_events_[3]));
+    BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, frame_events_[2], frame_events_[3]));
+    backward_time_pending_ = false;
+  }
 }
 
+// Device time of the last forward and backward frame calls; after async calls
+// this waits for their end events.
 void Engine::frame_timing(double* forward_s, double* backward_s) {
+  if (forward_time_pending_) {
+    BEADS_HIP(hipEventSynchronize(frame_events_[1]));
+    BEADS_HIP(hipEventElapsedTime(&last_forward_ms_, frame_events_[0], frame_events_[1]));
+    forward_time_pending_ = false;
+  }
+  if (backward_time_pending_) {
+    BEADS_HIP(hipEventSynchronize(frame_events_[3]));
+    BEADS_HIP(hipEventElapsedTime(&last_backward_ms_, frame_events_[2], frame_events_[3]));
+    backward_time_pending_ = false;
+  }
   if (forward_s != nullptr) *forward_s = last_forward_ms_ * 1e-3;
   if (backward_s != nullptr) *backward_s = last_backward_ms_ * 1e-3;
 }

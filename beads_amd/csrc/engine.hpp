@@ -96,12 +96,16 @@ class Engine {
   // caller in slot order, so particles are never reordered.
   void set_lj_params(const float* eps, const float* sigma);
   void set_bond_params(const float* k, const float* r0);
+  // async: enqueue and return without waiting (energies_out is then device
+  // memory, and a list overflow is reported by the next synchronize()).
   void frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
-                    bool reuse_list);
-  void frame_load(const float* positions_dev, bool reuse_list);
+                    bool reuse_list, bool async = false);
+  void frame_load(const float* positions_dev, bool reuse_list, bool async = false);
   void frame_backward(const float* grad_forces_dev, double grad_energy,
-                      float* grad_positions_dev, double* grad_params_dev);
+                      float* grad_positions_dev, double* grad_params_dev, bool async = false);
   void frame_timing(double* forward_s, double* backward_s);
+  // Enqueue frame-mode work on the caller's stream (nullptr: the engine's own).
+  void set_stream(hipStream_t stream);
   uint32_t n_particles() const { return n_; }
   int32_t n_types() const { return n_types_; }
   int32_t n_bond_types() const { return n_bond_types_; }
@@ -146,7 +150,8 @@ class Engine {
   bool ke_from_scale_ = false;  // which reduction last measured the stored velocities
   uint64_t excluded_pairs_ = 0;
 
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr;      // where work is enqueued: own_stream_ or the caller's
+  hipStream_t own_stream_ = nullptr;
 
   // device state
   DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
@@ -179,6 +184,7 @@ class Engine {
   bool frame_loaded_ = false;
   bool frame_list_valid_ = false;  // the stored list and its reference positions are current
   float last_forward_ms_ = 0.f, last_backward_ms_ = 0.f;
+  bool forward_time_pending_ = false, backward_time_pending_ = false;  // async calls
   hipEvent_t frame_events_[4] = {nullptr, nullptr, nullptr, nullptr};
 
   // kernel parameter blocks

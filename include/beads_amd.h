@@ -247,9 +247,28 @@ beads_status_t beads_engine_frame_load(beads_engine_t* engine, const float* posi
 beads_status_t beads_engine_frame_backward(beads_engine_t* engine, const float* grad_forces,
                                            double grad_energy, float* grad_positions,
                                            double* grad_params);
-/* Device time of the last frame_forces / frame_backward (HIP events). */
+/* Device time of the last frame_forces / frame_backward (HIP events); after
+ * the async variants this waits for their end events. */
 beads_status_t beads_engine_frame_timing(beads_engine_t* engine, double* forward_s,
                                          double* backward_s);
+/* Stream-ordered frame mode (no reference counterpart; for callers that keep
+ * tensors on their own stream, e.g. torch's current stream). set_stream puts
+ * the engine in frame mode and makes later calls enqueue on `stream` (a
+ * hipStream_t taken as given, NULL being HIP's null stream; until the first
+ * call the engine uses a non-blocking stream of its own), after the work
+ * already enqueued has completed. The _async variants take the same arguments as their
+ * synchronous counterparts but return without waiting: `energies` is then
+ * device memory (4 doubles, written in stream order), and a neighbour-list
+ * overflow is reported by the next beads_engine_synchronize. */
+beads_status_t beads_engine_set_stream(beads_engine_t* engine, void* stream);
+beads_status_t beads_engine_frame_forces_async(beads_engine_t* engine, const float* positions,
+                                               float* forces, double* energies,
+                                               int32_t reuse_list);
+beads_status_t beads_engine_frame_load_async(beads_engine_t* engine, const float* positions,
+                                             int32_t reuse_list);
+beads_status_t beads_engine_frame_backward_async(beads_engine_t* engine,
+                                                 const float* grad_forces, double grad_energy,
+                                                 float* grad_positions, double* grad_params);
 void beads_engine_destroy(beads_engine_t* engine);
 
 /* ---- batched replicas ----
