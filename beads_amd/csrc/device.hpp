@@ -143,6 +143,7 @@ struct BuildParams {
   const uint32_t* cell_start;
   const uint32_t* cell_count;
   const uint32_t* cell_atoms;
+  const float4* cell_pos;  // cell_atoms' positions, .w = the slot's bits
   const uint32_t* topo_count;
   const uint32_t* excl;
   uint32_t ecap;
@@ -181,6 +182,7 @@ struct ReorderParams {
   uint32_t* s_excl;
   uint32_t bcap, acap, dcap, ecap;
   uint32_t* cell_atoms;
+  float4* cell_pos;
   uint32_t* new_of_old;
   Ctrl* ctrl;
 };

@@ -478,6 +478,7 @@ void Engine::upload_state() {
   nbr_.resize(static_cast<size_t>(nbr_padded_capacity(plan_.max_neighbors)) * n);
   cell_of_.resize(n);
   cell_atoms_.resize(n);
+  cell_pos_.resize(n);
   cell_count_.resize(plan_.n_cells);
   cell_count_.zero(stream_);
   cell_start_.resize(static_cast<size_t>(plan_.n_cells) + 1);
@@ -628,6 +629,7 @@ void Engine::set_up_params() {
   b.cell_start = cell_start_.data();
   b.cell_count = cell_count_.data();
   b.cell_atoms = cell_atoms_.data();
+  b.cell_pos = cell_pos_.data();
   b.topo_count = topo_count_.data();
   b.excl = excl_.data();
   b.ecap = ecap_;
@@ -654,6 +656,8 @@ void Engine::set_up_params() {
   sp.cell_start = cell_start_.data();
   sp.cell_cursor = cell_cursor_.data();
   sp.cell_atoms = cell_atoms_.data();
+  sp.pos = pos_.data();
+  sp.cell_pos = cell_pos_.data();
   sp.ctrl = ctrl_.data();
 
   ReorderParams& r = reorder_params_;
@@ -685,6 +689,7 @@ void Engine::set_up_params() {
   r.dcap = dcap_;
   r.ecap = ecap_;
   r.cell_atoms = cell_atoms_.data();
+  r.cell_pos = cell_pos_.data();
   r.new_of_old = new_of_old_.data();
   r.ctrl = ctrl_.data();
 

@@ -164,8 +164,14 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(CellScanParams p) 
 // scatter (cell_list_build.cu:168-181). One wave per cell: for up to 64
 // members lane l holds member l, whose rank is the number of members with a
 // smaller slot (slots are distinct), and stores it at start + rank. A larger
-// cell is insertion-sorted by lane 0.
+// cell is insertion-sorted by lane 0. When no reorder follows, the kernel also
+// writes the members' positions in cell order with the slot in .w (the
+// build's candidate stream); after a reorder, reorder_commit writes them.
 constexpr uint32_t kSortCellsPerBlock = kBlock / 64;
+
+__device__ __forceinline__ float4 cell_entry(float4 x, uint32_t slot) {
+  return make_float4(x.x, x.y, x.z, __uint_as_float(slot));
+}
 
 __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
   if (p.ctrl->rebuild == 0) {
@@ -178,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
   }
   const uint32_t b = p.cell_start[c], e = p.cell_start[c + 1];
   const uint32_t cnt = e - b;
+  const bool stream = p.ctrl->reorder == 0;
   uint32_t* a = p.cell_atoms;
   if (cnt <= 64) {
     const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
@@ -189,6 +196,7 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
     }
     if (lane < cnt) {
       a[b + rank] = v;
+      if (stream) p.cell_pos[b + rank] = cell_entry(p.pos[v], v);
     }
   } else if (lane == 0) {
     for (uint32_t k = b + 1; k < e; ++k) {
@@ -199,6 +207,12 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
         --m;
       }
       a[m] = v;
+    }
+    if (stream) {
+      for (uint32_t k = b; k < e; ++k) {
+        const uint32_t v = a[k];
+        p.cell_pos[k] = cell_entry(p.pos[v], v);
+      }
     }
   }
 }
@@ -245,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void reorder_commit_kernel(ReorderParams p)
   p.molid[s] = p.s_molid[s];
   p.cell_of[s] = p.s_cell_of[s];
   p.cell_atoms[s] = s;
+  p.cell_pos[s] = cell_entry(p.s_pos[s], s);
   const uint32_t cw = p.s_topo_count[s];
   p.topo_count[s] = cw;
   const size_t n = p.n;
@@ -281,8 +296,10 @@ constexpr uint32_t kBuildAtoms = kBlock / kBuildTeam;
 // skip self, skip excluded, keep r2 <= r_s^2 (reference minimum image and
 // operation order), overflow past max_neighbors. Each atom has a team of
 // kBuildTeam adjacent lanes; lane t takes candidates k + u*kBuildTeam + t
-// (u < kBuildSlots) of every candidate run, so the team's id and position
-// loads are consecutive entries. Accepted candidates are appended in
+// (u < kBuildSlots) of every candidate run, so the team's loads are
+// consecutive 16-byte entries of the cell-ordered position stream (slot in
+// .w), one load per candidate with no index gather; slots past a run's end
+// reload its last entry and test nothing. Accepted candidates are appended in
 // candidate order (the order of a one-lane scan) through a ballot of the
 // team's lanes and written straight into the atom's chunks. EXCL: 0 = no
 // exclusions, 4 or 8 = up to that many partners held in registers (every
@@ -333,47 +350,43 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         dz = (xi.z - xj.z) + sh.z;
       }
       const float r2 = r2_strict(dx, dy, dz);
-      if (r2 > p.rs2 || (HALF ? j <= i : j == i)) {
-        return false;
-      }
+      // Non-short-circuit tests: selects, no branches.
+      bool ok = !(r2 > p.rs2) & (HALF ? j > i : j != i);
       if (EXCL > 0) {
         bool hit = false;
 #pragma unroll
         for (int k = 0; k < EXCL; ++k) {
           hit |= ex[k] == j;
         }
-        return !hit;
+        ok &= !hit;
       } else if (EXCL < 0) {
-        for (uint32_t k = 0; k < ne; ++k) {
-          if (p.excl[static_cast<size_t>(k) * n + i] == j) return false;
+        if (ok) {
+          for (uint32_t k = 0; k < ne; ++k) {
+            if (p.excl[static_cast<size_t>(k) * n + i] == j) ok = false;
+          }
         }
       }
-      return true;
+      return ok;
     };
-    // Candidates [b, e) of a list given by ids(k), kBuildTeam * kBuildSlots
-    // per round; slots past e load atom i itself and test nothing.
-    auto scan_range = [&](uint32_t b, uint32_t e, auto ids, auto generic, float3 sh) {
+    // Candidates [b, e) of a run (b < e) whose entry s is entry(s) = {x, y,
+    // z, slot bits}, kBuildTeam * kBuildSlots per round.
+    auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto generic, float3 sh) {
       for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
-        uint32_t j[kBuildSlots];
-#pragma unroll
-        for (int u = 0; u < kBuildSlots; ++u) {
-          const uint32_t s = k + u * kBuildTeam + t;
-          j[u] = s < e ? ids(s) : i;
-        }
         float4 x[kBuildSlots];
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
-          x[u] = p.pos[j[u]];
+          x[u] = entry(min(k + u * kBuildTeam + t, e - 1));
         }
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
-          const bool ok = k + u * kBuildTeam + t < e && accept(j[u], x[u], generic, sh);
+          const uint32_t j = __float_as_uint(x[u].w);
+          const bool ok = (k + u * kBuildTeam + t < e) & accept(j, x[u], generic, sh);
           const uint32_t bits =
               static_cast<uint32_t>(__ballot(ok) >> team_lane0) & ((1u << kBuildTeam) - 1u);
           if (ok) {
             const uint32_t slot = count + __builtin_popcount(bits & below);
             if (slot < maxn) {
-              p.nbr[nbr_chunk_base(slot / kNbrChunk, i, n) + (slot & (kNbrChunk - 1))] = j[u];
+              p.nbr[nbr_chunk_base(slot / kNbrChunk, i, n) + (slot & (kNbrChunk - 1))] = j;
             }
           }
           count += __builtin_popcount(bits);
@@ -382,13 +395,18 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     };
     const float3 no_shift = make_float3(0.f, 0.f, 0.f);
     if (p.grid.path == 0) {
-      scan_range(0, n, [](uint32_t k) { return k; }, std::true_type{}, no_shift);
+      const float4* pos = p.pos;
+      const auto entry = [pos](uint32_t k) {
+        const float4 q = pos[k];
+        return make_float4(q.x, q.y, q.z, __uint_as_float(k));
+      };
+      scan_range(0, n, entry, std::true_type{}, no_shift);
     } else {
       const uint32_t c = p.cell_of[i];
       const CellGridView& g = p.grid;
       const uint32_t cx = c % g.nx, cyz = c / g.nx, cy = cyz % g.ny, cz = cyz / g.ny;
-      const uint32_t* ca = p.cell_atoms;
-      const auto ids = [ca](uint32_t k) { return ca[k]; };
+      const float4* cp = p.cell_pos;
+      const auto entry = [cp](uint32_t k) { return cp[k]; };
       // Neighbour k (0, 1, 2) along an axis, in the order of unique_axis, and
       // its image shift: with at least 5 cells per axis a candidate in a
       // wrapped neighbour cell lies more than L/2 away (mic adds +L or -L),
@@ -429,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
             for (int d = 0; d < mx; ++d) {
               float sx;
               const uint32_t nc = axis(cx, g.nx, d, L0, sx) + rowc;
-              scan_range(p.cell_start[nc], p.cell_start[nc + 1], ids, generic,
+              scan_range(p.cell_start[nc], p.cell_start[nc + 1], entry, generic,
                          make_float3(sx, sy, sz));
             }
           }

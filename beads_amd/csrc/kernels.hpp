@@ -28,6 +28,8 @@ struct CellScanParams {
   uint32_t* cell_start;
   uint32_t* cell_cursor;
   uint32_t* cell_atoms;
+  const float4* pos;
+  float4* cell_pos;  // written in cell order when no reorder follows
   Ctrl* ctrl;
 };
 

@@ -118,6 +118,77 @@ def test_frame_forces_match_dynamics_engine_and_oracle():
     assert float(e) == pytest.approx(obs["pair_pe"] + obs["bond_pe"], rel=1e-5)
 
 
+def test_stream_ordered_frames_match_synchronous_calls():
+    """FrameModel on a side torch stream (stream-ordered entry points, no host
+    wait per call) against the synchronous entry points of a second engine
+    driven through the same call sequence: forward frames 0..2 keeping the
+    list when the skin allows, then backward 2, reload 1, backward 1, reload
+    0, backward 0. Forces, energies and dL/dx are bit-identical. Parameter
+    gradients are FP64 per frame (summed with per-workgroup atomics) and
+    autograd accumulates them over frames in the parameters' binary32 dtype,
+    so they agree with the FP64 sum of the engine's outputs to 1e-6."""
+    w = small_system(seed=7)
+    frames = _frames(w, 3, seed=11)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    xs = [torch.tensor(x, device="cuda") for x in frames]
+    gs = [torch.randn(x.shape, generator=gen, device="cuda") for x in xs]
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    with FrameModel(w) as m:
+        with torch.cuda.stream(side):
+            p = m.initial_params()
+            xr = [x.clone().requires_grad_(True) for x in xs]
+            fs, es, loss = [], [], None
+            for x, g in zip(xr, gs):
+                f, e = m.forces(x, p)
+                fs.append(f)
+                es.append(e)
+                term = (f.double() * g.double()).sum() + 0.5 * e
+                loss = term if loss is None else loss + term
+            loss.backward()
+            m.check()
+            out_f = [f.detach().cpu().numpy() for f in fs]
+            out_e = [float(e.detach()) for e in es]
+            out_gx = [x.grad.cpu().numpy() for x in xr]
+            out_gp = np.concatenate([t.grad.double().cpu().numpy().ravel()
+                                     for t in p.tensors()])
+        assert m.backward_calls == 3 and m.reload_calls == 2
+    n_p = 2 * w.n_types * w.n_types + 2 * int(w.bond_k.shape[0])
+    with Engine(w) as eng:
+        ref_f, ref_e, ref_gx = [], [], [None] * 3
+        gp_sum = np.zeros(n_p)
+        for x in xs:
+            f = torch.empty_like(x)
+            ref_e.append(float(np.sum(eng.frame_forces(x.data_ptr(), f.data_ptr(),
+                                                       reuse_list=True))))
+            ref_f.append(f.cpu().numpy())
+        for i in (2, 1, 0):
+            if i != 2:
+                eng.frame_load(xs[i].data_ptr(), reuse_list=True)
+            gx = torch.empty_like(xs[i])
+            gp = torch.empty(n_p, dtype=torch.float64, device="cuda")
+            eng.frame_backward(gs[i].data_ptr(), 0.5, gx.data_ptr(), gp.data_ptr())
+            ref_gx[i] = gx.cpu().numpy()
+            gp_sum += gp.cpu().numpy()
+    for i in range(3):
+        np.testing.assert_array_equal(out_f[i], ref_f[i])
+        assert out_e[i] == ref_e[i]
+        np.testing.assert_array_equal(out_gx[i], ref_gx[i])
+    # p.tensors() holds eps, sigma as [T, T] and k, r0 as [nb]: the same order
+    # as the engine's parameter vector.
+    np.testing.assert_allclose(out_gp, gp_sum, rtol=1e-6, atol=1e-6 * np.abs(gp_sum).max())
+
+
+def test_stream_ordered_overflow_reported_by_check():
+    """A list overflow in a stream-ordered frame call surfaces at check()."""
+    w = W.lj_box(1200, 0.8, 1.0, seed=4).with_(max_neighbors=8)
+    with FrameModel(w) as m:
+        p = m.initial_params(requires_grad=False)
+        m.forces(torch.tensor(w.positions, device="cuda"), p)
+        with pytest.raises(Exception, match="overflow"):
+            m.check()
+
+
 def test_force_matching_parameter_gradients_vs_fd():
     w = small_system()
     frames = _frames(w, 3, seed=9)
