@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <iostream>
@@ -436,6 +437,7 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
 }
 
 Engine::~Engine() {
+  output_.reset();  // the writer thread writes through the streams below
   if (step_graph_exec_ != nullptr) {
     if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
     (void)hipGraphExecDestroy(step_graph_exec_);
@@ -1193,16 +1195,58 @@ bool Engine::output_due(uint64_t step) const {
   return thermo || traj;
 }
 
+// Synchronous: read back and write at once. Asynchronous: pack the records on
+// the device, stage them with the control block in a pinned slot and let the
+// writer thread write; the compute stream never waits on the host.
 void Engine::stage_outputs(uint64_t step) {
+  const bool thermo = !thermo_path_.empty() && step % spec_.thermo_every == 0;
+  const bool traj = !trajectory_path_.empty() && step % spec_.trajectory_every == 0;
   materialize_velocities();
-  const Ctrl c = fetch_ctrl();
+  if (!async_output_) {
+    const Ctrl c = fetch_ctrl();
+    check_overflow(c);
+    if (thermo) write_thermo_row(step, c);
+    if (traj) write_trajectory_frame(step);
+    return;
+  }
+  if (!output_) {
+    const size_t bytes = trajectory_path_.empty() ? 0 : trajectory_record_bytes() * n_;
+    output_ = std::make_unique<AsyncOutput>(
+        bytes, [this](uint64_t s, bool t, bool j, const Ctrl& c, const unsigned char* r) {
+          write_outputs(s, t, j, c, r);
+        });
+  }
+  const int slot = output_->acquire();
+  try {
+    if (traj) {
+      PackParams pp{};
+      pp.n = n_;
+      pp.mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
+      pp.words = static_cast<uint32_t>(trajectory_record_bytes() / sizeof(uint32_t));
+      pp.pos = pos_.data();
+      pp.vel = vel_.data();
+      pp.force = force_.data();
+      pp.image = image_.data();
+      pp.tag = tag_.data();
+      pp.out = reinterpret_cast<uint32_t*>(output_->device_records(slot));
+      launch_pack_records(pp, stream_);
+      BEADS_HIP(hipGetLastError());
+    }
+    output_->submit(slot, ctrl_.data(), stream_, step, thermo, traj);
+  } catch (...) {
+    output_->cancel(slot);
+    throw;
+  }
+}
+
+// Writer side of an asynchronous output step (also the order of the
+// synchronous one): an overflowed list stops the run before anything of the
+// step is written.
+void Engine::write_outputs(uint64_t step, bool thermo, bool trajectory, const Ctrl& c,
+                           const unsigned char* records) {
   check_overflow(c);
-  if (!thermo_path_.empty() && step % spec_.thermo_every == 0) {
-    write_thermo_row(step, c);
-  }
-  if (!trajectory_path_.empty() && step % spec_.trajectory_every == 0) {
-    write_trajectory_frame(step);
-  }
+  if (thermo) write_thermo_row(step, c);
+  if (trajectory) write_trajectory_records(step, records);
 }
 
 // thermo_output.cu:437-509.
@@ -1243,8 +1287,8 @@ void Engine::write_thermo_row(uint64_t step, const Ctrl& c) {
   }
 }
 
-// trajectory_output.cu:33-46, 148-250, 555-595 (BEADS_TRAJECTORY v1).
-void Engine::write_trajectory_frame(uint64_t step) {
+// Bytes of one trajectory record (trajectory_output.cu:33-46).
+size_t Engine::trajectory_record_bytes() const {
   const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
   size_t rec = sizeof(uint32_t);
   if (mask & BEADS_TRAJ_TYPE) rec += sizeof(int32_t);
@@ -1252,6 +1296,14 @@ void Engine::write_trajectory_frame(uint64_t step) {
   if (mask & BEADS_TRAJ_IMAGE) rec += 3 * sizeof(int32_t);
   if (mask & BEADS_TRAJ_VELOCITY) rec += 3 * sizeof(float);
   if (mask & BEADS_TRAJ_FORCE) rec += 3 * sizeof(float);
+  return rec;
+}
+
+// trajectory_output.cu:148-250, 555-595 (BEADS_TRAJECTORY v1): the header on
+// first use, then one frame of n records already in file layout.
+void Engine::write_trajectory_records(uint64_t step, const unsigned char* records) {
+  const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
+  const size_t rec = trajectory_record_bytes();
   if (!trajectory_stream_.is_open()) {
     trajectory_stream_.open(trajectory_path_, std::ios::binary | std::ios::trunc);
     if (!trajectory_stream_) {
@@ -1272,6 +1324,19 @@ void Engine::write_trajectory_frame(uint64_t step) {
     write_scalar<uint64_t>(o, n_, "trajectory particle count");
     write_scalar<uint64_t>(o, rec, "trajectory record bytes");
   }
+  auto& o = trajectory_stream_;
+  write_scalar<uint64_t>(o, step, "trajectory frame step");
+  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
+  write_raw(o, lo, 3, "trajectory box lower");
+  write_raw(o, hi, 3, "trajectory box upper");
+  write_raw(o, records, rec * n_, "trajectory records");
+}
+
+// Synchronous frame: read the state back and lay the records out on the host.
+void Engine::write_trajectory_frame(uint64_t step) {
+  const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
+  const size_t rec = trajectory_record_bytes();
   const size_t n = n_;
   std::vector<float> pos(3 * n), vel(3 * n), force(3 * n);
   std::vector<int32_t> img(3 * n), type(n);
@@ -1299,13 +1364,7 @@ void Engine::write_trajectory_frame(uint64_t step) {
     if (mask & BEADS_TRAJ_VELOCITY) put(&vel[3 * i], 12);
     if (mask & BEADS_TRAJ_FORCE) put(&force[3 * i], 12);
   }
-  auto& o = trajectory_stream_;
-  write_scalar<uint64_t>(o, step, "trajectory frame step");
-  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
-  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
-  write_raw(o, lo, 3, "trajectory box lower");
-  write_raw(o, hi, 3, "trajectory box upper");
-  write_raw(o, buf.data(), buf.size(), "trajectory records");
+  write_trajectory_records(step, buf.data());
 }
 
 // final_state_output.cu:103-185 (BEADS_SNAPSHOT v1, SoA in slot order).
@@ -1406,6 +1465,8 @@ void Engine::run() {
     return;
   }
   run_start_ = std::chrono::steady_clock::now();
+  const char* sync_output = std::getenv("BEADS_AMD_SYNC_OUTPUT");
+  async_output_ = !(sync_output != nullptr && sync_output[0] == '1');
   {
     std::ostringstream m;
     m << "execute start\n  units: " << units_.name << "\n  particles: " << n_
@@ -1466,6 +1527,7 @@ void Engine::run() {
     }
   }
   BEADS_HIP(hipGetLastError());
+  if (output_) output_->drain();  // rethrows what the writer hit
   materialize_velocities();
   const Ctrl c = fetch_ctrl();
   if (initialized_) check_overflow(c);

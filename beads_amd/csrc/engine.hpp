@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
+#include "output.hpp"
 
 namespace beads_amd {
 
@@ -128,6 +129,10 @@ class Engine {
   void stage_outputs(uint64_t step);
   void write_thermo_row(uint64_t step, const Ctrl& c);
   void write_trajectory_frame(uint64_t step);
+  size_t trajectory_record_bytes() const;
+  void write_trajectory_records(uint64_t step, const unsigned char* records);
+  void write_outputs(uint64_t step, bool thermo, bool trajectory, const Ctrl& c,
+                     const unsigned char* records);
   void write_final_state();
   void log(const std::string& level, const std::string& message);
   void poll_interrupt();
@@ -198,7 +203,10 @@ class Engine {
   ScaleParams scale_params_{};
   NeighborControl control_{};
 
-  // outputs
+  // outputs: staged through output_ (pinned slots, copy stream, writer
+  // thread) unless BEADS_AMD_SYNC_OUTPUT=1 asks for host round trips
+  bool async_output_ = true;
+  std::unique_ptr<AsyncOutput> output_;
   std::ofstream thermo_stream_;
   std::ofstream trajectory_stream_;
   std::ofstream log_stream_;

@@ -29,6 +29,7 @@
 
 #include <type_traits>
 
+#include "beads_amd.h"
 #include "device_common.hpp"
 
 namespace beads_amd {
@@ -214,6 +215,44 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
         p.cell_pos[k] = cell_entry(p.pos[v], v);
       }
     }
+  }
+}
+
+// trajectory_output.cu:183-250: one record per atom in slot order, fields in
+// the file's order (tag, type, position, image, velocity, force); the type is
+// stored 1-based as in the spec.
+__global__ __launch_bounds__(kBlock) void pack_records_kernel(PackParams p) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) {
+    return;
+  }
+  uint32_t* r = p.out + static_cast<size_t>(i) * p.words;
+  const float4 x = p.pos[i];
+  uint32_t k = 0;
+  r[k++] = p.tag[i];
+  if (p.mask & BEADS_TRAJ_TYPE) r[k++] = __float_as_uint(x.w) + 1u;
+  if (p.mask & BEADS_TRAJ_POSITION) {
+    r[k++] = __float_as_uint(x.x);
+    r[k++] = __float_as_uint(x.y);
+    r[k++] = __float_as_uint(x.z);
+  }
+  if (p.mask & BEADS_TRAJ_IMAGE) {
+    const int4 m = p.image[i];
+    r[k++] = static_cast<uint32_t>(m.x);
+    r[k++] = static_cast<uint32_t>(m.y);
+    r[k++] = static_cast<uint32_t>(m.z);
+  }
+  if (p.mask & BEADS_TRAJ_VELOCITY) {
+    const float4 v = p.vel[i];
+    r[k++] = __float_as_uint(v.x);
+    r[k++] = __float_as_uint(v.y);
+    r[k++] = __float_as_uint(v.z);
+  }
+  if (p.mask & BEADS_TRAJ_FORCE) {
+    const float4 f = p.force[i];
+    r[k++] = __float_as_uint(f.x);
+    r[k++] = __float_as_uint(f.y);
+    r[k++] = __float_as_uint(f.z);
   }
 }
 
@@ -1021,6 +1060,10 @@ void launch_cell_sort(const CellScanParams& p, hipStream_t stream) {
 void launch_reorder(const ReorderParams& p, hipStream_t stream) {
   reorder_gather_kernel<<<grid_for(p.n), kBlock, 0, stream>>>(p);
   reorder_commit_kernel<<<grid_for(p.n), kBlock, 0, stream>>>(p);
+}
+
+void launch_pack_records(const PackParams& p, hipStream_t stream) {
+  pack_records_kernel<<<grid_for(p.n), kBlock, 0, stream>>>(p);
 }
 
 void launch_build(const BuildParams& p, hipStream_t stream) {

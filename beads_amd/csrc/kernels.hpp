@@ -63,6 +63,21 @@ void launch_cell_sort(const CellScanParams& p, hipStream_t stream);
 void launch_reorder(const ReorderParams& p, hipStream_t stream);
 void launch_build(const BuildParams& p, hipStream_t stream);
 
+// Trajectory records in file order (slot order), one `words`-word record per
+// atom: tag, then type, position, image, velocity, force as `mask` selects.
+struct PackParams {
+  uint32_t n;
+  uint32_t mask;   // BEADS_TRAJ_* bits
+  uint32_t words;  // record size in 32-bit words
+  const float4* pos;
+  const float4* vel;
+  const float4* force;
+  const int4* image;
+  const uint32_t* tag;
+  uint32_t* out;
+};
+void launch_pack_records(const PackParams& p, hipStream_t stream);
+
 // One step's work.
 void launch_force(const ForceParams& p, bool kick, bool observables, hipStream_t stream);
 void launch_integrate(const IntegrateParams& p, bool thermostat, bool check,

@@ -146,3 +146,49 @@ def test_execute_with_step_graphs_writes_the_same_files(tmp_path, monkeypatch):
     rows = blobs["1"][0].decode().splitlines()
     assert [r.split(",")[0] for r in rows[1:]] == ["0", "32", "64", "96"]
     assert blobs["1"] == blobs["0"]
+
+
+def test_async_outputs_match_synchronous_outputs(tmp_path, monkeypatch):
+    """Outputs staged through pinned slots and written by the writer thread
+    (the default) are byte-identical to outputs read back and written at each
+    output step (BEADS_AMD_SYNC_OUTPUT=1): thermo CSV with all listed-term
+    columns, every trajectory field, the snapshot and the THERMO log lines.
+    Output steps every 3 or 4 steps keep all staging slots in flight."""
+    w = cases()["melt"].with_(steps=60)
+    blobs = {}
+    for sync in ("0", "1"):
+        monkeypatch.setenv("BEADS_AMD_SYNC_OUTPUT", sync)
+        sim = w.to_simulation()
+        prefix = str(tmp_path / f"s{sync}")
+        sim.set_thermo(every=4, prefix=prefix)
+        sim.set_trajectory(every=3, prefix=prefix,
+                           fields=["tag", "type", "position", "image", "velocity", "force"])
+        sim.save_final_state(prefix=prefix)
+        sim.set_logging(echo="log", prefix=prefix)
+        sim.execute()
+        files = [(tmp_path / f"s{sync}.{ext}").read_bytes()
+                 for ext in ("thermo.csv", "trajectory.beadsbin", "state.beadsbin")]
+        log = (tmp_path / f"s{sync}.log").read_text().splitlines()
+        blobs[sync] = files + [[ln for ln in log if ln.startswith("[BEADS] THERMO")]]
+    rows = blobs["0"][0].decode().splitlines()
+    assert "dihedral_pe" in rows[0]
+    assert [r.split(",")[0] for r in rows[1:]] == [str(s) for s in range(0, 61, 4)]
+    assert len(blobs["0"][3]) == 16
+    traj = read_beads_trajectory(tmp_path / "s0.trajectory.beadsbin")
+    assert [int(s) for s in traj["steps"]] == list(range(0, 61, 3))
+    for k in range(4):
+        assert blobs["0"][k] == blobs["1"][k], k
+
+
+def test_async_outputs_report_a_list_overflow(tmp_path, monkeypatch):
+    """A list overflow found by the writer thread fails the run with the same
+    error as the synchronous path, and no row of the failing step is written."""
+    w = cases()["lj_regular"].with_(steps=20, max_neighbors=8)
+    for sync in ("0", "1"):
+        monkeypatch.setenv("BEADS_AMD_SYNC_OUTPUT", sync)
+        sim = w.to_simulation()
+        sim.set_thermo(every=5, prefix=str(tmp_path / f"o{sync}"))
+        sim.set_logging(echo="none")
+        with pytest.raises(RuntimeError, match="overflow"):
+            sim.execute()
+        assert not (tmp_path / f"o{sync}.thermo.csv").exists()
