@@ -4,7 +4,8 @@
 engine (trajectory_output.cu:33-46, 148-250, 555-595; reference reader
 python/beads/io.py:233-275, 755-940) into numpy structured arrays.
 ``write_lammps_dump`` converts one to LAMMPS ``dump custom`` text
-(python/beads/io.py:278-298, 943-1028).
+(python/beads/io.py:278-298, 943-1028). ``read_lammps_data`` imports a
+molecular LAMMPS data file as a ``System`` (python/beads/io.py:97-230).
 """
 
 from __future__ import annotations
@@ -15,7 +16,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["read_beads_trajectory", "read_trajectory", "write_lammps_dump"]
+from ._lammps_data import read_lammps_data
+
+__all__ = ["read_beads_trajectory", "read_lammps_data", "read_trajectory", "write_lammps_dump"]
 
 _MAGIC = b"BEADS_TRAJECTORY"
 _ENDIAN = 0x01020304

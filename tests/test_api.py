@@ -29,7 +29,7 @@ def test_public_surface():
     for name in ("System", "ForceField", "Dynamics", "Simulation", "build_info", "io"):
         assert hasattr(beads, name)
     assert beads.__version__
-    for name in ("read_beads_trajectory", "read_trajectory", "write_lammps_dump"):
+    for name in ("read_beads_trajectory", "read_lammps_data", "read_trajectory", "write_lammps_dump"):
         assert hasattr(beads.io, name)
 
 

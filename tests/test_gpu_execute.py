@@ -192,3 +192,48 @@ def test_async_outputs_report_a_list_overflow(tmp_path, monkeypatch):
         with pytest.raises(RuntimeError, match="overflow"):
             sim.execute()
         assert not (tmp_path / f"o{sync}.thermo.csv").exists()
+
+
+def test_lammps_data_import_runs_end_to_end(tmp_path):
+    """A molecular LAMMPS data file imported with ``beads.io.read_lammps_data``
+    runs through ``Simulation.execute()``; the step-0 trajectory frame holds
+    the file's positions (binary32) and velocities in tag order, and the dump
+    export writes one LAMMPS frame per trajectory frame."""
+    from test_lammps_data import _text
+
+    data = tmp_path / "in.data"
+    data.write_text(_text())
+    system = beads.io.read_lammps_data(data, units="nm_kjmol")
+    ff = beads.ForceField()
+    ff.pair_style("lj", cutoff=0.9)
+    ff.pair_coeff("lj", type_i=1, type_j=1, epsilon=1.0, sigma=0.34)
+    ff.pair_coeff("lj", type_i=1, type_j=2, epsilon=0.8, sigma=0.36)
+    ff.pair_coeff("lj", type_i=2, type_j=2, epsilon=0.6, sigma=0.38)
+    ff.bond_style("harmonic")
+    ff.bond_coeff("harmonic", type=1, k=500.0, r0=0.35)
+    ff.angle_style("harmonic")
+    ff.angle_coeff("harmonic", type=1, k=50.0, theta0=120.0)
+    ff.dihedral_style("harmonic")
+    ff.dihedral_coeff("harmonic", type=1, k=1.0, d=1, n=1)
+    sim = beads.Simulation(system=system, forcefield=ff,
+                           dynamics=beads.Dynamics("velocity_verlet", dt=0.001))
+    sim.set_runsteps(10)
+    sim.set_neighbor(cutoff_buffer=0.25, rebuild_check_every=1, sort_every_rebuild=2,
+                     max_neighbors=64)
+    prefix = str(tmp_path / "imp")
+    sim.set_thermo(every=5, prefix=prefix)
+    sim.set_trajectory(every=5, prefix=prefix,
+                       fields=["tag", "type", "position", "image", "velocity", "force"])
+    sim.set_logging(echo="none")
+    sim.execute()
+    rows = _rows(tmp_path / "imp.thermo.csv")
+    assert [r["step"] for r in rows] == ["0", "5", "10"]
+    traj = read_beads_trajectory(tmp_path / "imp.trajectory.beadsbin")
+    rec = traj["records"][0]
+    order = np.argsort(rec["tag"])
+    np.testing.assert_array_equal(rec["position"][order], system.positions)
+    np.testing.assert_array_equal(rec["velocity"][order], system.velocities)
+    np.testing.assert_array_equal(rec["type"][order], system.types)
+    dump = tmp_path / "imp.lammpstrj"
+    beads.io.write_lammps_dump(tmp_path / "imp.trajectory.beadsbin", dump)
+    assert dump.read_text().count("ITEM: TIMESTEP") == 3

@@ -1,0 +1,215 @@
+"""``beads.io.read_lammps_data``: a molecular LAMMPS data file becomes a
+``System`` with the file's box, atoms (sorted by id), masses per type,
+velocities, image flags and topology; every unsupported or inconsistent input
+is rejected with the reference importer's exception type and message
+(python/beads/io.py:97-230, 301-753). The reference ships no fixture for the
+importer, so the files here are written by the test; the expected values are
+the numbers written into them."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import beads_amd as beads
+
+BASE = """BEADS test data  # title line
+
+12 atoms
+10 bonds
+2 angles
+1 dihedrals
+2 atom types
+1 bond types
+1 angle types
+1 dihedral types
+
+0.0 4.0 xlo xhi
+-1.0 3.0 ylo yhi
+0.0 3.5 zlo zhi
+
+Pair Coeffs # lj/cut
+1 1.0 0.34
+2 0.8 0.36
+
+Masses
+
+2 83.798
+1 39.948
+
+Atoms # molecular
+
+{atoms}
+
+Velocities
+
+{velocities}
+
+Bonds
+
+{bonds}
+
+Angles
+
+1 1 1 2 3
+2 1 7 8 9
+
+Dihedrals
+
+1 1 1 2 3 4
+"""
+
+
+def _atoms(images=True):
+    rows = []
+    # written in reverse id order: the importer sorts by atom-ID
+    for tag in range(12, 0, -1):
+        chain, bead = divmod(tag - 1, 6)
+        x, y, z = 0.5 + 0.35 * bead, 0.8 + 0.9 * chain - 1.0, 0.8 + 0.03 * ((bead % 3) - 1)
+        img = f" {bead % 2} 0 {-chain}" if images else ""
+        rows.append(f"{tag} {chain + 1} {1 + bead % 2} {x:.8g} {y:.8g} {z:.8g}{img}")
+    return "\n".join(rows)
+
+
+def _velocities():
+    return "\n".join(f"{t} {0.01 * t:.8g} {-0.02 * t:.8g} {0.5:.8g}" for t in range(1, 13))
+
+
+def _bonds():
+    out, k = [], 1
+    for chain in range(2):
+        for bead in range(5):
+            a = chain * 6 + bead + 1
+            out.append(f"{k} 1 {a} {a + 1}")
+            k += 1
+    return "\n".join(out)
+
+
+def _text(**kw):
+    parts = {"atoms": _atoms(), "velocities": _velocities(), "bonds": _bonds()}
+    parts.update(kw)
+    return BASE.format(**parts)
+
+
+def _write(tmp_path, text):
+    p = tmp_path / "in.data"
+    p.write_text(text)
+    return p
+
+
+def test_reads_molecular_file(tmp_path):
+    s = beads.io.read_lammps_data(_write(tmp_path, _text()), units="nm_kjmol")
+    assert s.n_particles == 12 and s.units == "nm_kjmol"
+    np.testing.assert_array_equal(s.box_bound, np.array([[0, -1, 0], [4, 3, 3.5]], np.float32))
+    np.testing.assert_array_equal(s.tags, np.arange(1, 13))
+    bead = np.arange(12) % 6
+    chain = np.arange(12) // 6
+    np.testing.assert_array_equal(s.types, 1 + bead % 2)
+    np.testing.assert_array_equal(s.molecule_ids, chain + 1)
+    np.testing.assert_array_equal(s.masses, np.where(bead % 2 == 0, 39.948, 83.798).astype(np.float32))
+    np.testing.assert_array_equal(s.images, np.stack([bead % 2, 0 * bead, -chain], axis=1))
+    np.testing.assert_allclose(s.positions[:, 0], 0.5 + 0.35 * bead, rtol=1e-6)
+    np.testing.assert_allclose(s.velocities[:, 1], -0.02 * np.arange(1, 13), rtol=1e-6)
+    topo = s.topology
+    assert len(topo.bonds) == 10 and len(topo.angles) == 2 and len(topo.dihedrals) == 1
+
+
+def test_optional_sections_default(tmp_path):
+    text = _text(atoms=_atoms(images=False))
+    text = text.replace("Velocities\n\n" + _velocities() + "\n", "")
+    text = text[:text.index("Masses")] + text[text.index("Atoms # molecular"):]
+    s = beads.io.read_lammps_data(_write(tmp_path, text), units="reduced")
+    np.testing.assert_array_equal(s.velocities, 0)
+    np.testing.assert_array_equal(s.masses, 1)
+    np.testing.assert_array_equal(s.images, 0)
+
+
+def test_system_from_file_builds_a_simulation_spec(tmp_path):
+    s = beads.io.read_lammps_data(_write(tmp_path, _text()), units="nm_kjmol")
+    ff = beads.ForceField()
+    ff.pair_style("lj", cutoff=0.9)
+    ff.pair_coeff("lj", type_i=1, type_j=1, epsilon=1.0, sigma=0.34)
+    ff.pair_coeff("lj", type_i=1, type_j=2, epsilon=0.8, sigma=0.36)
+    ff.pair_coeff("lj", type_i=2, type_j=2, epsilon=0.6, sigma=0.38)
+    ff.bond_style("harmonic")
+    ff.bond_coeff("harmonic", type=1, k=500.0, r0=0.35)
+    ff.angle_style("harmonic")
+    ff.angle_coeff("harmonic", type=1, k=50.0, theta0=120.0)
+    ff.dihedral_style("harmonic")
+    ff.dihedral_coeff("harmonic", type=1, k=1.0, d=1, n=1)
+    sim = beads.Simulation(system=s, forcefield=ff,
+                           dynamics=beads.Dynamics("velocity_verlet", dt=0.001))
+    sim.set_runsteps(10)
+    spec = sim._native_spec()
+    system = spec["system"]
+    assert system["n_particles"] == 12
+    np.testing.assert_array_equal(system["tags"], np.arange(1, 13))
+    assert spec["runsteps"] == 10 and "topology" in system
+
+
+CASES = [
+    ("units", None, TypeError, "requires the units keyword"),
+    ("tilt", lambda t: t.replace("0.0 3.5 zlo zhi", "0.0 3.5 zlo zhi\n0.1 0.0 0.0 xy xz yz"),
+     ValueError, "triclinic tilt factors are unsupported"),
+    ("avec", lambda t: t.replace("0.0 3.5 zlo zhi", "0.0 3.5 zlo zhi\navec 1 0 0"),
+     ValueError, "general triclinic"),
+    ("bounds width", lambda t: t.replace("0.0 4.0 xlo xhi", "0.0 4.0 5.0 xlo xhi"),
+     ValueError, "x bounds expected 4 columns, got 5"),
+    ("missing bound", lambda t: t.replace("-1.0 3.0 ylo yhi\n", ""),
+     ValueError, "missing required orthogonal box bounds"),
+    ("inverted", lambda t: t.replace("0.0 3.5 zlo zhi", "3.5 0.0 zlo zhi"),
+     ValueError, "zhi must be greater than zlo"),
+    ("labels", lambda t: t + "\nAtom Type Labels\n\n1 A\n", ValueError,
+     "type labels are not supported: Atom Type Labels"),
+    ("ellipsoids", lambda t: t + "\nEllipsoids\n\n1 1 1 1\n", ValueError,
+     "does not support sections: Ellipsoids"),
+    ("duplicate", lambda t: t + "\nDihedrals\n\n2 1 5 6 7 8\n", ValueError,
+     "section 'Dihedrals' is duplicated"),
+    ("impropers", lambda t: t.replace("1 dihedral types", "1 dihedral types\n1 impropers"),
+     ValueError, "does not support Impropers"),
+    ("style", lambda t: t.replace("Atoms # molecular", "Atoms # full"), ValueError,
+     "supports only Atoms # molecular"),
+    ("mixed images", lambda t: t.replace(" 1 0 -1\n", "\n", 1), ValueError,
+     "must not mix image flags"),
+    ("atom count", lambda t: t.replace("12 atoms", "13 atoms"), ValueError,
+     "atoms count is 13, but Atoms contains 12 rows"),
+    ("negative count", lambda t: t.replace("10 bonds", "-10 bonds"), ValueError,
+     "bonds must be non-negative"),
+    ("type count", lambda t: t.replace("2 atom types", "1 atom types"), ValueError,
+     "atom type id 2 exceeds declared atom types count 1"),
+    ("bond rows", lambda t: t.replace("10 bonds", "9 bonds"), ValueError,
+     "bonds count is 9, but the section contains 10 rows"),
+    ("unknown atom", lambda t: t.replace("1 1 1 2 3\n", "1 1 1 2 30\n"), ValueError,
+     "Angles section references unknown atom ID 30"),
+    ("bond width", lambda t: t.replace("\n1 1 1 2\n", "\n1 1 1 2 3\n"), ValueError,
+     r"Bonds line \d+ expected 4 columns \(id type atom1 atom2\), got 5"),
+    ("velocities", lambda t: t.replace("\n12 0.12 -0.24 0.5", ""), ValueError,
+     "Velocities section must include every atom exactly once"),
+    ("mass", lambda t: t.replace("1 39.948", "1 -39.948"), ValueError,
+     "masses must be positive"),
+    ("mass missing", lambda t: t.replace("1 39.948\n", ""), ValueError,
+     "Masses section is missing atom types: 1"),
+    ("coordinate", lambda t: t.replace("Atoms # molecular\n\n12 2 2", "Atoms # molecular\n\n12 2 x"),
+     ValueError, "atom-type must be an integer"),
+    ("nan", lambda t: t.replace("0.0 4.0 xlo xhi", "0.0 nan xlo xhi"), ValueError,
+     "xhi must be finite"),
+]
+
+
+@pytest.mark.parametrize("name,edit,exc,match", CASES, ids=[c[0] for c in CASES])
+def test_rejections(tmp_path, name, edit, exc, match):
+    text = _text()
+    if edit is None:
+        with pytest.raises(exc, match=match):
+            beads.io.read_lammps_data(_write(tmp_path, text), units=None)
+        return
+    edited = edit(text)
+    assert edited != text, name
+    with pytest.raises(exc, match=match):
+        beads.io.read_lammps_data(_write(tmp_path, edited), units="nm_kjmol")
+
+
+def test_molecule_ids_must_be_dense(tmp_path):
+    text = _text().replace("\n1 1 1 ", "\n1 5 1 ", 1)
+    with pytest.raises(ValueError, match="positive dense 1-based ids that do not exceed"):
+        beads.io.read_lammps_data(_write(tmp_path, text), units="nm_kjmol")
