@@ -39,6 +39,7 @@ class Spec(C.Structure):
         ("n_angle_types", C.c_int32), ("angle_k", _f32p), ("angle_theta0_deg", _f32p),
         ("n_dihedral_types", C.c_int32), ("dihedral_k", _f32p), ("dihedral_d", _i32p),
         ("dihedral_n", _i32p), ("exclusion_distance", C.c_int32),
+        ("exclusion_policy", C.c_int32),
         ("dt", C.c_float), ("berendsen", C.c_int32), ("berendsen_temperature", C.c_float),
         ("berendsen_tau", C.c_float),
         ("cutoff_buffer", C.c_float), ("rebuild_check_every", C.c_uint64),
@@ -200,6 +201,7 @@ def make_spec(w: Workload) -> tuple[Spec, list]:
     s.dihedral_d = _ptr(arr(w.dihedral_d, np.int32), C.c_int32)
     s.dihedral_n = _ptr(arr(w.dihedral_n, np.int32), C.c_int32)
     s.exclusion_distance = int(w.exclusion_distance) if len(w.bonds) else 0
+    s.exclusion_policy = 1 if s.exclusion_distance else 0  # the default policy's resolution
     s.dt = w.dt
     s.berendsen = 1 if w.berendsen else 0
     s.berendsen_temperature = w.temperature

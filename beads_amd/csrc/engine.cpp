@@ -29,6 +29,34 @@ std::string fmt9(double v) {
   return s.str();
 }
 
+// Log layout (log_output.cpp:19-21, 72-90): "  label", padded so that ": "
+// starts at column 36, then the value.
+constexpr int kLogValueColumn = 2 + 34 + 3;
+constexpr int kThermoColumnWidth = 16;
+
+template <typename T>
+std::string label_value(const std::string& label, const T& value) {
+  std::ostringstream line;
+  const std::string head = "  " + label;
+  const int pad = kLogValueColumn - static_cast<int>(head.size()) - 3;
+  line << head << (pad > 0 ? std::string(static_cast<size_t>(pad), ' ') : std::string(" "))
+       << ": " << value;
+  return line.str();
+}
+
+// '(' x ',' y ',' z ')' at the stream's default precision (log_output.cpp:65-69).
+std::string real3(const float* v) {
+  std::ostringstream s;
+  s << '(' << v[0] << ',' << v[1] << ',' << v[2] << ')';
+  return s.str();
+}
+
+std::string prec9(double v) {
+  std::ostringstream s;
+  s << std::setprecision(9) << v;
+  return s.str();
+}
+
 template <typename T>
 void write_raw(std::ofstream& out, const T* data, size_t count, const std::string& what) {
   out.write(reinterpret_cast<const char*>(data), static_cast<std::streamsize>(count * sizeof(T)));
@@ -1283,7 +1311,24 @@ void Engine::write_thermo_row(uint64_t step, const Ctrl& c) {
     fail(BEADS_E_RUNTIME, "failed to write thermo output file " + thermo_path_ + ".");
   }
   if (spec_.log_echo != BEADS_LOG_NONE) {
-    log("THERMO", row.str());
+    const int w = kThermoColumnWidth;
+    if (!thermo_header_logged_) {
+      std::ostringstream h;
+      h << std::setw(w) << "Step" << std::setw(w) << "PotEng" << std::setw(w) << "PairEng";
+      if (has_bond_style_) h << std::setw(w) << "BondEng";
+      if (has_angle_style_) h << std::setw(w) << "AngleEng";
+      if (has_dihedral_style_) h << std::setw(w) << "DihedEng";
+      h << std::setw(w) << "KinEng" << std::setw(w) << "Temp" << std::setw(w) << "Press";
+      log("THERMO", h.str());
+      thermo_header_logged_ = true;
+    }
+    std::ostringstream t;
+    t << std::setw(w) << step << std::setw(w) << fmt9(pe) << std::setw(w) << fmt9(pair);
+    if (has_bond_style_) t << std::setw(w) << fmt9(bond);
+    if (has_angle_style_) t << std::setw(w) << fmt9(angle);
+    if (has_dihedral_style_) t << std::setw(w) << fmt9(dih);
+    t << std::setw(w) << fmt9(c.ke) << std::setw(w) << fmt9(temp) << std::setw(w) << fmt9(press);
+    log("THERMO", t.str());
   }
 }
 
@@ -1456,6 +1501,99 @@ void Engine::poll_interrupt() {
   }
 }
 
+// Run-start blocks (log_output.cpp:260-348; values as simulation_run.cpp:12-55
+// gathers them).
+void Engine::log_run_start() {
+  if (spec_.log_echo == BEADS_LOG_NONE) return;
+  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const float len[3] = {box_.L[0], box_.L[1], box_.L[2]};
+  {
+    std::ostringstream m;
+    m << "execute start" << '\n' << label_value("units", units_.name) << '\n'
+      << label_value("particles", n_) << '\n' << label_value("box_lo", real3(lo)) << '\n'
+      << label_value("box_hi", real3(spec_box_hi_)) << '\n'
+      << label_value("box_lengths", real3(len)) << '\n'
+      << label_value("runsteps", spec_.runsteps);
+    log("INFO", m.str());
+  }
+  {
+    std::ostringstream m;
+    m << "dynamics summary" << '\n' << label_value("style", "velocity_verlet") << '\n'
+      << label_value("dt", prec9(static_cast<float>(spec_.dt))) << '\n'
+      << label_value("thermostat", thermostat_ ? "berendsen" : "none");
+    log("INFO", m.str());
+  }
+  {
+    std::ostringstream m;
+    m << "neighbor summary" << '\n' << label_value("cutoff_buffer", spec_.cutoff_buffer) << '\n'
+      << label_value("path", plan_.path == 0 ? "direct"
+                                            : (plan_.path == 1 ? "regular_cell" : "small_dimension"))
+      << '\n' << label_value("rebuild_check_every", spec_.rebuild_check_every) << '\n'
+      << label_value("sort_every_rebuild", spec_.sort_every_rebuild) << '\n'
+      << label_value("max_neighbors", spec_.max_neighbors);
+    log("INFO", m.str());
+  }
+  {
+    const char* policy = spec_.exclusion_policy == BEADS_EXCLUSION_DEFAULT
+                             ? "default"
+                             : (spec_.exclusion_policy == BEADS_EXCLUSION_EXPLICIT ? "explicit"
+                                                                                   : "none");
+    const std::string distance =
+        spec_.exclusion_distance > 0 ? std::to_string(spec_.exclusion_distance) : "none";
+    std::ostringstream m;
+    m << "forcefield summary" << '\n' << label_value("pair", "lj") << '\n'
+      << label_value("bond", has_bond_style_ ? "harmonic" : "none") << '\n'
+      << label_value("angle", has_angle_style_ ? "harmonic" : "none") << '\n'
+      << label_value("dihedral", has_dihedral_style_ ? "harmonic" : "none") << '\n'
+      << label_value("bonded_exclusion_policy", policy) << '\n'
+      << label_value("bonded_exclusion_distance", distance) << '\n'
+      << label_value("bonded_excluded_pairs", excluded_pairs_);
+    log("INFO", m.str());
+  }
+  {
+    std::ostringstream m;
+    m << "topology summary" << '\n' << label_value("bonds", spec_.n_bonds) << '\n'
+      << label_value("angles", spec_.n_angles) << '\n'
+      << label_value("dihedrals", spec_.n_dihedrals);
+    log("INFO", m.str());
+  }
+  {
+    const int echo = spec_.log_echo;
+    const char* echo_name =
+        echo == BEADS_LOG_SCREEN ? "screen" : (echo == BEADS_LOG_FILE ? "log" : "both");
+    auto on = [](bool b) { return b ? "enabled" : "disabled"; };
+    std::ostringstream m;
+    m << "output summary" << '\n' << label_value("thermo", on(!thermo_path_.empty())) << '\n'
+      << label_value("final_state", on(!final_state_path_.empty())) << '\n'
+      << label_value("trajectory", on(!trajectory_path_.empty())) << '\n'
+      << label_value("log", echo_name);
+    log("INFO", m.str());
+  }
+}
+
+// Run-end block (log_output.cpp:359-402).
+void Engine::log_run_end(uint64_t runsteps, double wall_s, const Ctrl& c) {
+  if (spec_.log_echo == BEADS_LOG_NONE) return;
+  std::ostringstream m;
+  m << "execute end" << '\n' << label_value("runsteps", runsteps) << '\n'
+    << label_value("run_wall_s", prec9(wall_s));
+  if (wall_s >= 60.0) {
+    const int whole = static_cast<int>(wall_s);
+    const int h = whole / 3600, mins = (whole % 3600) / 60;
+    std::ostringstream t;
+    if (h > 0) t << h << "h ";
+    t << mins << "m " << std::fixed << std::setprecision(2) << (wall_s - h * 3600.0 - mins * 60.0)
+      << "s";
+    m << '\n' << label_value("run_wall_time", t.str());
+  }
+  m << '\n'
+    << label_value("steps_per_second",
+                   runsteps ? prec9(static_cast<double>(runsteps) / wall_s) : "not_applicable")
+    << '\n' << label_value("neighbor_build_count", c.build_count) << '\n'
+    << label_value("dangerous_rebuild_count", c.dangerous);
+  log("INFO", m.str());
+}
+
 // SimulationRun::execute (simulation_run.cpp:158-244).
 void Engine::run() {
   const uint64_t runsteps = spec_.runsteps;
@@ -1467,19 +1605,7 @@ void Engine::run() {
   run_start_ = std::chrono::steady_clock::now();
   const char* sync_output = std::getenv("BEADS_AMD_SYNC_OUTPUT");
   async_output_ = !(sync_output != nullptr && sync_output[0] == '1');
-  {
-    std::ostringstream m;
-    m << "execute start\n  units: " << units_.name << "\n  particles: " << n_
-      << "\n  runsteps: " << runsteps;
-    log("INFO", m.str());
-    std::ostringstream nm;
-    nm << "neighbor summary\n  cutoff_buffer: " << plan_.cutoff_buffer << "\n  path: "
-       << (plan_.path == 0 ? "direct" : (plan_.path == 1 ? "regular_cell" : "small_dimension"))
-       << "\n  rebuild_check_every: " << plan_.check_every
-       << "\n  sort_every_rebuild: " << plan_.sort_every
-       << "\n  max_neighbors: " << plan_.max_neighbors;
-    log("INFO", nm.str());
-  }
+  log_run_start();
   const bool thermo0 = !thermo_path_.empty();
   const bool traj_force0 = !trajectory_path_.empty() && (spec_.trajectory_fields & BEADS_TRAJ_FORCE);
   if (runsteps > 0 || thermo0 || traj_force0) {
@@ -1538,14 +1664,7 @@ void Engine::run() {
   if (trajectory_stream_.is_open()) trajectory_stream_.flush();
   const double wall =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - run_start_).count();
-  std::ostringstream m;
-  m << "execute end\n  runsteps: " << runsteps << "\n  run_wall_s: " << std::setprecision(9)
-    << wall << "\n  steps_per_second: ";
-  if (runsteps) m << std::setprecision(9) << static_cast<double>(runsteps) / wall;
-  else m << "not_applicable";
-  m << "\n  neighbor_build_count: " << c.build_count
-    << "\n  dangerous_rebuild_count: " << c.dangerous;
-  log("INFO", m.str());
+  log_run_end(runsteps, wall, c);
   if (log_stream_.is_open()) log_stream_.flush();
   std::cout.flush();
 }

@@ -135,6 +135,8 @@ class Engine {
                      const unsigned char* records);
   void write_final_state();
   void log(const std::string& level, const std::string& message);
+  void log_run_start();
+  void log_run_end(uint64_t runsteps, double wall_s, const Ctrl& c);
   void poll_interrupt();
 
   // configuration
@@ -211,6 +213,7 @@ class Engine {
   std::ofstream trajectory_stream_;
   std::ofstream log_stream_;
   bool log_wrote_any_ = false;
+  bool thermo_header_logged_ = false;
   uint64_t interrupt_interval_ = 1;
   std::chrono::steady_clock::time_point run_start_;
 };

@@ -344,16 +344,26 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     distance = static_cast<int32_t>(d);
   }
   const bool has_bonds = out.c.n_bonds > 0;
+  int32_t resolved = BEADS_EXCLUSION_NONE;
   if (policy == "auto") {
-    if (distance == 0 && has_bonds) distance = 2;
+    if (distance != 0) {
+      resolved = BEADS_EXCLUSION_EXPLICIT;
+    } else if (has_bonds) {
+      distance = 2;
+      resolved = BEADS_EXCLUSION_DEFAULT;
+    }
   } else if (policy == "default") {
     if (distance == 0 && has_bonds) distance = 2;
+    resolved = BEADS_EXCLUSION_DEFAULT;
   } else if (policy == "none") {
     distance = 0;
-  } else if (policy != "explicit") {
+  } else if (policy == "explicit") {
+    resolved = BEADS_EXCLUSION_EXPLICIT;
+  } else {
     value_error("forcefield.bonded_exclusion_policy is not supported.");
   }
   out.c.exclusion_distance = distance;
+  out.c.exclusion_policy = resolved;
 }
 
 void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
@@ -414,7 +424,7 @@ std::string prefix_of(const py::dict& d, const std::string& ctx) {
 
 void parse_output(const py::dict& o, ParsedSpec& out) {
   require_only_keys(o, {"log", "thermo", "final_state", "trajectory"}, "output");
-  out.c.log_echo = BEADS_LOG_SCREEN;
+  out.c.log_echo = BEADS_LOG_NONE;  // no output.log: no log output
   if (o.contains("log")) {
     const py::dict lg = require_dict(o["log"], "output.log");
     const std::string echo = require_str(require_item(lg, "echo", "output.log"), "output.log.echo");
@@ -533,7 +543,7 @@ void execute_simulation(const py::object& spec_obj) {
     parse_dynamics(require_dict(spec["dynamics"], "dynamics"), ps);
     parse_neighbor(require_dict(spec["neighbor"], "neighbor"), ps);
     if (spec.contains("output")) parse_output(require_dict(spec["output"], "output"), ps);
-    else ps.c.log_echo = BEADS_LOG_SCREEN;
+    else ps.c.log_echo = BEADS_LOG_NONE;
     ps.c.runsteps = require_uint64(spec["runsteps"], "runsteps");
   } catch (const TypeErr& e) {
     throw py::type_error(e.what());

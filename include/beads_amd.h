@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BEADS_AMD_ABI_VERSION 1
+#define BEADS_AMD_ABI_VERSION 2
 
 typedef enum beads_status {
   BEADS_OK = 0,
@@ -45,6 +45,14 @@ typedef enum beads_log_echo {
   BEADS_LOG_BOTH = 2,
   BEADS_LOG_NONE = 3
 } beads_log_echo_t;
+
+/* Resolved bonded-exclusion policy (native_spec_parser.cpp:730-753); reported
+ * in the run-start log. */
+typedef enum beads_exclusion_policy {
+  BEADS_EXCLUSION_NONE = 0,
+  BEADS_EXCLUSION_DEFAULT = 1,
+  BEADS_EXCLUSION_EXPLICIT = 2
+} beads_exclusion_policy_t;
 
 /* Trajectory field bits (trajectory_output.cu:39-46). */
 enum {
@@ -101,6 +109,7 @@ typedef struct beads_spec {
   const int32_t* dihedral_d;     /* +1 or -1 */
   const int32_t* dihedral_n;     /* >= 0 */
   int32_t exclusion_distance;    /* 0 none, else 1..3 */
+  int32_t exclusion_policy;      /* beads_exclusion_policy_t */
 
   /* dynamics "velocity_verlet" + optional thermostat "berendsen" */
   float dt;

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_build_info():
     lib = capi.library()
-    assert lib.beads_abi_version() == 1
+    assert lib.beads_abi_version() == 2
     info = capi.BuildInfo()
     assert lib.beads_build_info(ctypes.byref(info)) == 0
     assert info.real_dtype == b"float32" and info.device_arch == b"gfx950"

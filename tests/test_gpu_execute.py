@@ -173,7 +173,7 @@ def test_async_outputs_match_synchronous_outputs(tmp_path, monkeypatch):
     rows = blobs["0"][0].decode().splitlines()
     assert "dihedral_pe" in rows[0]
     assert [r.split(",")[0] for r in rows[1:]] == [str(s) for s in range(0, 61, 4)]
-    assert len(blobs["0"][3]) == 16
+    assert len(blobs["0"][3]) == 17  # the THERMO header and 16 rows
     traj = read_beads_trajectory(tmp_path / "s0.trajectory.beadsbin")
     assert [int(s) for s in traj["steps"]] == list(range(0, 61, 3))
     for k in range(4):
@@ -237,3 +237,61 @@ def test_lammps_data_import_runs_end_to_end(tmp_path):
     dump = tmp_path / "imp.lammpstrj"
     beads.io.write_lammps_dump(tmp_path / "imp.trajectory.beadsbin", dump)
     assert dump.read_text().count("ITEM: TIMESTEP") == 3
+
+
+def test_log_file_layout(tmp_path):
+    """The text log keeps the reference's layout (log_output.cpp:19-402): six
+    INFO blocks at run start with labels padded so that ': ' starts at column
+    36, a THERMO header written once, one THERMO line per thermo row with
+    every CSV value right-aligned in 16 columns, and the execute-end block.
+    Expected strings are formatted here from the workload and the CSV."""
+    w = cases()["melt"].with_(steps=20)
+    sim = w.to_simulation()
+    prefix = str(tmp_path / "lg")
+    sim.set_thermo(every=10, prefix=prefix)
+    sim.set_logging(echo="log", prefix=prefix)
+    sim.execute()
+    text = (tmp_path / "lg.log").read_text()
+    blocks = text.split("\n\n")
+
+    def lv(label, value):
+        return f"  {label}".ljust(36) + ": " + str(value)
+
+    lo, hi = w.box_bound.astype(np.float32)
+    g3 = lambda v: "(" + ",".join(f"{float(x):g}" for x in v) + ")"
+    assert blocks[0].splitlines() == [
+        "[BEADS] INFO execute start", lv("units", "nm_kjmol"), lv("particles", w.n),
+        lv("box_lo", g3(lo)), lv("box_hi", g3(hi)), lv("box_lengths", g3(hi - lo)),
+        lv("runsteps", 20)]
+    assert blocks[1].splitlines() == [
+        "[BEADS] INFO dynamics summary", lv("style", "velocity_verlet"),
+        lv("dt", f"{float(np.float32(w.dt)):.9g}"), lv("thermostat", "berendsen")]
+    assert blocks[2].splitlines()[0] == "[BEADS] INFO neighbor summary"
+    assert blocks[2].splitlines()[1] == lv("cutoff_buffer", f"{float(np.float32(w.cutoff_buffer)):g}")
+    assert blocks[2].splitlines()[2] == lv("path", "regular_cell")
+    assert blocks[3].splitlines() == [
+        "[BEADS] INFO forcefield summary", lv("pair", "lj"), lv("bond", "harmonic"),
+        lv("angle", "harmonic"), lv("dihedral", "harmonic"),
+        lv("bonded_exclusion_policy", "explicit"), lv("bonded_exclusion_distance", 2),
+        lv("bonded_excluded_pairs", len(O.exclusion_pairs(w.n, w.bonds, 2)))]
+    assert blocks[4].splitlines() == [
+        "[BEADS] INFO topology summary", lv("bonds", len(w.bonds)), lv("angles", len(w.angles)),
+        lv("dihedrals", len(w.dihedrals))]
+    # THERMO lines follow the last INFO block without a blank line.
+    out = blocks[5].splitlines()
+    assert out[:5] == [
+        "[BEADS] INFO output summary", lv("thermo", "enabled"), lv("final_state", "disabled"),
+        lv("trajectory", "disabled"), lv("log", "log")]
+    thermo = out[5:]
+    header = ["Step", "PotEng", "PairEng", "BondEng", "AngleEng", "DihedEng", "KinEng", "Temp",
+              "Press"]
+    assert thermo[0] == "[BEADS] THERMO " + "".join(h.rjust(16) for h in header)
+    rows = (tmp_path / "lg.thermo.csv").read_text().splitlines()[1:]
+    assert thermo[1:] == ["[BEADS] THERMO " + "".join(v.rjust(16) for v in r.split(","))
+                          for r in rows]
+    end = blocks[6].splitlines()
+    assert end[0] == "[BEADS] INFO execute end" and end[1] == lv("runsteps", 20)
+    assert [ln.split(":")[0] for ln in end[2:]] == [
+        "  run_wall_s".ljust(36), "  steps_per_second".ljust(36),
+        "  neighbor_build_count".ljust(36), "  dangerous_rebuild_count".ljust(36)]
+    assert len(blocks) == 7
