@@ -14,7 +14,7 @@ import sys
 import numpy as np
 
 from ._checks import IMAGE, INDEX, REAL, TYPE, integer_array, real_array, rows_of_three
-from ._topology import EMPTY, Topology, make_topology
+from ._topology import EMPTY, Topology, make_topology, recheck
 
 UNITS = ("reduced", "nm_kjmol")
 
@@ -170,9 +170,7 @@ class System:
                             ("molecule_ids", INDEX), ("images", IMAGE)):
             object.__setattr__(self, name, _owned(getattr(self, name), dtype, readonly=True))
         topo: Topology = self.topology
-        object.__setattr__(self, "topology", make_topology(
-            bonds=topo.bonds, angles=topo.angles, dihedrals=topo.dihedrals,
-            tags=self.tags.tolist()))
+        object.__setattr__(self, "topology", recheck(topo, self.tags))
         object.__setattr__(self, "_frozen", True)
         return self
 

@@ -108,9 +108,9 @@ class Workload:
                               molecule_ids=self.molecule_ids, images=self.images,
                               box_bound=self.box_bound, units=self.units)
         if len(self.bonds) or len(self.angles) or len(self.dihedrals):
-            system.set_topology(bonds=[tuple(int(v) for v in r) for r in self.bonds],
-                                angles=[tuple(int(v) for v in r) for r in self.angles],
-                                dihedrals=[tuple(int(v) for v in r) for r in self.dihedrals])
+            system.set_topology(bonds=np.asarray(self.bonds, np.int64).reshape(-1, 3),
+                                angles=np.asarray(self.angles, np.int64).reshape(-1, 4),
+                                dihedrals=np.asarray(self.dihedrals, np.int64).reshape(-1, 5))
         ff = beads.ForceField()
         ff.pair_style("lj", cutoff=float(self.cutoff), shift=bool(self.shift))
         for a in range(self.n_types):
