@@ -82,7 +82,8 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(BinParams p) {
 // copy to the scatter cursor and clear the counts for the next rebuild. The
 // counts pass through LDS in tiles of kScanTile cells: coalesced loads (all
 // of a thread's loads in flight), a scan of kScanPer consecutive cells per
-// thread plus a Hillis-Steele scan of the thread sums, coalesced stores. Rows
+// thread, a shuffle scan of the thread sums within each wave and the wave
+// totals added from LDS (four barriers per tile), coalesced stores. Rows
 // are padded by one word per kScanPer so a thread's row starts in its own
 // bank.
 constexpr int kScanBlock = 1024;
@@ -95,9 +96,11 @@ __global__ __launch_bounds__(kScanBlock) void cell_scan_kernel(CellScanParams p)
   if (p.ctrl->rebuild == 0) {
     return;
   }
+  constexpr uint32_t kWaves = kScanBlock / 64;
   __shared__ uint32_t tile[kScanTile + kScanBlock];
-  __shared__ uint32_t sums[kScanBlock];
+  __shared__ uint32_t wave_total[kWaves];
   const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63u, wave = t >> 6;
   uint32_t carry = 0;
   for (uint32_t base = 0; base < p.n_cells; base += kScanTile) {
     const uint32_t len = min(kScanTile, p.n_cells - base);
@@ -112,15 +115,22 @@ __global__ __launch_bounds__(kScanBlock) void cell_scan_kernel(CellScanParams p)
     for (uint32_t m = 0; m < kScanPer; ++m) {
       local += tile[scan_slot(t * kScanPer + m)];
     }
-    sums[t] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
-      const uint32_t v = t >= off ? sums[t - off] : 0u;
-      __syncthreads();
-      sums[t] += v;
-      __syncthreads();
+    uint32_t incl = local;  // inclusive scan of the thread sums within the wave
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, 64);
+      incl += lane >= off ? v : 0u;
     }
-    uint32_t run = carry + sums[t] - local;
+    if (lane == 63) wave_total[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, tile_total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      const uint32_t v = wave_total[w];
+      before += w < wave ? v : 0u;
+      tile_total += v;
+    }
+    uint32_t run = carry + before + incl - local;
 #pragma unroll
     for (uint32_t m = 0; m < kScanPer; ++m) {
       const uint32_t s = scan_slot(t * kScanPer + m);
@@ -128,7 +138,7 @@ __global__ __launch_bounds__(kScanBlock) void cell_scan_kernel(CellScanParams p)
       tile[s] = run;
       run += cnt;
     }
-    carry += sums[kScanBlock - 1];
+    carry += tile_total;
     __syncthreads();
 #pragma unroll
     for (uint32_t m = 0; m < kScanPer; ++m) {
@@ -179,40 +189,41 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
     return;
   }
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t c = blockIdx.x * kSortCellsPerBlock + threadIdx.x / 64;
-  if (c >= p.n_cells) {
-    return;  // uniform over the wave
-  }
-  const uint32_t b = p.cell_start[c], e = p.cell_start[c + 1];
-  const uint32_t cnt = e - b;
   const bool stream = p.ctrl->reorder == 0;
-  uint32_t* a = p.cell_atoms;
-  if (cnt <= 64) {
-    const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
-    uint32_t rank = 0;
-    for (uint32_t m = 0; m < cnt; ++m) {
-      const uint32_t w = static_cast<uint32_t>(
-          __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(m)));
-      rank += w < v ? 1u : 0u;
-    }
-    if (lane < cnt) {
-      a[b + rank] = v;
-      if (stream) p.cell_pos[b + rank] = cell_entry(p.pos[v], v);
-    }
-  } else if (lane == 0) {
-    for (uint32_t k = b + 1; k < e; ++k) {
-      const uint32_t v = a[k];
-      uint32_t m = k;
-      while (m > b && a[m - 1] > v) {
-        a[m] = a[m - 1];
-        --m;
+  // A capped grid loops over the cells, so the launch stays small when the
+  // rebuild flag makes it exit at once.
+  for (uint32_t c = blockIdx.x * kSortCellsPerBlock + threadIdx.x / 64; c < p.n_cells;
+       c += gridDim.x * kSortCellsPerBlock) {  // uniform over the wave
+    const uint32_t b = p.cell_start[c], e = p.cell_start[c + 1];
+    const uint32_t cnt = e - b;
+    uint32_t* a = p.cell_atoms;
+    if (cnt <= 64) {
+      const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
+      uint32_t rank = 0;
+      for (uint32_t m = 0; m < cnt; ++m) {
+        const uint32_t w = static_cast<uint32_t>(
+            __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(m)));
+        rank += w < v ? 1u : 0u;
       }
-      a[m] = v;
-    }
-    if (stream) {
-      for (uint32_t k = b; k < e; ++k) {
+      if (lane < cnt) {
+        a[b + rank] = v;
+        if (stream) p.cell_pos[b + rank] = cell_entry(p.pos[v], v);
+      }
+    } else if (lane == 0) {
+      for (uint32_t k = b + 1; k < e; ++k) {
         const uint32_t v = a[k];
-        p.cell_pos[k] = cell_entry(p.pos[v], v);
+        uint32_t m = k;
+        while (m > b && a[m - 1] > v) {
+          a[m] = a[m - 1];
+          --m;
+        }
+        a[m] = v;
+      }
+      if (stream) {
+        for (uint32_t k = b; k < e; ++k) {
+          const uint32_t v = a[k];
+          p.cell_pos[k] = cell_entry(p.pos[v], v);
+        }
       }
     }
   }
@@ -1054,7 +1065,9 @@ void launch_cell_scatter(const CellScanParams& p, hipStream_t stream) {
 }
 
 void launch_cell_sort(const CellScanParams& p, hipStream_t stream) {
-  cell_sort_kernel<<<grid_for(p.n_cells, kSortCellsPerBlock), kBlock, 0, stream>>>(p);
+  constexpr unsigned kMaxSortBlocks = 4096;
+  const unsigned g = grid_for(p.n_cells, kSortCellsPerBlock);
+  cell_sort_kernel<<<g < kMaxSortBlocks ? g : kMaxSortBlocks, kBlock, 0, stream>>>(p);
 }
 
 void launch_reorder(const ReorderParams& p, hipStream_t stream) {
