@@ -180,6 +180,30 @@ class FrameModel:
                                        *params.tensors())
 
 
+def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
+                                  targets: list[torch.Tensor],
+                                  params: Params) -> tuple[torch.Tensor, Params]:
+    """force_matching_loss and its parameter gradients without an autograd
+    graph: each frame's adjoint pass runs right after its forward, while its
+    neighbour list is current, on dL/dF = 2 (F - F_target) / (frames * n * 3)
+    (the gradient autograd hands the same kernels). The per-frame FP64
+    parameter gradients are summed in FP64 and returned in the parameters'
+    dtypes. For training loops that need only dL/dtheta."""
+    eps, sigma, k, r0 = (t.detach() for t in params.tensors())
+    model._set_params(eps, sigma, k, r0)
+    scale = 1.0 / (len(frames) * model.n * 3)
+    n_p = 2 * model.n_types * model.n_types + 2 * model.n_bond_types
+    total = torch.zeros((), dtype=torch.float64, device=model.device)
+    g_sum = torch.zeros(max(n_p, 1), dtype=torch.float64, device=model.device)
+    for x, ft in zip(frames, targets):
+        f, _, _ = model._forward_raw(model._positions(x))
+        diff = f.double() - ft.to(model.device).double()
+        total += (diff * diff).sum()
+        _, gp = model._backward_raw((2.0 * scale * diff).float().contiguous(), 0.0)
+        g_sum += gp
+    return total * scale, Params(*model._split(g_sum, (eps, sigma, k, r0)))
+
+
 def force_matching_loss(model: FrameModel, frames: list[torch.Tensor],
                         targets: list[torch.Tensor], params: Params) -> torch.Tensor:
     """mean over frames, atoms and components of (F(x; params) - F_target)^2."""

@@ -114,7 +114,7 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
     import torch
 
     from beads_amd import workloads as W
-    from beads_amd.adjoint import FrameModel, force_matching_loss
+    from beads_amd.adjoint import FrameModel, force_matching_loss, force_matching_value_and_grad
 
     w = W.polymer_solvent()
     print(f"bench: adjoint leg, {w.n} beads, {frames} frames", file=sys.stderr, flush=True)
@@ -164,6 +164,13 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
         torch.cuda.synchronize()
         t_fb = time.perf_counter() - t0
         grad_norm = float(sum((t.grad.double() ** 2).sum() for t in p.tensors()) ** 0.5)
+        # the same loss and parameter gradients without an autograd graph
+        force_matching_value_and_grad(m, xs[:2], targets[:2], p)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fused_loss, _ = force_matching_value_and_grad(m, xs, targets, p)
+        torch.cuda.synchronize()
+        t_fused = time.perf_counter() - t0
         # trajectory backprop
         x0 = torch.tensor(w.positions, device="cuda", requires_grad=True)
         v0 = torch.tensor(w.velocities, device="cuda", requires_grad=True)
@@ -182,6 +189,10 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
             "forward_s": t_fwd, "forward_backward_s": t_fb,
             "gradient_over_forward": t_fb / t_fwd,
             "backward_over_forward": (t_fb - t_fwd) / t_fwd,
+            "fused_value_and_grad_s": t_fused,
+            "fused_gradient_over_forward": t_fused / t_fwd,
+            "fused_loss_rel_diff": abs(float(fused_loss) - float(loss.detach()))
+            / abs(float(loss.detach())),
             "kernel_backward_over_forward": bwd_dev / fwd_dev,
             "frame_forward_device_us": fwd_dev / calls * 1e6,
             "frame_backward_device_us": bwd_dev / calls * 1e6,

@@ -424,3 +424,30 @@ def test_frame_mode_rejects_angles_and_dynamics_mixing():
         m.forces(x, m.initial_params(requires_grad=False))
         with pytest.raises(Exception):
             m.engine.initialize()
+
+
+def test_fused_value_and_grad_matches_autograd():
+    """force_matching_value_and_grad (adjoint right after each forward, no
+    autograd graph) gives the autograd loss to FP64 rounding and the autograd
+    parameter gradients to 1e-6 (autograd accumulates the per-frame FP64
+    gradients over frames in binary32)."""
+    from beads_amd.adjoint import force_matching_value_and_grad
+
+    w = small_system(seed=11)
+    frames = _frames(w, 4, seed=13)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        xs = [torch.tensor(x, device="cuda") for x in frames]
+        with torch.no_grad():
+            ts = [m.forces(x, p)[0] * 1.05 + 0.1 * torch.randn(x.shape, generator=gen, device="cuda")
+                  for x in xs]
+        loss = force_matching_loss(m, xs, ts, p)
+        loss.backward()
+        ref = [t.grad.double().cpu().numpy() for t in p.tensors()]
+        value, grads = force_matching_value_and_grad(m, xs, ts, p)
+        got = [t.double().cpu().numpy() for t in grads.tensors()]
+        m.check()
+    assert float(value) == pytest.approx(float(loss.detach()), rel=1e-12)
+    for g, r in zip(got, ref):
+        np.testing.assert_allclose(g, r, rtol=1e-6, atol=1e-6 * max(np.abs(r).max(), 1e-30))
