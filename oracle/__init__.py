@@ -101,6 +101,8 @@ def library() -> C.CDLL:
         lib.orc_berendsen_lambda.restype = C.c_double
         lib.orc_berendsen_lambda.argtypes = [C.c_double, C.c_uint32, C.c_double, C.c_double,
                                              C.c_double, C.c_double]
+        lib.orc_displacement_exceeds.argtypes = [C.c_uint32, _f32p, _i32p, _f32p, _i32p, _f32p,
+                                                 _f32p, C.c_float]
         lib.orc_run.argtypes = [C.POINTER(System), C.POINTER(RunParams), _f64p, _f64p, _i32p, _u32p,
                                 C.POINTER(RunStats)]
         _lib = lib
@@ -288,6 +290,19 @@ def kinetic_energy(vel, masses) -> float:
 
 def berendsen_lambda(ke: float, n: int, kb: float, temperature: float, dt: float, tau: float) -> float:
     return library().orc_berendsen_lambda(ke, n, kb, temperature, dt, tau)
+
+
+def displacement_exceeds(w, pos, images, ref_pos, ref_images) -> bool:
+    """Binary32 displacement test of the rebuild tracker over all atoms."""
+    lib = library()
+    a = [np.ascontiguousarray(x, dt) for x, dt in ((pos, np.float32), (images, np.int32),
+                                                   (ref_pos, np.float32), (ref_images, np.int32))]
+    lo = np.ascontiguousarray(w.box_bound[0], np.float32)
+    hi = np.ascontiguousarray(w.box_bound[1], np.float32)
+    return bool(lib.orc_displacement_exceeds(a[0].shape[0], _p(a[0], C.c_float), _p(a[1], C.c_int32),
+                                             _p(a[2], C.c_float), _p(a[3], C.c_int32),
+                                             _p(lo, C.c_float), _p(hi, C.c_float),
+                                             w.cutoff_buffer))
 
 
 def run(w, steps: int | None = None, threads: int = 0):

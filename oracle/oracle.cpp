@@ -16,7 +16,13 @@
 //   neighbour build three paths agree with each other and with an O(N^2) scan
 //   forces          known answers (LJ minimum, U(sigma)=0, shift), central FD of
 //                   the energy, sum of forces = 0, f64-vs-f32-input agreement
+//   virials         two-body LJ and one-bond closed forms, W = -dU/dlambda under
+//                   uniform scaling of positions and box (central difference),
+//                   zero angle/dihedral virial (scale invariance)
 //   vv / berendsen  closed forms, lambda = 1 at T = T0, NVE energy drift
+//   displacement    binary32 strict threshold at the last representable value,
+//   test, cadence   image unwrapping, closed-form build/dangerous counts of a
+//                   drifting bead for check intervals 1, 3, 10 and zero skin
 #include "oracle.h"
 
 #include <algorithm>
@@ -776,6 +782,31 @@ double orc_berendsen_lambda(double ke, uint32_t n, double kb, double temperature
   const double target = kb * temperature;
   const double arg = 1.0 + (dt / tau) * (target / current - 1.0);
   return std::sqrt(std::max(0.0, arg));
+}
+
+// rebuild_tracker.cu:80-92 with the threshold of neighbor_system.cpp:213-216,
+// in binary32 like the reference's real_t: the unwrapped displacement
+// (x - x_ref) + (float)(image - image_ref) * L per axis, L = hi - lo
+// (box_geometry.cpp:9-17), and the strict test dx*dx + dy*dy + dz*dz >
+// (0.5 * skin)^2, evaluated in source order without contraction.
+int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
+                             const float* ref_pos, const int32_t* ref_image,
+                             const float lo[3], const float hi[3], float cutoff_buffer) {
+  const BoxF box = make_box(lo, hi);
+  const float half_buffer = 0.5f * cutoff_buffer;
+  const float threshold_sq = half_buffer * half_buffer;
+  for (uint32_t i = 0; i < n; ++i) {
+    float d[3];
+    for (int a = 0; a < 3; ++a) {
+      const long long di = static_cast<long long>(image[3 * i + a]) -
+                           static_cast<long long>(ref_image[3 * i + a]);
+      d[a] = (pos[3 * i + a] - ref_pos[3 * i + a]) + static_cast<float>(di) * box.L[a];
+    }
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > threshold_sq) {
+      return 1;
+    }
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -142,6 +142,13 @@ double orc_kinetic_energy(uint32_t n, const float* vel, const float* mass);
 double orc_berendsen_lambda(double ke, uint32_t n, double kb, double temperature,
                             double dt, double tau);
 
+/* rebuild_tracker.cu:80-92 + neighbor_system.cpp:213-216 in binary32: 1 if
+ * any atom's unwrapped displacement from its reference exceeds half the
+ * cutoff buffer (strict), else 0. pos/ref_pos [n][3], image/ref_image [n][3]. */
+int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
+                             const float* ref_pos, const int32_t* ref_image,
+                             const float lo[3], const float hi[3], float cutoff_buffer);
+
 /* A whole FP64 run (CPU baseline; NVE/NVT invariants). Final state in slot
  * order of the oracle's own reordering; tags_out gives the identity. */
 int orc_run(const orc_system_t* sys, const orc_run_params_t* params,

@@ -16,6 +16,7 @@ import pytest
 
 import oracle as O
 from beads_amd import workloads as W
+from gpu_cases import drifting_lattice, expected_builds
 
 
 def two_body(r: float, eps: float = 1.0, sig: float = 1.0, cutoff: float = 2.5,
@@ -304,3 +305,165 @@ def test_planted_mistake_is_caught_by_finite_differences():
     right = 24.0 / r * (2 * r ** -12 - r ** -6)
     assert f[1, 0] == pytest.approx(right, rel=1e-6)
     assert abs(f[1, 0] - wrong) > 0.4 * abs(right)
+
+
+# ------------------------------------------------------------- virials -----
+#
+# The thermo pressure is scale * (2 KE + W) / (3 V) with W the sum over the
+# terms of d . f (pair_lj.cu:138-149, bond_harmonic.cu:77). W is pinned three
+# ways: the two-body closed form of LJ, the one-bond closed form, and the
+# volume derivative W = -dU(lambda)/dlambda at lambda = 1 when every position
+# and the box are scaled by lambda (a central difference of the binary64
+# energy). Angle and dihedral energies do not change under that scaling, so
+# the total virial equals the pair plus bond parts.
+
+def _lj_pair_virial(r: float, eps: float = 1.0, sig: float = 1.0) -> float:
+    sr6 = (sig / r) ** 6
+    return 24.0 * eps * (2.0 * sr6 * sr6 - sr6)
+
+
+@pytest.mark.parametrize("shift", [False, True])
+@pytest.mark.parametrize("r,eps,sig", [(1.0, 1.0, 1.0), (1.3, 2.0, 1.1), (0.95, 0.7, 1.0),
+                                       (2.2, 1.5, 0.9)])
+def test_pair_virial_two_body_closed_form(r, eps, sig, shift):
+    w = two_body(r, eps=eps, sig=sig, shift=shift)
+    rr = float(w.positions[1, 0]) - float(w.positions[0, 0])  # binary32 inputs
+    e32, s32 = float(np.float32(eps)), float(np.float32(sig))
+    _, obs = O.forces(w)
+    assert obs["pair_virial"] == pytest.approx(_lj_pair_virial(rr, e32, s32), rel=1e-11)
+    _, obs64 = O.forces_f64(w, w.positions.astype(np.float64))
+    assert obs64["pair_virial"] == pytest.approx(_lj_pair_virial(rr, e32, s32), rel=1e-11)
+
+
+@pytest.mark.parametrize("r", [0.40, 0.47, 0.55])
+def test_bond_virial_closed_form(r):
+    pos = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+    w = W.Workload(name="bond", units="nm_kjmol", positions=pos, velocities=np.zeros((2, 3)),
+                   masses=np.ones(2), types=np.ones(2, np.int32),
+                   box_bound=np.array([[0, 0, 0], [6, 6, 6]]), lj_epsilon=[[1.0]],
+                   lj_sigma=[[0.1]], cutoff=0.2, bonds=np.array([[1, 2, 1]]), bond_k=[1250.0],
+                   bond_r0=[0.47], exclusion_distance=1, cutoff_buffer=0.0)
+    rr = float(w.positions[1, 0]) - float(w.positions[0, 0])
+    r0 = float(np.float32(0.47))
+    _, obs = O.forces(w)
+    want = -1250.0 * (rr - r0) * rr
+    assert obs["bond_virial"] == pytest.approx(want, rel=1e-11, abs=1e-12)
+    assert obs["pair_virial"] == 0.0  # the bonded pair is excluded
+
+
+def _scaled_melt() -> W.Workload:
+    """A small melt moved into a cubic box of edge 23/4 nm with lo = 0, so that
+    L (1 +- 2^-14) is exact in binary32 and positions scale about the origin."""
+    w = W.polymer_melt(n_chains=30, beads_per_chain=50, seed=5)
+    L0 = float(w.box_bound[1, 0] - w.box_bound[0, 0])
+    L = 5.75
+    pos = (w.positions.astype(np.float64) - w.box_bound[0].astype(np.float64)) * (L / L0)
+    return w.with_(positions=pos, box_bound=np.array([[0, 0, 0], [L, L, L]]), shift=True)
+
+
+def _energies_at_scale(w: W.Workload, lam: float) -> dict:
+    L = float(w.box_bound[1, 0]) * lam
+    assert float(np.float32(L)) == L
+    ws = w.with_(box_bound=np.array([[0, 0, 0], [L, L, L]]))
+    return O.forces_f64(ws, w.positions.astype(np.float64) * lam)[1]
+
+
+def test_virials_equal_minus_volume_derivative_of_energy():
+    w = _scaled_melt()
+    h = 2.0 ** -14
+    up = _energies_at_scale(w, 1.0 + h)
+    dn = _energies_at_scale(w, 1.0 - h)
+    _, obs = O.forces_f64(w, w.positions.astype(np.float64))
+    for term in ("pair", "bond"):
+        dudl = (up[f"{term}_pe"] - dn[f"{term}_pe"]) / (2.0 * h)
+        assert obs[f"{term}_virial"] == pytest.approx(-dudl, rel=2e-6), term
+        assert abs(obs[f"{term}_virial"]) > 1.0  # a non-trivial value is being checked
+    total = sum(obs[f"{t}_virial"] for t in ("pair", "bond", "angle", "dihedral"))
+    dtot = sum((up[f"{t}_pe"] - dn[f"{t}_pe"]) for t in ("pair", "bond", "angle", "dihedral"))
+    assert total == pytest.approx(-dtot / (2.0 * h), rel=2e-6)
+    # the binary32-input path reports the same virials
+    _, o32 = O.forces(w)
+    assert o32["pair_virial"] == pytest.approx(obs["pair_virial"], rel=1e-9)
+    assert o32["bond_virial"] == pytest.approx(obs["bond_virial"], rel=1e-9)
+
+
+def test_planted_virial_mistakes_fail_the_volume_derivative():
+    # A full-list pair virial that forgets the 1/2 (pair_lj.cu:149), or a bond
+    # virial with the wrong sign, must fail the pin above.
+    w = _scaled_melt()
+    h = 2.0 ** -14
+    up = _energies_at_scale(w, 1.0 + h)
+    dn = _energies_at_scale(w, 1.0 - h)
+    _, obs = O.forces_f64(w, w.positions.astype(np.float64))
+    d_pair = -(up["pair_pe"] - dn["pair_pe"]) / (2.0 * h)
+    d_bond = -(up["bond_pe"] - dn["bond_pe"]) / (2.0 * h)
+    assert 2.0 * obs["pair_virial"] != pytest.approx(d_pair, rel=2e-6)
+    assert -obs["bond_virial"] != pytest.approx(d_bond, rel=2e-6)
+
+
+# -------------------------------------------- displacement test, cadence ---
+
+def _tracker_case(dx: float, image_dx: int = 0, L: float = 16.0, skin: float = 0.3):
+    w = two_body(3.0).with_(box_bound=np.array([[0, 0, 0], [L, L, L]]), cutoff_buffer=skin)
+    ref = np.array([[5.0, 5.0, 5.0], [9.0, 9.0, 9.0]], np.float32)
+    pos = ref.copy()
+    pos[0, 0] = np.float32(np.float32(5.0) + np.float32(dx))
+    img = np.zeros((2, 3), np.int32)
+    img[0, 0] = image_dx
+    return w, pos, img, ref, np.zeros((2, 3), np.int32)
+
+
+def test_displacement_test_is_strict_binary32():
+    thr = np.float32(np.float32(0.5) * np.float32(0.3))
+    thr2 = np.float32(thr * thr)
+    # largest binary32 d with d*d <= thr2 and the next one up
+    d = np.float32(thr)
+    while np.float32(d * d) > thr2:
+        d = np.nextafter(d, np.float32(0))
+    while np.float32(np.nextafter(d, np.float32(1)) ** 2) <= thr2:
+        d = np.nextafter(d, np.float32(1))
+    up = np.nextafter(d, np.float32(1))
+    # x_ref = 0 makes x - x_ref exactly the displacement under test
+    for disp, want in ((d, False), (up, True)):
+        w, pos, img, ref, rimg = _tracker_case(0.0)
+        ref[0, 0] = np.float32(0.0)
+        pos[0, 0] = disp
+        assert O.displacement_exceeds(w, pos, img, ref, rimg) is want
+    # the same displacement split over the three axes
+    w, pos, img, ref, rimg = _tracker_case(0.0)
+    ref[0] = 0.0
+    pos[0] = np.float32(0.1)
+    assert O.displacement_exceeds(w, pos, img, ref, rimg) is True  # 0.173 > 0.15
+    pos[0] = np.float32(0.08)
+    assert O.displacement_exceeds(w, pos, img, ref, rimg) is False  # 0.139
+
+
+def test_displacement_test_unwraps_images():
+    # crossed the +x face: wrapped coordinate near 0.05, image +1, reference at 15.95
+    w, pos, img, ref, rimg = _tracker_case(0.0)
+    ref[0, 0] = np.float32(15.95)
+    pos[0, 0] = np.float32(0.05)
+    img[0, 0] = 1
+    assert O.displacement_exceeds(w, pos, img, ref, rimg) is False  # moved 0.1 < 0.15
+    img[0, 0] = 0
+    assert O.displacement_exceeds(w, pos, img, ref, rimg) is True   # 15.9 without the image
+
+
+@pytest.mark.parametrize("v_dt,check_every,steps,want", [
+    (0.15 / 10.5, 1, 60, (6, 0)),   # builds at 0, 11, 22, 33, 44, 55
+    (0.15 / 10.5, 3, 60, (6, 0)),   # builds at 0, 12, 24, 36, 48, 60
+    (0.15 / 2.5, 3, 24, (9, 8)),    # every check exceeds: all dangerous
+    (0.15 / 10.5, 10, 60, (4, 0)),  # builds at 0, 20, 40, 60, each at the second check
+    (0.15 / 9.5, 10, 60, (7, 6)),   # every check exceeds: all dangerous
+])
+def test_rebuild_cadence_closed_form(v_dt, check_every, steps, want):
+    assert expected_builds(0.15 / v_dt, check_every, steps, 0.3) == want
+    w = drifting_lattice(v_dt, check_every=check_every, steps=steps)
+    _, st = O.run(w)
+    assert (st.build_count, st.dangerous_rebuild_count) == want
+
+
+def test_zero_skin_rebuilds_every_step():
+    w = drifting_lattice(0.01, skin=0.0, steps=25)
+    _, st = O.run(w)
+    assert (st.build_count, st.dangerous_rebuild_count) == (26, 0)
