@@ -40,6 +40,51 @@ def jittered_fcc(cells: int = 5, amplitude: float = 0.05, seed: int = 1) -> W.Wo
     return w.with_(name="jittered_fcc", positions=W._wrap_into(pos, lo, hi))
 
 
+def drifting_lattice(v_dt: float, check_every: int = 1, steps: int = 60, skin: float = 0.3,
+                     spacing: float = 5.0, side: int = 4) -> W.Workload:
+    """side^3 LJ beads (reduced units, r_c 2.5) on a simple-cubic lattice whose
+    spacing exceeds r_c plus the largest drift, so no pair is ever inside r_c
+    and every force is zero. All beads rest except bead 1, which drifts along
+    +x by v_dt per step: its displacement from the last build is k * v_dt after
+    k steps, so the rebuild steps follow in closed form (expected_builds)."""
+    n = side ** 3
+    g = np.indices((side, side, side)).reshape(3, -1).T.astype(np.float64)
+    L = side * spacing
+    pos = (g + 0.5) * spacing
+    vel = np.zeros((n, 3))
+    dt = 0.005
+    vel[0, 0] = v_dt / dt
+    lo, hi = np.zeros(3), np.full(3, L)
+    return W.Workload(
+        name=f"drifting_lattice_{n}", units="reduced", positions=W._wrap_into(pos, lo, hi),
+        velocities=vel, masses=np.ones(n), types=np.ones(n, np.int32),
+        box_bound=np.stack([lo, hi]), lj_epsilon=np.ones((1, 1)), lj_sigma=np.ones((1, 1)),
+        cutoff=2.5, shift=True, dt=dt, cutoff_buffer=skin, rebuild_check_every=check_every,
+        sort_every_rebuild=10, max_neighbors=64, steps=steps)
+
+
+def expected_builds(steps_to_cross: float, check_every: int, steps: int,
+                    skin: float) -> tuple[int, int]:
+    """(build_count, dangerous_rebuild_count) after `steps` steps when the
+    largest displacement since the last build is k / steps_to_cross times half
+    the skin after k steps. The rule of neighbor_system.cpp:79-105, 198-216:
+    skin 0 rebuilds after every step; otherwise the displacement is checked
+    when step % check_every == 0, a rebuild follows when it exceeds half the
+    skin (strictly), and a rebuild at the first check after a build counts as
+    dangerous when check_every > 1. The initial build counts once."""
+    if skin == 0.0:
+        return steps + 1, 0
+    builds, dangerous, last, seen = 1, 0, 0, False
+    for step in range(1, steps + 1):
+        if step % check_every:
+            continue
+        first, seen = not seen, True
+        if step - last > steps_to_cross:
+            dangerous += int(check_every > 1 and first)
+            builds, last, seen = builds + 1, step, False
+    return builds, dangerous
+
+
 def cases() -> dict[str, W.Workload]:
     return {
         "fcc_direct": jittered_fcc(),
