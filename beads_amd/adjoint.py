@@ -189,6 +189,7 @@ def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
     (the gradient autograd hands the same kernels). The per-frame FP64
     parameter gradients are summed in FP64 and returned in the parameters'
     dtypes. For training loops that need only dL/dtheta."""
+    _check_frames(frames, targets)
     eps, sigma, k, r0 = (t.detach() for t in params.tensors())
     model._set_params(eps, sigma, k, r0)
     scale = 1.0 / (len(frames) * model.n * 3)
@@ -204,9 +205,16 @@ def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
     return total * scale, Params(*model._split(g_sum, (eps, sigma, k, r0)))
 
 
+def _check_frames(frames, targets) -> None:
+    if len(frames) == 0 or len(frames) != len(targets):
+        raise ValueError(f"force matching needs one target per frame and at least one frame; "
+                         f"got {len(frames)} frames and {len(targets)} targets")
+
+
 def force_matching_loss(model: FrameModel, frames: list[torch.Tensor],
                         targets: list[torch.Tensor], params: Params) -> torch.Tensor:
     """mean over frames, atoms and components of (F(x; params) - F_target)^2."""
+    _check_frames(frames, targets)
     total = None
     for x, ft in zip(frames, targets):
         f, _ = model.forces(x, params)

@@ -40,6 +40,95 @@ def jittered_fcc(cells: int = 5, amplitude: float = 0.05, seed: int = 1) -> W.Wo
     return w.with_(name="jittered_fcc", positions=W._wrap_into(pos, lo, hi))
 
 
+def example_chain(distance: int, beads: int = 8) -> W.Workload:
+    """The reference's bonded-exclusion example (examples/05_bonded_exclusion_switch.py
+    with examples/_systems.py:123-183): one zig-zag chain of 8 beads, bond
+    length 0.35 nm, LJ eps 0.5 sigma 0.34 r_c 1.2, harmonic bonds k 0 r0 0.35,
+    nm_kjmol, exclusions to the given bond distance. Every axis has fewer than
+    three cells, so the engine takes the direct path."""
+    pos = np.array([[1.0 + b * 0.35, 1.0 + (0.12 if b % 2 == 0 else -0.12),
+                     1.0 + 0.08 * float((b % 3) - 1)] for b in range(beads)])
+    box = np.array([[0.0, 0.0, 0.0], [2.0 + 0.35 * (beads - 1), 4.0, 4.0]])
+    tags = np.arange(1, beads + 1, dtype=np.uint32)
+    bonds = np.stack([tags[:-1], tags[1:], np.ones(beads - 1, np.uint32)], axis=1)
+    rng = np.random.default_rng(5)
+    return W.Workload(
+        name=f"example_chain_d{distance}", units="nm_kjmol", positions=pos,
+        velocities=rng.uniform(-0.01, 0.01, (beads, 3)), masses=np.ones(beads),
+        types=np.ones(beads, np.int32), box_bound=box, lj_epsilon=[[0.5]], lj_sigma=[[0.34]],
+        cutoff=1.2, bonds=bonds, bond_k=[0.0], bond_r0=[0.35], exclusion_distance=distance,
+        dt=0.001, steps=0)
+
+
+def bonded_lattice(kind: str, distance: int, side: int = 15, density: float = 0.8442,
+                   seed: int = 31) -> W.Workload:
+    """LJ beads (reduced units) on a jittered full cubic lattice of side^3 sites,
+    with harmonic bonds (k 100, r0 = lattice spacing) between lattice
+    neighbours, exclusions to the given bond distance:
+      chains  serpentine runs of `side` beads: 2 d partners per interior bead
+              (distance 3: six, the engine's 8-entry register variant);
+      stars   a hub every 5 sites per axis with six arms of two beads along
+              the lattice axes: the hub has 12 partners at distance 2 (the
+              engine's variant that reads exclusions from memory)."""
+    n = side ** 3
+    L = (n / density) ** (1.0 / 3.0)
+    spacing = L / side
+    rng = np.random.default_rng(seed)
+    if kind == "chains":
+        g = W._serpentine(side).astype(np.float64)
+        tags = np.arange(1, n + 1, dtype=np.uint32).reshape(-1, side)
+        bonds = np.stack([tags[:, :-1], tags[:, 1:], np.ones_like(tags[:, 1:])],
+                         axis=-1).reshape(-1, 3)
+    elif kind == "stars":
+        assert side % 5 == 0
+        g = np.indices((side, side, side)).reshape(3, -1).T[:, ::-1].astype(np.float64)
+        # site index s = x + side (y + side z); tag = s + 1
+        def tag(x, y, z):
+            return int((x % side) + side * ((y % side) + side * (z % side))) + 1
+        bl = []
+        for hx in range(2, side, 5):
+            for hy in range(2, side, 5):
+                for hz in range(2, side, 5):
+                    hub = tag(hx, hy, hz)
+                    for ax in range(3):
+                        for sgn in (-1, 1):
+                            d1 = [0, 0, 0]
+                            d1[ax] = sgn
+                            a1 = tag(hx + d1[0], hy + d1[1], hz + d1[2])
+                            a2 = tag(hx + 2 * d1[0], hy + 2 * d1[1], hz + 2 * d1[2])
+                            bl.append((min(hub, a1), max(hub, a1), 1))
+                            bl.append((min(a1, a2), max(a1, a2), 1))
+        bonds = np.array(sorted(bl), np.uint32)
+    else:
+        raise ValueError(kind)
+    pos = (g + 0.5) * spacing + rng.uniform(-0.05, 0.05, (n, 3)) * spacing
+    lo, hi = np.zeros(3), np.full(3, L)
+    masses = np.ones(n)
+    return W.Workload(
+        name=f"bonded_{kind}_d{distance}", units="reduced", positions=W._wrap_into(pos, lo, hi),
+        velocities=W.maxwell_boltzmann(rng, masses.astype(np.float32), 1.0, 1.0), masses=masses,
+        types=np.ones(n, np.int32), box_bound=np.stack([lo, hi]), lj_epsilon=np.ones((1, 1)),
+        lj_sigma=np.ones((1, 1)), cutoff=2.5, shift=True, bonds=bonds, bond_k=[100.0],
+        bond_r0=[spacing], exclusion_distance=distance, dt=0.005, steps=100)
+
+
+def exclusion_counts(w: W.Workload) -> np.ndarray:
+    """Per-tag number of beads within the exclusion bond distance (brute-force
+    breadth-first search over the bond graph)."""
+    adj = [[] for _ in range(w.n + 1)]
+    for a, b, _ in np.asarray(w.bonds, np.int64):
+        adj[a].append(b)
+        adj[b].append(a)
+    out = np.zeros(w.n + 1, np.int64)
+    for s in range(1, w.n + 1):
+        seen, frontier = {s}, [s]
+        for _ in range(w.exclusion_distance):
+            frontier = [q for t in frontier for q in adj[t] if q not in seen]
+            seen.update(frontier)
+        out[s] = len(seen) - 1
+    return out[1:]
+
+
 def drifting_lattice(v_dt: float, check_every: int = 1, steps: int = 60, skin: float = 0.3,
                      spacing: float = 5.0, side: int = 4) -> W.Workload:
     """side^3 LJ beads (reduced units, r_c 2.5) on a simple-cubic lattice whose

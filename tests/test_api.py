@@ -258,3 +258,16 @@ def test_native_parser_accepts_a_full_spec(tmp_path):
         _native.execute_simulation(sim._native_spec())
     except RuntimeError as exc:
         assert "HIP" in str(exc) or "device" in str(exc), exc
+
+
+def test_force_matching_rejects_mismatched_or_empty_frames():
+    import torch
+
+    from beads_amd.adjoint import force_matching_loss, force_matching_value_and_grad
+
+    x = torch.zeros((4, 3))
+    for fn in (force_matching_loss, force_matching_value_and_grad):
+        with pytest.raises(ValueError, match="one target per frame"):
+            fn(None, [x, x], [x], None)
+        with pytest.raises(ValueError, match="at least one frame"):
+            fn(None, [], [], None)

@@ -85,3 +85,39 @@ def test_autotuner_picks_a_candidate_and_restores_state():
         np.testing.assert_array_equal(np.sort(st["tags"]), np.arange(1, ws[2].n + 1))
         # after the autotuner the batch restarts from the initial state
         assert b.stats(2).step == 0
+
+
+def test_full_replica_set_against_single_engines_and_oracle():
+    """The 512-replica set of the batched benchmark (N log-uniform in
+    [500, 20000], 2.6M atoms in all) for 20 steps in one launch. On a sample
+    that includes the largest and the smallest replica: build and reorder
+    counts equal to a single engine's over the same 20 steps, positions within
+    1e-4 of its trajectory, and forces within the force bar of the FP64
+    oracle at the batch's own final positions. Every replica ends without
+    overflow at step 20."""
+    ws = W.replica_set(512, steps=20)
+    sizes = np.array([w.n for w in ws])
+    sample = sorted({int(np.argmax(sizes)), int(np.argmin(sizes)), 1, 100, 257, 511})
+    steps = 20
+    with Batch(ws) as b:
+        b.run(steps, observables_on_last=True)
+        for r in range(len(ws)):
+            s = b.stats(r)
+            assert s.step == steps and s.overflow == 0, r
+        for r in sample:
+            w = ws[r]
+            st = b.state(r)
+            bs = b.stats(r)
+            with Engine(w) as e:
+                e.initialize(observables=False)
+                e.advance(steps, observables_on_last=True)
+                single = e.state()
+                ss = e.stats()
+            assert (bs.build_count, bs.reorder_count) == (ss.build_count, ss.reorder_count), r
+            order_b, order_s = np.argsort(st["tags"]), np.argsort(single["tags"])
+            L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+            dx = st["positions"][order_b].astype(np.float64) - single["positions"][order_s]
+            dx -= np.round(dx / L) * L
+            assert np.max(np.abs(dx)) < 1e-4, r
+            f_ref, _ = O.forces(w, positions=st["positions"], tags=st["tags"], types=st["types"])
+            assert _force_error(st["forces"].astype(np.float64), f_ref) <= FORCE_RTOL, r

@@ -197,3 +197,32 @@ def test_step_graph_replays_match_eager_launches(name, monkeypatch):
         np.testing.assert_array_equal(a["velocities"], b["velocities"])
         np.testing.assert_array_equal(a["forces"], b["forces"])
         assert list(sa.pe) == list(sb.pe) and sa.build_count == sb.build_count
+
+
+def test_lj_fluid_fcc_config_full_run():
+    """The 4,000-bead FCC LJ fluid exactly as configured for the small
+    benchmark (check every 10 steps, 1000 steps of dt 0.005, NVE): total
+    energy drift below 1e-6 per step, and at the end the neighbour sets are
+    bit-exact and the forces within the force bar of the FP64 oracle at the
+    engine's final positions. The lattice starts with zero force on every
+    bead, so the force bar's scale there is the end state's rms force."""
+    w = W.lj_fluid_fcc()
+    assert w.n == 4000 and w.rebuild_check_every == 10 and w.steps == 1000
+    with _engine(w) as e:
+        s = e.stats()
+        e0 = s.ke + sum(s.pe)
+        for _ in range(10):
+            e.advance(100, observables_on_last=True)
+        s = e.stats()
+        st = e.state()
+        counts, lst = e.neighbors()
+    assert s.step == 1000 and s.overflow == 0
+    assert s.build_count >= 2 and s.dangerous_rebuild_count < s.build_count
+    assert abs(s.ke + sum(s.pe) - e0) / abs(e0) / 1000 < 1e-6
+    rc, rl, _ = O.neighbor_lists(w, positions=st["positions"], tags=st["tags"], types=st["types"])
+    np.testing.assert_array_equal(counts, rc)
+    for i in range(w.n):
+        assert np.array_equal(np.sort(lst[: counts[i], i]), np.sort(rl[i, : rc[i]])), i
+    f_ref, obs = O.forces(w, positions=st["positions"], tags=st["tags"], types=st["types"])
+    assert _force_error(st["forces"].astype(np.float64), f_ref) <= FORCE_RTOL
+    assert abs(s.pe[0] - obs["pair_pe"]) <= ENERGY_RTOL * abs(obs["pair_pe"])
