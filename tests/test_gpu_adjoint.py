@@ -491,7 +491,7 @@ def test_full_size_force_matching_and_energy_gradients():
         loss.backward()
         m.check()
         grads = [t.grad.cpu().numpy().astype(np.float64) for t in p.tensors()]
-        print(f"loss {float(loss):.6e}", flush=True)
+        print(f"loss {float(loss.detach()):.6e}", flush=True)
         q = m.initial_params(requires_grad=False)
         x = xs[0].clone().requires_grad_(True)
         f, e = m.forces(x, q)
