@@ -113,6 +113,8 @@ SIGNATURES = {
     "beads_engine_frame_load_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "beads_engine_frame_backward_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double,
                                                     C.c_void_p, C.c_void_p]),
+    "beads_engine_frame_backward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double,
+                                                      C.c_void_p, C.c_void_p]),
     "beads_engine_destroy": (None, [C.c_void_p]),
     "beads_batch_create": (C.c_int, [C.POINTER(Spec), C.c_uint32, C.POINTER(C.c_void_p)]),
     "beads_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32),
@@ -348,6 +350,13 @@ class Engine:
     def frame_backward_async(self, grad_forces_ptr: int, grad_energy: float,
                              grad_positions_ptr: int, grad_params_ptr: int) -> None:
         _check(self.lib.beads_engine_frame_backward_async(
+            self.handle, C.c_void_p(grad_forces_ptr), float(grad_energy),
+            C.c_void_p(grad_positions_ptr), C.c_void_p(grad_params_ptr)))
+
+    def frame_backward_noalloc(self, grad_forces_ptr: int, grad_energy: float,
+                               grad_positions_ptr: int, grad_params_ptr: int) -> None:
+        """frame_backward_async without any device allocation (static scratch)."""
+        _check(self.lib.beads_engine_frame_backward_noalloc(
             self.handle, C.c_void_p(grad_forces_ptr), float(grad_energy),
             C.c_void_p(grad_positions_ptr), C.c_void_p(grad_params_ptr)))
 

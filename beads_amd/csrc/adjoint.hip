@@ -23,12 +23,25 @@
 // Every bond is seen from both atoms, so each side adds half of dE/dk, dE/dr0.
 #include "adjoint.hpp"
 
+#include <mutex>
+
+#include "common.hpp"
 #include "device_common.hpp"
 
 namespace beads_amd {
 namespace {
 
 using namespace dev;
+
+// Staging of dL/dF (float4 per atom) for frame_backward's allocation-free
+// variant: one copy per GPU, held until the process ends. Host code must use
+// the address from hipGetSymbolAddress (scratch_address below), never this
+// name, which in host code is only a stand-in for the device symbol.
+static __device__ __align__(256) float4 scratch[kNoAllocMaxAtoms];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float4* g_scratch_ptr[kMaxDevices] = {};
 
 template <bool CHECK>
 __global__ __launch_bounds__(kBlock) void frame_load_kernel(FrameLoadParams p) {
@@ -73,6 +86,11 @@ __global__ __launch_bounds__(kBlock) void frame_store_kernel(uint32_t n, const f
   xyz[3 * i] = f.x;
   xyz[3 * i + 1] = f.y;
   xyz[3 * i + 2] = f.z;
+}
+
+__global__ __launch_bounds__(kBlock) void fill_f64_kernel(double* out, uint32_t count) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < count) out[k] = 0.0;
 }
 
 __global__ __launch_bounds__(kBlock) void pack_xyz_kernel(uint32_t n, const float* xyz,
@@ -257,6 +275,29 @@ void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream)
 
 void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream) {
   frame_store_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, force, xyz);
+}
+
+void launch_fill_f64(double* out, uint32_t count, hipStream_t stream) {
+  if (count == 0) return;
+  fill_f64_kernel<<<grid_for(count), kBlock, 0, stream>>>(out, count);
+}
+
+// Device address of this GPU's scratch array, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+float4* scratch_address() {
+  int device = 0;
+  BEADS_HIP(hipGetDevice(&device));
+  require(device >= 0 && device < kMaxDevices, "device index out of range for the scratch table",
+          BEADS_E_RUNTIME);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    BEADS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, "hipGetSymbolAddress returned no address for the scratch array",
+            BEADS_E_DEVICE);
+    g_scratch_ptr[device] = static_cast<float4*>(ptr);
+  }
+  return g_scratch_ptr[device];
 }
 
 void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream) {

@@ -10,6 +10,10 @@ namespace beads_amd {
 
 constexpr int kMaxBondTypesAdjoint = 64;
 
+// Largest system frame_backward's allocation-free variant serves: its dL/dF
+// staging lives in a static device array of this many float4 (16 MiB per GPU).
+constexpr uint32_t kNoAllocMaxAtoms = 1u << 20;
+
 struct AdjointParams {
   uint32_t n;
   const float4* pos;        // positions, .w = type bits
@@ -52,6 +56,11 @@ struct FrameLoadParams {
 void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream);
 void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream);
 void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream);
+// out[0, count) = 0 with a kernel (no memset node when a caller captures the stream).
+void launch_fill_f64(double* out, uint32_t count, hipStream_t stream);
+// This GPU's static staging array (kNoAllocMaxAtoms float4), address resolved
+// once per device; allocates nothing.
+float4* scratch_address();
 void launch_fm_backward(const AdjointParams& p, hipStream_t stream);
 
 }  // namespace beads_amd

@@ -237,6 +237,18 @@ beads_status_t beads_engine_frame_backward_async(beads_engine_t* e, const float*
   });
 }
 
+beads_status_t beads_engine_frame_backward_noalloc(beads_engine_t* e, const float* grad_forces,
+                                                   double grad_energy, float* grad_positions,
+                                                   double* grad_params) {
+  return guarded([&] {
+    beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",
+                       BEADS_E_TYPE);
+    beads_amd::require(std::isfinite(grad_energy), "grad_energy must be finite");
+    engine_of(e).frame_backward(grad_forces, grad_energy, grad_positions, grad_params, true,
+                                true);
+  });
+}
+
 void beads_engine_destroy(beads_engine_t* e) {
   if (e != nullptr) {
     delete e->impl;

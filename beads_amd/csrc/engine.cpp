@@ -1142,7 +1142,8 @@ void Engine::frame_load(const float* positions_dev, bool reuse_list, bool async)
 // grad_energy scales dE/d(.), outputs dL/dx [n][3] binary32 and the FP64
 // parameter gradients [eps T*T | sigma T*T | k nb | r0 nb] (device).
 void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
-                            float* grad_positions_dev, double* grad_params_dev, bool async) {
+                            float* grad_positions_dev, double* grad_params_dev, bool async,
+                            bool noalloc) {
   require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
   require(!half_list_, "the adjoint reads the full neighbour list; create the engine with "
           "half_list = 0.", BEADS_E_VALUE);
@@ -1160,14 +1161,30 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
     }
   }
   const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
-  grad_force4_.resize(n_);
-  BEADS_HIP(hipEventRecord(frame_events_[2], stream_));
-  BEADS_HIP(hipMemsetAsync(grad_params_dev, 0, n_params * sizeof(double), stream_));
-  launch_pack_xyz(n_, grad_forces_dev, grad_force4_.data(), stream_);
+  float4* staging = nullptr;
+  if (noalloc) {
+    require(n_ <= kNoAllocMaxAtoms,
+            "frame_backward_noalloc serves at most " + std::to_string(kNoAllocMaxAtoms) +
+                " particles (its static scratch); this engine has " + std::to_string(n_) + ".",
+            BEADS_E_VALUE);
+    staging = scratch_address();
+    async = true;
+  } else {
+    grad_force4_.resize(n_);
+    staging = grad_force4_.data();
+  }
+  if (noalloc) {
+    // no timing events either: nothing but kernel launches on the stream
+    launch_fill_f64(grad_params_dev, static_cast<uint32_t>(n_params), stream_);
+  } else {
+    BEADS_HIP(hipEventRecord(frame_events_[2], stream_));
+    BEADS_HIP(hipMemsetAsync(grad_params_dev, 0, n_params * sizeof(double), stream_));
+  }
+  launch_pack_xyz(n_, grad_forces_dev, staging, stream_);
   AdjointParams p{};
   p.n = n_;
   p.pos = pos_.data();
-  p.g = grad_force4_.data();
+  p.g = staging;
   p.nbr_count = nbr_count_.data();
   p.nbr = nbr_.data();
   p.box = box_;
@@ -1187,6 +1204,8 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   p.grad_pos = grad_positions_dev;
   p.grad_params = grad_params_dev;
   launch_fm_backward(p, stream_);
+  BEADS_HIP(hipGetLastError());
+  if (noalloc) return;
   BEADS_HIP(hipEventRecord(frame_events_[3], stream_));
   BEADS_HIP(hipGetLastError());
   if (async) {

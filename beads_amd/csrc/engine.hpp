@@ -102,8 +102,11 @@ class Engine {
   void frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
                     bool reuse_list, bool async = false);
   void frame_load(const float* positions_dev, bool reuse_list, bool async = false);
+  // noalloc: stage dL/dF in the library's static per-GPU scratch array instead
+  // of an engine buffer and clear grad_params with a kernel; implies async.
   void frame_backward(const float* grad_forces_dev, double grad_energy,
-                      float* grad_positions_dev, double* grad_params_dev, bool async = false);
+                      float* grad_positions_dev, double* grad_params_dev, bool async = false,
+                      bool noalloc = false);
   void frame_timing(double* forward_s, double* backward_s);
   // Enqueue frame-mode work on the caller's stream (nullptr: the engine's own).
   void set_stream(hipStream_t stream);

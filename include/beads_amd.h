@@ -278,6 +278,20 @@ beads_status_t beads_engine_frame_load_async(beads_engine_t* engine, const float
 beads_status_t beads_engine_frame_backward_async(beads_engine_t* engine,
                                                  const float* grad_forces, double grad_energy,
                                                  float* grad_positions, double* grad_params);
+/* Allocation-free variant of beads_engine_frame_backward_async. No counterpart
+ * in the reference; it is for callers that cannot allocate device memory once
+ * set up: code under HIP graph capture (a captured training step), tight
+ * serving loops, harnesses that forbid hipMalloc. Same arguments, meaning and
+ * stream order as the async call, but dL/dF is staged in the library's static
+ * per-GPU scratch array (16 MiB, at most 1,048,576 particles; a larger engine
+ * gets BEADS_E_VALUE) instead of an engine buffer allocated on first use,
+ * grad_params is cleared by a kernel rather than a memset, and no timing
+ * events are recorded (beads_engine_frame_timing does not cover these calls).
+ * The scratch keeps the last call's contents and is shared by every engine on
+ * that GPU: calls that overlap on one GPU (engines on concurrent streams) race. */
+beads_status_t beads_engine_frame_backward_noalloc(beads_engine_t* engine,
+                                                   const float* grad_forces, double grad_energy,
+                                                   float* grad_positions, double* grad_params);
 void beads_engine_destroy(beads_engine_t* engine);
 
 /* ---- batched replicas ----
