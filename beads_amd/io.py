@@ -10,6 +10,7 @@ molecular LAMMPS data file as a ``System`` (python/beads/io.py:97-230).
 
 from __future__ import annotations
 
+import operator
 import os
 import struct
 from dataclasses import dataclass
@@ -29,7 +30,7 @@ _KNOWN = sum(bit for _, bit in _FIELD_BITS)
 @dataclass(frozen=True)
 class _Header:
     version: int
-    endian: int
+    endian: str  # "<" or ">", from the marker
     real_size: int
     index_size: int
     image_size: int
@@ -45,66 +46,75 @@ class _Header:
     frame_dtype: np.dtype
 
 
-def _int_dtype(size: int, signed: bool, label: str) -> np.dtype:
-    table = {(4, True): np.int32, (8, True): np.int64, (4, False): np.uint32, (8, False): np.uint64}
-    if (size, signed) not in table:
-        raise ValueError(f"Trajectory {label} size {size} is not supported.")
-    return np.dtype(table[(size, signed)])
+# Sized numpy kinds of the header's element sizes; the messages are the
+# reference reader's (python/beads/io.py:895-918).
+def _sized(kind: str, size: int, endian: str, label: str) -> np.dtype:
+    if size not in (4, 8):
+        raise ValueError(f"Unsupported BEADS trajectory {label} size: {size}.")
+    return np.dtype(f"{endian}{kind}{size}")
 
 
 def _read_header(handle) -> _Header:
+    """Header of a BEADS_TRAJECTORY v1 file in either byte order: the marker
+    0x01020304 read back in little- or big-endian order names the order of
+    every later field (the reference reader's rule, python/beads/io.py:788-893)."""
     def take(n: int, label: str) -> bytes:
         data = handle.read(n)
         if len(data) != n:
-            raise ValueError(f"Trajectory header ended while reading {label}.")
+            raise ValueError(f"Trajectory file is truncated while reading {label}.")
         return data
 
-    if take(len(_MAGIC), "magic") != _MAGIC:
-        raise ValueError("Not a BEADS trajectory file (magic mismatch).")
-    version, endian, real_size, index_size, image_size, type_size, tag_size = struct.unpack(
-        "=7I", take(28, "sizes"))
+    magic = take(len(_MAGIC), "magic")
+    if magic != _MAGIC:
+        raise ValueError(f"Unsupported BEADS trajectory magic: {magic!r}.")
+    version_raw = take(4, "version")
+    marker_raw = take(4, "endian marker")
+    endian = next((e for e in "<>" if struct.unpack(e + "I", marker_raw)[0] == _ENDIAN), None)
+    if endian is None:
+        raise ValueError("Unsupported BEADS trajectory endian marker.")
+    (version,) = struct.unpack(endian + "I", version_raw)
     if version != 1:
-        raise ValueError(f"Trajectory version {version} is not supported.")
-    if endian != _ENDIAN:
-        raise ValueError("Trajectory endian marker is not supported.")
-    (units_len,) = struct.unpack("=Q", take(8, "units length"))
+        raise ValueError(f"Unsupported BEADS trajectory version: {version}. "
+                         f"This reader supports version 1.")
+    sizes = []
+    for label in ("real size", "index size", "image size", "type size", "tag size"):
+        sizes.append(struct.unpack(endian + "I", take(4, label))[0])
+    real_size, index_size, image_size, type_size, tag_size = sizes
+    (units_len,) = struct.unpack(endian + "Q", take(8, "units length"))
+    if units_len > 4096:
+        raise ValueError("BEADS trajectory units string is unreasonably long.")
     units = take(units_len, "units").decode("ascii")
-    (mask,) = struct.unpack("=I", take(4, "field mask"))
-    n, record_bytes = struct.unpack("=QQ", take(16, "counts"))
-    if mask & ~_KNOWN or not mask & 1:
-        raise ValueError("Trajectory field mask is not supported.")
-    if real_size not in (4, 8):
-        raise ValueError(f"Trajectory real size {real_size} is not supported.")
-    real = np.dtype(np.float32 if real_size == 4 else np.float64)
-    parts = [("tag", _int_dtype(tag_size, False, "tag"))]
-    fields = ["tag"]
-    if mask & 2:
-        parts.append(("type", _int_dtype(type_size, True, "type")))
-        fields.append("type")
-    if mask & 4:
-        parts.append(("position", real, (3,)))
-        fields.append("position")
-    if mask & 8:
-        parts.append(("image", _int_dtype(image_size, True, "image"), (3,)))
-        fields.append("image")
-    if mask & 16:
-        parts.append(("velocity", real, (3,)))
-        fields.append("velocity")
-    if mask & 32:
-        parts.append(("force", real, (3,)))
-        fields.append("force")
+    (mask,) = struct.unpack(endian + "I", take(4, "field mask"))
+    (n,) = struct.unpack(endian + "Q", take(8, "particle count"))
+    (record_bytes,) = struct.unpack(endian + "Q", take(8, "record bytes"))
+    if mask & ~_KNOWN:
+        raise ValueError("BEADS trajectory field mask contains unsupported bits.")
+    if not mask & 1:
+        raise ValueError("BEADS trajectory field mask must include tag.")
+    if n <= 0:
+        raise ValueError("BEADS trajectory particle count must be positive.")
+    real = _sized("f", real_size, endian, "real")
+    parts = [("tag", _sized("u", tag_size, endian, "tag"))]
+    image = _sized("i", image_size, endian, "image")
+    kinds = {"type": _sized("i", type_size, endian, "type"), "position": (real, (3,)),
+             "image": (image, (3,)), "velocity": (real, (3,)), "force": (real, (3,))}
+    fields = tuple(name for name, bit in _FIELD_BITS if mask & bit)
+    for name in fields[1:]:
+        kind = kinds[name]
+        parts.append((name, *kind) if isinstance(kind, tuple) else (name, kind))
     record = np.dtype(parts)
     if record.itemsize != record_bytes:
-        raise ValueError("Trajectory record size does not match its field mask.")
-    frame = np.dtype([("step", np.uint64), ("box_bound", real, (2, 3)), ("records", record, (n,))])
+        raise ValueError("BEADS trajectory record byte count is inconsistent with its field mask.")
+    frame = np.dtype([("step", np.dtype(endian + "u8")), ("box_bound", real, (2, 3)),
+                      ("records", record, (n,))])
     start = handle.tell()
     handle.seek(0, os.SEEK_END)
     payload = handle.tell() - start
     handle.seek(start)
-    if frame.itemsize == 0 or payload % frame.itemsize:
-        raise ValueError("Trajectory payload is not a whole number of frames.")
+    if payload % frame.itemsize:
+        raise ValueError("BEADS trajectory file size is inconsistent with its header.")
     return _Header(version, endian, real_size, index_size, image_size, type_size, tag_size, units,
-                   mask, tuple(fields), int(n), int(record_bytes), payload // frame.itemsize,
+                   mask, fields, int(n), int(record_bytes), payload // frame.itemsize,
                    record, frame)
 
 
@@ -139,8 +149,14 @@ def write_lammps_dump(trajectory_path: object, dump_path: object, *, sort_by_tag
     """Convert a BEADS trajectory to LAMMPS ``dump custom`` text."""
     if not isinstance(sort_by_tag, bool):
         raise TypeError("write_lammps_dump(...) sort_by_tag must be a bool.")
-    if isinstance(precision, bool) or not isinstance(precision, int) or not 1 <= precision <= 17:
-        raise ValueError("write_lammps_dump(...) precision must be an integer in 1..17.")
+    if isinstance(precision, (bool, np.bool_)):
+        raise TypeError("write_lammps_dump(...) precision must be an integer, not bool.")
+    try:
+        precision = int(operator.index(precision))
+    except TypeError as exc:
+        raise TypeError("write_lammps_dump(...) precision must be an integer.") from exc
+    if not 1 <= precision <= 17:
+        raise ValueError("write_lammps_dump(...) precision must be between 1 and 17.")
     traj = read_beads_trajectory(trajectory_path)
     fields = traj["fields"]
     columns = ["id"]

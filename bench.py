@@ -6,7 +6,10 @@ One step = one velocity-Verlet step of the whole system (pre-force half-kick,
 drift, wrap, displacement check, predicated neighbour rebuild, fused LJ +
 bonded forces, post-force half-kick, Berendsen factor). Inputs are resident in
 HBM before the timed region; the timed region is bracketed by device
-synchronisation. Prints one JSON line.
+synchronisation. Before the warm-up, --equilibrate untimed steps (default 300)
+take the lattice-built melt past its relaxation, so the timed steps see the
+steady rebuild rate (the equilibration's own rate is reported beside it).
+Prints one JSON line.
 
 roofline: the fused force kernel (the dominant kernel) against HBM peak.
   algorithmic bytes per atom = 60 + 4 K + A_bonded (read x+type 16, count 4,
@@ -24,6 +27,8 @@ batched: 512 reduced-unit LJ replicas (N log-uniform in [500, 20000]) advanced
 adjoint: the 100,000-bead polymer/solvent force-matching workload (64 seeded
   frames, MSE to seeded target forces) forward vs forward+backward, plus a
   200-step velocity-Verlet trajectory backprop with checkpoints every 20 steps.
+other_workloads: the 4,000-bead FCC LJ fluid (1000 steps) and the 4,000,000-bead
+  four-type mixture (200 steps), each with its own FP64 oracle sample.
 """
 
 from __future__ import annotations
@@ -58,7 +63,21 @@ def force_bytes_per_atom(w, mean_neighbors: float) -> float:
     return 60.0 + 4.0 * mean_neighbors + bonded
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, sample_steps: int) -> dict:
+    # Thread placement for the OpenMP runtime, which reads it when the oracle
+    # library is first loaded (nothing in this process loads it earlier).
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -66,7 +85,9 @@ def cpu_baseline(w, sample_steps: int) -> dict:
     value = w.n * sample_steps / st.wall_seconds
     return {"value": value, "unit": "atom-steps/s", "cores": int(st.threads_used), "kind": "port",
             "sample": f"{w.name}: {sample_steps} steps of the same system, FP64 OpenMP oracle, "
-                      f"{st.wall_seconds:.1f} s step loop"}
+                      f"{st.wall_seconds:.1f} s step loop",
+            "cpu_model": cpu_model(), "omp_proc_bind": os.environ["OMP_PROC_BIND"],
+            "omp_places": os.environ["OMP_PLACES"]}
 
 
 def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
@@ -91,7 +112,9 @@ def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
     for w in ws:
         with Engine(w) as e:
             e.initialize(observables=False)
-            e.profile(5)
+            # two 16-step graphs: the step graph is captured, instantiated and
+            # replayed before the timed steps
+            e.profile(32)
             e.synchronize()
             t0 = time.perf_counter()
             e.profile(serial_steps)
@@ -171,9 +194,13 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
         fused_loss, _ = force_matching_value_and_grad(m, xs, targets, p)
         torch.cuda.synchronize()
         t_fused = time.perf_counter() - t0
-        # trajectory backprop
+        # trajectory backprop, after one short warm-up trajectory
         x0 = torch.tensor(w.positions, device="cuda", requires_grad=True)
         v0 = torch.tensor(w.velocities, device="cuda", requires_grad=True)
+        xw, vw = m.trajectory(x0, v0, p, steps=every, checkpoint_every=every)
+        ((xw.double() ** 2).sum() * 1e-3 + (vw.double() ** 2).sum()).backward()
+        for t in (x0, v0, *p.tensors()):
+            t.grad = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         xT, vT = m.trajectory(x0, v0, p, steps=traj_steps, checkpoint_every=every)
@@ -201,17 +228,53 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
                            "backward_over_forward": t_traj_b / t_traj_f}}
 
 
-def measured_traffic(workload: str):
+def measured_traffic(workload: str, mean_neighbors: float):
     """HBM bytes per force-kernel launch from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration), or None
-    when no summary covers this workload."""
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration) and the
+    mean neighbour count that profile ran at. The bytes are reported only when
+    that count is within 2 % of this run's (the traffic scales with it);
+    otherwise None."""
     path = ROOT / "profiles" / f"force_kernel_{workload}_hbm_traffic.md"
     if not path.exists():
-        return None
+        return None, None
+    traffic, k = None, None
     for line in path.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":", 1)[1])
-    return None
+            traffic = int(line.split(":", 1)[1])
+        if line.startswith("profiled_mean_neighbors:"):
+            k = float(line.split(":", 1)[1])
+    note = {"file": str(path.relative_to(ROOT)), "profiled_mean_neighbors": k}
+    if traffic is None or k is None or abs(k - mean_neighbors) > 0.02 * mean_neighbors:
+        return None, note
+    return traffic, note
+
+
+def other_workload_leg(name: str, steps: int, cpu_steps: int) -> dict:
+    """A further benchmark configuration: atom-steps/s of `steps` steps after a
+    32-step warm-up (which captures and replays the step graph) and the FP64
+    oracle on `cpu_steps` steps of the same system."""
+    from beads_amd.capi import Engine
+
+    w, description = make_workload(name)
+    print(f"bench: {description}", file=sys.stderr, flush=True)
+    with Engine(w) as e:
+        e.initialize(observables=False)
+        e.profile(32)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.profile(steps)
+        e.synchronize()
+        wall = time.perf_counter() - t0
+        st = e.stats()
+    out = {"workload": description, "n_beads": w.n, "steps": steps,
+           "atom_steps_per_s": w.n * steps / wall, "ms_per_step": wall / steps * 1e3,
+           "neighbor_builds": int(st.build_count), "reorders": int(st.reorder_count),
+           "mean_neighbors": round(st.mean_neighbors, 2),
+           "timing": "steps 33 to steps + 32 of a run from the initial state"}
+    if cpu_steps:
+        out["cpu_baseline"] = cpu_baseline(w, cpu_steps)
+        out["gpu_over_cpu"] = out["atom_steps_per_s"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def copy_calibration(mib: int) -> None:
@@ -235,6 +298,12 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="melt1m")
     ap.add_argument("--cpu-sample-steps", type=int, default=64)
+    ap.add_argument("--equilibrate", type=int, default=300,
+                    help="untimed steps run before the warm-up, so that the timed steps see "
+                         "the steady rebuild rate rather than the relaxation of the lattice "
+                         "start")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the 4,000-bead FCC fluid and 4M-bead mixture lines")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serial-steps", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=512)
@@ -270,6 +339,14 @@ def main() -> None:
     eng.synchronize()
     setup_s = time.perf_counter() - t_setup
     eng.initialize(observables=False)
+    eq = None
+    if args.equilibrate:
+        t_eq = time.perf_counter()
+        eng.profile(args.equilibrate)
+        eng.synchronize()
+        t_eq = time.perf_counter() - t_eq
+        eq = {"steps": args.equilibrate, "atom_steps_per_s": w.n * args.equilibrate / t_eq,
+              "neighbor_builds": int(eng.stats().build_count)}
     eng.profile(args.warmup)
     eng.synchronize()
     s0 = eng.stats()
@@ -292,9 +369,10 @@ def main() -> None:
         force_avg = pt.force_s / max(1, pt.force_launches)
         per_atom = force_bytes_per_atom(w, k_mean)
         achieved = per_atom * w.n / force_avg / 1e9
+        traffic, traffic_src = measured_traffic(args.workload, k_mean)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(args.workload),
+                    "traffic": traffic, "traffic_profile": traffic_src,
                     "kernel": "force_kernel", "bytes_per_atom": round(per_atom, 1),
                     "algorithmic_bytes_per_launch": round(per_atom * w.n),
                     "mean_launch_us": round(force_avg * 1e6, 2)}
@@ -316,6 +394,12 @@ def main() -> None:
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w, args.cpu_sample_steps)
 
+    others = None
+    if not args.no_other_workloads and args.workload == "melt1m":
+        cpu_on = not args.no_cpu_baseline
+        others = {"fcc4000": other_workload_leg("fcc4000", 1000, 1000 if cpu_on else 0),
+                  "mixture4m": other_workload_leg("mixture4m", 200, 8 if cpu_on else 0)}
+
     line = {
         "metric": "atom-steps/s at 1M beads; batched vs serial replicas/s; adjoint/forward cost",
         "value": value,
@@ -332,12 +416,14 @@ def main() -> None:
         "config": {"workload": description, "n_beads": w.n, "steps": args.steps,
                    "neighbor_builds_in_timed_steps": builds,
                    "mean_neighbors": round(s1.mean_neighbors, 2), "phases": phases,
+                   "equilibration": eq,
                    "setup_s": round(setup_s, 3),
                    "host_inclusive_atom_steps_per_s": w.n * args.steps / (wall + setup_s)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "other_workloads": others,
     }
     print(json.dumps(line))
 

@@ -212,23 +212,54 @@ def test_snapshot_reader_round_trip(tmp_path):
         beads.System.from_state(bad)
 
 
-def test_trajectory_reader_and_lammps_dump(tmp_path):
-    n, frames = 4, 3
-    rec = np.dtype([("tag", np.uint32), ("position", np.float32, (3,)), ("image", np.int32, (3,))])
-    p = tmp_path / "t.trajectory.beadsbin"
+def _write_trajectory(p, endian="<", n=4, frames=3, marker=0x01020304, version=1):
+    e = endian
+    rec = np.dtype([("tag", e + "u4"), ("position", e + "f4", (3,)), ("image", e + "i4", (3,))])
     with open(p, "wb") as f:
         f.write(b"BEADS_TRAJECTORY")
-        f.write(struct.pack("=7I", 1, 0x01020304, 4, 4, 4, 4, 4))
-        f.write(struct.pack("=Q", 8) + b"nm_kjmol")
-        f.write(struct.pack("=IQQ", 1 | 4 | 8, n, rec.itemsize))
+        f.write(struct.pack(e + "7I", version, marker, 4, 4, 4, 4, 4))
+        f.write(struct.pack(e + "Q", 8) + b"nm_kjmol")
+        f.write(struct.pack(e + "IQQ", 1 | 4 | 8, n, rec.itemsize))
         for k in range(frames):
-            f.write(struct.pack("=Q", 10 * k))
-            f.write(np.array([0, 0, 0, 5, 5, 5], np.float32).tobytes())
+            f.write(struct.pack(e + "Q", 10 * k))
+            f.write(np.array([0, 0, 0, 5, 5, 5], e + "f4").tobytes())
             r = np.zeros(n, rec)
             r["tag"] = [4, 3, 2, 1]
             r["position"] = k + np.arange(12, dtype=np.float32).reshape(4, 3) / 10
             r["image"][:, 0] = k
             f.write(r.tobytes())
+
+
+def test_trajectory_reader_either_byte_order(tmp_path):
+    """The marker read back in either order names the file's byte order
+    (python/beads/io.py:795-802); both files give the same values."""
+    out = {}
+    for e in "<>":
+        p = tmp_path / f"t{'le' if e == '<' else 'be'}.trajectory.beadsbin"
+        _write_trajectory(p, e)
+        out[e] = beads.io.read_beads_trajectory(p)
+        assert out[e]["endian"] == e
+    for key in ("steps", "box_bounds"):
+        np.testing.assert_array_equal(out["<"][key], out[">"][key])
+    for field in ("tag", "position", "image"):
+        np.testing.assert_array_equal(out["<"]["records"][field], out[">"]["records"][field])
+    bad = tmp_path / "bad.trajectory.beadsbin"
+    _write_trajectory(bad, "<", marker=0x05060708)
+    with pytest.raises(ValueError, match="Unsupported BEADS trajectory endian marker"):
+        beads.io.read_beads_trajectory(bad)
+    _write_trajectory(bad, ">", version=2)
+    with pytest.raises(ValueError, match="Unsupported BEADS trajectory version: 2"):
+        beads.io.read_beads_trajectory(bad)
+    good = tmp_path / "tle.trajectory.beadsbin"
+    bad.write_bytes(good.read_bytes()[:-3])
+    with pytest.raises(ValueError, match="file size is inconsistent"):
+        beads.io.read_beads_trajectory(bad)
+
+
+def test_trajectory_reader_and_lammps_dump(tmp_path):
+    n, frames = 4, 3
+    p = tmp_path / "t.trajectory.beadsbin"
+    _write_trajectory(p, "=", n, frames)
     t = beads.io.read_beads_trajectory(p)
     assert t["fields"] == ["tag", "position", "image"] and t["frame_count"] == frames
     assert list(t["steps"]) == [0, 10, 20] and t["units"] == "nm_kjmol"
