@@ -26,9 +26,13 @@ namespace beads_amd {
 constexpr int kMaxLdsTypes = 16;  // LJ type tables up to 16x16 live in LDS
 constexpr uint32_t kNbrChunk = 16;  // neighbour entries per stored chunk (64 bytes)
 
-// Entries allocated per atom for a capacity of max_neighbors.
+// Entries allocated per atom for a capacity of max_neighbors: whole pairs of
+// chunks, so that the force pass's rounds of 2 * kNbrChunk slots can load
+// every slot of a round without a bounds test (entries past the count are
+// read and ignored).
+constexpr uint32_t kNbrRound = 2 * kNbrChunk;
 inline uint32_t nbr_padded_capacity(uint32_t max_neighbors) {
-  return (max_neighbors + kNbrChunk - 1) / kNbrChunk * kNbrChunk;
+  return (max_neighbors + kNbrRound - 1) / kNbrRound * kNbrRound;
 }
 
 // LJ coefficients of one ordered type pair, formed in FP64 from the binary32

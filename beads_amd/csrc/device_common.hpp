@@ -95,6 +95,13 @@ __device__ __forceinline__ double rcp_f64(double x) {
   return r * (2.0 - x * r);
 }
 
+// 1/sqrt(x) in FP64 for x > 0: the hardware reciprocal square root refined by
+// one Newton step (the hardware value's relative error squared, about 1e-15).
+__device__ __forceinline__ double rsq_f64(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  return r * (1.5 - 0.5 * x * r * r);
+}
+
 // Sum over a BLOCK-wide workgroup (BLOCK a multiple of 64, at most 1024);
 // result valid in thread 0. lds needs BLOCK / 64 doubles.
 template <int BLOCK>
@@ -239,6 +246,11 @@ __device__ __forceinline__ d3 dcross(d3 a, d3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// The listed terms take square roots and quotients as products with the
+// FP64 reciprocal (square root) of rcp_f64 / rsq_f64 (one Newton step each,
+// about 1e-15 relative) instead of IEEE division and square root, whose
+// correctly rounded sequences cost several times the instructions.
+
 // Harmonic bond U = k/2 (r - r0)^2 (bond_harmonic.cu:41-79) on the displacement
 // d = x_self - x_partner; returns the force on self and the term's energy and
 // virial d.f.
@@ -247,14 +259,15 @@ __device__ __forceinline__ float3 bond_force(float3 df, float2 kc, double& energ
   const d3 d = dbl(df);
   const double k = kc.x, r0 = kc.y;
   const double r2 = ddot(d, d);
-  const double rr = sqrt(r2);
-  const double stretch = rr - r0;
-  energy = 0.5 * k * stretch * stretch;
-  if (rr == 0.0) {
+  if (r2 == 0.0) {
+    energy = 0.5 * k * r0 * r0;
     virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const double s = -k * stretch / rr;
+  const double inv_r = rsq_f64(r2);
+  const double stretch = r2 * inv_r - r0;
+  energy = 0.5 * k * stretch * stretch;
+  const double s = -k * stretch * inv_r;
   virial = s * r2;
   return flt(dscale(d, s));
 }
@@ -277,19 +290,21 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
     virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const double ra = sqrt(ra2), rb = sqrt(rb2);
+  const double ia = rsq_f64(ra2), ib = rsq_f64(rb2);
   const d3 nrm = dcross(a, b);
-  const double nn = sqrt(ddot(nrm, nrm));
+  const double nn2 = ddot(nrm, nrm);
+  const double nn = nn2 > 0.0 ? nn2 * rsq_f64(nn2) : 0.0;
   // The FP64 arguments are well conditioned; a binary32 atan2 of them is
   // accurate to a few 1e-8 relative and keeps the pass's register budget.
   const double theta = atan2f(static_cast<float>(nn), static_cast<float>(ddot(a, b)));
   const double dth = theta - theta0;
   energy = 0.5 * k * dth * dth;
-  const double sn_true = nn / (ra * rb);
+  const double sn_true = nn * ia * ib;
   const double sn = sn_true <= 0.001 ? 0.001 : sn_true;
-  const double A = -k * dth / sn;
-  const d3 fi = dscale(dcross(a, nrm), A / (ra2 * ra * rb));
-  const d3 fk = dscale(dcross(b, nrm), -A / (rb2 * rb * ra));
+  const double A = -k * dth * rcp_f64(sn);
+  const double aib = A * ia * ib;
+  const d3 fi = dscale(dcross(a, nrm), aib * ia * ia);
+  const d3 fk = dscale(dcross(b, nrm), -aib * ib * ib);
   virial = ddot(a, fi) + ddot(b, fk);
   if (role == 0) return flt(fi);
   if (role == 2) return flt(fk);
@@ -313,12 +328,14 @@ __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, floa
     virial = 0.0;
     return make_float3(0.f, 0.f, 0.f);
   }
-  const double nb = sqrt(n1);
+  const double inb = rsq_f64(n1);
+  const double nb = n1 * inb;
   // cos(phi) and sin(phi) from the cross products (sin carries the sign of
   // v0.c1), then cos(n phi), sin(n phi) by the angle-addition recurrence:
   // the reference's k[1 + d cos(n phi)] and its derivative without a
   // transcendental call.
-  const double inv_c = 1.0 / sqrt(nc0 * nc1);
+  const double ic0 = rsq_f64(nc0), ic1 = rsq_f64(nc1);
+  const double inv_c = ic0 * ic1;
   const double c1phi = ddot(c0, c1) * inv_c;
   const double s1phi = nb * ddot(v0, c1) * inv_c;
   double cs = 1.0, sn = 0.0;
@@ -329,9 +346,9 @@ __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, floa
   }
   energy = k * (1.0 + dsign * cs);
   const double dE = -k * dsign * nmul * sn;
-  const d3 fi = dscale(c0, -dE * nb / nc0);
-  const d3 fl = dscale(c1, dE * nb / nc1);
-  const double inv_n1 = 1.0 / n1;
+  const d3 fi = dscale(c0, -dE * nb * ic0 * ic0);
+  const d3 fl = dscale(c1, dE * nb * ic1 * ic1);
+  const double inv_n1 = inb * inb;
   const d3 sv = dsub(dscale(fi, ddot(v0, v1) * inv_n1), dscale(fl, ddot(v2, v1) * inv_n1));
   const d3 fj = dsub(sv, fi);
   const d3 fk = dscale(dadd(sv, fl), -1.0);

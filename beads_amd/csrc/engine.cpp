@@ -494,6 +494,7 @@ void Engine::upload_state() {
                          s.masses[i]);
     img[i] = make_int4(s.images[3 * i], s.images[3 * i + 1], s.images[3 * i + 2], 0);
   }
+  pos_.resize(static_cast<size_t>(n) + kCandidatePad);  // the direct build's candidate stream
   pos_.upload(pos.data(), n, stream_);
   vel_.upload(vel.data(), n, stream_);
   image_.upload(img.data(), n, stream_);
@@ -508,7 +509,7 @@ void Engine::upload_state() {
   nbr_.resize(static_cast<size_t>(nbr_padded_capacity(plan_.max_neighbors)) * n);
   cell_of_.resize(n);
   cell_atoms_.resize(n);
-  cell_pos_.resize(n);
+  cell_pos_.resize(static_cast<size_t>(n) + kCandidatePad);
   cell_count_.resize(plan_.n_cells);
   cell_count_.zero(stream_);
   cell_start_.resize(static_cast<size_t>(plan_.n_cells) + 1);

@@ -349,7 +349,8 @@ constexpr uint32_t kBuildAtoms = kBlock / kBuildTeam;
 // (u < kBuildSlots) of every candidate run, so the team's loads are
 // consecutive 16-byte entries of the cell-ordered position stream (slot in
 // .w), one load per candidate with no index gather; slots past a run's end
-// reload its last entry and test nothing. Accepted candidates are appended in
+// load whatever follows it (the streams are padded past the last atom) and
+// test nothing. Accepted candidates are appended in
 // candidate order (the order of a one-lane scan) through a ballot of the
 // team's lanes and written straight into the atom's chunks. EXCL: 0 = no
 // exclusions, 4 or 8 = up to that many partners held in registers (every
@@ -422,10 +423,13 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     // z, slot bits}, kBuildTeam * kBuildSlots per round.
     auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto generic, float3 sh) {
       for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
+        // Entries past the run (at most kCandidatePad - 1 past the stream's
+        // last atom) are loaded and then ignored.
+        static_assert(kBuildTeam * kBuildSlots <= kCandidatePad, "candidate padding");
         float4 x[kBuildSlots];
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
-          x[u] = entry(min(k + u * kBuildTeam + t, e - 1));
+          x[u] = entry(k + u * kBuildTeam + t);
         }
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
@@ -607,6 +611,7 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
                                         double& fyd, double& fzd, double& e_pair,
                                         double& w_pair) {
   static_assert(kNbrChunk % kTeam == 0, "a team's slots of one round share a chunk");
+  static_assert(kTeam * kTeamSlots == kNbrRound, "a round covers the padded capacity's unit");
   const int T = p.n_types;
   const int ti = __float_as_int(xi4.w);
   const uint32_t n = p.n;
@@ -614,11 +619,17 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
   const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
   const uint32_t cnt = p.nbr_count[i];
   for (uint32_t base = 0; base < cnt; base += kTeam * kTeamSlots) {
+    // Every slot of the round lies inside the atom's padded capacity; slots
+    // at or past the count read whatever the chunk holds and become i.
     uint32_t jj[kTeamSlots];
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
       const uint32_t s = base + u * kTeam + t;
-      jj[u] = s < cnt ? p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))] : i;
+      jj[u] = p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))];
+    }
+#pragma unroll
+    for (int u = 0; u < kTeamSlots; ++u) {
+      jj[u] = base + u * kTeam + t < cnt ? jj[u] : i;
     }
     float4 xj[kTeamSlots];
 #pragma unroll

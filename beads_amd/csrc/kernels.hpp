@@ -53,6 +53,11 @@ constexpr int kForceAtoms = kBlock / kTeam;
 
 inline unsigned force_grid(uint32_t n) { return grid_for(n, kForceAtoms); }
 
+// Entries allocated past the last atom in the build's candidate streams (the
+// position array and its cell-ordered copy): a build round loads 16
+// consecutive candidates without a bounds test and ignores those past the run.
+constexpr uint32_t kCandidatePad = 16;
+
 // Neighbour-rebuild sequence; every kernel is predicated on ctrl->rebuild so
 // the host never waits for the displacement check.
 void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, hipStream_t stream);

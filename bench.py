@@ -212,7 +212,7 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
         t_traj_b = time.perf_counter() - t0
     return {"workload": f"polymer/solvent {w.n} beads (500 x 100 chains + 50,000 solvent), "
                         f"4 types, 22 parameters, nm_kjmol",
-            "frames": frames, "loss": float(loss), "grad_norm": grad_norm,
+            "frames": frames, "loss": float(loss.detach()), "grad_norm": grad_norm,
             "forward_s": t_fwd, "forward_backward_s": t_fb,
             "gradient_over_forward": t_fb / t_fwd,
             "backward_over_forward": (t_fb - t_fwd) / t_fwd,

@@ -536,7 +536,7 @@ def test_full_size_trajectory_gradient_independent_of_checkpoint_interval():
                           v0.grad.cpu().numpy().astype(np.float64),
                           np.concatenate([t.grad.double().cpu().numpy().ravel()
                                           for t in p.tensors()]))
-            print(f"every {every}: loss {float(loss):.9e}", flush=True)
+            print(f"every {every}: loss {float(loss.detach()):.9e}", flush=True)
     ref = out[20]
     for every in (1, 200):
         got = out[every]
