@@ -50,6 +50,15 @@ struct BoxView {
   float invL[3];
 };
 
+// Members of a cell are ordered by (x, slot). Each cell's x extent is cut
+// into kXBins equal bins; cell_xbins holds, for m = 1..kXBins, the number of
+// members whose bin is below m, as 16-bit counts (word (m-1)/2, half (m-1)%2),
+// so that the build can skip the members of an x-neighbour cell that lie
+// farther than r_s along x. Cells of more than kXBinsMaxCount members carry no
+// table and are scanned whole.
+constexpr int kXBins = 8;
+constexpr uint32_t kXBinsMaxCount = 65535;
+
 struct CellGridView {
   uint32_t nx, ny, nz, n_cells;
   float inv_cell[3];
@@ -148,6 +157,7 @@ struct BuildParams {
   const uint32_t* cell_count;
   const uint32_t* cell_atoms;
   const float4* cell_pos;  // cell_atoms' positions, .w = the slot's bits
+  const uint4* cell_xbins; // per cell: x-bin prefix counts (cell_sort_kernel)
   const uint32_t* topo_count;
   const uint32_t* excl;
   uint32_t ecap;

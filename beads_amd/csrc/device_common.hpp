@@ -37,6 +37,17 @@ __device__ __forceinline__ float mic_diff(float xi, float xj, float L, float hL)
   return d > hL ? (xi - L) - xj : (d < -hL ? xi - (xj - L) : d);
 }
 
+// mic_diff with both folded forms evaluated and one selected (no branches):
+// xi_m = xi - L is the caller's, hoisted out of a loop over partners.
+__device__ __forceinline__ float mic_diff_sel(float xi, float xi_m, float xj, float L, float hL) {
+  const float d = xi - xj;
+  const float up = xi_m - xj;
+  const float dn = xi - (xj - L);
+  float r = __builtin_unpredictable(d > hL) ? up : d;
+  r = __builtin_unpredictable(d < -hL) ? dn : r;
+  return r;
+}
+
 // Squared distance exactly as written in the reference source (no FMA), so the
 // r^2 <= r_s^2 and r^2 < r_c^2 decisions match a binary32 host evaluation.
 __device__ __forceinline__ float r2_strict(float dx, float dy, float dz) {

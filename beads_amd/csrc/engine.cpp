@@ -512,6 +512,8 @@ void Engine::upload_state() {
   cell_pos_.resize(static_cast<size_t>(n) + kCandidatePad);
   cell_count_.resize(plan_.n_cells);
   cell_count_.zero(stream_);
+  cell_xbins_.resize(plan_.n_cells);
+  cell_xbins_.zero(stream_);
   cell_start_.resize(static_cast<size_t>(plan_.n_cells) + 1);
   cell_start_.zero(stream_);
   cell_cursor_.resize(plan_.n_cells);
@@ -661,6 +663,7 @@ void Engine::set_up_params() {
   b.cell_count = cell_count_.data();
   b.cell_atoms = cell_atoms_.data();
   b.cell_pos = cell_pos_.data();
+  b.cell_xbins = cell_xbins_.data();
   b.topo_count = topo_count_.data();
   b.excl = excl_.data();
   b.ecap = ecap_;
@@ -689,6 +692,9 @@ void Engine::set_up_params() {
   sp.cell_atoms = cell_atoms_.data();
   sp.pos = pos_.data();
   sp.cell_pos = cell_pos_.data();
+  sp.cell_xbins = cell_xbins_.data();
+  sp.box = box_;
+  sp.grid = grid;
   sp.ctrl = ctrl_.data();
 
   ReorderParams& r = reorder_params_;

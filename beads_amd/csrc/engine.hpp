@@ -166,6 +166,7 @@ class Engine {
   // device state
   DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
   DeviceBuffer<float4> cell_pos_;  // positions in cell order, .w = slot bits
+  DeviceBuffer<uint4> cell_xbins_;  // per-cell x-bin prefix counts
   DeviceBuffer<int4> image_, ref_image_, s_image_;
   DeviceBuffer<uint32_t> tag_, molid_, s_tag_, s_molid_;
   DeviceBuffer<uint32_t> nbr_count_, nbr_;

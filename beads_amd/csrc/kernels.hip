@@ -170,12 +170,16 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(CellScanParams p) 
   p.cell_atoms[slot] = i;
 }
 
-// Order each cell's members by slot: together with the counting sort this is a
-// stable sort by (cell, slot), deterministic unlike the reference's atomic
-// scatter (cell_list_build.cu:168-181). One wave per cell: for up to 64
-// members lane l holds member l, whose rank is the number of members with a
-// smaller slot (slots are distinct), and stores it at start + rank. A larger
-// cell is insertion-sorted by lane 0. When no reorder follows, the kernel also
+// Order each cell's members by (x, slot): with the counting sort this is a
+// deterministic order, where the reference's atomic scatter
+// (cell_list_build.cu:168-181) leaves each cell's members in arbitrary order;
+// the sets a build finds are the same for any order inside a cell. Ordering
+// by x makes the members of an x-row of cells ascending in x, so a build can
+// scan a row as one run and skip its ends beyond r_s (cell_xbins). One wave
+// per cell: for up to 64 members lane l holds member l, whose rank is the
+// number of members before it in (x, slot) order, and stores it at start +
+// rank; the x-bin prefix counts come from one ballot per bin. A larger cell
+// is insertion-sorted by lane 0. When no reorder follows, the kernel also
 // writes the members' positions in cell order with the slot in .w (the
 // build's candidate stream); after a reorder, reorder_commit writes them.
 constexpr uint32_t kSortCellsPerBlock = kBlock / 64;
@@ -184,46 +188,82 @@ __device__ __forceinline__ float4 cell_entry(float4 x, uint32_t slot) {
   return make_float4(x.x, x.y, x.z, __uint_as_float(slot));
 }
 
+// Bin of coordinate x inside x-cell cx (0 .. kXBins - 1), monotone in x: the
+// cell coordinate of axis_cell, offset by the cell and scaled by the bin count
+// (a power of two).
+__device__ __forceinline__ int x_bin(float x, float lo, float inv, uint32_t cx) {
+#pragma clang fp contract(off)
+  const float f = ((x - lo) * inv - static_cast<float>(cx)) * static_cast<float>(kXBins);
+  return f <= 0.f ? 0 : (f >= static_cast<float>(kXBins - 1) ? kXBins - 1 : static_cast<int>(f));
+}
+
+__device__ __forceinline__ bool before_in_cell(float xa, uint32_t sa, float xb, uint32_t sb) {
+  return xa < xb || (xa == xb && sa < sb);
+}
+
 __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
   if (p.ctrl->rebuild == 0) {
     return;
   }
   const uint32_t lane = threadIdx.x & 63u;
   const bool stream = p.ctrl->reorder == 0;
+  const float lo = p.box.lo[0], inv = p.grid.inv_cell[0];
   // A capped grid loops over the cells, so the launch stays small when the
   // rebuild flag makes it exit at once.
   for (uint32_t c = blockIdx.x * kSortCellsPerBlock + threadIdx.x / 64; c < p.n_cells;
        c += gridDim.x * kSortCellsPerBlock) {  // uniform over the wave
     const uint32_t b = p.cell_start[c], e = p.cell_start[c + 1];
     const uint32_t cnt = e - b;
+    const uint32_t cx = c % p.grid.nx;
     uint32_t* a = p.cell_atoms;
     if (cnt <= 64) {
       const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
+      const float4 xv4 = lane < cnt ? p.pos[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float xv = xv4.x;
       uint32_t rank = 0;
       for (uint32_t m = 0; m < cnt; ++m) {
         const uint32_t w = static_cast<uint32_t>(
             __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(m)));
-        rank += w < v ? 1u : 0u;
+        const float xw = __int_as_float(
+            __builtin_amdgcn_readlane(__float_as_int(xv), static_cast<int>(m)));
+        rank += before_in_cell(xw, w, xv, v) ? 1u : 0u;
       }
       if (lane < cnt) {
         a[b + rank] = v;
-        if (stream) p.cell_pos[b + rank] = cell_entry(p.pos[v], v);
+        if (stream) p.cell_pos[b + rank] = cell_entry(xv4, v);
+      }
+      const int bin = lane < cnt ? x_bin(xv, lo, inv, cx) : kXBins;
+      uint32_t pre[kXBins];
+#pragma unroll
+      for (int m = 1; m <= kXBins; ++m) {
+        pre[m - 1] = static_cast<uint32_t>(__builtin_popcountll(__ballot(bin < m)));
+      }
+      if (lane == 0) {
+        p.cell_xbins[c] = make_uint4(pre[0] | (pre[1] << 16), pre[2] | (pre[3] << 16),
+                                     pre[4] | (pre[5] << 16), pre[6] | (pre[7] << 16));
       }
     } else if (lane == 0) {
       for (uint32_t k = b + 1; k < e; ++k) {
         const uint32_t v = a[k];
+        const float xv = p.pos[v].x;
         uint32_t m = k;
-        while (m > b && a[m - 1] > v) {
+        while (m > b && before_in_cell(xv, v, p.pos[a[m - 1]].x, a[m - 1])) {
           a[m] = a[m - 1];
           --m;
         }
         a[m] = v;
       }
-      if (stream) {
-        for (uint32_t k = b; k < e; ++k) {
-          const uint32_t v = a[k];
-          p.cell_pos[k] = cell_entry(p.pos[v], v);
-        }
+      uint32_t pre[kXBins] = {};
+      for (uint32_t k = b; k < e; ++k) {
+        const uint32_t v = a[k];
+        const float4 xv4 = p.pos[v];
+        if (stream) p.cell_pos[k] = cell_entry(xv4, v);
+        const int bin = x_bin(xv4.x, lo, inv, cx);
+        for (int m = bin + 1; m <= kXBins; ++m) ++pre[m - 1];
+      }
+      if (cnt <= kXBinsMaxCount) {
+        p.cell_xbins[c] = make_uint4(pre[0] | (pre[1] << 16), pre[2] | (pre[3] << 16),
+                                     pre[4] | (pre[5] << 16), pre[6] | (pre[7] << 16));
       }
     }
   }
@@ -488,6 +528,50 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
       const int mx = g.nx < 3 ? static_cast<int>(g.nx) : 3;
       const int my = g.ny < 3 ? static_cast<int>(g.ny) : 3;
       const int mz = g.nz < 3 ? static_cast<int>(g.nz) : 3;
+      // Members of cell cc with x-bin below m (cell_xbins; m in 0..kXBins).
+      auto below = [&](uint32_t cc, int m) -> uint32_t {
+        if (m <= 0) return 0u;
+        const uint4 q = p.cell_xbins[cc];
+        const int f = m - 1;
+        const uint32_t word = (f >> 1) == 0 ? q.x : ((f >> 1) == 1 ? q.y : ((f >> 1) == 2 ? q.z : q.w));
+        return (word >> (16 * (f & 1))) & 0xffffu;
+      };
+      // Regular grid, at least 5 cells per axis, atom not in an x-boundary
+      // cell: each row of three x-adjacent cells is one run of the candidate
+      // stream, ascending in x (members are in (x, slot) order), and the run
+      // starts and ends at the x-bins within r_s of the atom, one bin of
+      // margin on each side (a bin is an eighth of a cell, far above any
+      // rounding). The members skipped at the ends lie farther than r_s along
+      // x alone, so their r^2 exceeds r_s^2 and the build would reject them:
+      // the accepted candidates and their order are those of the full scan.
+      auto rows = [&]() {
+        const float rs = sqrtf(p.rs2);
+        const float lo = p.box.lo[0], inv = g.inv_cell[0];
+        const float fl = ((xi.x - rs) - lo) * inv - static_cast<float>(cx - 1);
+        const float fr = ((xi.x + rs) - lo) * inv - static_cast<float>(cx + 1);
+        const int m_left = static_cast<int>(floorf(fl * static_cast<float>(kXBins))) - 1;
+        const int m_right = static_cast<int>(floorf(fr * static_cast<float>(kXBins))) + 2;
+        const int ml = m_left < 0 ? 0 : (m_left > kXBins ? kXBins : m_left);
+        const int mr = m_right < 0 ? 0 : (m_right > kXBins ? kXBins : m_right);
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {
+          float sz;
+          const uint32_t zc = axis(cz, g.nz, a, L2, sz);
+#pragma unroll 1
+          for (int b = 0; b < 3; ++b) {
+            float sy;
+            const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
+            const uint32_t cl = rowc + cx - 1, cr = rowc + cx + 1;
+            const uint32_t s_l = p.cell_start[cl], s_m = p.cell_start[cl + 1];
+            const uint32_t s_r = p.cell_start[cr], e_r = p.cell_start[cr + 1];
+            const uint32_t n_l = s_m - s_l, n_r = e_r - s_r;
+            const uint32_t skip_l = n_l <= kXBinsMaxCount ? below(cl, ml) : 0u;
+            const uint32_t keep_r = n_r <= kXBinsMaxCount ? below(cr, mr) : n_r;
+            scan_range(s_l + skip_l, s_r + keep_r, entry, std::false_type{},
+                       make_float3(0.f, sy, sz));
+          }
+        }
+      };
       auto cells = [&](auto generic) {
 #pragma unroll 1
         for (int a = 0; a < mz; ++a) {
@@ -508,7 +592,11 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
       };
       if (g.path == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {
-        cells(std::false_type{});
+        if (cx >= 1 && cx + 1 < g.nx) {
+          rows();
+        } else {
+          cells(std::false_type{});
+        }
       } else {
         cells(std::true_type{});
       }
@@ -617,6 +705,7 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
   const uint32_t n = p.n;
   const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
   const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+  const float xm0 = xi4.x - L0, xm1 = xi4.y - L1, xm2 = xi4.z - L2;
   const uint32_t cnt = p.nbr_count[i];
   for (uint32_t base = 0; base < cnt; base += kTeam * kTeamSlots) {
     // Every slot of the round lies inside the atom's padded capacity; slots
@@ -627,22 +716,26 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
       const uint32_t s = base + u * kTeam + t;
       jj[u] = p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))];
     }
+    // slot base + u * kTeam + t holds a neighbour iff u * kTeam < rem
+    const int rem = static_cast<int>(cnt) - static_cast<int>(base + t);
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      jj[u] = base + u * kTeam + t < cnt ? jj[u] : i;
+      jj[u] = u * kTeam < rem ? jj[u] : i;
     }
+    // Gathers as a 32-bit byte offset from the array base (n * 16 < 2^32).
+    const char* pos_bytes = reinterpret_cast<const char*>(p.pos);
     float4 xj[kTeamSlots];
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      xj[u] = p.pos[jj[u]];
+      xj[u] = *reinterpret_cast<const float4*>(pos_bytes + (jj[u] << 4));
     }
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
-      const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
-      const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
+      const float dx = mic_diff_sel(xi4.x, xm0, xj[u].x, L0, h0);
+      const float dy = mic_diff_sel(xi4.y, xm1, xj[u].y, L1, h1);
+      const float dz = mic_diff_sel(xi4.z, xm2, xj[u].z, L2, h2);
       const float r2 = r2_strict(dx, dy, dz);
-      if (base + u * kTeam + t < cnt && r2 < p.rc2) {
+      if (u * kTeam < rem && r2 < p.rc2) {
         LjCoeff c;
         if (TYPES == 0) {
           c = c0;

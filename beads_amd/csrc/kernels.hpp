@@ -30,6 +30,9 @@ struct CellScanParams {
   uint32_t* cell_atoms;
   const float4* pos;
   float4* cell_pos;  // written in cell order when no reorder follows
+  uint4* cell_xbins; // per cell: x-bin prefix counts of its x-sorted members
+  BoxView box;
+  CellGridView grid;
   Ctrl* ctrl;
 };
 

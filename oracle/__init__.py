@@ -85,7 +85,8 @@ def library() -> C.CDLL:
         lib.orc_last_error.restype = C.c_char_p
         lib.orc_plan.argtypes = [_f32p, _f32p, C.c_float, C.c_float, C.c_uint32, C.POINTER(Plan)]
         lib.orc_cell_ids.argtypes = [C.c_uint32, _f32p, _f32p, C.POINTER(Plan), _u32p]
-        lib.orc_stable_cell_order.argtypes = [C.c_uint32, _u32p, C.c_uint32, _u32p, _u32p, _u32p]
+        lib.orc_stable_cell_order.argtypes = [C.c_uint32, _u32p, C.c_uint32, _f32p, _u32p, _u32p,
+                                              _u32p]
         lib.orc_exclusion_pairs.restype = C.c_uint64
         lib.orc_exclusion_pairs.argtypes = [C.c_uint32, C.c_uint32, _u32p, C.c_int32, _u32p, C.c_uint64]
         lib.orc_neighbor_build.argtypes = [C.POINTER(System), C.c_float, C.c_uint32, C.c_int32, _u32p,
@@ -183,14 +184,21 @@ def cell_ids(w, positions: np.ndarray | None = None) -> np.ndarray:
     return out
 
 
-def stable_cell_order(cells: np.ndarray, n_cells: int):
+def stable_cell_order(cells: np.ndarray, n_cells: int, positions: np.ndarray | None = None):
+    """Counting sort by cell; members of a cell by (x, slot) when positions
+    are given, else by slot. Returns (order, counts, offsets)."""
     lib = library()
     cells = np.ascontiguousarray(cells, np.uint32)
     order = np.zeros(cells.size, np.uint32)
     counts = np.zeros(n_cells, np.uint32)
     offsets = np.zeros(n_cells, np.uint32)
-    lib.orc_stable_cell_order(cells.size, _p(cells, C.c_uint32), n_cells, _p(order, C.c_uint32),
-                              _p(counts, C.c_uint32), _p(offsets, C.c_uint32))
+    pos = None
+    if positions is not None:
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    lib.orc_stable_cell_order(cells.size, _p(cells, C.c_uint32), n_cells,
+                              _p(pos, C.c_float) if pos is not None else None,
+                              _p(order, C.c_uint32), _p(counts, C.c_uint32),
+                              _p(offsets, C.c_uint32))
     return order, counts, offsets
 
 

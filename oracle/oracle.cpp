@@ -198,12 +198,15 @@ inline int unique_axis(uint32_t c, uint32_t count, uint32_t out[3]) {
   return 3;
 }
 
-// Stable counting sort by (cell, slot). The reference's scatter
-// (cell_list_build.cu:168-181) is atomic and unordered inside a cell; the
-// engine under test orders each cell by slot, which this restates.
+// Counting sort by cell; inside a cell by (x, slot) when x is given, else by
+// slot. The reference's scatter (cell_list_build.cu:168-181) is atomic and
+// leaves a cell's members in arbitrary order; the engine under test orders
+// them by (x, slot), which this restates. Any order inside a cell gives the
+// same neighbour sets; it fixes the candidate order of a build.
 void stable_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
                   std::vector<uint32_t>& order, std::vector<uint32_t>& counts,
-                  std::vector<uint32_t>& offsets) {
+                  std::vector<uint32_t>& offsets, const float* x = nullptr,
+                  size_t x_stride = 1) {
   counts.assign(n_cells, 0);
   for (uint32_t i = 0; i < n; ++i) {
     ++counts[cell[i]];
@@ -218,6 +221,12 @@ void stable_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
   order.assign(n, 0);
   for (uint32_t i = 0; i < n; ++i) {
     order[cursor[cell[i]]++] = i;
+  }
+  if (x != nullptr) {
+    for (uint32_t c = 0; c < n_cells; ++c) {
+      std::stable_sort(order.begin() + offsets[c], order.begin() + offsets[c] + counts[c],
+                       [&](uint32_t a, uint32_t b) { return x[a * x_stride] < x[b * x_stride]; });
+    }
   }
 }
 
@@ -324,7 +333,9 @@ void build_lists(uint32_t n, const Pos& pos, const BoxF& box, const orc_plan_t& 
       const uint32_t iz = axis_cell(static_cast<float>(pos.z(i)), box.lo[2], plan.inv_cell[2], plan.nz);
       cell[i] = ix + plan.nx * (iy + plan.ny * iz);
     }
-    stable_order(n, cell.data(), plan.n_cells, order, counts, offsets);
+    std::vector<float> xs(n);
+    for (uint32_t i = 0; i < n; ++i) xs[i] = static_cast<float>(pos.x(i));
+    stable_order(n, cell.data(), plan.n_cells, order, counts, offsets, xs.data());
   }
   const Real L[3] = {static_cast<Real>(box.L[0]), static_cast<Real>(box.L[1]), static_cast<Real>(box.L[2])};
   const Real hL[3] = {static_cast<Real>(box.hL[0]), static_cast<Real>(box.hL[1]), static_cast<Real>(box.hL[2])};
@@ -642,10 +653,10 @@ void orc_cell_ids(uint32_t n, const float* pos, const float lo[3],
   }
 }
 
-void orc_stable_cell_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
+void orc_stable_cell_order(uint32_t n, const uint32_t* cell, uint32_t n_cells, const float* pos,
                            uint32_t* order, uint32_t* counts, uint32_t* offsets) {
   std::vector<uint32_t> o, c, off;
-  stable_order(n, cell, n_cells, o, c, off);
+  stable_order(n, cell, n_cells, o, c, off, pos, 3);
   std::copy(o.begin(), o.end(), order);
   std::copy(c.begin(), c.end(), counts);
   std::copy(off.begin(), off.end(), offsets);

@@ -103,8 +103,9 @@ const char* orc_last_error(void);
 void orc_cell_ids(uint32_t n, const float* pos, const float lo[3],
                   const orc_plan_t* plan, uint32_t* cell);
 
-/* Stable counting sort by (cell, old slot): order[new] = old. */
-void orc_stable_cell_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
+/* Counting sort by cell, inside a cell by (x, old slot) when pos ([n][3]) is
+ * given, else by old slot: order[new] = old. */
+void orc_stable_cell_order(uint32_t n, const uint32_t* cell, uint32_t n_cells, const float* pos,
                            uint32_t* order, uint32_t* counts, uint32_t* offsets);
 
 /* Bond-graph exclusions, exclusions.cu:514-598: all tag pairs (a<b) within

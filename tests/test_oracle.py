@@ -76,6 +76,11 @@ def test_stable_cell_order_is_a_stable_sort():
     np.testing.assert_array_equal(order, np.argsort(cells, kind="stable"))
     np.testing.assert_array_equal(counts, np.bincount(cells, minlength=50))
     np.testing.assert_array_equal(offsets, np.concatenate([[0], np.cumsum(counts)[:-1]]))
+    # with positions: by cell, then x, then slot (ties in x on purpose)
+    pos = rng.uniform(0, 4, (2000, 3)).astype(np.float32)
+    pos[::7, 0] = pos[1::7, 0][: len(pos[::7])]
+    order, _, _ = O.stable_cell_order(cells, 50, pos)
+    np.testing.assert_array_equal(order, np.lexsort((np.arange(2000), pos[:, 0], cells)))
 
 
 # ------------------------------------------------------------ exclusions ---
