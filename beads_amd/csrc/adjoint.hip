@@ -216,10 +216,19 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
       }
     }
     if (p.bcap > 0) {
-      const uint32_t nbond = p.topo_count[i] & 255u;
+      const uint32_t nbond_inline = p.topo_count[i] & 255u;
+      uint32_t nbond = nbond_inline, spill_off = 0;
+      if (p.spill_index != nullptr) {
+        const uint32_t si = p.spill_index[i];
+        if (si != kNoSpill) {
+          nbond += p.spill[si].cnt[0];
+          spill_off = p.spill[si].off[0];
+        }
+      }
       const float3 x3 = f3(xi);
       for (uint32_t k = t; k < nbond; k += kAdjTeam) {
-        const uint2 r = p.bond[static_cast<size_t>(k) * n + i];
+        const uint2 r = k < nbond_inline ? p.bond[static_cast<size_t>(k) * n + i]
+                                         : p.spill_bond[spill_off + (k - nbond_inline)];
         const uint32_t bt = r.y & 0x7fffffffu;
         const double kk = p.bond_k[bt], r0 = p.bond_r0[bt];
         const float3 d = mic3(x3, f3(p.pos[r.x]), p.box);

@@ -30,6 +30,9 @@ struct AdjointParams {
   const uint32_t* topo_count;
   const uint2* bond;
   uint32_t bcap;
+  const uint32_t* spill_index;  // null when no particle spills
+  const SpillEntry* spill;
+  const uint2* spill_bond;
   int32_t n_bond_types;
   const double* bond_k;
   const double* bond_r0;

@@ -91,6 +91,18 @@ struct NeighborControl {
   int32_t path;
 };
 
+// Listed-term roles and exclusions are stored per slot, slot-major, up to an
+// inline capacity per kind (bond, angle, dihedral, exclusion); a particle
+// with more records of a kind keeps the rest in its own contiguous segment of
+// that kind's spill array. spill_index[slot] names the particle's SpillEntry
+// (kNoSpill when it has none); the entry moves with the particle's slot and
+// its records' partner slots are remapped by every reorder.
+constexpr uint32_t kNoSpill = 0xffffffffu;
+struct SpillEntry {
+  uint32_t off[4];  // first record in the spill array of each kind
+  uint32_t cnt[4];  // records of each kind beyond the inline capacity
+};
+
 struct TopologyView {
   const uint32_t* count;
   const uint2* bond;
@@ -102,6 +114,12 @@ struct TopologyView {
   const float2* angle_coeff;  // k, theta0 (rad)
   const float4* dih_coeff;    // k, d, n, unused
   int32_t has_listed;
+  // null when no particle spills
+  const uint32_t* spill_index;
+  const SpillEntry* spill;
+  const uint2* spill_bond;
+  const uint4* spill_angle;
+  const uint4* spill_dih;
 };
 
 struct ForceParams {
@@ -161,6 +179,9 @@ struct BuildParams {
   const uint32_t* topo_count;
   const uint32_t* excl;
   uint32_t ecap;
+  const uint32_t* spill_index;  // null unless some particle spills exclusions
+  const SpillEntry* spill;
+  const uint32_t* spill_excl;
   uint32_t max_neighbors;
   uint32_t* nbr_count;
   uint32_t* nbr;
@@ -195,6 +216,13 @@ struct ReorderParams {
   uint4* s_dih;
   uint32_t* s_excl;
   uint32_t bcap, acap, dcap, ecap;
+  uint32_t* spill_index;        // null when no particle spills
+  uint32_t* s_spill_index;
+  const SpillEntry* spill;
+  uint2* spill_bond;
+  uint4* spill_angle;
+  uint4* spill_dih;
+  uint32_t* spill_excl;
   uint32_t* cell_atoms;
   float4* cell_pos;
   uint32_t* new_of_old;

@@ -203,6 +203,36 @@ std::vector<std::pair<uint32_t, uint32_t>> bond_graph_exclusions(uint32_t n, uin
   return out;
 }
 
+uint32_t inline_capacity(const std::vector<uint32_t>& counts) {
+  constexpr uint32_t kSmall = 16;
+  const uint32_t mx = counts.empty() ? 0u : *std::max_element(counts.begin(), counts.end());
+  if (mx <= kSmall) return mx;
+  const size_t allow = std::max<size_t>(16, counts.size() / 256);
+  std::vector<size_t> above(static_cast<size_t>(mx) + 2, 0);  // above[c] = #(count > c)
+  for (const uint32_t c : counts) ++above[c];
+  size_t run = 0;
+  for (int64_t c = mx; c >= 0; --c) {
+    const size_t here = above[static_cast<size_t>(c)];
+    above[static_cast<size_t>(c)] = run;
+    run += here;
+  }
+  uint32_t cap = mx;
+  for (uint32_t c = kSmall; c <= mx; ++c) {
+    if (above[c] <= allow) {
+      cap = c;
+      break;
+    }
+  }
+  return std::min<uint32_t>(cap, 255u);
+}
+
+// Per-slot records of the listed terms (every role of every term, so each
+// atom evaluates its own terms) and of the excluded partners. The reference
+// keeps bonded terms as term lists (bond_harmonic.cu:24-99) and per-slot
+// exclusions as up to 6 inline partners plus an overflow region
+// (exclusions.cu:131-210, 337-445); here every kind has an inline capacity
+// (inline_capacity) and records past it go to the particle's spill segment,
+// so a particle may take part in any number of terms and exclusions.
 HostTopology build_host_topology(const beads_spec_t& s) {
   HostTopology h;
   const uint32_t n = s.n_particles;
@@ -227,16 +257,39 @@ HostTopology build_host_topology(const beads_spec_t& s) {
     ++ne[slot_of_tag[p.first]];
     ++ne[slot_of_tag[p.second]];
   }
-  auto max_of = [](const std::vector<uint32_t>& v) {
-    return v.empty() ? 0u : *std::max_element(v.begin(), v.end());
-  };
-  h.bcap = max_of(nb);
-  h.acap = max_of(na);
-  h.dcap = max_of(nd);
-  h.ecap = max_of(ne);
-  require(h.bcap <= 255 && h.acap <= 255 && h.dcap <= 255 && h.ecap <= 255,
-          "a particle takes part in more than 255 bonds, angles, dihedrals or exclusions; "
-          "this engine supports at most 255 of each per particle.");
+  h.bcap = inline_capacity(nb);
+  h.acap = inline_capacity(na);
+  h.dcap = inline_capacity(nd);
+  h.ecap = inline_capacity(ne);
+  const uint32_t caps[4] = {h.bcap, h.acap, h.dcap, h.ecap};
+  const std::vector<uint32_t>* all[4] = {&nb, &na, &nd, &ne};
+  // Spill entries and segment offsets.
+  std::vector<uint32_t> entry(n, kNoSpill);
+  size_t spill_total[4] = {0, 0, 0, 0};
+  for (uint32_t i = 0; i < n; ++i) {
+    bool spills = false;
+    for (int q = 0; q < 4; ++q) spills |= (*all[q])[i] > caps[q];
+    if (!spills) continue;
+    SpillEntry e{};
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t extra = (*all[q])[i] > caps[q] ? (*all[q])[i] - caps[q] : 0u;
+      require(spill_total[q] + extra <= 0xffffffffull, "topology records exceed 2^32 per kind.",
+              BEADS_E_OVERFLOW);
+      e.off[q] = static_cast<uint32_t>(spill_total[q]);
+      e.cnt[q] = extra;
+      spill_total[q] += extra;
+    }
+    entry[i] = static_cast<uint32_t>(h.spill.size());
+    h.spill.push_back(e);
+  }
+  if (!h.spill.empty()) {
+    h.spill_index = entry;
+    h.spill_bond.assign(spill_total[0], uint2{0, 0});
+    h.spill_angle.assign(spill_total[1], uint4{0, 0, 0, 0});
+    h.spill_dih.assign(spill_total[2], uint4{0, 0, 0, 0});
+    h.spill_excl.assign(spill_total[3], 0u);
+    h.spills_exclusions = spill_total[3] > 0;
+  }
   h.count.assign(n, 0);
   h.bond.assign(static_cast<size_t>(h.bcap) * n, uint2{0, 0});
   h.angle.assign(static_cast<size_t>(h.acap) * n, uint4{0, 0, 0, 0});
@@ -244,19 +297,40 @@ HostTopology build_host_topology(const beads_spec_t& s) {
   h.excl.assign(static_cast<size_t>(h.ecap) * n, 0u);
   std::vector<uint32_t> cb(n, 0), ca(n, 0), cd(n, 0), ce(n, 0);
   const size_t N = n;
+  // k-th record of slot i of kind q: inline below the capacity, else spilled.
+  auto put_bond = [&](uint32_t i, uint2 r) {
+    const uint32_t k = cb[i]++;
+    if (k < h.bcap) h.bond[k * N + i] = r;
+    else h.spill_bond[h.spill[entry[i]].off[0] + (k - h.bcap)] = r;
+  };
+  auto put_angle = [&](uint32_t i, uint4 r) {
+    const uint32_t k = ca[i]++;
+    if (k < h.acap) h.angle[k * N + i] = r;
+    else h.spill_angle[h.spill[entry[i]].off[1] + (k - h.acap)] = r;
+  };
+  auto put_dih = [&](uint32_t i, uint4 r) {
+    const uint32_t k = cd[i]++;
+    if (k < h.dcap) h.dih[k * N + i] = r;
+    else h.spill_dih[h.spill[entry[i]].off[2] + (k - h.dcap)] = r;
+  };
+  auto put_excl = [&](uint32_t i, uint32_t partner) {
+    const uint32_t k = ce[i]++;
+    if (k < h.ecap) h.excl[k * N + i] = partner;
+    else h.spill_excl[h.spill[entry[i]].off[3] + (k - h.ecap)] = partner;
+  };
   for (uint32_t b = 0; b < s.n_bonds; ++b) {
     const uint32_t si = slot_of_tag[s.bonds[3 * b]], sj = slot_of_tag[s.bonds[3 * b + 1]];
     const uint32_t ty = s.bonds[3 * b + 2] - 1;
-    h.bond[cb[si]++ * N + si] = uint2{sj, ty | 0x80000000u};
-    h.bond[cb[sj]++ * N + sj] = uint2{si, ty};
+    put_bond(si, uint2{sj, ty | 0x80000000u});
+    put_bond(sj, uint2{si, ty});
   }
   for (uint32_t a = 0; a < s.n_angles; ++a) {
     const uint32_t si = slot_of_tag[s.angles[4 * a]], sj = slot_of_tag[s.angles[4 * a + 1]],
                    sk = slot_of_tag[s.angles[4 * a + 2]];
     const uint32_t ty = s.angles[4 * a + 3] - 1;
-    h.angle[ca[si]++ * N + si] = uint4{sj, sk, ty, 0};
-    h.angle[ca[sj]++ * N + sj] = uint4{si, sk, ty, 1};
-    h.angle[ca[sk]++ * N + sk] = uint4{si, sj, ty, 2};
+    put_angle(si, uint4{sj, sk, ty, 0});
+    put_angle(sj, uint4{si, sk, ty, 1});
+    put_angle(sk, uint4{si, sj, ty, 2});
   }
   for (uint32_t d = 0; d < s.n_dihedrals; ++d) {
     uint32_t sl[4];
@@ -268,16 +342,18 @@ HostTopology build_host_topology(const beads_spec_t& s) {
       for (int q = 0; q < 4; ++q) {
         if (q != static_cast<int>(r)) others[m++] = sl[q];
       }
-      h.dih[cd[sl[r]]++ * N + sl[r]] = uint4{others[0], others[1], others[2], ty | (r << 30)};
+      put_dih(sl[r], uint4{others[0], others[1], others[2], ty | (r << 30)});
     }
   }
   for (const auto& p : pairs) {
     const uint32_t a = slot_of_tag[p.first], b = slot_of_tag[p.second];
-    h.excl[ce[a]++ * N + a] = b;
-    h.excl[ce[b]++ * N + b] = a;
+    put_excl(a, b);
+    put_excl(b, a);
   }
   for (uint32_t i = 0; i < n; ++i) {
-    h.count[i] = nb[i] | (na[i] << 8) | (nd[i] << 16) | (ne[i] << 24);
+    const uint32_t ib = std::min(nb[i], h.bcap), ia = std::min(na[i], h.acap);
+    const uint32_t id = std::min(nd[i], h.dcap), ie = std::min(ne[i], h.ecap);
+    h.count[i] = ib | (ia << 8) | (id << 16) | (ie << 24);
   }
   return h;
 }
@@ -536,6 +612,16 @@ void Engine::upload_state() {
   angle_.upload(topo.angle.data(), topo.angle.size(), stream_);
   dih_.upload(topo.dih.data(), topo.dih.size(), stream_);
   excl_.upload(topo.excl.data(), topo.excl.size(), stream_);
+  has_spill_ = !topo.spill.empty();
+  spill_excl_any_ = topo.spills_exclusions;
+  if (has_spill_) {
+    spill_index_.upload(topo.spill_index.data(), n, stream_);
+    spill_.upload(topo.spill.data(), topo.spill.size(), stream_);
+    spill_bond_.upload(topo.spill_bond.data(), topo.spill_bond.size(), stream_);
+    spill_angle_.upload(topo.spill_angle.data(), topo.spill_angle.size(), stream_);
+    spill_dih_.upload(topo.spill_dih.data(), topo.spill_dih.size(), stream_);
+    spill_excl_.upload(topo.spill_excl.data(), topo.spill_excl.size(), stream_);
+  }
 
   if (plan_.path != 0) {
     s_pos_.resize(n);
@@ -549,6 +635,7 @@ void Engine::upload_state() {
     s_angle_.resize(topo.angle.size());
     s_dih_.resize(topo.dih.size());
     s_excl_.resize(topo.excl.size());
+    if (has_spill_) s_spill_index_.resize(n);
     new_of_old_.resize(n);
   }
 
@@ -602,6 +689,11 @@ void Engine::set_up_params() {
   topo.angle_coeff = angle_coeff_.data();
   topo.dih_coeff = dih_coeff_.data();
   topo.has_listed = (bcap_ | acap_ | dcap_) != 0;
+  topo.spill_index = has_spill_ ? spill_index_.data() : nullptr;
+  topo.spill = spill_.data();
+  topo.spill_bond = spill_bond_.data();
+  topo.spill_angle = spill_angle_.data();
+  topo.spill_dih = spill_dih_.data();
 
   ForceParams& f = force_params_;
   f.n = n_;
@@ -667,6 +759,9 @@ void Engine::set_up_params() {
   b.topo_count = topo_count_.data();
   b.excl = excl_.data();
   b.ecap = ecap_;
+  b.spill_index = spill_excl_any_ ? spill_index_.data() : nullptr;
+  b.spill = spill_.data();
+  b.spill_excl = spill_excl_.data();
   b.max_neighbors = plan_.max_neighbors;
   b.nbr_count = nbr_count_.data();
   b.nbr = nbr_.data();
@@ -725,6 +820,13 @@ void Engine::set_up_params() {
   r.acap = acap_;
   r.dcap = dcap_;
   r.ecap = ecap_;
+  r.spill_index = has_spill_ ? spill_index_.data() : nullptr;
+  r.s_spill_index = s_spill_index_.data();
+  r.spill = spill_.data();
+  r.spill_bond = spill_bond_.data();
+  r.spill_angle = spill_angle_.data();
+  r.spill_dih = spill_dih_.data();
+  r.spill_excl = spill_excl_.data();
   r.cell_atoms = cell_atoms_.data();
   r.cell_pos = cell_pos_.data();
   r.new_of_old = new_of_old_.data();
@@ -1204,6 +1306,9 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   p.topo_count = topo_count_.data();
   p.bond = bond_.data();
   p.bcap = bcap_;
+  p.spill_index = has_spill_ ? spill_index_.data() : nullptr;
+  p.spill = spill_.data();
+  p.spill_bond = spill_bond_.data();
   p.n_bond_types = n_bond_types_;
   p.bond_k = bond_k_d_.data();
   p.bond_r0 = bond_r0_d_.data();

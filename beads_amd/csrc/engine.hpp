@@ -49,14 +49,27 @@ BoxView make_box_view(const float* box_bound);
 // Per-slot listed-term roles and exclusion partners built on the host from the
 // tag-based topology (the device keeps them current across reorders).
 struct HostTopology {
-  uint32_t bcap = 0, acap = 0, dcap = 0, ecap = 0;
-  std::vector<uint32_t> count;  // packed counts per slot
+  uint32_t bcap = 0, acap = 0, dcap = 0, ecap = 0;  // inline capacities
+  std::vector<uint32_t> count;  // packed inline counts per slot (8 bits each)
   std::vector<uint2> bond;      // [bcap][n]
   std::vector<uint4> angle;     // [acap][n]
   std::vector<uint4> dih;       // [dcap][n]
   std::vector<uint32_t> excl;   // [ecap][n]
   uint64_t excluded_pairs = 0;
+  // Records past the inline capacities (all empty when nothing spills).
+  std::vector<uint32_t> spill_index;  // [n], kNoSpill or an entry index
+  std::vector<SpillEntry> spill;
+  std::vector<uint2> spill_bond;
+  std::vector<uint4> spill_angle;
+  std::vector<uint4> spill_dih;
+  std::vector<uint32_t> spill_excl;
+  bool spills_exclusions = false;
 };
+
+// Inline capacity for one kind of per-slot record: the largest count when it
+// is small; otherwise the smallest capacity that at most max(16, n / 256)
+// particles exceed (those spill the rest), never above 255.
+uint32_t inline_capacity(const std::vector<uint32_t>& counts);
 
 HostTopology build_host_topology(const beads_spec_t& spec);
 
@@ -173,6 +186,12 @@ class Engine {
   DeviceBuffer<uint32_t> cell_of_, s_cell_of_, cell_count_, cell_start_, cell_cursor_, cell_atoms_;
   DeviceBuffer<uint32_t> new_of_old_;
   DeviceBuffer<uint32_t> topo_count_, s_topo_count_, excl_, s_excl_;
+  // records past the inline capacities (empty when nothing spills)
+  DeviceBuffer<uint32_t> spill_index_, s_spill_index_, spill_excl_;
+  DeviceBuffer<SpillEntry> spill_;
+  DeviceBuffer<uint2> spill_bond_;
+  DeviceBuffer<uint4> spill_angle_, spill_dih_;
+  bool has_spill_ = false, spill_excl_any_ = false;
   DeviceBuffer<uint2> bond_, s_bond_;
   DeviceBuffer<uint4> angle_, s_angle_, dih_, s_dih_;
   DeviceBuffer<LjCoeff> lj_;

@@ -331,6 +331,7 @@ __global__ __launch_bounds__(kBlock) void reorder_gather_kernel(ReorderParams p)
   for (uint32_t k = 0; k < p.acap; ++k) p.s_angle[k * n + s] = p.angle[k * n + o];
   for (uint32_t k = 0; k < p.dcap; ++k) p.s_dih[k * n + s] = p.dih[k * n + o];
   for (uint32_t k = 0; k < p.ecap; ++k) p.s_excl[k * n + s] = p.excl[k * n + o];
+  if (p.spill_index != nullptr) p.s_spill_index[s] = p.spill_index[o];
 }
 
 // Copy back and translate partner slots through the inverse permutation.
@@ -376,6 +377,34 @@ __global__ __launch_bounds__(kBlock) void reorder_commit_kernel(ReorderParams p)
   for (uint32_t k = 0; k < ne; ++k) {
     p.excl[k * n + s] = m[p.s_excl[k * n + s]];
   }
+  if (p.spill_index != nullptr) {
+    // The particle's spill segments stay where they are; their partner slots
+    // are remapped here by their one owner.
+    const uint32_t si = p.s_spill_index[s];
+    p.spill_index[s] = si;
+    if (si != kNoSpill) {
+      const SpillEntry e = p.spill[si];
+      for (uint32_t k = 0; k < e.cnt[0]; ++k) {
+        uint2& r = p.spill_bond[e.off[0] + k];
+        r.x = m[r.x];
+      }
+      for (uint32_t k = 0; k < e.cnt[1]; ++k) {
+        uint4& r = p.spill_angle[e.off[1] + k];
+        r.x = m[r.x];
+        r.y = m[r.y];
+      }
+      for (uint32_t k = 0; k < e.cnt[2]; ++k) {
+        uint4& r = p.spill_dih[e.off[2] + k];
+        r.x = m[r.x];
+        r.y = m[r.y];
+        r.z = m[r.z];
+      }
+      for (uint32_t k = 0; k < e.cnt[3]; ++k) {
+        uint32_t& r = p.spill_excl[e.off[3] + k];
+        r = m[r];
+      }
+    }
+  }
 }
 
 constexpr int kBuildTeam = 4;
@@ -414,6 +443,14 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     const uint32_t n = p.n;
     uint32_t ex[EXCL > 0 ? EXCL : 1];
     uint32_t ne = 0;
+    uint32_t spill_ne = 0, spill_off = 0;  // exclusions past the inline capacity
+    if (EXCL < 0 && p.spill_index != nullptr) {
+      const uint32_t si = p.spill_index[i];
+      if (si != kNoSpill) {
+        spill_off = p.spill[si].off[3];
+        spill_ne = p.spill[si].cnt[3];
+      }
+    }
     if (EXCL != 0) {
       ne = p.topo_count[i] >> 24;
       if (EXCL > 0) {
@@ -454,6 +491,9 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         if (ok) {
           for (uint32_t k = 0; k < ne; ++k) {
             if (p.excl[static_cast<size_t>(k) * n + i] == j) ok = false;
+          }
+          for (uint32_t k = 0; k < spill_ne; ++k) {
+            if (p.spill_excl[spill_off + k] == j) ok = false;
           }
         }
       }
@@ -813,9 +853,8 @@ struct StepSums {
 // dihedral_harmonic.cu). A term's energy and virial are counted once, by role
 // 0 (bonds: the flag bit of the coefficient word).
 template <bool OBS>
-__device__ __forceinline__ float3 bond_term(const ForceParams& p, uint32_t i, float3 xi,
-                                            uint32_t k, StepSums& sums) {
-  const uint2 r = p.topo.bond[static_cast<size_t>(k) * p.n + i];
+__device__ __forceinline__ float3 bond_term(const ForceParams& p, float3 xi, uint2 r,
+                                            StepSums& sums) {
   double e, w;
   const float3 f =
       bond_force(mic3(xi, f3(p.pos[r.x]), p.box), p.topo.bond_coeff[r.y & 0x7fffffffu], e, w);
@@ -827,9 +866,8 @@ __device__ __forceinline__ float3 bond_term(const ForceParams& p, uint32_t i, fl
 }
 
 template <bool OBS>
-__device__ __forceinline__ float3 angle_term(const ForceParams& p, uint32_t i, float3 xi,
-                                             uint32_t k, StepSums& sums) {
-  const uint4 r = p.topo.angle[static_cast<size_t>(k) * p.n + i];
+__device__ __forceinline__ float3 angle_term(const ForceParams& p, float3 xi, uint4 r,
+                                             StepSums& sums) {
   const int role = static_cast<int>(r.w);
   const float3 a = f3(p.pos[r.x]);
   const float3 b = f3(p.pos[r.y]);
@@ -846,9 +884,8 @@ __device__ __forceinline__ float3 angle_term(const ForceParams& p, uint32_t i, f
 }
 
 template <bool OBS>
-__device__ __forceinline__ float3 dihedral_term(const ForceParams& p, uint32_t i, float3 xi,
-                                                uint32_t k, StepSums& sums) {
-  const uint4 r = p.topo.dih[static_cast<size_t>(k) * p.n + i];
+__device__ __forceinline__ float3 dihedral_term(const ForceParams& p, float3 xi, uint4 r,
+                                                StepSums& sums) {
   const int role = static_cast<int>(r.w >> 30);
   const float3 a = f3(p.pos[r.x]);
   const float3 b = f3(p.pos[r.y]);
@@ -870,7 +907,7 @@ __device__ __forceinline__ float3 dihedral_term(const ForceParams& p, uint32_t i
 
 // The listed terms of atom i shared over its team: lane t takes terms t,
 // t + kTeam, ... of each kind and adds their forces to its FP64 partial sums.
-template <bool OBS>
+template <bool OBS, bool SPILL>
 __device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, uint32_t t,
                                             float4 xi4, double& fxd, double& fyd, double& fzd,
                                             StepSums& sums) {
@@ -880,23 +917,35 @@ __device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, ui
   const uint32_t cw = p.topo.count[i];
   const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
   const float3 xi = f3(xi4);
-  for (uint32_t k = t; k < nbond; k += kTeam) {
-    const float3 f = bond_term<OBS>(p, i, xi, k, sums);
+  const size_t n = p.n;
+  auto add = [&](float3 f) {
     fxd += f.x;
     fyd += f.y;
     fzd += f.z;
+  };
+  for (uint32_t k = t; k < nbond; k += kTeam) {
+    add(bond_term<OBS>(p, xi, p.topo.bond[k * n + i], sums));
   }
   for (uint32_t k = t; k < nang; k += kTeam) {
-    const float3 f = angle_term<OBS>(p, i, xi, k, sums);
-    fxd += f.x;
-    fyd += f.y;
-    fzd += f.z;
+    add(angle_term<OBS>(p, xi, p.topo.angle[k * n + i], sums));
   }
   for (uint32_t k = t; k < ndih; k += kTeam) {
-    const float3 f = dihedral_term<OBS>(p, i, xi, k, sums);
-    fxd += f.x;
-    fyd += f.y;
-    fzd += f.z;
+    add(dihedral_term<OBS>(p, xi, p.topo.dih[k * n + i], sums));
+  }
+  if (SPILL) {  // some particle spills: its records past the inline capacities
+    const uint32_t si = p.topo.spill_index[i];
+    if (si != kNoSpill) {  // uniform over the team
+      const SpillEntry e = p.topo.spill[si];
+      for (uint32_t k = t; k < e.cnt[0]; k += kTeam) {
+        add(bond_term<OBS>(p, xi, p.topo.spill_bond[e.off[0] + k], sums));
+      }
+      for (uint32_t k = t; k < e.cnt[1]; k += kTeam) {
+        add(angle_term<OBS>(p, xi, p.topo.spill_angle[e.off[1] + k], sums));
+      }
+      for (uint32_t k = t; k < e.cnt[2]; k += kTeam) {
+        add(dihedral_term<OBS>(p, xi, p.topo.spill_dih[e.off[2] + k], sums));
+      }
+    }
   }
 }
 
@@ -1015,7 +1064,7 @@ __global__ __launch_bounds__(kReduceBlock) void step_reduce_kernel(ForceParams p
 // consecutive slots of one cell, a few cache lines per atom instead of one
 // line per lane. The team's FP64 partial sums meet in a shuffle butterfly and
 // the atom's total is rounded to binary32 once.
-template <bool KICK, bool OBS, int TYPES, bool HALF>
+template <bool KICK, bool OBS, int TYPES, bool HALF, bool SPILL>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   __shared__ LjCoeff lj_lds[TYPES == 1 && !HALF ? kMaxLdsTypes * kMaxLdsTypes : 1];
   __shared__ double red_lds[4];
@@ -1040,7 +1089,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
     } else {
       lj_team<OBS, TYPES>(p, i, t, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
     }
-    listed_team<OBS>(p, i, t, xi4, fxd, fyd, fzd, sums);
+    listed_team<OBS, SPILL>(p, i, t, xi4, fxd, fyd, fzd, sums);
     fxd = team_sum<kTeam>(fxd);
     fyd = team_sum<kTeam>(fyd);
     fzd = team_sum<kTeam>(fzd);
@@ -1187,7 +1236,9 @@ void launch_build(const BuildParams& p, hipStream_t stream) {
   const unsigned g = grid_for(p.n, kBuildAtoms);
   // Exclusion registers: the fewest of 4 / 8 that hold every slot's
   // exclusions (ecap is the largest count), else read from memory.
-  const int ex = p.ecap == 0 ? 0 : (p.ecap <= 4 ? 4 : (p.ecap <= 8 ? 8 : -1));
+  const int ex = p.spill_index != nullptr
+                     ? -1
+                     : (p.ecap == 0 ? 0 : (p.ecap <= 4 ? 4 : (p.ecap <= 8 ? 8 : -1)));
 #define BEADS_BUILD_CASE(E, H) \
   if (ex == E && (p.half != 0) == H) build_kernel<E, H><<<g, kBlock, 0, stream>>>(p);
   BEADS_BUILD_CASE(0, false)
@@ -1217,16 +1268,25 @@ void launch_force(const ForceParams& p, bool kick, bool obs, hipStream_t stream)
 #undef BEADS_HALF_CASE
   }
   const unsigned gf = force_grid(p.n);
-#define BEADS_FORCE_CASE(K, O, T)                                             \
-  if (kick == K && obs == O && (half || types == T)) {                        \
-    if (half) {                                                               \
-      force_kernel<K, O, 0, true><<<gf, kBlock, 0, stream>>>(p);              \
-      step_reduce_kernel<K, O, true><<<1, kReduceBlock, 0, stream>>>(p, gf);  \
-    } else {                                                                  \
-      force_kernel<K, O, T, false><<<gf, kBlock, 0, stream>>>(p);             \
-      step_reduce_kernel<K, O, false><<<1, kReduceBlock, 0, stream>>>(p, gf); \
-    }                                                                         \
-    return;                                                                   \
+  const bool spill = p.topo.spill_index != nullptr;
+#define BEADS_FORCE_CASE(K, O, T)                                                      \
+  if (kick == K && obs == O && (half || types == T)) {                                 \
+    if (half) {                                                                        \
+      if (spill) {                                                                     \
+        force_kernel<K, O, 0, true, true><<<gf, kBlock, 0, stream>>>(p);               \
+      } else {                                                                         \
+        force_kernel<K, O, 0, true, false><<<gf, kBlock, 0, stream>>>(p);              \
+      }                                                                                \
+      step_reduce_kernel<K, O, true><<<1, kReduceBlock, 0, stream>>>(p, gf);           \
+    } else {                                                                           \
+      if (spill) {                                                                     \
+        force_kernel<K, O, T, false, true><<<gf, kBlock, 0, stream>>>(p);              \
+      } else {                                                                         \
+        force_kernel<K, O, T, false, false><<<gf, kBlock, 0, stream>>>(p);             \
+      }                                                                                \
+      step_reduce_kernel<K, O, false><<<1, kReduceBlock, 0, stream>>>(p, gf);          \
+    }                                                                                  \
+    return;                                                                            \
   }
   BEADS_FORCE_CASE(true, false, 0)
   BEADS_FORCE_CASE(true, false, 1)

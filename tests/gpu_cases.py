@@ -112,6 +112,61 @@ def bonded_lattice(kind: str, distance: int, side: int = 15, density: float = 0.
         bond_r0=[spacing], exclusion_distance=distance, dt=0.005, steps=100)
 
 
+def hub_lattice(arms: int, arm_length: int, distance: int, listed: bool, side: int = 15,
+                density: float = 0.8442, seed: int = 37) -> W.Workload:
+    """LJ beads (reduced units) on a jittered full cubic lattice with one hub
+    at the centre site carrying `arms` arms of `arm_length` beads each (the
+    sites nearest the hub, bonded hub-a1-a2...), harmonic bonds (k 100, r0 =
+    spacing) and, when `listed`, every angle and dihedral the arms define
+    (a1-hub-b1 for every pair of arms, hub-a1-a2, a2-a1-hub-b1 ...), with
+    exclusions to the given distance. The hub takes part in far more bonds,
+    angles, dihedrals and exclusions than the other beads, so its records go
+    past the engine's inline capacities into spill segments."""
+    n = side ** 3
+    L = (n / density) ** (1.0 / 3.0)
+    spacing = L / side
+    rng = np.random.default_rng(seed)
+    g = np.indices((side, side, side)).reshape(3, -1).T[:, ::-1].astype(np.float64)
+    pos = (g + 0.5) * spacing + rng.uniform(-0.05, 0.05, (n, 3)) * spacing
+    c = side // 2
+    hub = c + side * (c + side * c)
+    d = np.linalg.norm((g - g[hub]), axis=1)
+    order = [int(k) for k in np.argsort(d, kind="stable") if k != hub]
+    tags = np.arange(1, n + 1)
+    chains = []
+    for a in range(arms):
+        chains.append([hub] + [order[a + arms * m] for m in range(arm_length)])
+    bonds, angles, dihedrals = set(), set(), set()
+    for ch in chains:
+        for u, v in zip(ch[:-1], ch[1:]):
+            bonds.add((min(tags[u], tags[v]), max(tags[u], tags[v])))
+    if listed:
+        for a, ca in enumerate(chains):
+            for b, cb in enumerate(chains):
+                if a < b:
+                    angles.add((tags[ca[1]], tags[hub], tags[cb[1]]))
+                if a != b and arm_length >= 2:
+                    dihedrals.add((tags[ca[2]], tags[ca[1]], tags[hub], tags[cb[1]]))
+            for k in range(len(ca) - 2):
+                angles.add((tags[ca[k]], tags[ca[k + 1]], tags[ca[k + 2]]))
+    b = np.array([(i, j, 1) for i, j in sorted(bonds)], np.uint32)
+    an = np.array([(i, j, k, 1) for i, j, k in sorted(angles)], np.uint32).reshape(-1, 4)
+    dh = np.array([(i, j, k, m, 1) for i, j, k, m in sorted(dihedrals)], np.uint32).reshape(-1, 5)
+    lo, hi = np.zeros(3), np.full(3, L)
+    masses = np.ones(n)
+    extra = {}
+    if listed:
+        extra = dict(angles=an, dihedrals=dh, angle_k=[5.0], angle_theta0=[109.5],
+                     dihedral_k=[0.5], dihedral_d=[1], dihedral_n=[2])
+    return W.Workload(
+        name=f"hub_{arms}x{arm_length}_d{distance}", units="reduced",
+        positions=W._wrap_into(pos, lo, hi),
+        velocities=W.maxwell_boltzmann(rng, masses.astype(np.float32), 1.0, 1.0), masses=masses,
+        types=np.ones(n, np.int32), box_bound=np.stack([lo, hi]), lj_epsilon=np.ones((1, 1)),
+        lj_sigma=np.ones((1, 1)), cutoff=2.5, shift=True, bonds=b, bond_k=[100.0],
+        bond_r0=[spacing], exclusion_distance=distance, dt=0.002, steps=50, **extra)
+
+
 def exclusion_counts(w: W.Workload) -> np.ndarray:
     """Per-tag number of beads within the exclusion bond distance (brute-force
     breadth-first search over the bond graph)."""
