@@ -33,6 +33,9 @@ class Spec(C.Structure):
         ("types", _i32p), ("tags", _u32p), ("molecule_ids", _u32p), ("images", _i32p),
         ("n_bonds", C.c_uint32), ("bonds", _u32p), ("n_angles", C.c_uint32), ("angles", _u32p),
         ("n_dihedrals", C.c_uint32), ("dihedrals", _u32p),
+        ("pair_style", C.c_char_p), ("bond_style", C.c_char_p), ("angle_style", C.c_char_p),
+        ("dihedral_style", C.c_char_p), ("integrator_style", C.c_char_p),
+        ("thermostat_style", C.c_char_p),
         ("n_types", C.c_int32), ("lj_cutoff", C.c_float), ("lj_shift", C.c_int32),
         ("lj_epsilon", _f32p), ("lj_sigma", _f32p),
         ("n_bond_types", C.c_int32), ("bond_k", _f32p), ("bond_r0", _f32p),

@@ -358,36 +358,6 @@ HostTopology build_host_topology(const beads_spec_t& s) {
   return h;
 }
 
-// pair_lj.cu:171-192 coefficient packing in binary32:
-// {sigma^2, 24 eps, 4 eps, shift} per ordered type pair.
-std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                   float cutoff, bool shift_energy) {
-  const int T = n_types;
-  std::vector<LjCoeff> lj(static_cast<size_t>(T) * T);
-  const double rc = cutoff;
-  for (int a = 0; a < T; ++a) {
-    for (int b = 0; b < T; ++b) {
-      const double eps = epsilon[a * T + b], sig = sigma[a * T + b];
-      LjCoeff c;
-      c.sigma2 = sig * sig;
-      c.eps4 = 4.0 * eps;
-      c.eps24 = 24.0 * eps;
-      c.shift = 0.0;
-      if (shift_energy) {
-        const double sr2 = c.sigma2 / (rc * rc);
-        const double sr6 = sr2 * sr2 * sr2;
-        c.shift = c.eps4 * (sr6 * sr6 - sr6);
-      }
-      lj[static_cast<size_t>(a) * T + b] = c;
-    }
-  }
-  return lj;
-}
-
-std::vector<LjCoeff> pack_lj_table(const beads_spec_t& s) {
-  return pack_lj_table(s.n_types, s.lj_epsilon, s.lj_sigma, s.lj_cutoff, s.lj_shift != 0);
-}
-
 // Shell validation of the typed spec (native_spec_validation.hpp:23-716,
 // restated for the fields this engine consumes).
 void validate_spec(const beads_spec_t& s) {
@@ -419,16 +389,7 @@ void validate_spec(const beads_spec_t& s) {
     seen[t] = 1;
     require(s.molecule_ids[i] >= 1, "system.molecule_ids must be one-based.");
   }
-  require(s.n_types >= 1 && s.lj_epsilon && s.lj_sigma,
-          "forcefield requires pair_style \"lj\" with coefficients.");
-  require(finite(s.lj_cutoff) && s.lj_cutoff > 0.f,
-          "Pair(\"lj\") pair_style.cutoff must be finite and positive.");
-  for (int32_t k = 0; k < s.n_types * s.n_types; ++k) {
-    require(finite(s.lj_epsilon[k]) && s.lj_epsilon[k] > 0.f,
-            "Pair(\"lj\") pair_coeff.epsilon must be finite and positive.");
-    require(finite(s.lj_sigma[k]) && s.lj_sigma[k] > 0.f,
-            "Pair(\"lj\") pair_coeff.sigma must be finite and positive.");
-  }
+  require(s.n_types >= 1, "system.types must be covered by the pair coefficient table.");
   auto check_tag = [&](uint32_t t, const char* what) {
     require(t >= 1 && t <= n, std::string(what) + " references a tag outside the system.");
   };
@@ -454,33 +415,14 @@ void validate_spec(const beads_spec_t& s) {
                 s.dihedrals[5 * d + 4] <= static_cast<uint32_t>(s.n_dihedral_types),
             "dihedral type is not covered by dihedral_coeffs.");
   }
-  for (int32_t t = 0; t < s.n_bond_types; ++t) {
-    require(finite(s.bond_k[t]) && s.bond_k[t] >= 0.f, "Bond(\"harmonic\") bond_coeff.k must be non-negative.");
-    require(finite(s.bond_r0[t]) && s.bond_r0[t] >= 0.f, "Bond(\"harmonic\") bond_coeff.r0 must be non-negative.");
-  }
-  for (int32_t t = 0; t < s.n_angle_types; ++t) {
-    require(finite(s.angle_k[t]) && s.angle_k[t] >= 0.f, "Angle(\"harmonic\") angle_coeff.k must be non-negative.");
-    require(finite(s.angle_theta0_deg[t]) && s.angle_theta0_deg[t] >= 0.f && s.angle_theta0_deg[t] <= 180.f,
-            "Angle(\"harmonic\") angle_coeff.theta0 must be between 0 and 180 degrees.");
-  }
-  for (int32_t t = 0; t < s.n_dihedral_types; ++t) {
-    require(finite(s.dihedral_k[t]) && s.dihedral_k[t] >= 0.f,
-            "Dihedral(\"harmonic\") dihedral_coeff.k must be non-negative.");
-    require(s.dihedral_d[t] == 1 || s.dihedral_d[t] == -1,
-            "Dihedral(\"harmonic\") dihedral_coeff.d must be +1 or -1.");
-    require(s.dihedral_n[t] >= 0, "Dihedral(\"harmonic\") dihedral_coeff.n must be non-negative.");
-  }
   require(s.exclusion_distance >= 0 && s.exclusion_distance <= 3,
           "bond graph exclusion distance must be 1, 2, or 3.");
   require(s.exclusion_distance == 0 || s.n_bonds > 0,
           "ForceField.exclude_bonded(...) requires System topology bonds.");
-  require(finite(s.dt) && s.dt > 0.f, "VelocityVerletIntegrator dt must be finite and positive.");
-  if (s.berendsen) {
-    require(finite(s.berendsen_temperature) && s.berendsen_temperature >= 0.f,
-            "Thermostat(\"berendsen\").temperature must be non-negative.");
-    require(finite(s.berendsen_tau) && s.berendsen_tau > 0.f,
-            "Thermostat(\"berendsen\").tau must be positive.");
-  }
+  // Style parameters are checked by their models (models.cpp).
+  ModelContext ctx;
+  ctx.kb = unit_system_from(s.units).kb;
+  (void)create_models(s, ctx);
   require(!(s.cutoff_buffer == 0.f && s.rebuild_check_every != 1),
           "neighbor cutoff_buffer=0 requires rebuild_check_every=1");
   require(s.half_list == 0 || s.half_list == 1, "neighbor half_list must be 0 or 1.");
@@ -508,7 +450,12 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
   box_ = make_box_view(spec.box_bound);
   for (int a = 0; a < 3; ++a) spec_box_hi_[a] = spec.box_bound[3 + a];
   n_ = spec.n_particles;
-  thermostat_ = spec.berendsen != 0;
+  {
+    ModelContext ctx;
+    ctx.kb = units_.kb;
+    models_ = create_models(spec, ctx);
+  }
+  thermostat_ = models_.thermostat != nullptr;
   half_list_ = spec.half_list != 0;
   {
     // Diagnostics switch: BEADS_AMD_GRAPHS=0 launches every step's kernels
@@ -516,10 +463,11 @@ Engine::Engine(const beads_spec_t& spec) : spec_(spec) {
     const char* env = std::getenv("BEADS_AMD_GRAPHS");
     use_graphs_ = !(env != nullptr && env[0] == '0');
   }
-  has_bond_style_ = spec.n_bond_types > 0;
-  has_angle_style_ = spec.n_angle_types > 0;
-  has_dihedral_style_ = spec.n_dihedral_types > 0;
-  plan_ = make_neighbor_plan(box_, spec.lj_cutoff, spec.cutoff_buffer, n_, spec.max_neighbors,
+  has_bond_style_ = models_.bond != nullptr;
+  has_angle_style_ = models_.angle != nullptr;
+  has_dihedral_style_ = models_.dihedral != nullptr;
+  plan_ = make_neighbor_plan(box_, models_.pair->max_cutoff(), spec.cutoff_buffer, n_,
+                             spec.max_neighbors,
                              spec.rebuild_check_every, spec.sort_every_rebuild);
   interrupt_interval_ = std::max<uint64_t>(1, 100000 / std::max<uint32_t>(1, n_));
 
@@ -639,25 +587,9 @@ void Engine::upload_state() {
     new_of_old_.resize(n);
   }
 
-  n_types_ = s.n_types;
-  n_bond_types_ = s.n_bond_types;
-  lj_shift_ = s.lj_shift != 0;
-  lj_eps_.assign(s.lj_epsilon, s.lj_epsilon + static_cast<size_t>(n_types_) * n_types_);
-  lj_sig_.assign(s.lj_sigma, s.lj_sigma + static_cast<size_t>(n_types_) * n_types_);
-  bond_k_.assign(s.bond_k, s.bond_k + n_bond_types_);
-  bond_r0_.assign(s.bond_r0, s.bond_r0 + n_bond_types_);
+  n_types_ = models_.pair->type_count();
+  n_bond_types_ = models_.bond ? models_.bond->type_count() : 0;
   upload_parameters();
-  std::vector<float2> ac(std::max(1, s.n_angle_types));
-  std::vector<float4> dc(std::max(1, s.n_dihedral_types));
-  for (int t = 0; t < s.n_angle_types; ++t) {
-    ac[t] = make_float2(s.angle_k[t], s.angle_theta0_deg[t] * static_cast<float>(kPi) / 180.f);
-  }
-  for (int t = 0; t < s.n_dihedral_types; ++t) {
-    dc[t] = make_float4(s.dihedral_k[t], static_cast<float>(s.dihedral_d[t]),
-                        static_cast<float>(s.dihedral_n[t]), 0.f);
-  }
-  angle_coeff_.upload(ac.data(), ac.size(), stream_);
-  dih_coeff_.upload(dc.data(), dc.size(), stream_);
 
   // Per-workgroup partial sums of the force pass (the widest grid) and of scale.
   partials_.resize(9 * static_cast<size_t>(std::max(force_grid(n_), grid_for(n_))));
@@ -685,9 +617,9 @@ void Engine::set_up_params() {
   topo.acap = acap_;
   topo.dcap = dcap_;
   topo.ecap = ecap_;
-  topo.bond_coeff = bond_coeff_.data();
-  topo.angle_coeff = angle_coeff_.data();
-  topo.dih_coeff = dih_coeff_.data();
+  if (models_.bond) models_.bond->bind(topo);
+  if (models_.angle) models_.angle->bind(topo);
+  if (models_.dihedral) models_.dihedral->bind(topo);
   topo.has_listed = (bcap_ | acap_ | dcap_) != 0;
   topo.spill_index = has_spill_ ? spill_index_.data() : nullptr;
   topo.spill = spill_.data();
@@ -703,14 +635,13 @@ void Engine::set_up_params() {
   f.nbr_count = nbr_count_.data();
   f.nbr = nbr_.data();
   f.box = box_;
-  f.rc2 = plan_.force_cutoff * plan_.force_cutoff;
-  f.n_types = spec_.n_types;
-  f.lj = lj_.data();
+  models_.pair->bind(f);
   f.topo = topo;
-  f.dt = spec_.dt;
-  f.berendsen = thermostat_ ? 1 : 0;
-  f.target_thermal = units_.kb * static_cast<double>(spec_.berendsen_temperature);
-  f.dt_over_tau = thermostat_ ? static_cast<double>(spec_.dt) / spec_.berendsen_tau : 0.0;
+  f.berendsen = 0;
+  f.target_thermal = 0.0;
+  f.dt_over_tau = 0.0;
+  models_.integrator->bind(integrate_params_, f);
+  if (models_.thermostat) models_.thermostat->bind(f, models_.integrator->dt());
   f.dof = std::max(0.0, 3.0 * static_cast<double>(n_) - 3.0);
   f.partials = partials_.data();
   f.ctrl = ctrl_.data();
@@ -728,7 +659,7 @@ void Engine::set_up_params() {
   in.ref_pos = ref_pos_.data();
   in.ref_image = ref_image_.data();
   in.box = box_;
-  in.dt = spec_.dt;
+  in.dt = models_.integrator->dt();
   const float half_buffer = 0.5f * plan_.cutoff_buffer;
   in.threshold_sq = half_buffer * half_buffer;
   in.ctrl = ctrl_.data();
@@ -1118,43 +1049,21 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
 // Binary32 tables as the forward uses them ({sigma^2, 24 eps, 4 eps, shift},
 // {k, r0}) plus FP64 copies of the same binary32 values for the adjoint.
 void Engine::upload_parameters() {
-  const std::vector<LjCoeff> lj =
-      pack_lj_table(n_types_, lj_eps_.data(), lj_sig_.data(), plan_.force_cutoff, lj_shift_);
-  lj_.upload(lj.data(), lj.size(), stream_);
-  std::vector<float2> bc(std::max(1, n_bond_types_), make_float2(0.f, 0.f));
-  for (int t = 0; t < n_bond_types_; ++t) bc[t] = make_float2(bond_k_[t], bond_r0_[t]);
-  bond_coeff_.upload(bc.data(), bc.size(), stream_);
-  const std::vector<double> e(lj_eps_.begin(), lj_eps_.end()), s(lj_sig_.begin(), lj_sig_.end());
-  std::vector<double> k(std::max(1, n_bond_types_), 0.0), r0(std::max(1, n_bond_types_), 0.0);
-  for (int t = 0; t < n_bond_types_; ++t) {
-    k[t] = bond_k_[t];
-    r0[t] = bond_r0_[t];
-  }
-  eps_d_.upload(e.data(), e.size(), stream_);
-  sig_d_.upload(s.data(), s.size(), stream_);
-  bond_k_d_.upload(k.data(), k.size(), stream_);
-  bond_r0_d_.upload(r0.data(), r0.size(), stream_);
+  models_.pair->upload(stream_);
+  if (models_.bond) models_.bond->upload(stream_);
+  if (models_.angle) models_.angle->upload(stream_);
+  if (models_.dihedral) models_.dihedral->upload(stream_);
   BEADS_HIP(hipStreamSynchronize(stream_));
 }
 
 void Engine::set_lj_params(const float* eps, const float* sigma) {
-  const size_t tt = static_cast<size_t>(n_types_) * n_types_;
-  for (size_t q = 0; q < tt; ++q) {
-    require(std::isfinite(eps[q]) && eps[q] >= 0.f, "lj epsilon must be finite and >= 0.");
-    require(std::isfinite(sigma[q]) && sigma[q] > 0.f, "lj sigma must be finite and > 0.");
-  }
-  lj_eps_.assign(eps, eps + tt);
-  lj_sig_.assign(sigma, sigma + tt);
+  models_.pair->set_type_pair_parameters(eps, sigma);
   upload_parameters();
 }
 
 void Engine::set_bond_params(const float* k, const float* r0) {
-  for (int t = 0; t < n_bond_types_; ++t) {
-    require(std::isfinite(k[t]) && k[t] >= 0.f && std::isfinite(r0[t]) && r0[t] >= 0.f,
-            "bond k and r0 must be finite and non-negative.");
-  }
-  bond_k_.assign(k, k + n_bond_types_);
-  bond_r0_.assign(r0, r0 + n_bond_types_);
+  require(models_.bond != nullptr, "this engine has no bond style.", BEADS_E_VALUE);
+  models_.bond->set_parameters(k, r0);
   upload_parameters();
 }
 
@@ -1260,15 +1169,9 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
           "the adjoint differentiates LJ pairs and harmonic bonds; systems with angle or "
           "dihedral terms are not supported.",
           BEADS_E_VALUE);
-  require(n_types_ <= kMaxLdsTypes && n_bond_types_ <= kMaxBondTypesAdjoint,
-          "too many types for the adjoint kernels.", BEADS_E_VALUE);
-  for (int a = 0; a < n_types_; ++a) {
-    for (int b = 0; b < a; ++b) {
-      require(lj_eps_[a * n_types_ + b] == lj_eps_[b * n_types_ + a] &&
-                  lj_sig_[a * n_types_ + b] == lj_sig_[b * n_types_ + a],
-              "the adjoint requires symmetric LJ tables.", BEADS_E_VALUE);
-    }
-  }
+  AdjointParams p{};
+  models_.pair->bind_adjoint(p);
+  if (models_.bond) models_.bond->bind_adjoint(p);
   const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
   float4* staging = nullptr;
   if (noalloc) {
@@ -1290,28 +1193,18 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
     BEADS_HIP(hipMemsetAsync(grad_params_dev, 0, n_params * sizeof(double), stream_));
   }
   launch_pack_xyz(n_, grad_forces_dev, staging, stream_);
-  AdjointParams p{};
   p.n = n_;
   p.pos = pos_.data();
   p.g = staging;
   p.nbr_count = nbr_count_.data();
   p.nbr = nbr_.data();
   p.box = box_;
-  p.rc2 = plan_.force_cutoff * plan_.force_cutoff;
-  p.rc2_f64 = static_cast<double>(plan_.force_cutoff) * plan_.force_cutoff;
-  p.lj_shift = lj_shift_ ? 1 : 0;
-  p.n_types = n_types_;
-  p.eps = eps_d_.data();
-  p.sigma = sig_d_.data();
   p.topo_count = topo_count_.data();
   p.bond = bond_.data();
   p.bcap = bcap_;
   p.spill_index = has_spill_ ? spill_index_.data() : nullptr;
   p.spill = spill_.data();
   p.spill_bond = spill_bond_.data();
-  p.n_bond_types = n_bond_types_;
-  p.bond_k = bond_k_d_.data();
-  p.bond_r0 = bond_r0_d_.data();
   p.grad_energy = grad_energy;
   p.grad_pos = grad_positions_dev;
   p.grad_params = grad_params_dev;
@@ -1649,9 +1542,11 @@ void Engine::log_run_start() {
   }
   {
     std::ostringstream m;
-    m << "dynamics summary" << '\n' << label_value("style", "velocity_verlet") << '\n'
-      << label_value("dt", prec9(static_cast<float>(spec_.dt))) << '\n'
-      << label_value("thermostat", thermostat_ ? "berendsen" : "none");
+    m << "dynamics summary" << '\n'
+      << label_value("style", models_.integrator->style_name()) << '\n'
+      << label_value("dt", prec9(models_.integrator->dt())) << '\n'
+      << label_value("thermostat",
+                     models_.thermostat ? models_.thermostat->style_name() : "none");
     log("INFO", m.str());
   }
   {
@@ -1672,10 +1567,13 @@ void Engine::log_run_start() {
     const std::string distance =
         spec_.exclusion_distance > 0 ? std::to_string(spec_.exclusion_distance) : "none";
     std::ostringstream m;
-    m << "forcefield summary" << '\n' << label_value("pair", "lj") << '\n'
-      << label_value("bond", has_bond_style_ ? "harmonic" : "none") << '\n'
-      << label_value("angle", has_angle_style_ ? "harmonic" : "none") << '\n'
-      << label_value("dihedral", has_dihedral_style_ ? "harmonic" : "none") << '\n'
+    auto style = [](const std::unique_ptr<ListedModel>& m) {
+      return m ? m->style_name() : "none";
+    };
+    m << "forcefield summary" << '\n' << label_value("pair", models_.pair->style_name()) << '\n'
+      << label_value("bond", style(models_.bond)) << '\n'
+      << label_value("angle", style(models_.angle)) << '\n'
+      << label_value("dihedral", style(models_.dihedral)) << '\n'
       << label_value("bonded_exclusion_policy", policy) << '\n'
       << label_value("bonded_exclusion_distance", distance) << '\n'
       << label_value("bonded_excluded_pairs", excluded_pairs_);

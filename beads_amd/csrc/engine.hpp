@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
+#include "models.hpp"
 #include "output.hpp"
 
 namespace beads_amd {
@@ -76,11 +77,7 @@ HostTopology build_host_topology(const beads_spec_t& spec);
 // Shell validation of one spec (throws Error with the matching status).
 void validate_spec(const beads_spec_t& spec);
 
-// FP64 LJ coefficients {sigma^2, 24 eps, 4 eps, shift} for all ordered type
-// pairs, from the binary32 parameters.
-std::vector<LjCoeff> pack_lj_table(const beads_spec_t& spec);
-std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                   float cutoff, bool shift_energy);
+
 
 // Bond-graph exclusion pairs (tag_a < tag_b) within the given distance
 // (exclusions.cu:514-598).
@@ -194,9 +191,6 @@ class Engine {
   bool has_spill_ = false, spill_excl_any_ = false;
   DeviceBuffer<uint2> bond_, s_bond_;
   DeviceBuffer<uint4> angle_, s_angle_, dih_, s_dih_;
-  DeviceBuffer<LjCoeff> lj_;
-  DeviceBuffer<float2> bond_coeff_, angle_coeff_;
-  DeviceBuffer<float4> dih_coeff_;
   DeviceBuffer<double> partials_;
   DeviceBuffer<double> pair_acc_;  // half list: partner-side LJ sums [3 N]
 
@@ -205,11 +199,10 @@ class Engine {
   uint32_t bcap_ = 0, acap_ = 0, dcap_ = 0, ecap_ = 0;
   bool has_topology_ = false;
 
-  // Parameters as given (binary32) and their FP64 copies for the adjoint.
+  // Force field, integrator and thermostat, created from the spec's style
+  // names (models.hpp); they own their parameter tables.
+  ModelSet models_;
   int32_t n_types_ = 0, n_bond_types_ = 0;
-  bool lj_shift_ = false;
-  std::vector<float> lj_eps_, lj_sig_, bond_k_, bond_r0_;
-  DeviceBuffer<double> eps_d_, sig_d_, bond_k_d_, bond_r0_d_;
   DeviceBuffer<float4> grad_force4_;
   bool frame_mode_ = false;   // positions supplied per frame; never reordered
   bool frame_loaded_ = false;

@@ -244,6 +244,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
   const py::dict ps = require_dict(require_item(ff, "pair_style", "forcefield"), "forcefield.pair_style");
   const std::string pstyle = require_str(require_item(ps, "style", "pair_style"), "pair_style.style");
   if (pstyle != "lj") value_error("Unknown pair_style \"" + pstyle + "\".");
+  out.c.pair_style = "lj";  // the style whose parameter block this parser fills
   const Params pp = read_params(require_dict(require_item(ps, "params", "pair_style"), "pair_style.params"),
                                 "pair_style");
   require_param_keys(pp, {"cutoff"}, {"shift"}, "pair_style");
@@ -287,6 +288,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
   out.c.lj_sigma = out.lj_sig.data();
 
   if (!optional_style(ff, "bond_style", "harmonic").empty()) {
+    out.c.bond_style = "harmonic";
     const auto m = typed_coeffs(require_item(ff, "bond_coeffs", "forcefield"), "harmonic", "bond_coeffs");
     for (const auto& kv : m) {
       require_param_keys(kv.second, {"k", "r0"}, {}, "bond_coeff");
@@ -295,6 +297,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     }
   }
   if (!optional_style(ff, "angle_style", "harmonic").empty()) {
+    out.c.angle_style = "harmonic";
     const auto m = typed_coeffs(require_item(ff, "angle_coeffs", "forcefield"), "harmonic", "angle_coeffs");
     for (const auto& kv : m) {
       require_param_keys(kv.second, {"k", "theta0"}, {}, "angle_coeff");
@@ -305,6 +308,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     }
   }
   if (!optional_style(ff, "dihedral_style", "harmonic").empty()) {
+    out.c.dihedral_style = "harmonic";
     const auto m = typed_coeffs(require_item(ff, "dihedral_coeffs", "forcefield"), "harmonic",
                                 "dihedral_coeffs");
     for (const auto& kv : m) {
@@ -374,6 +378,7 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
                                "dynamics");
   require_param_keys(p, {"dt"}, {}, "Dynamics(\"velocity_verlet\")");
   out.c.dt = static_cast<float>(positive_param(p, "dt", "dynamics"));
+  out.c.integrator_style = "velocity_verlet";
   out.c.berendsen = 0;
   if (dyn.contains("thermostat") && !dyn["thermostat"].is_none()) {
     const py::dict th = require_dict(dyn["thermostat"], "dynamics.thermostat");
@@ -383,6 +388,7 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
                                   "thermostat");
     require_param_keys(tp, {"temperature", "tau"}, {}, "Thermostat(\"berendsen\")");
     out.c.berendsen = 1;
+    out.c.thermostat_style = "berendsen";
     out.c.berendsen_temperature =
         static_cast<float>(nonnegative_param(tp, "temperature", "Thermostat(\"berendsen\")"));
     out.c.berendsen_tau = static_cast<float>(positive_param(tp, "tau", "Thermostat(\"berendsen\")"));

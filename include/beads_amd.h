@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BEADS_AMD_ABI_VERSION 2
+#define BEADS_AMD_ABI_VERSION 3
 
 typedef enum beads_status {
   BEADS_OK = 0,
@@ -89,6 +89,20 @@ typedef struct beads_spec {
   const uint32_t* angles;        /* [n_angles][4]: tag_i, tag_j, tag_k, type */
   uint32_t n_dihedrals;
   const uint32_t* dihedrals;     /* [n_dihedrals][5]: tag_i..tag_l, type */
+
+  /* Style names (NULL: the default) select each model from its registry
+   * (pair_factory.cpp:30-47 and the bond/angle/dihedral factories): pair
+   * "lj" (default); bond, angle, dihedral "harmonic" (default when the
+   * kind's type count is positive, else none); integrator "velocity_verlet"
+   * (default); thermostat "berendsen" (default when berendsen is set, else
+   * none). An unregistered name is rejected with BEADS_E_VALUE. The
+   * parameter blocks below are those of the built-in styles. */
+  const char* pair_style;
+  const char* bond_style;
+  const char* angle_style;
+  const char* dihedral_style;
+  const char* integrator_style;
+  const char* thermostat_style;
 
   /* pair style "lj" */
   int32_t n_types;

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_build_info():
     lib = capi.library()
-    assert lib.beads_abi_version() == 2
+    assert lib.beads_abi_version() == 3
     info = capi.BuildInfo()
     assert lib.beads_build_info(ctypes.byref(info)) == 0
     assert info.real_dtype == b"float32" and info.device_arch == b"gfx950"
@@ -67,3 +67,23 @@ def test_engine_rejects_invalid_spec_without_device_work():
     assert status == 1  # BEADS_E_VALUE
     assert b"half shortest box length" in capi.library().beads_last_error()
     del keep
+
+
+def test_engine_rejects_unregistered_styles_before_any_device_call():
+    """Models come from per-kind registries keyed by style name
+    (pair_factory.cpp:30-47): an unknown name is BEADS_E_VALUE with the
+    reference's message, raised while the spec is validated (no device)."""
+    from beads_amd import workloads as W
+
+    w = W.polymer_melt(n_chains=2, beads_per_chain=10, seed=1)
+    lib = capi.library()
+    for field, kind in (("pair_style", "pair"), ("bond_style", "bond"), ("angle_style", "angle"),
+                        ("dihedral_style", "dihedral"), ("integrator_style", "integrator"),
+                        ("thermostat_style", "thermostat")):
+        spec, keep = capi.make_spec(w)
+        setattr(spec, field, b"nope")
+        handle = ctypes.c_void_p()
+        status = lib.beads_engine_create(ctypes.byref(spec), ctypes.byref(handle))
+        assert status == 1, field  # BEADS_E_VALUE
+        assert lib.beads_last_error().decode() == f'unsupported {kind} style "nope".'
+        assert not handle.value
