@@ -12,11 +12,13 @@ from typing import Any, Mapping
 
 import numpy as np
 
-REAL = np.dtype(np.float32)
-TYPE = np.dtype(np.int32)
-INDEX = np.dtype(np.uint32)
-IMAGE = np.dtype(np.int32)
-RUNSTEP = np.dtype(np.uint64)
+from ._build_config import dtype_for
+
+REAL = dtype_for("real_dtype")
+TYPE = dtype_for("type_dtype")
+INDEX = dtype_for("index_dtype")
+IMAGE = dtype_for("image_dtype")
+RUNSTEP = dtype_for("runstep_dtype")
 
 _NON_REAL_KINDS = frozenset("bOSUc")
 _NON_INT_KINDS = frozenset("bOSUfc")
@@ -115,8 +117,8 @@ def nonnegative_real(name: str, value: object) -> float:
 
 
 def freeze_params(context: str, params: Mapping[str, object], *, allow_bool: bool) -> dict[str, Any]:
-    """Scalar style parameters: bool (if allowed), int64, finite float32-representable
-    float, or str."""
+    """Scalar style parameters: bool (if allowed), int64, finite float representable
+    in the build's real dtype, or str."""
     out: dict[str, Any] = {}
     for key, value in params.items():
         if not isinstance(key, str) or not key:
@@ -141,9 +143,9 @@ def _scalar(context: str, key: str, value: object, allow_bool: bool) -> Any:
         if not np.isfinite(fv):
             raise ValueError(f"{context} {key} must be finite.")
         with np.errstate(over="ignore", under="ignore"):
-            narrowed = float(np.float32(fv))
+            narrowed = float(REAL.type(fv))
         if not np.isfinite(narrowed):
-            raise ValueError(f"{context} {key} must be representable as float32.")
+            raise ValueError(f"{context} {key} must be representable as {REAL.name}.")
         return fv
     if isinstance(value, str):
         return value

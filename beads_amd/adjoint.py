@@ -15,8 +15,9 @@ gradients. This module puts the engine's frame-mode entry points
   hand-written reverse sweep that recomputes each segment of ``checkpoint_every``
   steps from its stored (x, v) checkpoint.
 
-Forces are binary32 as in the dynamics path; the adjoint kernel evaluates in
-FP64 from the binary32 coordinates. Engine calls are enqueued on torch's
+Forces and positions are in the engine's real dtype as in the dynamics path
+(float32, or float64 with ``precision="double"``, the BEADS_REAL64 build); the
+adjoint kernel evaluates in FP64 from those coordinates. Engine calls are enqueued on torch's
 current stream (the engine's stream-ordered frame entry points), so tensors
 flow between torch and the engine without host waits; energies come back as
 device tensors. A neighbour-list overflow surfaces at ``FrameModel.check()``.
@@ -52,17 +53,19 @@ class FrameModel:
     caller-supplied positions. Positions are [n, 3] in the workload's slot
     order; the engine never reorders them."""
 
-    def __init__(self, workload: Workload, device: str | torch.device = "cuda"):
+    def __init__(self, workload: Workload, device: str | torch.device = "cuda",
+                 precision: str | None = None):
         if workload.angles.shape[0] or workload.dihedrals.shape[0]:
             raise ValueError("the adjoint covers LJ pairs and harmonic bonds; remove angle and "
                              "dihedral terms")
         self.workload = workload
         self.device = torch.device(device)
-        self.engine = Engine(workload)
+        self.engine = Engine(workload, precision)
+        self.real = torch.float64 if self.engine.precision == "double" else torch.float32
         self.n = workload.n
         self.n_types = workload.n_types
         self.n_bond_types = int(workload.bond_k.shape[0])
-        self.masses = torch.as_tensor(workload.masses, dtype=torch.float32,
+        self.masses = torch.as_tensor(workload.masses, dtype=self.real,
                                       device=self.device).reshape(-1, 1)
         self._params_key: bytes | None = None
         self._params_ident: tuple | None = None
@@ -88,7 +91,7 @@ class FrameModel:
         w = self.workload
 
         def t(a):
-            return torch.tensor(np.asarray(a, np.float32), dtype=torch.float32,
+            return torch.tensor(np.asarray(a), dtype=self.real,
                                 device=self.device, requires_grad=requires_grad)
 
         return Params(t(w.lj_epsilon), t(w.lj_sigma), t(w.bond_k), t(w.bond_r0))
@@ -100,7 +103,7 @@ class FrameModel:
                       for p in (eps, sigma, k, r0))
         if ident == self._params_ident:
             return
-        arrs = [np.ascontiguousarray(p.detach().to("cpu", torch.float32).numpy())
+        arrs = [np.ascontiguousarray(p.detach().to("cpu", self.real).numpy())
                 for p in (eps, sigma, k, r0)]
         key = b"".join(a.tobytes() for a in arrs)
         if key != self._params_key:
@@ -145,7 +148,7 @@ class FrameModel:
     def _backward_raw(self, g_forces: torch.Tensor, g_energy: float):
         self._on_current_stream()
         n_p = 2 * self.n_types * self.n_types + 2 * self.n_bond_types
-        gx = torch.empty((self.n, 3), dtype=torch.float32, device=self.device)
+        gx = torch.empty((self.n, 3), dtype=self.real, device=self.device)
         gp = torch.empty(max(n_p, 1), dtype=torch.float64, device=self.device)
         self.engine.frame_backward_async(g_forces.data_ptr(), g_energy, gx.data_ptr(),
                                          gp.data_ptr())
@@ -162,12 +165,13 @@ class FrameModel:
     def _positions(self, x: torch.Tensor) -> torch.Tensor:
         if x.shape != (self.n, 3):
             raise ValueError(f"positions must have shape ({self.n}, 3)")
-        return x.detach().to(self.device, torch.float32).contiguous()
+        return x.detach().to(self.device, self.real).contiguous()
 
     # ------------------------------------------------------------- public
 
     def forces(self, x: torch.Tensor, params: Params) -> tuple[torch.Tensor, torch.Tensor]:
-        """Forces [n, 3] (binary32) and total potential energy (FP64 scalar)."""
+        """Forces [n, 3] (the engine's real dtype) and total potential energy (FP64
+        scalar)."""
         return _FrameForces.apply(self, x, *params.tensors())
 
     def trajectory(self, x0: torch.Tensor, v0: torch.Tensor, params: Params, steps: int,
@@ -200,7 +204,7 @@ def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
         f, _, _ = model._forward_raw(model._positions(x))
         diff = f.double() - ft.to(model.device).double()
         total += (diff * diff).sum()
-        _, gp = model._backward_raw((2.0 * scale * diff).float().contiguous(), 0.0)
+        _, gp = model._backward_raw((2.0 * scale * diff).to(model.real).contiguous(), 0.0)
         g_sum += gp
     return total * scale, Params(*model._split(g_sum, (eps, sigma, k, r0)))
 
@@ -245,7 +249,7 @@ class _FrameForces(torch.autograd.Function):
         model._set_params(eps, sigma, k, r0)
         model._load_raw(pos, ctx.token)
         gF = (torch.zeros_like(pos) if g_forces is None
-              else g_forces.detach().to(model.device, torch.float32).contiguous())
+              else g_forces.detach().to(model.device, model.real).contiguous())
         gE = 0.0 if g_energy is None else float(g_energy)
         gx, gp = model._backward_raw(gF, gE)
         g_eps, g_sig, g_k, g_r0 = model._split(gp, (eps, sigma, k, r0))
@@ -270,7 +274,7 @@ class _VerletTrajectory(torch.autograd.Function):
         dt = float(model.workload.dt)
         a = (0.5 * dt) / model.masses
         x = model._positions(x0)
-        v = v0.detach().to(model.device, torch.float32).contiguous()
+        v = v0.detach().to(model.device, model.real).contiguous()
         checkpoints = [(x.clone(), v.clone())]
         f, _, _ = model._forward_raw(x)
         for s in range(steps):
@@ -296,10 +300,10 @@ class _VerletTrajectory(torch.autograd.Function):
         a = (0.5 * dt) / model.masses
         steps, every = ctx.steps, ctx.every
         dev = model.device
-        xbar = (torch.zeros((model.n, 3), dtype=torch.float32, device=dev) if gx_T is None
-                else gx_T.detach().to(dev, torch.float32).clone())
-        vbar = (torch.zeros((model.n, 3), dtype=torch.float32, device=dev) if gv_T is None
-                else gv_T.detach().to(dev, torch.float32).clone())
+        xbar = (torch.zeros((model.n, 3), dtype=model.real, device=dev) if gx_T is None
+                else gx_T.detach().to(dev, model.real).clone())
+        vbar = (torch.zeros((model.n, 3), dtype=model.real, device=dev) if gv_T is None
+                else gv_T.detach().to(dev, model.real).clone())
         carry = torch.zeros_like(xbar)
         n_p = 2 * model.n_types * model.n_types + 2 * model.n_bond_types
         pbar = torch.zeros(max(n_p, 1), dtype=torch.float64, device=dev)

@@ -4,7 +4,7 @@
 //   * dL/dtheta for the LJ epsilon/sigma of every type pair and the harmonic
 //     bond k/r0 of every bond type, accumulated in FP64, and
 //   * dL/dx = J^T g - gE F, J = dF/dx (the Hessian-vector product), in FP64
-//     per atom, rounded to binary32 on store.
+//     per atom, rounded to real_t on store.
 // The same pass serves force matching (g = dL/dF) and the reverse sweep of
 // velocity-Verlet trajectory backprop (g = adjoint of the force).
 //
@@ -33,30 +33,30 @@ namespace {
 
 using namespace dev;
 
-// Staging of dL/dF (float4 per atom) for frame_backward's allocation-free
+// Staging of dL/dF (real4 per atom) for frame_backward's allocation-free
 // variant: one copy per GPU, held until the process ends. Host code must use
 // the address from hipGetSymbolAddress (scratch_address below), never this
 // name, which in host code is only a stand-in for the device symbol.
-static __device__ __align__(256) float4 scratch[kNoAllocMaxAtoms];
+static __device__ __align__(256) real4 scratch[kNoAllocMaxAtoms];
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mutex;
-float4* g_scratch_ptr[kMaxDevices] = {};
+real4* g_scratch_ptr[kMaxDevices] = {};
 
 template <bool CHECK>
 __global__ __launch_bounds__(kBlock) void frame_load_kernel(FrameLoadParams p) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= p.n) return;
-  float4 x = p.pos[i];
+  real4 x = p.pos[i];
   x.x = p.xyz[3 * i];
   x.y = p.xyz[3 * i + 1];
   x.z = p.xyz[3 * i + 2];
-  float c[3];
+  real_t c[3];
   {
 #pragma clang fp contract(off)
-    c[0] = floorf((x.x - p.box.lo[0]) * p.box.invL[0]);
-    c[1] = floorf((x.y - p.box.lo[1]) * p.box.invL[1]);
-    c[2] = floorf((x.z - p.box.lo[2]) * p.box.invL[2]);
+    c[0] = floor((x.x - p.box.lo[0]) * p.box.invL[0]);
+    c[1] = floor((x.y - p.box.lo[1]) * p.box.invL[1]);
+    c[2] = floor((x.z - p.box.lo[2]) * p.box.invL[2]);
     x.x -= c[0] * p.box.L[0];
     x.y -= c[1] * p.box.L[1];
     x.z -= c[2] * p.box.L[2];
@@ -67,22 +67,22 @@ __global__ __launch_bounds__(kBlock) void frame_load_kernel(FrameLoadParams p) {
   p.image[i] = im;
   if (CHECK) {
 #pragma clang fp contract(off)
-    const float4 r = p.ref_pos[i];
+    const real4 r = p.ref_pos[i];
     const int4 ri = p.ref_image[i];
-    const float dx = (x.x - r.x) + static_cast<float>(im.x - ri.x) * p.box.L[0];
-    const float dy = (x.y - r.y) + static_cast<float>(im.y - ri.y) * p.box.L[1];
-    const float dz = (x.z - r.z) + static_cast<float>(im.z - ri.z) * p.box.L[2];
+    const real_t dx = (x.x - r.x) + static_cast<real_t>(im.x - ri.x) * p.box.L[0];
+    const real_t dy = (x.y - r.y) + static_cast<real_t>(im.y - ri.y) * p.box.L[1];
+    const real_t dz = (x.z - r.z) + static_cast<real_t>(im.z - ri.z) * p.box.L[2];
     if (dx * dx + dy * dy + dz * dz > p.threshold_sq) {
       __hip_atomic_store(&p.ctrl->rebuild, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void frame_store_kernel(uint32_t n, const float4* force,
-                                                             float* xyz) {
+__global__ __launch_bounds__(kBlock) void frame_store_kernel(uint32_t n, const real4* force,
+                                                             real_t* xyz) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float4 f = force[i];
+  const real4 f = force[i];
   xyz[3 * i] = f.x;
   xyz[3 * i + 1] = f.y;
   xyz[3 * i + 2] = f.z;
@@ -93,11 +93,11 @@ __global__ __launch_bounds__(kBlock) void fill_f64_kernel(double* out, uint32_t 
   if (k < count) out[k] = 0.0;
 }
 
-__global__ __launch_bounds__(kBlock) void pack_xyz_kernel(uint32_t n, const float* xyz,
-                                                          float4* out) {
+__global__ __launch_bounds__(kBlock) void pack_xyz_kernel(uint32_t n, const real_t* xyz,
+                                                          real4* out) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+  out[i] = make_real4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0);
 }
 
 // Lanes per atom and list slots per lane per round of the backward pass (the
@@ -123,12 +123,12 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kAdjAtoms + threadIdx.x / kAdjTeam;
   const double gE = p.grad_energy;
   if (i < n) {  // uniform over a team
-    const float4 xi = p.pos[i];
-    const int ti = __float_as_int(xi.w);
-    const float4 gi = p.g[i];
+    const real4 xi = p.pos[i];
+    const int ti = static_cast<int>(id_of(xi.w));
+    const real4 gi = p.g[i];
     const double gix = gi.x, giy = gi.y, giz = gi.z;
-    const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
-    const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+    const real_t L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+    const real_t h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     const double inv_rc2 = rcp_f64(p.rc2_f64);
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
         const uint32_t s = base + u * kAdjTeam + t;
         jj[u] = s < cnt ? p.nbr[nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1))] : i;
       }
-      float4 xj[kAdjSlots], gj[kAdjSlots];
+      real4 xj[kAdjSlots], gj[kAdjSlots];
 #pragma unroll
       for (int u = 0; u < kAdjSlots; ++u) {
         xj[u] = p.pos[jj[u]];
@@ -151,11 +151,11 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
       }
 #pragma unroll
       for (int u = 0; u < kAdjSlots; ++u) {
-        const float dx = mic_diff(xi.x, xj[u].x, L0, h0);
-        const float dy = mic_diff(xi.y, xj[u].y, L1, h1);
-        const float dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
+        const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
+        const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
         if (base + u * kAdjTeam + t >= cnt || r2_strict(dx, dy, dz) >= p.rc2) continue;
-        const int tj = __float_as_int(xj[u].w);
+        const int tj = static_cast<int>(id_of(xj[u].w));
         const double eps = p.eps[ti * T + tj];
         const double sig = p.sigma[ti * T + tj];
         const double inv_sig = rcp_f64(sig);
@@ -225,20 +225,20 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
           spill_off = p.spill[si].off[0];
         }
       }
-      const float3 x3 = f3(xi);
+      const real3 x3 = f3(xi);
       for (uint32_t k = t; k < nbond; k += kAdjTeam) {
         const uint2 r = k < nbond_inline ? p.bond[static_cast<size_t>(k) * n + i]
                                          : p.spill_bond[spill_off + (k - nbond_inline)];
         const uint32_t bt = r.y & 0x7fffffffu;
         const double kk = p.bond_k[bt], r0 = p.bond_r0[bt];
-        const float3 d = mic3(x3, f3(p.pos[r.x]), p.box);
+        const real3 d = mic3(x3, f3(p.pos[r.x]), p.box);
         const double ddx = d.x, ddy = d.y, ddz = d.z;
         const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
         if (rr == 0.0) continue;
         const double inv_r = rcp_f64(rr);
         const double phi = -kk * (rr - r0) * inv_r;
         const double dphi_r = -kk * r0 * inv_r * inv_r * inv_r;  // phi' / r
-        const float4 gp = p.g[r.x];
+        const real4 gp = p.g[r.x];
         const double ex = gix - gp.x, ey = giy - gp.y, ez = giz - gp.z;
         const double dd = ddx * ex + ddy * ey + ddz * ez;
         gx += phi * ex + dphi_r * ddx * dd;
@@ -261,9 +261,9 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
     fy = team_sum<kAdjTeam>(fy);
     fz = team_sum<kAdjTeam>(fz);
     if (t == 0) {
-      p.grad_pos[3 * i] = static_cast<float>(gx - gE * fx);
-      p.grad_pos[3 * i + 1] = static_cast<float>(gy - gE * fy);
-      p.grad_pos[3 * i + 2] = static_cast<float>(gz - gE * fz);
+      p.grad_pos[3 * i] = static_cast<real_t>(gx - gE * fx);
+      p.grad_pos[3 * i + 1] = static_cast<real_t>(gy - gE * fy);
+      p.grad_pos[3 * i + 2] = static_cast<real_t>(gz - gE * fz);
     }
   }
   __syncthreads();
@@ -282,7 +282,7 @@ void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream)
   }
 }
 
-void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream) {
+void launch_frame_store(uint32_t n, const real4* force, real_t* xyz, hipStream_t stream) {
   frame_store_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, force, xyz);
 }
 
@@ -293,7 +293,7 @@ void launch_fill_f64(double* out, uint32_t count, hipStream_t stream) {
 
 // Device address of this GPU's scratch array, looked up once per device with
 // that device current (each GPU holds its own copy at its own address).
-float4* scratch_address() {
+real4* scratch_address() {
   int device = 0;
   BEADS_HIP(hipGetDevice(&device));
   require(device >= 0 && device < kMaxDevices, "device index out of range for the scratch table",
@@ -304,12 +304,12 @@ float4* scratch_address() {
     BEADS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
     require(ptr != nullptr, "hipGetSymbolAddress returned no address for the scratch array",
             BEADS_E_DEVICE);
-    g_scratch_ptr[device] = static_cast<float4*>(ptr);
+    g_scratch_ptr[device] = static_cast<real4*>(ptr);
   }
   return g_scratch_ptr[device];
 }
 
-void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream) {
+void launch_pack_xyz(uint32_t n, const real_t* xyz, real4* out, hipStream_t stream) {
   pack_xyz_kernel<<<grid_for(n), kBlock, 0, stream>>>(n, xyz, out);
 }
 

@@ -11,17 +11,18 @@ namespace beads_amd {
 constexpr int kMaxBondTypesAdjoint = 64;
 
 // Largest system frame_backward's allocation-free variant serves: its dL/dF
-// staging lives in a static device array of this many float4 (16 MiB per GPU).
+// staging lives in a static device array of this many real4 (16 MiB per GPU,
+// 32 MiB in the BEADS_REAL64 build).
 constexpr uint32_t kNoAllocMaxAtoms = 1u << 20;
 
 struct AdjointParams {
   uint32_t n;
-  const float4* pos;        // positions, .w = type bits
-  const float4* g;          // dL/dF per atom
+  const real4* pos;        // positions, .w = type bits
+  const real4* g;          // dL/dF per atom
   const uint32_t* nbr_count;
   const uint32_t* nbr;      // chunked full list
   BoxView box;
-  float rc2;                // binary32 cutoff decision, as in the forward
+  real_t rc2;               // cutoff decision in real_t, as in the forward
   double rc2_f64;           // FP64 rc^2 for the energy-shift derivative
   int32_t lj_shift;
   int32_t n_types;
@@ -37,7 +38,7 @@ struct AdjointParams {
   const double* bond_k;
   const double* bond_r0;
   double grad_energy;       // dL/dE for the total potential energy
-  float* grad_pos;          // [n][3]
+  real_t* grad_pos;          // [n][3]
   double* grad_params;      // eps[T*T], sigma[T*T], k[nb], r0[nb]; zeroed by the caller
 };
 
@@ -46,24 +47,24 @@ struct AdjointParams {
 // position at the last build raises ctrl->rebuild (integrate_kernel's test).
 struct FrameLoadParams {
   uint32_t n;
-  const float* xyz;
-  float4* pos;
+  const real_t* xyz;
+  real4* pos;
   int4* image;
-  const float4* ref_pos;
+  const real4* ref_pos;
   const int4* ref_image;
   BoxView box;
-  float threshold_sq;
+  real_t threshold_sq;
   Ctrl* ctrl;
 };
 
 void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream);
-void launch_frame_store(uint32_t n, const float4* force, float* xyz, hipStream_t stream);
-void launch_pack_xyz(uint32_t n, const float* xyz, float4* out, hipStream_t stream);
+void launch_frame_store(uint32_t n, const real4* force, real_t* xyz, hipStream_t stream);
+void launch_pack_xyz(uint32_t n, const real_t* xyz, real4* out, hipStream_t stream);
 // out[0, count) = 0 with a kernel (no memset node when a caller captures the stream).
 void launch_fill_f64(double* out, uint32_t count, hipStream_t stream);
-// This GPU's static staging array (kNoAllocMaxAtoms float4), address resolved
+// This GPU's static staging array (kNoAllocMaxAtoms real4), address resolved
 // once per device; allocates nothing.
-float4* scratch_address();
+real4* scratch_address();
 void launch_fm_backward(const AdjointParams& p, hipStream_t stream);
 
 }  // namespace beads_amd

@@ -59,7 +59,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     d.grid.path = p.path;
     d.rs2 = p.search_cutoff_sq;
     d.rc2 = p.force_cutoff * p.force_cutoff;
-    const float half = 0.5f * p.cutoff_buffer;
+    const real_t half = real_t(0.5) * p.cutoff_buffer;
     d.thr2 = half * half;
     d.dt = s.dt;
     d.cutoff_buffer = p.cutoff_buffer;
@@ -93,13 +93,11 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     const beads_spec_t& s = specs[order_[k]];
     const uint32_t a0 = desc_host_[k].atom0;
     for (uint32_t i = 0; i < s.n_particles; ++i) {
-      int32_t t = s.types[i] - 1;
-      float tw;
-      std::memcpy(&tw, &t, sizeof(float));
-      init_pos_[a0 + i] = make_float4(s.positions[3 * i], s.positions[3 * i + 1],
-                                      s.positions[3 * i + 2], tw);
-      init_vel_[a0 + i] = make_float4(s.velocities[3 * i], s.velocities[3 * i + 1],
-                                      s.velocities[3 * i + 2], s.masses[i]);
+      const real_t tw = real_of_id(static_cast<uint32_t>(s.types[i] - 1));
+      init_pos_[a0 + i] = make_real4(s.positions[3 * i], s.positions[3 * i + 1],
+                                     s.positions[3 * i + 2], tw);
+      init_vel_[a0 + i] = make_real4(s.velocities[3 * i], s.velocities[3 * i + 1],
+                                     s.velocities[3 * i + 2], s.masses[i]);
       init_img_[a0 + i] = make_int4(s.images[3 * i], s.images[3 * i + 1], s.images[3 * i + 2], 0);
       init_tag_[a0 + i] = s.tags[i];
       init_cell_atoms_[a0 + i] = i;
@@ -277,11 +275,11 @@ void Batch::read_state(uint32_t r, const beads_state_out_t& o) {
   require(r < count_, "replica index out of range.", BEADS_E_INDEX);
   const ReplicaDesc& d = desc_host_[position_of_[r]];
   const size_t n = d.n, a0 = d.atom0;
-  std::vector<float4> pos(n), vel(n), force(n);
+  std::vector<real4> pos(n), vel(n), force(n);
   std::vector<int4> img(n);
-  BEADS_HIP(hipMemcpyAsync(pos.data(), pos_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-  BEADS_HIP(hipMemcpyAsync(vel.data(), vel_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-  BEADS_HIP(hipMemcpyAsync(force.data(), force_.data() + a0, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(pos.data(), pos_.data() + a0, n * sizeof(real4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(vel.data(), vel_.data() + a0, n * sizeof(real4), hipMemcpyDeviceToHost, stream_));
+  BEADS_HIP(hipMemcpyAsync(force.data(), force_.data() + a0, n * sizeof(real4), hipMemcpyDeviceToHost, stream_));
   BEADS_HIP(hipMemcpyAsync(img.data(), image_.data() + a0, n * sizeof(int4), hipMemcpyDeviceToHost, stream_));
   if (o.tags) {
     BEADS_HIP(hipMemcpyAsync(o.tags, tag_.data() + a0, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
@@ -293,11 +291,7 @@ void Batch::read_state(uint32_t r, const beads_state_out_t& o) {
       o.positions[3 * i + 1] = pos[i].y;
       o.positions[3 * i + 2] = pos[i].z;
     }
-    if (o.types) {
-      int32_t t;
-      std::memcpy(&t, &pos[i].w, sizeof(int32_t));
-      o.types[i] = t + 1;
-    }
+    if (o.types) o.types[i] = static_cast<int32_t>(id_of(pos[i].w)) + 1;
     if (o.velocities) {
       o.velocities[3 * i] = vel[i].x;
       o.velocities[3 * i + 1] = vel[i].y;

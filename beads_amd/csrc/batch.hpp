@@ -42,7 +42,7 @@ class Batch {
   int block_ = 512;
   std::vector<uint32_t> order_, position_of_;
   std::vector<ReplicaDesc> desc_host_;
-  std::vector<float4> init_pos_, init_vel_;
+  std::vector<real4> init_pos_, init_vel_;
   std::vector<int4> init_img_;
   std::vector<uint32_t> init_tag_, init_cell_atoms_;
   std::vector<CachedGraph> graphs_;
@@ -51,7 +51,7 @@ class Batch {
   DeviceBuffer<ReplicaDesc> desc_;
   DeviceBuffer<ReplicaState> state_;
   DeviceBuffer<LjCoeff> lj_;
-  DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
+  DeviceBuffer<real4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
   DeviceBuffer<int4> image_, ref_image_, s_image_;
   DeviceBuffer<uint32_t> tag_, s_tag_, cell_of_, s_cell_of_, cell_atoms_, nbr_count_, nbr_;
   DeviceBuffer<uint32_t> cell_count_, cell_start_, cell_cursor_;

@@ -78,8 +78,9 @@ int beads_abi_version(void) { return BEADS_AMD_ABI_VERSION; }
 beads_status_t beads_build_info(beads_build_info_t* out) {
   return guarded([&] {
     beads_amd::require(out != nullptr, "build_info output is null");
-    out->real_precision = "single";
-    out->real_dtype = "float32";
+    // module.cpp:19-21: derived from real_t itself.
+    out->real_precision = beads_amd::kRealPrecision;
+    out->real_dtype = beads_amd::kRealDtype;
     out->type_dtype = "int32";
     out->index_dtype = "uint32";
     out->image_dtype = "int32";
@@ -156,15 +157,15 @@ beads_status_t beads_engine_profile_advance(beads_engine_t* e, uint64_t n, int32
   });
 }
 
-beads_status_t beads_engine_set_lj_params(beads_engine_t* e, const float* epsilon,
-                                          const float* sigma) {
+beads_status_t beads_engine_set_lj_params(beads_engine_t* e, const beads_real_t* epsilon,
+                                          const beads_real_t* sigma) {
   return guarded([&] {
     beads_amd::require(epsilon && sigma, "lj parameter arrays are null", BEADS_E_TYPE);
     engine_of(e).set_lj_params(epsilon, sigma);
   });
 }
 
-beads_status_t beads_engine_set_bond_params(beads_engine_t* e, const float* k, const float* r0) {
+beads_status_t beads_engine_set_bond_params(beads_engine_t* e, const beads_real_t* k, const beads_real_t* r0) {
   return guarded([&] {
     beads_amd::Engine& eng = engine_of(e);
     beads_amd::require(eng.n_bond_types() == 0 || (k && r0), "bond parameter arrays are null",
@@ -173,7 +174,7 @@ beads_status_t beads_engine_set_bond_params(beads_engine_t* e, const float* k, c
   });
 }
 
-beads_status_t beads_engine_frame_forces(beads_engine_t* e, const float* positions, float* forces,
+beads_status_t beads_engine_frame_forces(beads_engine_t* e, const beads_real_t* positions, beads_real_t* forces,
                                          double* energies, int32_t reuse_list) {
   return guarded([&] {
     beads_amd::require(positions && forces, "frame arrays are null", BEADS_E_TYPE);
@@ -181,7 +182,7 @@ beads_status_t beads_engine_frame_forces(beads_engine_t* e, const float* positio
   });
 }
 
-beads_status_t beads_engine_frame_load(beads_engine_t* e, const float* positions,
+beads_status_t beads_engine_frame_load(beads_engine_t* e, const beads_real_t* positions,
                                        int32_t reuse_list) {
   return guarded([&] {
     beads_amd::require(positions != nullptr, "frame positions are null", BEADS_E_TYPE);
@@ -189,8 +190,8 @@ beads_status_t beads_engine_frame_load(beads_engine_t* e, const float* positions
   });
 }
 
-beads_status_t beads_engine_frame_backward(beads_engine_t* e, const float* grad_forces,
-                                           double grad_energy, float* grad_positions,
+beads_status_t beads_engine_frame_backward(beads_engine_t* e, const beads_real_t* grad_forces,
+                                           double grad_energy, beads_real_t* grad_positions,
                                            double* grad_params) {
   return guarded([&] {
     beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",
@@ -209,8 +210,8 @@ beads_status_t beads_engine_set_stream(beads_engine_t* e, void* stream) {
   return guarded([&] { engine_of(e).set_stream(static_cast<hipStream_t>(stream)); });
 }
 
-beads_status_t beads_engine_frame_forces_async(beads_engine_t* e, const float* positions,
-                                               float* forces, double* energies,
+beads_status_t beads_engine_frame_forces_async(beads_engine_t* e, const beads_real_t* positions,
+                                               beads_real_t* forces, double* energies,
                                                int32_t reuse_list) {
   return guarded([&] {
     beads_amd::require(positions && forces, "frame arrays are null", BEADS_E_TYPE);
@@ -218,7 +219,7 @@ beads_status_t beads_engine_frame_forces_async(beads_engine_t* e, const float* p
   });
 }
 
-beads_status_t beads_engine_frame_load_async(beads_engine_t* e, const float* positions,
+beads_status_t beads_engine_frame_load_async(beads_engine_t* e, const beads_real_t* positions,
                                              int32_t reuse_list) {
   return guarded([&] {
     beads_amd::require(positions != nullptr, "frame positions are null", BEADS_E_TYPE);
@@ -226,8 +227,8 @@ beads_status_t beads_engine_frame_load_async(beads_engine_t* e, const float* pos
   });
 }
 
-beads_status_t beads_engine_frame_backward_async(beads_engine_t* e, const float* grad_forces,
-                                                 double grad_energy, float* grad_positions,
+beads_status_t beads_engine_frame_backward_async(beads_engine_t* e, const beads_real_t* grad_forces,
+                                                 double grad_energy, beads_real_t* grad_positions,
                                                  double* grad_params) {
   return guarded([&] {
     beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",
@@ -237,8 +238,8 @@ beads_status_t beads_engine_frame_backward_async(beads_engine_t* e, const float*
   });
 }
 
-beads_status_t beads_engine_frame_backward_noalloc(beads_engine_t* e, const float* grad_forces,
-                                                   double grad_energy, float* grad_positions,
+beads_status_t beads_engine_frame_backward_noalloc(beads_engine_t* e, const beads_real_t* grad_forces,
+                                                   double grad_energy, beads_real_t* grad_positions,
                                                    double* grad_params) {
   return guarded([&] {
     beads_amd::require(grad_forces && grad_positions && grad_params, "adjoint arrays are null",

@@ -12,10 +12,10 @@
 #include <utility>
 
 #include "beads_amd.h"
+#include "real.hpp"
 
 namespace beads_amd {
 
-using real_t = float;      // types.hpp:12-16 (single precision build)
 using type_id_t = int32_t;
 using index_t = uint32_t;
 using image_t = int32_t;

@@ -1,9 +1,10 @@
 // Parameter blocks shared by the host engine and the gfx950 kernels.
 //
-// Device layout (slot order, N atoms, all 16-byte vectors coalesced):
-//   pos    float4[N]  x, y, z, w = bit pattern of the 0-based type id
-//   vel    float4[N]  vx, vy, vz, w = mass
-//   force  float4[N]  fx, fy, fz, w unused
+// Device layout (slot order, N atoms, all records coalesced vectors):
+//   pos    real4[N]   x, y, z, w = bits of the 0-based type id (id_of)
+//   vel    real4[N]   vx, vy, vz, w = mass
+//   force  real4[N]   fx, fy, fz, w unused
+//   (real4 is float4, 16 bytes; double4, 32 bytes, in the BEADS_REAL64 build)
 //   image  int4[N]    ix, iy, iz, unused
 //   nbr_count u32[N], nbr u32[ceil(max_neighbors / 16) * 16 * N] in 64-byte
 //     chunks: entries 16c..16c+15 of atom i at ((c * N) + i) * 16 (the
@@ -20,6 +21,8 @@
 #pragma once
 
 #include <cstdint>
+
+#include "real.hpp"
 
 namespace beads_amd {
 
@@ -44,10 +47,10 @@ struct LjCoeff {
 };
 
 struct BoxView {
-  float lo[3];
-  float L[3];
-  float hL[3];
-  float invL[3];
+  real_t lo[3];
+  real_t L[3];
+  real_t hL[3];
+  real_t invL[3];
 };
 
 // Members of a cell are ordered by (x, slot). Each cell's x extent is cut
@@ -61,7 +64,7 @@ constexpr uint32_t kXBinsMaxCount = 65535;
 
 struct CellGridView {
   uint32_t nx, ny, nz, n_cells;
-  float inv_cell[3];
+  real_t inv_cell[3];
   int32_t path;  // 0 direct, 1 regular, 2 small dimension
 };
 
@@ -110,9 +113,9 @@ struct TopologyView {
   const uint4* dih;
   const uint32_t* excl;
   uint32_t bcap, acap, dcap, ecap;
-  const float2* bond_coeff;   // k, r0
-  const float2* angle_coeff;  // k, theta0 (rad)
-  const float4* dih_coeff;    // k, d, n, unused
+  const real2* bond_coeff;   // k, r0
+  const real2* angle_coeff;  // k, theta0 (rad)
+  const real4* dih_coeff;    // k, d, n, unused
   int32_t has_listed;
   // null when no particle spills
   const uint32_t* spill_index;
@@ -124,17 +127,17 @@ struct TopologyView {
 
 struct ForceParams {
   uint32_t n;
-  const float4* pos;
-  float4* vel;
-  float4* force;
+  const real4* pos;
+  real4* vel;
+  real4* force;
   const uint32_t* nbr_count;
   const uint32_t* nbr;
   BoxView box;
-  float rc2;
+  real_t rc2;
   int32_t n_types;
   const LjCoeff* lj;  // [T*T]
   TopologyView topo;
-  float dt;
+  real_t dt;
   int32_t berendsen;
   double target_thermal;  // kB * T0
   double dt_over_tau;
@@ -149,32 +152,32 @@ struct ForceParams {
 
 struct IntegrateParams {
   uint32_t n;
-  float4* pos;
-  float4* vel;
-  const float4* force;
+  real4* pos;
+  real4* vel;
+  const real4* force;
   int4* image;
-  const float4* ref_pos;
+  const real4* ref_pos;
   const int4* ref_image;
   BoxView box;
-  float dt;
-  float threshold_sq;
+  real_t dt;
+  real_t threshold_sq;
   Ctrl* ctrl;
 };
 
 struct BuildParams {
   uint32_t n;
-  const float4* pos;
+  const real4* pos;
   const int4* image;
-  float4* ref_pos;
+  real4* ref_pos;
   int4* ref_image;
   BoxView box;
   CellGridView grid;
-  float rs2;
+  real_t rs2;
   const uint32_t* cell_of;
   const uint32_t* cell_start;
   const uint32_t* cell_count;
   const uint32_t* cell_atoms;
-  const float4* cell_pos;  // cell_atoms' positions, .w = the slot's bits
+  const real4* cell_pos;  // cell_atoms' positions, .w = the slot's bits
   const uint4* cell_xbins; // per cell: x-bin prefix counts (cell_sort_kernel)
   const uint32_t* topo_count;
   const uint32_t* excl;
@@ -192,8 +195,8 @@ struct BuildParams {
 struct ReorderParams {
   uint32_t n;
   // primary arrays
-  float4* pos;
-  float4* vel;
+  real4* pos;
+  real4* vel;
   int4* image;
   uint32_t* tag;
   uint32_t* molid;
@@ -204,8 +207,8 @@ struct ReorderParams {
   uint4* dih;
   uint32_t* excl;
   // scratch
-  float4* s_pos;
-  float4* s_vel;
+  real4* s_pos;
+  real4* s_vel;
   int4* s_image;
   uint32_t* s_tag;
   uint32_t* s_molid;
@@ -224,7 +227,7 @@ struct ReorderParams {
   uint4* spill_dih;
   uint32_t* spill_excl;
   uint32_t* cell_atoms;
-  float4* cell_pos;
+  real4* cell_pos;
   uint32_t* new_of_old;
   Ctrl* ctrl;
 };

@@ -1,6 +1,7 @@
 // Device helpers shared by the per-step kernels (kernels.hip) and the batched
-// replica kernel (replicas.hip): minimum image, binary32 decision arithmetic,
-// cell indexing, wavefront/workgroup reductions, listed-term algebra.
+// replica kernel (replicas.hip): minimum image, decision arithmetic in the
+// storage precision real_t, cell indexing, wavefront/workgroup reductions,
+// listed-term algebra.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,8 +13,6 @@
 namespace beads_amd {
 namespace dev {
 
-constexpr float kPiF = 3.14159265358979323846f;
-
 // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
 // contiguous logical range [x * nb / 8, (x + 1) * nb / 8).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
@@ -23,7 +22,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 }
 
 // box_geometry_view.cuh:50-62 (minimum image of an already-formed difference).
-__device__ __forceinline__ float mic(float d, float L, float hL) {
+__device__ __forceinline__ real_t mic(real_t d, real_t L, real_t hL) {
   return d > hL ? d - L : (d < -hL ? d + L : d);
 }
 
@@ -32,25 +31,26 @@ __device__ __forceinline__ float mic(float d, float L, float hL) {
 // exact by Sterbenz, so only the final small difference rounds. mic(xi - xj)
 // instead rounds at the scale of L before folding back (about 1e-6 nm at
 // L ~ 50 nm), which the r^-13 force amplifies past 1e-5.
-__device__ __forceinline__ float mic_diff(float xi, float xj, float L, float hL) {
-  const float d = xi - xj;
+__device__ __forceinline__ real_t mic_diff(real_t xi, real_t xj, real_t L, real_t hL) {
+  const real_t d = xi - xj;
   return d > hL ? (xi - L) - xj : (d < -hL ? xi - (xj - L) : d);
 }
 
 // mic_diff with both folded forms evaluated and one selected (no branches):
 // xi_m = xi - L is the caller's, hoisted out of a loop over partners.
-__device__ __forceinline__ float mic_diff_sel(float xi, float xi_m, float xj, float L, float hL) {
-  const float d = xi - xj;
-  const float up = xi_m - xj;
-  const float dn = xi - (xj - L);
-  float r = __builtin_unpredictable(d > hL) ? up : d;
+__device__ __forceinline__ real_t mic_diff_sel(real_t xi, real_t xi_m, real_t xj, real_t L,
+                                              real_t hL) {
+  const real_t d = xi - xj;
+  const real_t up = xi_m - xj;
+  const real_t dn = xi - (xj - L);
+  real_t r = __builtin_unpredictable(d > hL) ? up : d;
   r = __builtin_unpredictable(d < -hL) ? dn : r;
   return r;
 }
 
 // Squared distance exactly as written in the reference source (no FMA), so the
-// r^2 <= r_s^2 and r^2 < r_c^2 decisions match a binary32 host evaluation.
-__device__ __forceinline__ float r2_strict(float dx, float dy, float dz) {
+// r^2 <= r_s^2 and r^2 < r_c^2 decisions match a host evaluation in real_t.
+__device__ __forceinline__ real_t r2_strict(real_t dx, real_t dy, real_t dz) {
 #pragma clang fp contract(off)
   return dx * dx + dy * dy + dz * dz;
 }
@@ -132,14 +132,14 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return v;
 }
 
-// cell_list_build.cu:110-123: binary32 cell index, clamped.
-__device__ __forceinline__ uint32_t axis_cell(float x, float lo, float inv, uint32_t count) {
-  const float s = (x - lo) * inv;
-  uint32_t i = s <= 0.f ? 0u : static_cast<uint32_t>(s);
+// cell_list_build.cu:110-123: cell index in real_t, clamped.
+__device__ __forceinline__ uint32_t axis_cell(real_t x, real_t lo, real_t inv, uint32_t count) {
+  const real_t s = (x - lo) * inv;
+  uint32_t i = s <= real_t(0) ? 0u : static_cast<uint32_t>(s);
   return i >= count ? count - 1 : i;
 }
 
-__device__ __forceinline__ uint32_t cell_of_position(float4 x, const BoxView& box,
+__device__ __forceinline__ uint32_t cell_of_position(real4 x, const BoxView& box,
                                                      const CellGridView& g) {
   const uint32_t ix = axis_cell(x.x, box.lo[0], g.inv_cell[0], g.nx);
   const uint32_t iy = axis_cell(x.y, box.lo[1], g.inv_cell[1], g.ny);
@@ -198,11 +198,11 @@ __device__ __forceinline__ CellNeighborhood neighborhood(uint32_t c, const CellG
   return h;
 }
 
-// LJ pair value in FP64 from an exact binary32 displacement (FP64 VALU runs at
-// the non-packed FP32 rate on CDNA4; binary32 would amplify the rounding of r^2
-// six-fold through (sigma^2/r^2)^6). Returns the force scale s with
-// f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift.
-__device__ __forceinline__ double lj_pair(float dx, float dy, float dz, const LjCoeff& c,
+// LJ pair value in FP64 from the displacement in real_t (exact in binary32
+// too: FP64 VALU runs at the non-packed FP32 rate on CDNA4, and binary32 would
+// amplify the rounding of r^2 six-fold through (sigma^2/r^2)^6). Returns the
+// force scale s with f_i = s * d; energy gets 4 eps (sr12 - sr6) - shift.
+__device__ __forceinline__ double lj_pair(real_t dx, real_t dy, real_t dz, const LjCoeff& c,
                                           double& rr, double& energy) {
   const double ddx = dx, ddy = dy, ddz = dz;
   rr = ddx * ddx + ddy * ddy + ddz * ddz;
@@ -216,38 +216,25 @@ __device__ __forceinline__ double lj_pair(float dx, float dy, float dz, const Lj
 
 // ------------------------------------------------------- listed terms ------
 
-__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
-__device__ __forceinline__ float3 sub3(float3 a, float3 b) {
-  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
-}
-__device__ __forceinline__ float3 add3(float3 a, float3 b) {
-  return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
-}
-__device__ __forceinline__ float3 scale3(float3 a, float s) {
-  return make_float3(a.x * s, a.y * s, a.z * s);
-}
-__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
-  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
+__device__ __forceinline__ real3 f3(real4 v) { return make_real3(v.x, v.y, v.z); }
 
 // Minimum-image displacement p - q (wrapped coordinates), see mic_diff.
-__device__ __forceinline__ float3 mic3(float3 p, float3 q, const BoxView& b) {
-  return make_float3(mic_diff(p.x, q.x, b.L[0], b.hL[0]), mic_diff(p.y, q.y, b.L[1], b.hL[1]),
-                     mic_diff(p.z, q.z, b.L[2], b.hL[2]));
+__device__ __forceinline__ real3 mic3(real3 p, real3 q, const BoxView& b) {
+  return make_real3(mic_diff(p.x, q.x, b.L[0], b.hL[0]), mic_diff(p.y, q.y, b.L[1], b.hL[1]),
+                    mic_diff(p.z, q.z, b.L[2], b.hL[2]));
 }
 
 // FP64 vectors for the listed terms. Their displacements come exact (or
-// rounded once) from the binary32 positions through mic3; everything after
-// that is FP64, because near-collinear triples make the angle and dihedral
+// rounded once) from the positions through mic3; everything after that is
+// FP64, because near-collinear triples make the angle and dihedral
 // algebra ill-conditioned in binary32 (|v0 x v1| ~ 1e-5 |v0||v1| occurs in a
 // million-bead melt and costs binary32 three significant digits there).
 struct d3 {
   double x, y, z;
 };
-__device__ __forceinline__ d3 dbl(float3 v) { return {v.x, v.y, v.z}; }
-__device__ __forceinline__ float3 flt(d3 v) {
-  return make_float3(static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z));
+__device__ __forceinline__ d3 dbl(real3 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ real3 flt(d3 v) {
+  return make_real3(static_cast<real_t>(v.x), static_cast<real_t>(v.y), static_cast<real_t>(v.z));
 }
 __device__ __forceinline__ d3 dsub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ d3 dadd(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -265,15 +252,15 @@ __device__ __forceinline__ d3 dcross(d3 a, d3 b) {
 // Harmonic bond U = k/2 (r - r0)^2 (bond_harmonic.cu:41-79) on the displacement
 // d = x_self - x_partner; returns the force on self and the term's energy and
 // virial d.f.
-__device__ __forceinline__ float3 bond_force(float3 df, float2 kc, double& energy,
-                                             double& virial) {
+__device__ __forceinline__ real3 bond_force(real3 df, real2 kc, double& energy,
+                                            double& virial) {
   const d3 d = dbl(df);
   const double k = kc.x, r0 = kc.y;
   const double r2 = ddot(d, d);
   if (r2 == 0.0) {
     energy = 0.5 * k * r0 * r0;
     virial = 0.0;
-    return make_float3(0.f, 0.f, 0.f);
+    return make_real3(0, 0, 0);
   }
   const double inv_r = rsq_f64(r2);
   const double stretch = r2 * inv_r - r0;
@@ -289,9 +276,9 @@ __device__ __forceinline__ float3 bond_force(float3 df, float2 kc, double& energ
 // and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
 // instead of differences of nearly parallel vectors. The reference's floor
 // (sin theta replaced by 0.001 at or below 0.001) is kept.
-__device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 xk, float2 kc,
-                                                   int role, const BoxView& box, double& energy,
-                                                   double& virial) {
+__device__ __forceinline__ real3 angle_role_force(real3 xi, real3 xj, real3 xk, real2 kc,
+                                                  int role, const BoxView& box, double& energy,
+                                                  double& virial) {
   const d3 a = dbl(mic3(xi, xj, box));
   const d3 b = dbl(mic3(xk, xj, box));
   const double k = kc.x, theta0 = kc.y;
@@ -299,15 +286,17 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
   if (ra2 == 0.0 || rb2 == 0.0) {
     energy = 0.5 * k * theta0 * theta0;
     virial = 0.0;
-    return make_float3(0.f, 0.f, 0.f);
+    return make_real3(0, 0, 0);
   }
   const double ia = rsq_f64(ra2), ib = rsq_f64(rb2);
   const d3 nrm = dcross(a, b);
   const double nn2 = ddot(nrm, nrm);
   const double nn = nn2 > 0.0 ? nn2 * rsq_f64(nn2) : 0.0;
-  // The FP64 arguments are well conditioned; a binary32 atan2 of them is
-  // accurate to a few 1e-8 relative and keeps the pass's register budget.
-  const double theta = atan2f(static_cast<float>(nn), static_cast<float>(ddot(a, b)));
+  // The FP64 arguments are well conditioned; in the binary32 build an atan2
+  // of them in binary32 is accurate to a few 1e-8 relative (below the
+  // force's own rounding) and keeps the pass's register budget.
+  const double theta = kReal64 ? atan2(nn, ddot(a, b))
+                               : atan2f(static_cast<float>(nn), static_cast<float>(ddot(a, b)));
   const double dth = theta - theta0;
   energy = 0.5 * k * dth * dth;
   const double sn_true = nn * ia * ib;
@@ -325,9 +314,9 @@ __device__ __forceinline__ float3 angle_role_force(float3 xi, float3 xj, float3 
 // Harmonic dihedral k[1 + d cos(n phi)] (dihedral_harmonic.cu:83-206), one role.
 // phi = atan2(|v1| v0.c1, c0.c1) carries the reference's sign convention
 // (negative when v0.c1 < 0) without the ill-conditioned acos near 0 and pi.
-__device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, float3 xk, float3 xl,
-                                                      float4 kc, int role, const BoxView& box,
-                                                      double& energy, double& virial) {
+__device__ __forceinline__ real3 dihedral_role_force(real3 xi, real3 xj, real3 xk, real3 xl,
+                                                     real4 kc, int role, const BoxView& box,
+                                                     double& energy, double& virial) {
   const d3 v0 = dbl(mic3(xi, xj, box));
   const d3 v1 = dbl(mic3(xk, xj, box));
   const d3 v2 = dbl(mic3(xk, xl, box));
@@ -337,7 +326,7 @@ __device__ __forceinline__ float3 dihedral_role_force(float3 xi, float3 xj, floa
   if (n1 == 0.0 || nc0 == 0.0 || nc1 == 0.0) {
     energy = nmul == 0.0 ? k * (1.0 + dsign) : k;
     virial = 0.0;
-    return make_float3(0.f, 0.f, 0.f);
+    return make_real3(0, 0, 0);
   }
   const double inb = rsq_f64(n1);
   const double nb = n1 * inb;

@@ -18,14 +18,13 @@ namespace beads_amd {
 
 namespace {
 
-constexpr double kPi = 3.141592653589793238462643383279502884;
 constexpr uint64_t kOverflowPollSteps = 256;
 
-bool finite(float v) { return std::isfinite(v); }
+bool finite(double v) { return std::isfinite(v); }
 
 std::string fmt9(double v) {
   std::ostringstream s;
-  s << std::setprecision(9) << static_cast<float>(v);
+  s << std::setprecision(9) << static_cast<real_t>(v);
   return s.str();
 }
 
@@ -45,7 +44,7 @@ std::string label_value(const std::string& label, const T& value) {
 }
 
 // '(' x ',' y ',' z ')' at the stream's default precision (log_output.cpp:65-69).
-std::string real3(const float* v) {
+std::string triple(const real_t* v) {
   std::ostringstream s;
   s << '(' << v[0] << ',' << v[1] << ',' << v[2] << ')';
   return s.str();
@@ -77,30 +76,30 @@ UnitSystem unit_system_from(int32_t units) {
     return {"reduced", 1.0, 1.0};
   }
   if (units == BEADS_UNITS_NM_KJMOL) {
-    return {"nm_kjmol", static_cast<double>(static_cast<float>(0.00831446261815324)),
-            static_cast<double>(static_cast<float>(16.60539067))};
+    return {"nm_kjmol", static_cast<double>(static_cast<real_t>(0.00831446261815324)),
+            static_cast<double>(static_cast<real_t>(16.60539067))};
   }
   fail(BEADS_E_VALUE, "system.units must be \"reduced\" or \"nm_kjmol\".");
 }
 
-BoxView make_box_view(const float* bb) {
+BoxView make_box_view(const real_t* bb) {
   BoxView b{};
   for (int a = 0; a < 3; ++a) {
     b.lo[a] = bb[a];
     b.L[a] = bb[3 + a] - bb[a];
-    b.hL[a] = 0.5f * b.L[a];
-    b.invL[a] = 1.0f / b.L[a];
+    b.hL[a] = real_t(0.5) * b.L[a];
+    b.invL[a] = real_t(1) / b.L[a];
   }
   return b;
 }
 
-NeighborPlan make_neighbor_plan(const BoxView& box, float rc, float buffer, uint32_t n,
+NeighborPlan make_neighbor_plan(const BoxView& box, real_t rc, real_t buffer, uint32_t n,
                                 uint32_t max_neighbors, uint64_t check_every,
                                 uint64_t sort_every) {
   NeighborPlan p;
   require(n > 0, "neighbor n_particles must be positive.");
-  require(finite(rc) && rc > 0.f, "neighbor force_cutoff must be finite and positive.");
-  require(finite(buffer) && buffer >= 0.f,
+  require(finite(rc) && rc > 0, "neighbor force_cutoff must be finite and positive.");
+  require(finite(buffer) && buffer >= 0,
           "neighbor cutoff_buffer must be finite and non-negative.");
   require(max_neighbors > 0, "neighbor max_neighbors must be positive.");
   require(check_every > 0, "neighbor rebuild_check_every must be positive.");
@@ -112,7 +111,7 @@ NeighborPlan make_neighbor_plan(const BoxView& box, float rc, float buffer, uint
   p.max_neighbors = max_neighbors;
   p.check_every = check_every;
   p.sort_every = sort_every;
-  const float half_shortest = 0.5f * std::min(box.L[0], std::min(box.L[1], box.L[2]));
+  const real_t half_shortest = real_t(0.5) * std::min(box.L[0], std::min(box.L[1], box.L[2]));
   require(p.search_cutoff <= half_shortest,
           "neighbor search_cutoff must not exceed half shortest box length.");
   uint32_t counts[3];
@@ -123,10 +122,10 @@ NeighborPlan make_neighbor_plan(const BoxView& box, float rc, float buffer, uint
     require(raw <= static_cast<double>(std::numeric_limits<uint32_t>::max()),
             "neighbor cell count exceeds index_t range.");
     counts[a] = raw < 1.0 ? 1u : static_cast<uint32_t>(raw);
-    p.cell_size[a] = box.L[a] / static_cast<float>(counts[a]);
-    p.inv_cell[a] = 1.0f / p.cell_size[a];
+    p.cell_size[a] = box.L[a] / static_cast<real_t>(counts[a]);
+    p.inv_cell[a] = real_t(1) / p.cell_size[a];
     if (counts[a] >= 3) {
-      const float tol = std::max(1.0e-6f, p.search_cutoff * 1.0e-5f);
+      const real_t tol = std::max(real_t(1.0e-6), p.search_cutoff * real_t(1.0e-5));
       require(p.cell_size[a] + tol >= p.search_cutoff,
               "neighbor cell size must be at least neighbor search_cutoff.");
     }
@@ -363,6 +362,12 @@ HostTopology build_host_topology(const beads_spec_t& s) {
 void validate_spec(const beads_spec_t& s) {
   (void)unit_system_from(s.units);
   require(s.n_particles > 0, "system.n_particles must be positive.");
+  // The force pass gathers neighbour positions by a 32-bit byte offset.
+  constexpr uint64_t kMaxParticles = (uint64_t{1} << 32) / sizeof(real4) - 1;
+  require(s.n_particles <= kMaxParticles,
+          "system.n_particles must be at most " + std::to_string(kMaxParticles) +
+              " in this build.",
+          BEADS_E_OVERFLOW);
   require(s.box_bound && s.positions && s.velocities && s.masses && s.types && s.tags &&
               s.molecule_ids && s.images,
           "system arrays must not be null.", BEADS_E_TYPE);
@@ -375,13 +380,13 @@ void validate_spec(const beads_spec_t& s) {
   std::vector<uint8_t> seen(static_cast<size_t>(n) + 1, 0);
   for (uint32_t i = 0; i < n; ++i) {
     for (int a = 0; a < 3; ++a) {
-      const float x = s.positions[3 * i + a];
+      const real_t x = s.positions[3 * i + a];
       require(finite(x), "system.positions must be finite.");
       require(x >= s.box_bound[a] && x < s.box_bound[3 + a],
               "system.positions must be inside box_bound.");
       require(finite(s.velocities[3 * i + a]), "system.velocities must be finite.");
     }
-    require(finite(s.masses[i]) && s.masses[i] > 0.f, "system.masses must be finite and positive.");
+    require(finite(s.masses[i]) && s.masses[i] > 0, "system.masses must be finite and positive.");
     require(s.types[i] >= 1 && s.types[i] <= s.n_types,
             "system.types must be covered by the pair coefficient table.");
     const uint32_t t = s.tags[i];
@@ -507,15 +512,13 @@ Engine::~Engine() {
 void Engine::upload_state() {
   const beads_spec_t& s = spec_;
   const size_t n = n_;
-  std::vector<float4> pos(n), vel(n);
+  std::vector<real4> pos(n), vel(n);
   std::vector<int4> img(n);
   for (size_t i = 0; i < n; ++i) {
-    int32_t t = s.types[i] - 1;
-    float tw;
-    std::memcpy(&tw, &t, sizeof(float));
-    pos[i] = make_float4(s.positions[3 * i], s.positions[3 * i + 1], s.positions[3 * i + 2], tw);
-    vel[i] = make_float4(s.velocities[3 * i], s.velocities[3 * i + 1], s.velocities[3 * i + 2],
-                         s.masses[i]);
+    const real_t tw = real_of_id(static_cast<uint32_t>(s.types[i] - 1));
+    pos[i] = make_real4(s.positions[3 * i], s.positions[3 * i + 1], s.positions[3 * i + 2], tw);
+    vel[i] = make_real4(s.velocities[3 * i], s.velocities[3 * i + 1], s.velocities[3 * i + 2],
+                        s.masses[i]);
     img[i] = make_int4(s.images[3 * i], s.images[3 * i + 1], s.images[3 * i + 2], 0);
   }
   pos_.resize(static_cast<size_t>(n) + kCandidatePad);  // the direct build's candidate stream
@@ -660,7 +663,7 @@ void Engine::set_up_params() {
   in.ref_image = ref_image_.data();
   in.box = box_;
   in.dt = models_.integrator->dt();
-  const float half_buffer = 0.5f * plan_.cutoff_buffer;
+  const real_t half_buffer = real_t(0.5) * plan_.cutoff_buffer;
   in.threshold_sq = half_buffer * half_buffer;
   in.ctrl = ctrl_.data();
 
@@ -924,7 +927,7 @@ void Engine::stats(beads_engine_stats_t* out) {
 
 void Engine::read_state(const beads_state_out_t& o) {
   const size_t n = n_;
-  std::vector<float4> pos(n), vel(n), force(n);
+  std::vector<real4> pos(n), vel(n), force(n);
   std::vector<int4> img(n);
   pos_.download(pos.data(), n, stream_);
   vel_.download(vel.data(), n, stream_);
@@ -939,11 +942,7 @@ void Engine::read_state(const beads_state_out_t& o) {
       o.positions[3 * i + 1] = pos[i].y;
       o.positions[3 * i + 2] = pos[i].z;
     }
-    if (o.types) {
-      int32_t t;
-      std::memcpy(&t, &pos[i].w, sizeof(int32_t));
-      o.types[i] = t + 1;
-    }
+    if (o.types) o.types[i] = static_cast<int32_t>(id_of(pos[i].w)) + 1;
     if (o.velocities) {
       o.velocities[3 * i] = vel[i].x;
       o.velocities[3 * i + 1] = vel[i].y;
@@ -1047,7 +1046,7 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
 // -------------------------------------------------------------- frame mode --
 
 // Binary32 tables as the forward uses them ({sigma^2, 24 eps, 4 eps, shift},
-// {k, r0}) plus FP64 copies of the same binary32 values for the adjoint.
+// {k, r0}) plus FP64 copies of the same real_t values for the adjoint.
 void Engine::upload_parameters() {
   models_.pair->upload(stream_);
   if (models_.bond) models_.bond->upload(stream_);
@@ -1056,18 +1055,18 @@ void Engine::upload_parameters() {
   BEADS_HIP(hipStreamSynchronize(stream_));
 }
 
-void Engine::set_lj_params(const float* eps, const float* sigma) {
+void Engine::set_lj_params(const real_t* eps, const real_t* sigma) {
   models_.pair->set_type_pair_parameters(eps, sigma);
   upload_parameters();
 }
 
-void Engine::set_bond_params(const float* k, const float* r0) {
+void Engine::set_bond_params(const real_t* k, const real_t* r0) {
   require(models_.bond != nullptr, "this engine has no bond style.", BEADS_E_VALUE);
   models_.bond->set_parameters(k, r0);
   upload_parameters();
 }
 
-// One force evaluation at caller-supplied positions ([n][3] binary32 in slot
+// One force evaluation at caller-supplied positions ([n][3] real_t in slot
 // order, device memory): wrap into the box, neighbour rebuild without
 // reordering (forced, or with reuse_list only when some atom moved more than
 // half the skin since the last build, the integrator's test), forces +
@@ -1082,7 +1081,7 @@ void Engine::enter_frame_mode() {
   }
 }
 
-void Engine::load_frame(const float* positions_dev, bool reuse_list) {
+void Engine::load_frame(const real_t* positions_dev, bool reuse_list) {
   enter_frame_mode();
   // The list may be kept only if one was built in frame mode and the last
   // call completed (so its reference positions are the list's).
@@ -1102,7 +1101,7 @@ void Engine::load_frame(const float* positions_dev, bool reuse_list) {
   rebuild_sequence(!check);
 }
 
-void Engine::frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
+void Engine::frame_forces(const real_t* positions_dev, real_t* forces_dev, double* energies_out,
                           bool reuse_list, bool async) {
   enter_frame_mode();
   frame_loaded_ = false;
@@ -1145,7 +1144,7 @@ void Engine::set_stream(hipStream_t stream) {
 
 // Positions and neighbour list of a frame without evaluating forces: what
 // frame_backward needs when another frame was evaluated in between.
-void Engine::frame_load(const float* positions_dev, bool reuse_list, bool async) {
+void Engine::frame_load(const real_t* positions_dev, bool reuse_list, bool async) {
   frame_loaded_ = false;
   load_frame(positions_dev, reuse_list);
   // The force pass normally clears the rebuild flags; no force pass runs here.
@@ -1156,11 +1155,11 @@ void Engine::frame_load(const float* positions_dev, bool reuse_list, bool async)
   frame_list_valid_ = true;
 }
 
-// Adjoint of the last frame_forces call: grad_forces [n][3] binary32 (device),
-// grad_energy scales dE/d(.), outputs dL/dx [n][3] binary32 and the FP64
+// Adjoint of the last frame_forces call: grad_forces [n][3] real_t (device),
+// grad_energy scales dE/d(.), outputs dL/dx [n][3] real_t and the FP64
 // parameter gradients [eps T*T | sigma T*T | k nb | r0 nb] (device).
-void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
-                            float* grad_positions_dev, double* grad_params_dev, bool async,
+void Engine::frame_backward(const real_t* grad_forces_dev, double grad_energy,
+                            real_t* grad_positions_dev, double* grad_params_dev, bool async,
                             bool noalloc) {
   require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
   require(!half_list_, "the adjoint reads the full neighbour list; create the engine with "
@@ -1173,7 +1172,7 @@ void Engine::frame_backward(const float* grad_forces_dev, double grad_energy,
   models_.pair->bind_adjoint(p);
   if (models_.bond) models_.bond->bind_adjoint(p);
   const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
-  float4* staging = nullptr;
+  real4* staging = nullptr;
   if (noalloc) {
     require(n_ <= kNoAllocMaxAtoms,
             "frame_backward_noalloc serves at most " + std::to_string(kNoAllocMaxAtoms) +
@@ -1314,11 +1313,11 @@ void Engine::write_thermo_row(uint64_t step, const Ctrl& c) {
     if (has_dihedral_style_) thermo_stream_ << ",dihedral_pe";
     thermo_stream_ << ",ke,temp,press\n";
   }
-  const float pair = static_cast<float>(c.pe[0]);
-  const float bond = has_bond_style_ ? static_cast<float>(c.pe[1]) : 0.f;
-  const float angle = has_angle_style_ ? static_cast<float>(c.pe[2]) : 0.f;
-  const float dih = has_dihedral_style_ ? static_cast<float>(c.pe[3]) : 0.f;
-  const float pe = pair + bond + angle + dih;
+  const real_t pair = static_cast<real_t>(c.pe[0]);
+  const real_t bond = has_bond_style_ ? static_cast<real_t>(c.pe[1]) : real_t(0);
+  const real_t angle = has_angle_style_ ? static_cast<real_t>(c.pe[2]) : real_t(0);
+  const real_t dih = has_dihedral_style_ ? static_cast<real_t>(c.pe[3]) : real_t(0);
+  const real_t pe = pair + bond + angle + dih;
   const double dof = std::max(0.0, 3.0 * n_ - 3.0);
   const double temp = dof > 0 ? 2.0 * c.ke / (dof * units_.kb) : 0.0;
   const double vol = static_cast<double>(box_.L[0]) * box_.L[1] * box_.L[2];
@@ -1361,10 +1360,10 @@ size_t Engine::trajectory_record_bytes() const {
   const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
   size_t rec = sizeof(uint32_t);
   if (mask & BEADS_TRAJ_TYPE) rec += sizeof(int32_t);
-  if (mask & BEADS_TRAJ_POSITION) rec += 3 * sizeof(float);
+  if (mask & BEADS_TRAJ_POSITION) rec += 3 * sizeof(real_t);
   if (mask & BEADS_TRAJ_IMAGE) rec += 3 * sizeof(int32_t);
-  if (mask & BEADS_TRAJ_VELOCITY) rec += 3 * sizeof(float);
-  if (mask & BEADS_TRAJ_FORCE) rec += 3 * sizeof(float);
+  if (mask & BEADS_TRAJ_VELOCITY) rec += 3 * sizeof(real_t);
+  if (mask & BEADS_TRAJ_FORCE) rec += 3 * sizeof(real_t);
   return rec;
 }
 
@@ -1382,7 +1381,7 @@ void Engine::write_trajectory_records(uint64_t step, const unsigned char* record
     write_raw(o, "BEADS_TRAJECTORY", 16, "trajectory magic");
     write_scalar<uint32_t>(o, 1, "trajectory version");
     write_scalar<uint32_t>(o, 0x01020304u, "trajectory endian marker");
-    write_scalar<uint32_t>(o, sizeof(float), "trajectory real size");
+    write_scalar<uint32_t>(o, sizeof(real_t), "trajectory real size");
     write_scalar<uint32_t>(o, sizeof(uint32_t), "trajectory index size");
     write_scalar<uint32_t>(o, sizeof(int32_t), "trajectory image size");
     write_scalar<uint32_t>(o, sizeof(int32_t), "trajectory type size");
@@ -1395,8 +1394,8 @@ void Engine::write_trajectory_records(uint64_t step, const unsigned char* record
   }
   auto& o = trajectory_stream_;
   write_scalar<uint64_t>(o, step, "trajectory frame step");
-  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
-  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
+  const real_t lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const real_t hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
   write_raw(o, lo, 3, "trajectory box lower");
   write_raw(o, hi, 3, "trajectory box upper");
   write_raw(o, records, rec * n_, "trajectory records");
@@ -1407,7 +1406,7 @@ void Engine::write_trajectory_frame(uint64_t step) {
   const uint32_t mask = spec_.trajectory_fields | BEADS_TRAJ_TAG;
   const size_t rec = trajectory_record_bytes();
   const size_t n = n_;
-  std::vector<float> pos(3 * n), vel(3 * n), force(3 * n);
+  std::vector<real_t> pos(3 * n), vel(3 * n), force(3 * n);
   std::vector<int32_t> img(3 * n), type(n);
   std::vector<uint32_t> tag(n);
   beads_state_out_t out{};
@@ -1428,10 +1427,10 @@ void Engine::write_trajectory_frame(uint64_t step) {
     };
     put(&tag[i], 4);
     if (mask & BEADS_TRAJ_TYPE) put(&type[i], 4);
-    if (mask & BEADS_TRAJ_POSITION) put(&pos[3 * i], 12);
+    if (mask & BEADS_TRAJ_POSITION) put(&pos[3 * i], 3 * sizeof(real_t));
     if (mask & BEADS_TRAJ_IMAGE) put(&img[3 * i], 12);
-    if (mask & BEADS_TRAJ_VELOCITY) put(&vel[3 * i], 12);
-    if (mask & BEADS_TRAJ_FORCE) put(&force[3 * i], 12);
+    if (mask & BEADS_TRAJ_VELOCITY) put(&vel[3 * i], 3 * sizeof(real_t));
+    if (mask & BEADS_TRAJ_FORCE) put(&force[3 * i], 3 * sizeof(real_t));
   }
   write_trajectory_records(step, buf.data());
 }
@@ -1439,7 +1438,7 @@ void Engine::write_trajectory_frame(uint64_t step) {
 // final_state_output.cu:103-185 (BEADS_SNAPSHOT v1, SoA in slot order).
 void Engine::write_final_state() {
   const size_t n = n_;
-  std::vector<float> pos(3 * n), vel(3 * n), mass(n);
+  std::vector<real_t> pos(3 * n), vel(3 * n), mass(n);
   std::vector<int32_t> img(3 * n), type(n);
   std::vector<uint32_t> tag(n), molid(n);
   beads_state_out_t out{};
@@ -1457,7 +1456,7 @@ void Engine::write_final_state() {
   }
   write_raw(o, "BEADS_SNAPSHOT", 14, "snapshot magic");
   write_scalar<uint32_t>(o, 1, "snapshot version");
-  write_scalar<uint32_t>(o, sizeof(float), "snapshot real size");
+  write_scalar<uint32_t>(o, sizeof(real_t), "snapshot real size");
   write_scalar<uint32_t>(o, sizeof(uint32_t), "snapshot index size");
   write_scalar<uint32_t>(o, sizeof(int32_t), "snapshot image size");
   write_scalar<uint32_t>(o, sizeof(int32_t), "snapshot type size");
@@ -1467,15 +1466,15 @@ void Engine::write_final_state() {
   write_raw(o, units_.name.data(), units_.name.size(), "snapshot units");
   write_scalar<uint64_t>(o, step_, "snapshot step");
   write_scalar<uint64_t>(o, n, "snapshot particle count");
-  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
-  const float hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
+  const real_t lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const real_t hi[3] = {spec_box_hi_[0], spec_box_hi_[1], spec_box_hi_[2]};
   write_raw(o, lo, 3, "snapshot box lower");
   write_raw(o, hi, 3, "snapshot box upper");
   write_raw(o, tag.data(), n, "snapshot tag");
   write_raw(o, type.data(), n, "snapshot type");
   write_raw(o, molid.data(), n, "snapshot molecule_id");
   write_raw(o, mass.data(), n, "snapshot mass");
-  std::vector<float> col(n);
+  std::vector<real_t> col(n);
   for (int a = 0; a < 3; ++a) {
     for (size_t i = 0; i < n; ++i) col[i] = pos[3 * i + a];
     write_raw(o, col.data(), n, "snapshot position");
@@ -1529,14 +1528,14 @@ void Engine::poll_interrupt() {
 // gathers them).
 void Engine::log_run_start() {
   if (spec_.log_echo == BEADS_LOG_NONE) return;
-  const float lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
-  const float len[3] = {box_.L[0], box_.L[1], box_.L[2]};
+  const real_t lo[3] = {box_.lo[0], box_.lo[1], box_.lo[2]};
+  const real_t len[3] = {box_.L[0], box_.L[1], box_.L[2]};
   {
     std::ostringstream m;
     m << "execute start" << '\n' << label_value("units", units_.name) << '\n'
-      << label_value("particles", n_) << '\n' << label_value("box_lo", real3(lo)) << '\n'
-      << label_value("box_hi", real3(spec_box_hi_)) << '\n'
-      << label_value("box_lengths", real3(len)) << '\n'
+      << label_value("particles", n_) << '\n' << label_value("box_lo", triple(lo)) << '\n'
+      << label_value("box_hi", triple(spec_box_hi_)) << '\n'
+      << label_value("box_lengths", triple(len)) << '\n'
       << label_value("runsteps", spec_.runsteps);
     log("INFO", m.str());
   }

@@ -29,23 +29,23 @@ UnitSystem unit_system_from(int32_t units);
 
 // neighbor_plan.cpp:182-236: search cutoff, cell grid, path choice.
 struct NeighborPlan {
-  float force_cutoff = 0.f;
-  float cutoff_buffer = 0.f;
-  float search_cutoff = 0.f;
-  float search_cutoff_sq = 0.f;
+  real_t force_cutoff = 0;
+  real_t cutoff_buffer = 0;
+  real_t search_cutoff = 0;
+  real_t search_cutoff_sq = 0;
   uint32_t nx = 1, ny = 1, nz = 1, n_cells = 1;
-  float cell_size[3] = {0, 0, 0};
-  float inv_cell[3] = {0, 0, 0};
+  real_t cell_size[3] = {0, 0, 0};
+  real_t inv_cell[3] = {0, 0, 0};
   int32_t path = 0;  // 0 direct, 1 regular cell, 2 small dimension
   uint32_t max_neighbors = 0;
   uint64_t check_every = 1;
   uint64_t sort_every = 10;
 };
 
-NeighborPlan make_neighbor_plan(const BoxView& box, float force_cutoff, float buffer,
+NeighborPlan make_neighbor_plan(const BoxView& box, real_t force_cutoff, real_t buffer,
                                 uint32_t n, uint32_t max_neighbors, uint64_t check_every,
                                 uint64_t sort_every);
-BoxView make_box_view(const float* box_bound);
+BoxView make_box_view(const real_t* box_bound);
 
 // Per-slot listed-term roles and exclusion partners built on the host from the
 // tag-based topology (the device keeps them current across reorders).
@@ -105,17 +105,17 @@ class Engine {
 
   // Frame mode (force matching, trajectory backprop): positions come from the
   // caller in slot order, so particles are never reordered.
-  void set_lj_params(const float* eps, const float* sigma);
-  void set_bond_params(const float* k, const float* r0);
+  void set_lj_params(const real_t* eps, const real_t* sigma);
+  void set_bond_params(const real_t* k, const real_t* r0);
   // async: enqueue and return without waiting (energies_out is then device
   // memory, and a list overflow is reported by the next synchronize()).
-  void frame_forces(const float* positions_dev, float* forces_dev, double* energies_out,
+  void frame_forces(const real_t* positions_dev, real_t* forces_dev, double* energies_out,
                     bool reuse_list, bool async = false);
-  void frame_load(const float* positions_dev, bool reuse_list, bool async = false);
+  void frame_load(const real_t* positions_dev, bool reuse_list, bool async = false);
   // noalloc: stage dL/dF in the library's static per-GPU scratch array instead
   // of an engine buffer and clear grad_params with a kernel; implies async.
-  void frame_backward(const float* grad_forces_dev, double grad_energy,
-                      float* grad_positions_dev, double* grad_params_dev, bool async = false,
+  void frame_backward(const real_t* grad_forces_dev, double grad_energy,
+                      real_t* grad_positions_dev, double* grad_params_dev, bool async = false,
                       bool noalloc = false);
   void frame_timing(double* forward_s, double* backward_s);
   // Enqueue frame-mode work on the caller's stream (nullptr: the engine's own).
@@ -127,7 +127,7 @@ class Engine {
  private:
   void upload_parameters();
   void enter_frame_mode();
-  void load_frame(const float* positions_dev, bool reuse_list);
+  void load_frame(const real_t* positions_dev, bool reuse_list);
   void upload_state();
   void set_up_params();
   void rebuild_sequence(bool forced);
@@ -157,7 +157,7 @@ class Engine {
   std::string thermo_path_, trajectory_path_, final_state_path_, log_path_;
   UnitSystem units_;
   BoxView box_{};
-  float spec_box_hi_[3] = {0.f, 0.f, 0.f};  // upper bounds exactly as given
+  real_t spec_box_hi_[3] = {0, 0, 0};  // upper bounds exactly as given
   NeighborPlan plan_;
   uint32_t n_ = 0;
   bool thermostat_ = false;
@@ -174,8 +174,8 @@ class Engine {
   hipStream_t own_stream_ = nullptr;
 
   // device state
-  DeviceBuffer<float4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
-  DeviceBuffer<float4> cell_pos_;  // positions in cell order, .w = slot bits
+  DeviceBuffer<real4> pos_, vel_, force_, ref_pos_, s_pos_, s_vel_;
+  DeviceBuffer<real4> cell_pos_;  // positions in cell order, .w = slot bits
   DeviceBuffer<uint4> cell_xbins_;  // per-cell x-bin prefix counts
   DeviceBuffer<int4> image_, ref_image_, s_image_;
   DeviceBuffer<uint32_t> tag_, molid_, s_tag_, s_molid_;
@@ -203,7 +203,7 @@ class Engine {
   // names (models.hpp); they own their parameter tables.
   ModelSet models_;
   int32_t n_types_ = 0, n_bond_types_ = 0;
-  DeviceBuffer<float4> grad_force4_;
+  DeviceBuffer<real4> grad_force4_;
   bool frame_mode_ = false;   // positions supplied per frame; never reordered
   bool frame_loaded_ = false;
   bool frame_list_valid_ = false;  // the stored list and its reference positions are current

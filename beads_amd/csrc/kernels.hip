@@ -1,8 +1,9 @@
 // gfx950 kernels of the per-step MD path.
 //
 // Design (CDNA4): 256-wide workgroups (4 wave64s), one work-item per atom
-// except in the force pass (a team of kTeam lanes per atom), float4 SoA
-// streams, FP64 for LJ pair values and global reductions. Kernels that gather
+// except in the force pass (a team of kTeam lanes per atom), real4 SoA
+// streams (float4, or double4 in the BEADS_REAL64 build), FP64 for LJ pair
+// values and global reductions. Kernels that gather
 // neighbour positions map workgroups onto atoms XCD by XCD so each XCD's L2
 // holds one contiguous (spatially sorted) slab of positions. Every kernel of
 // the neighbour-rebuild sequence reads the device control block first and
@@ -184,20 +185,35 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(CellScanParams p) 
 // build's candidate stream); after a reorder, reorder_commit writes them.
 constexpr uint32_t kSortCellsPerBlock = kBlock / 64;
 
-__device__ __forceinline__ float4 cell_entry(float4 x, uint32_t slot) {
-  return make_float4(x.x, x.y, x.z, __uint_as_float(slot));
+__device__ __forceinline__ real4 cell_entry(real4 x, uint32_t slot) {
+  return make_real4(x.x, x.y, x.z, real_of_id(slot));
+}
+
+// v of lane `lane` (uniform over the wave).
+__device__ __forceinline__ real_t read_lane(real_t v, int lane) {
+#ifdef BEADS_REAL64
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), lane));
+  const uint32_t hi = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), lane));
+  return __builtin_bit_cast(real_t, (static_cast<uint64_t>(hi) << 32) | lo);
+#else
+  return __builtin_bit_cast(real_t, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+#endif
 }
 
 // Bin of coordinate x inside x-cell cx (0 .. kXBins - 1), monotone in x: the
 // cell coordinate of axis_cell, offset by the cell and scaled by the bin count
 // (a power of two).
-__device__ __forceinline__ int x_bin(float x, float lo, float inv, uint32_t cx) {
+__device__ __forceinline__ int x_bin(real_t x, real_t lo, real_t inv, uint32_t cx) {
 #pragma clang fp contract(off)
-  const float f = ((x - lo) * inv - static_cast<float>(cx)) * static_cast<float>(kXBins);
-  return f <= 0.f ? 0 : (f >= static_cast<float>(kXBins - 1) ? kXBins - 1 : static_cast<int>(f));
+  const real_t f = ((x - lo) * inv - static_cast<real_t>(cx)) * static_cast<real_t>(kXBins);
+  return f <= real_t(0) ? 0
+                        : (f >= static_cast<real_t>(kXBins - 1) ? kXBins - 1 : static_cast<int>(f));
 }
 
-__device__ __forceinline__ bool before_in_cell(float xa, uint32_t sa, float xb, uint32_t sb) {
+__device__ __forceinline__ bool before_in_cell(real_t xa, uint32_t sa, real_t xb, uint32_t sb) {
   return xa < xb || (xa == xb && sa < sb);
 }
 
@@ -207,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
   }
   const uint32_t lane = threadIdx.x & 63u;
   const bool stream = p.ctrl->reorder == 0;
-  const float lo = p.box.lo[0], inv = p.grid.inv_cell[0];
+  const real_t lo = p.box.lo[0], inv = p.grid.inv_cell[0];
   // A capped grid loops over the cells, so the launch stays small when the
   // rebuild flag makes it exit at once.
   for (uint32_t c = blockIdx.x * kSortCellsPerBlock + threadIdx.x / 64; c < p.n_cells;
@@ -218,14 +234,13 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
     uint32_t* a = p.cell_atoms;
     if (cnt <= 64) {
       const uint32_t v = lane < cnt ? a[b + lane] : 0xffffffffu;
-      const float4 xv4 = lane < cnt ? p.pos[v] : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float xv = xv4.x;
+      const real4 xv4 = lane < cnt ? p.pos[v] : make_real4(0, 0, 0, 0);
+      const real_t xv = xv4.x;
       uint32_t rank = 0;
       for (uint32_t m = 0; m < cnt; ++m) {
         const uint32_t w = static_cast<uint32_t>(
             __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(m)));
-        const float xw = __int_as_float(
-            __builtin_amdgcn_readlane(__float_as_int(xv), static_cast<int>(m)));
+        const real_t xw = read_lane(xv, static_cast<int>(m));
         rank += before_in_cell(xw, w, xv, v) ? 1u : 0u;
       }
       if (lane < cnt) {
@@ -245,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
     } else if (lane == 0) {
       for (uint32_t k = b + 1; k < e; ++k) {
         const uint32_t v = a[k];
-        const float xv = p.pos[v].x;
+        const real_t xv = p.pos[v].x;
         uint32_t m = k;
         while (m > b && before_in_cell(xv, v, p.pos[a[m - 1]].x, a[m - 1])) {
           a[m] = a[m - 1];
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void cell_sort_kernel(CellScanParams p) {
       uint32_t pre[kXBins] = {};
       for (uint32_t k = b; k < e; ++k) {
         const uint32_t v = a[k];
-        const float4 xv4 = p.pos[v];
+        const real4 xv4 = p.pos[v];
         if (stream) p.cell_pos[k] = cell_entry(xv4, v);
         const int bin = x_bin(xv4.x, lo, inv, cx);
         for (int m = bin + 1; m <= kXBins; ++m) ++pre[m - 1];
@@ -278,14 +293,24 @@ __global__ __launch_bounds__(kBlock) void pack_records_kernel(PackParams p) {
     return;
   }
   uint32_t* r = p.out + static_cast<size_t>(i) * p.words;
-  const float4 x = p.pos[i];
+  const real4 x = p.pos[i];
   uint32_t k = 0;
+  // A real as one word (binary32) or two, low word first (binary64).
+  auto put = [&](real_t v) {
+#ifdef BEADS_REAL64
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    r[k++] = static_cast<uint32_t>(b);
+    r[k++] = static_cast<uint32_t>(b >> 32);
+#else
+    r[k++] = __builtin_bit_cast(uint32_t, v);
+#endif
+  };
   r[k++] = p.tag[i];
-  if (p.mask & BEADS_TRAJ_TYPE) r[k++] = __float_as_uint(x.w) + 1u;
+  if (p.mask & BEADS_TRAJ_TYPE) r[k++] = id_of(x.w) + 1u;
   if (p.mask & BEADS_TRAJ_POSITION) {
-    r[k++] = __float_as_uint(x.x);
-    r[k++] = __float_as_uint(x.y);
-    r[k++] = __float_as_uint(x.z);
+    put(x.x);
+    put(x.y);
+    put(x.z);
   }
   if (p.mask & BEADS_TRAJ_IMAGE) {
     const int4 m = p.image[i];
@@ -294,16 +319,16 @@ __global__ __launch_bounds__(kBlock) void pack_records_kernel(PackParams p) {
     r[k++] = static_cast<uint32_t>(m.z);
   }
   if (p.mask & BEADS_TRAJ_VELOCITY) {
-    const float4 v = p.vel[i];
-    r[k++] = __float_as_uint(v.x);
-    r[k++] = __float_as_uint(v.y);
-    r[k++] = __float_as_uint(v.z);
+    const real4 v = p.vel[i];
+    put(v.x);
+    put(v.y);
+    put(v.z);
   }
   if (p.mask & BEADS_TRAJ_FORCE) {
-    const float4 f = p.force[i];
-    r[k++] = __float_as_uint(f.x);
-    r[k++] = __float_as_uint(f.y);
-    r[k++] = __float_as_uint(f.z);
+    const real4 f = p.force[i];
+    put(f.x);
+    put(f.y);
+    put(f.z);
   }
 }
 
@@ -439,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBuildAtoms + threadIdx.x / kBuildTeam;
   uint32_t count = 0;  // the same in every lane of a team
   if (i < p.n) {       // uniform over a team
-    const float4 xi = p.pos[i];
+    const real4 xi = p.pos[i];
     const uint32_t n = p.n;
     uint32_t ex[EXCL > 0 ? EXCL : 1];
     uint32_t ne = 0;
@@ -460,14 +485,14 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
       }
     }
-    const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
-    const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+    const real_t L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+    const real_t h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
     const uint32_t maxn = p.max_neighbors;
     // sh: the image shift of the candidate's cell, with mic(xi - xj) equal to
     // (xi - xj) + sh on every axis (cells of a regular grid with at least 5
     // cells per axis); generic: mic() per candidate.
-    auto accept = [&](uint32_t j, float4 xj, auto generic, float3 sh) -> bool {
-      float dx, dy, dz;
+    auto accept = [&](uint32_t j, real4 xj, auto generic, real3 sh) -> bool {
+      real_t dx, dy, dz;
       if (decltype(generic)::value) {
         dx = mic(xi.x - xj.x, L0, h0);
         dy = mic(xi.y - xj.y, L1, h1);
@@ -477,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         dy = (xi.y - xj.y) + sh.y;
         dz = (xi.z - xj.z) + sh.z;
       }
-      const float r2 = r2_strict(dx, dy, dz);
+      const real_t r2 = r2_strict(dx, dy, dz);
       // Non-short-circuit tests: selects, no branches.
       bool ok = !(r2 > p.rs2) & (HALF ? j > i : j != i);
       if (EXCL > 0) {
@@ -501,19 +526,19 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     };
     // Candidates [b, e) of a run (b < e) whose entry s is entry(s) = {x, y,
     // z, slot bits}, kBuildTeam * kBuildSlots per round.
-    auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto generic, float3 sh) {
+    auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto generic, real3 sh) {
       for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
         // Entries past the run (at most kCandidatePad - 1 past the stream's
         // last atom) are loaded and then ignored.
         static_assert(kBuildTeam * kBuildSlots <= kCandidatePad, "candidate padding");
-        float4 x[kBuildSlots];
+        real4 x[kBuildSlots];
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
           x[u] = entry(k + u * kBuildTeam + t);
         }
 #pragma unroll
         for (int u = 0; u < kBuildSlots; ++u) {
-          const uint32_t j = __float_as_uint(x[u].w);
+          const uint32_t j = id_of(x[u].w);
           const bool ok = (k + u * kBuildTeam + t < e) & accept(j, x[u], generic, sh);
           const uint32_t bits =
               static_cast<uint32_t>(__ballot(ok) >> team_lane0) & ((1u << kBuildTeam) - 1u);
@@ -527,26 +552,26 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
         }
       }
     };
-    const float3 no_shift = make_float3(0.f, 0.f, 0.f);
+    const real3 no_shift = make_real3(0, 0, 0);
     if (p.grid.path == 0) {
-      const float4* pos = p.pos;
+      const real4* pos = p.pos;
       const auto entry = [pos](uint32_t k) {
-        const float4 q = pos[k];
-        return make_float4(q.x, q.y, q.z, __uint_as_float(k));
+        const real4 q = pos[k];
+        return make_real4(q.x, q.y, q.z, real_of_id(k));
       };
       scan_range(0, n, entry, std::true_type{}, no_shift);
     } else {
       const uint32_t c = p.cell_of[i];
       const CellGridView& g = p.grid;
       const uint32_t cx = c % g.nx, cyz = c / g.nx, cy = cyz % g.ny, cz = cyz / g.ny;
-      const float4* cp = p.cell_pos;
+      const real4* cp = p.cell_pos;
       const auto entry = [cp](uint32_t k) { return cp[k]; };
       // Neighbour k (0, 1, 2) along an axis, in the order of unique_axis, and
       // its image shift: with at least 5 cells per axis a candidate in a
       // wrapped neighbour cell lies more than L/2 away (mic adds +L or -L),
       // and in any other neighbour cell less than L/2 away (mic adds 0).
-      auto axis = [](uint32_t cc, uint32_t count, int k, float L, float& shift) -> uint32_t {
-        shift = 0.f;
+      auto axis = [](uint32_t cc, uint32_t count, int k, real_t L, real_t& shift) -> uint32_t {
+        shift = 0;
         if (count <= 1) return 0u;
         if (count == 2) return k == 0 ? (cc == 0 ? 1u : 0u) : cc;
         if (k == 0) {
@@ -585,21 +610,21 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
       // x alone, so their r^2 exceeds r_s^2 and the build would reject them:
       // the accepted candidates and their order are those of the full scan.
       auto rows = [&]() {
-        const float rs = sqrtf(p.rs2);
-        const float lo = p.box.lo[0], inv = g.inv_cell[0];
-        const float fl = ((xi.x - rs) - lo) * inv - static_cast<float>(cx - 1);
-        const float fr = ((xi.x + rs) - lo) * inv - static_cast<float>(cx + 1);
-        const int m_left = static_cast<int>(floorf(fl * static_cast<float>(kXBins))) - 1;
-        const int m_right = static_cast<int>(floorf(fr * static_cast<float>(kXBins))) + 2;
+        const real_t rs = sqrt(p.rs2);
+        const real_t lo = p.box.lo[0], inv = g.inv_cell[0];
+        const real_t fl = ((xi.x - rs) - lo) * inv - static_cast<real_t>(cx - 1);
+        const real_t fr = ((xi.x + rs) - lo) * inv - static_cast<real_t>(cx + 1);
+        const int m_left = static_cast<int>(floor(fl * static_cast<real_t>(kXBins))) - 1;
+        const int m_right = static_cast<int>(floor(fr * static_cast<real_t>(kXBins))) + 2;
         const int ml = m_left < 0 ? 0 : (m_left > kXBins ? kXBins : m_left);
         const int mr = m_right < 0 ? 0 : (m_right > kXBins ? kXBins : m_right);
 #pragma unroll 1
         for (int a = 0; a < 3; ++a) {
-          float sz;
+          real_t sz;
           const uint32_t zc = axis(cz, g.nz, a, L2, sz);
 #pragma unroll 1
           for (int b = 0; b < 3; ++b) {
-            float sy;
+            real_t sy;
             const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
             const uint32_t cl = rowc + cx - 1, cr = rowc + cx + 1;
             const uint32_t s_l = p.cell_start[cl], s_m = p.cell_start[cl + 1];
@@ -608,25 +633,25 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
             const uint32_t skip_l = n_l <= kXBinsMaxCount ? below(cl, ml) : 0u;
             const uint32_t keep_r = n_r <= kXBinsMaxCount ? below(cr, mr) : n_r;
             scan_range(s_l + skip_l, s_r + keep_r, entry, std::false_type{},
-                       make_float3(0.f, sy, sz));
+                       make_real3(0, sy, sz));
           }
         }
       };
       auto cells = [&](auto generic) {
 #pragma unroll 1
         for (int a = 0; a < mz; ++a) {
-          float sz;
+          real_t sz;
           const uint32_t zc = axis(cz, g.nz, a, L2, sz);
 #pragma unroll 1
           for (int b = 0; b < my; ++b) {
-            float sy;
+            real_t sy;
             const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
 #pragma unroll 1
             for (int d = 0; d < mx; ++d) {
-              float sx;
+              real_t sx;
               const uint32_t nc = axis(cx, g.nx, d, L0, sx) + rowc;
               scan_range(p.cell_start[nc], p.cell_start[nc + 1], entry, generic,
-                         make_float3(sx, sy, sz));
+                         make_real3(sx, sy, sz));
             }
           }
         }
@@ -669,15 +694,15 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
 // TYPES: 0 = single type (coefficients in registers), 1 = table in LDS,
 // 2 = table in global memory.
 template <bool OBS, int TYPES, bool REACT>
-__device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4 xi4,
+__device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, real4 xi4,
                                         const LjCoeff* lj_lds, const LjCoeff& c0, double& fxd,
                                         double& fyd, double& fzd, double& e_pair,
                                         double& w_pair) {
   const int T = p.n_types;
-  const int ti = __float_as_int(xi4.w);
+  const int ti = static_cast<int>(id_of(xi4.w));
   const uint32_t n = p.n;
-  const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
-  const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+  const real_t L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+  const real_t h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
   const uint32_t cnt = p.nbr_count[i];
   // Eight slots per iteration: both index quads of a half chunk, then eight
   // gathers in flight before any arithmetic. Slots at or past cnt gather
@@ -689,25 +714,25 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
     const uint4 qa = *reinterpret_cast<const uint4*>(cp);
     const uint4 qb = *reinterpret_cast<const uint4*>(cp + 4);
     const uint32_t jj[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-    float4 xj[8];
+    real4 xj[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       xj[u] = p.pos[base + u < cnt ? jj[u] : i];
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const float dx = mic_diff(xi4.x, xj[u].x, L0, h0);
-      const float dy = mic_diff(xi4.y, xj[u].y, L1, h1);
-      const float dz = mic_diff(xi4.z, xj[u].z, L2, h2);
-      const float r2 = r2_strict(dx, dy, dz);
+      const real_t dx = mic_diff(xi4.x, xj[u].x, L0, h0);
+      const real_t dy = mic_diff(xi4.y, xj[u].y, L1, h1);
+      const real_t dz = mic_diff(xi4.z, xj[u].z, L2, h2);
+      const real_t r2 = r2_strict(dx, dy, dz);
       if (base + u < cnt && r2 < p.rc2) {
         LjCoeff c;
         if (TYPES == 0) {
           c = c0;
         } else if (TYPES == 1) {
-          c = lj_lds[ti * T + __float_as_int(xj[u].w)];
+          c = lj_lds[ti * T + static_cast<int>(id_of(xj[u].w))];
         } else {
-          c = p.lj[ti * T + __float_as_int(xj[u].w)];
+          c = p.lj[ti * T + static_cast<int>(id_of(xj[u].w))];
         }
         double rr, u_pair;
         const double s = lj_pair(dx, dy, dz, c, rr, u_pair);
@@ -734,18 +759,18 @@ __device__ __forceinline__ void lj_list(const ForceParams& p, uint32_t i, float4
 // (pair_lj.cu:21-167; same decision and pair value as lj_list). Slots at or
 // past cnt neither load an index nor count; they gather atom i itself.
 template <bool OBS, int TYPES>
-__device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32_t t, float4 xi4,
+__device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32_t t, real4 xi4,
                                         const LjCoeff* lj_lds, const LjCoeff& c0, double& fxd,
                                         double& fyd, double& fzd, double& e_pair,
                                         double& w_pair) {
   static_assert(kNbrChunk % kTeam == 0, "a team's slots of one round share a chunk");
   static_assert(kTeam * kTeamSlots == kNbrRound, "a round covers the padded capacity's unit");
   const int T = p.n_types;
-  const int ti = __float_as_int(xi4.w);
+  const int ti = static_cast<int>(id_of(xi4.w));
   const uint32_t n = p.n;
-  const float L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
-  const float h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
-  const float xm0 = xi4.x - L0, xm1 = xi4.y - L1, xm2 = xi4.z - L2;
+  const real_t L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
+  const real_t h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
+  const real_t xm0 = xi4.x - L0, xm1 = xi4.y - L1, xm2 = xi4.z - L2;
   const uint32_t cnt = p.nbr_count[i];
   for (uint32_t base = 0; base < cnt; base += kTeam * kTeamSlots) {
     // Every slot of the round lies inside the atom's padded capacity; slots
@@ -762,27 +787,29 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
     for (int u = 0; u < kTeamSlots; ++u) {
       jj[u] = u * kTeam < rem ? jj[u] : i;
     }
-    // Gathers as a 32-bit byte offset from the array base (n * 16 < 2^32).
+    // Gathers as a 32-bit byte offset from the array base (n * sizeof(real4)
+    // < 2^32, checked at set-up).
     const char* pos_bytes = reinterpret_cast<const char*>(p.pos);
-    float4 xj[kTeamSlots];
+    real4 xj[kTeamSlots];
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      xj[u] = *reinterpret_cast<const float4*>(pos_bytes + (jj[u] << 4));
+      xj[u] = *reinterpret_cast<const real4*>(
+          pos_bytes + jj[u] * static_cast<uint32_t>(sizeof(real4)));
     }
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      const float dx = mic_diff_sel(xi4.x, xm0, xj[u].x, L0, h0);
-      const float dy = mic_diff_sel(xi4.y, xm1, xj[u].y, L1, h1);
-      const float dz = mic_diff_sel(xi4.z, xm2, xj[u].z, L2, h2);
-      const float r2 = r2_strict(dx, dy, dz);
+      const real_t dx = mic_diff_sel(xi4.x, xm0, xj[u].x, L0, h0);
+      const real_t dy = mic_diff_sel(xi4.y, xm1, xj[u].y, L1, h1);
+      const real_t dz = mic_diff_sel(xi4.z, xm2, xj[u].z, L2, h2);
+      const real_t r2 = r2_strict(dx, dy, dz);
       if (u * kTeam < rem && r2 < p.rc2) {
         LjCoeff c;
         if (TYPES == 0) {
           c = c0;
         } else if (TYPES == 1) {
-          c = lj_lds[ti * T + __float_as_int(xj[u].w)];
+          c = lj_lds[ti * T + static_cast<int>(id_of(xj[u].w))];
         } else {
-          c = p.lj[ti * T + __float_as_int(xj[u].w)];
+          c = p.lj[ti * T + static_cast<int>(id_of(xj[u].w))];
         }
         double rr, u_pair;
         const double s = lj_pair(dx, dy, dz, c, rr, u_pair);
@@ -853,10 +880,10 @@ struct StepSums {
 // dihedral_harmonic.cu). A term's energy and virial are counted once, by role
 // 0 (bonds: the flag bit of the coefficient word).
 template <bool OBS>
-__device__ __forceinline__ float3 bond_term(const ForceParams& p, float3 xi, uint2 r,
+__device__ __forceinline__ real3 bond_term(const ForceParams& p, real3 xi, uint2 r,
                                             StepSums& sums) {
   double e, w;
-  const float3 f =
+  const real3 f =
       bond_force(mic3(xi, f3(p.pos[r.x]), p.box), p.topo.bond_coeff[r.y & 0x7fffffffu], e, w);
   if (OBS && (r.y >> 31) != 0) {
     sums.e_bond += e;
@@ -866,16 +893,16 @@ __device__ __forceinline__ float3 bond_term(const ForceParams& p, float3 xi, uin
 }
 
 template <bool OBS>
-__device__ __forceinline__ float3 angle_term(const ForceParams& p, float3 xi, uint4 r,
+__device__ __forceinline__ real3 angle_term(const ForceParams& p, real3 xi, uint4 r,
                                              StepSums& sums) {
   const int role = static_cast<int>(r.w);
-  const float3 a = f3(p.pos[r.x]);
-  const float3 b = f3(p.pos[r.y]);
-  const float3 pi = role == 0 ? xi : a;
-  const float3 pj = role == 0 ? a : (role == 1 ? xi : b);
-  const float3 pk = role == 2 ? xi : b;
+  const real3 a = f3(p.pos[r.x]);
+  const real3 b = f3(p.pos[r.y]);
+  const real3 pi = role == 0 ? xi : a;
+  const real3 pj = role == 0 ? a : (role == 1 ? xi : b);
+  const real3 pk = role == 2 ? xi : b;
   double e, w;
-  const float3 f = angle_role_force(pi, pj, pk, p.topo.angle_coeff[r.z], role, p.box, e, w);
+  const real3 f = angle_role_force(pi, pj, pk, p.topo.angle_coeff[r.z], role, p.box, e, w);
   if (OBS && role == 0) {
     sums.e_angle += e;
     sums.w_angle += w;
@@ -884,19 +911,19 @@ __device__ __forceinline__ float3 angle_term(const ForceParams& p, float3 xi, ui
 }
 
 template <bool OBS>
-__device__ __forceinline__ float3 dihedral_term(const ForceParams& p, float3 xi, uint4 r,
+__device__ __forceinline__ real3 dihedral_term(const ForceParams& p, real3 xi, uint4 r,
                                                 StepSums& sums) {
   const int role = static_cast<int>(r.w >> 30);
-  const float3 a = f3(p.pos[r.x]);
-  const float3 b = f3(p.pos[r.y]);
-  const float3 c = f3(p.pos[r.z]);
-  float3 q0, q1, q2, q3;
+  const real3 a = f3(p.pos[r.x]);
+  const real3 b = f3(p.pos[r.y]);
+  const real3 c = f3(p.pos[r.z]);
+  real3 q0, q1, q2, q3;
   if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
   else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
   else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
   else { q0 = a; q1 = b; q2 = c; q3 = xi; }
   double e, w;
-  const float3 f = dihedral_role_force(q0, q1, q2, q3, p.topo.dih_coeff[r.w & 0x3fffffffu], role,
+  const real3 f = dihedral_role_force(q0, q1, q2, q3, p.topo.dih_coeff[r.w & 0x3fffffffu], role,
                                        p.box, e, w);
   if (OBS && role == 0) {
     sums.e_dih += e;
@@ -909,16 +936,16 @@ __device__ __forceinline__ float3 dihedral_term(const ForceParams& p, float3 xi,
 // t + kTeam, ... of each kind and adds their forces to its FP64 partial sums.
 template <bool OBS, bool SPILL>
 __device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, uint32_t t,
-                                            float4 xi4, double& fxd, double& fyd, double& fzd,
+                                            real4 xi4, double& fxd, double& fyd, double& fzd,
                                             StepSums& sums) {
   if (!p.topo.has_listed) {
     return;
   }
   const uint32_t cw = p.topo.count[i];
   const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
-  const float3 xi = f3(xi4);
+  const real3 xi = f3(xi4);
   const size_t n = p.n;
-  auto add = [&](float3 f) {
+  auto add = [&](real3 f) {
     fxd += f.x;
     fyd += f.y;
     fzd += f.z;
@@ -955,13 +982,13 @@ __device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, ui
 template <bool KICK>
 __device__ __forceinline__ void finish_atom(const ForceParams& p, uint32_t i, double fxd,
                                             double fyd, double fzd, StepSums& sums) {
-  const float fx = static_cast<float>(fxd);
-  const float fy = static_cast<float>(fyd);
-  const float fz = static_cast<float>(fzd);
-  p.force[i] = make_float4(fx, fy, fz, 0.f);
-  float4 v = p.vel[i];
+  const real_t fx = static_cast<real_t>(fxd);
+  const real_t fy = static_cast<real_t>(fyd);
+  const real_t fz = static_cast<real_t>(fzd);
+  p.force[i] = make_real4(fx, fy, fz, 0);
+  real4 v = p.vel[i];
   if (KICK) {
-    const float h = 0.5f * p.dt / v.w;
+    const real_t h = real_t(0.5) * p.dt / v.w;
     v.x += h * fx;
     v.y += h * fy;
     v.z += h * fz;
@@ -1074,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kForceAtoms + threadIdx.x / kTeam;
   StepSums sums;
   if (i < p.n) {  // uniform over a team
-    const float4 xi4 = p.pos[i];
+    const real4 xi4 = p.pos[i];
     double fxd = 0.0, fyd = 0.0, fzd = 0.0;
     if (HALF) {
       if (t == 0) {
@@ -1109,9 +1136,9 @@ __global__ __launch_bounds__(kBlock) void scale_kernel(ScaleParams p) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   double ke = 0.0;
   if (i < p.n) {
-    float4 v = p.vel[i];
+    real4 v = p.vel[i];
     if (THERMO) {
-      const float lam = static_cast<float>(p.ctrl->lambda);
+      const real_t lam = static_cast<real_t>(p.ctrl->lambda);
       v.x *= lam;
       v.y *= lam;
       v.z *= lam;
@@ -1143,38 +1170,38 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
   if (i >= p.n) {
     return;
   }
-  float4 v = p.vel[i];
-  const float4 f = p.force[i];
-  float4 x = p.pos[i];
+  real4 v = p.vel[i];
+  const real4 f = p.force[i];
+  real4 x = p.pos[i];
   if (THERMO) {
     // Rounded on its own, as the reference's separate rescale kernel and
     // scale_kernel (output steps) round it: never fused into the kick.
 #pragma clang fp contract(off)
-    const float lam = static_cast<float>(p.ctrl->lambda);
+    const real_t lam = static_cast<real_t>(p.ctrl->lambda);
     v.x *= lam;
     v.y *= lam;
     v.z *= lam;
   }
-  const float h = 0.5f * p.dt / v.w;
+  const real_t h = real_t(0.5) * p.dt / v.w;
   v.x += h * f.x;
   v.y += h * f.y;
   v.z += h * f.z;
   x.x += p.dt * v.x;
   x.y += p.dt * v.y;
   x.z += p.dt * v.z;
-  float c[3];
+  real_t c[3];
   {
 #pragma clang fp contract(off)
-    c[0] = floorf((x.x - p.box.lo[0]) * p.box.invL[0]);
-    c[1] = floorf((x.y - p.box.lo[1]) * p.box.invL[1]);
-    c[2] = floorf((x.z - p.box.lo[2]) * p.box.invL[2]);
+    c[0] = floor((x.x - p.box.lo[0]) * p.box.invL[0]);
+    c[1] = floor((x.y - p.box.lo[1]) * p.box.invL[1]);
+    c[2] = floor((x.z - p.box.lo[2]) * p.box.invL[2]);
     x.x -= c[0] * p.box.L[0];
     x.y -= c[1] * p.box.L[1];
     x.z -= c[2] * p.box.L[2];
   }
   p.pos[i] = x;
   p.vel[i] = v;
-  const bool wrapped = c[0] != 0.f || c[1] != 0.f || c[2] != 0.f;
+  const bool wrapped = c[0] != real_t(0) || c[1] != real_t(0) || c[2] != real_t(0);
   int4 im = make_int4(0, 0, 0, 0);
   if (wrapped || CHECK) {
     im = p.image[i];
@@ -1187,11 +1214,11 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
   }
   if (CHECK) {
 #pragma clang fp contract(off)
-    const float4 r = p.ref_pos[i];
+    const real4 r = p.ref_pos[i];
     const int4 ri = p.ref_image[i];
-    const float dx = (x.x - r.x) + static_cast<float>(im.x - ri.x) * p.box.L[0];
-    const float dy = (x.y - r.y) + static_cast<float>(im.y - ri.y) * p.box.L[1];
-    const float dz = (x.z - r.z) + static_cast<float>(im.z - ri.z) * p.box.L[2];
+    const real_t dx = (x.x - r.x) + static_cast<real_t>(im.x - ri.x) * p.box.L[0];
+    const real_t dy = (x.y - r.y) + static_cast<real_t>(im.y - ri.y) * p.box.L[1];
+    const real_t dz = (x.z - r.z) + static_cast<real_t>(im.z - ri.z) * p.box.L[2];
     if (dx * dx + dy * dy + dz * dz > p.threshold_sq) {
       __hip_atomic_store(&p.ctrl->rebuild, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

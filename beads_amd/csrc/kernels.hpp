@@ -11,7 +11,7 @@ constexpr int kBlock = 256;
 
 struct BinParams {
   uint32_t n;
-  const float4* pos;
+  const real4* pos;
   BoxView box;
   CellGridView grid;
   uint32_t* cell_of;
@@ -28,8 +28,8 @@ struct CellScanParams {
   uint32_t* cell_start;
   uint32_t* cell_cursor;
   uint32_t* cell_atoms;
-  const float4* pos;
-  float4* cell_pos;  // written in cell order when no reorder follows
+  const real4* pos;
+  real4* cell_pos;  // written in cell order when no reorder follows
   uint4* cell_xbins; // per cell: x-bin prefix counts of its x-sorted members
   BoxView box;
   CellGridView grid;
@@ -38,7 +38,7 @@ struct CellScanParams {
 
 struct ScaleParams {
   uint32_t n;
-  float4* vel;
+  real4* vel;
   double* partials;
   Ctrl* ctrl;
 };
@@ -77,9 +77,9 @@ struct PackParams {
   uint32_t n;
   uint32_t mask;   // BEADS_TRAJ_* bits
   uint32_t words;  // record size in 32-bit words
-  const float4* pos;
-  const float4* vel;
-  const float4* force;
+  const real4* pos;
+  const real4* vel;
+  const real4* force;
   const int4* image;
   const uint32_t* tag;
   uint32_t* out;

@@ -12,24 +12,24 @@ namespace {
 
 constexpr double kPiD = 3.14159265358979323846;
 
-bool finite(float x) { return std::isfinite(x); }
+bool finite(double x) { return std::isfinite(x); }
 
 // ------------------------------------------------------------------ pair ----
 
 // pair_lj.cu: one global cutoff, per type pair {sigma^2, 24 eps, 4 eps, shift}
-// in FP64 from the binary32 parameters, plus FP64 copies of epsilon and sigma
+// in FP64 from the real_t parameters, plus FP64 copies of epsilon and sigma
 // for the adjoint.
 class LjPairModel final : public PairModel {
  public:
   LjPairModel(const beads_spec_t& s, const ModelContext&) {
     require(s.n_types >= 1 && s.lj_epsilon && s.lj_sigma,
             "forcefield requires pair_style \"lj\" with coefficients.");
-    require(finite(s.lj_cutoff) && s.lj_cutoff > 0.f,
+    require(finite(s.lj_cutoff) && s.lj_cutoff > 0,
             "Pair(\"lj\") pair_style.cutoff must be finite and positive.");
     for (int32_t k = 0; k < s.n_types * s.n_types; ++k) {
-      require(finite(s.lj_epsilon[k]) && s.lj_epsilon[k] > 0.f,
+      require(finite(s.lj_epsilon[k]) && s.lj_epsilon[k] > 0,
               "Pair(\"lj\") pair_coeff.epsilon must be finite and positive.");
-      require(finite(s.lj_sigma[k]) && s.lj_sigma[k] > 0.f,
+      require(finite(s.lj_sigma[k]) && s.lj_sigma[k] > 0,
               "Pair(\"lj\") pair_coeff.sigma must be finite and positive.");
     }
     n_types_ = s.n_types;
@@ -40,7 +40,7 @@ class LjPairModel final : public PairModel {
     sig_.assign(s.lj_sigma, s.lj_sigma + tt);
   }
   const char* style_name() const override { return "lj"; }
-  float max_cutoff() const override { return cutoff_; }
+  real_t max_cutoff() const override { return cutoff_; }
   int32_t type_count() const override { return n_types_; }
   bool shift_energy() const override { return shift_; }
   void upload(hipStream_t stream) override {
@@ -55,11 +55,11 @@ class LjPairModel final : public PairModel {
     f.n_types = n_types_;
     f.rc2 = cutoff_ * cutoff_;
   }
-  void set_type_pair_parameters(const float* eps, const float* sigma) override {
+  void set_type_pair_parameters(const real_t* eps, const real_t* sigma) override {
     const size_t tt = static_cast<size_t>(n_types_) * n_types_;
     for (size_t q = 0; q < tt; ++q) {
-      require(std::isfinite(eps[q]) && eps[q] >= 0.f, "lj epsilon must be finite and >= 0.");
-      require(std::isfinite(sigma[q]) && sigma[q] > 0.f, "lj sigma must be finite and > 0.");
+      require(std::isfinite(eps[q]) && eps[q] >= 0, "lj epsilon must be finite and >= 0.");
+      require(std::isfinite(sigma[q]) && sigma[q] > 0, "lj sigma must be finite and > 0.");
     }
     eps_.assign(eps, eps + tt);
     sig_.assign(sigma, sigma + tt);
@@ -83,9 +83,9 @@ class LjPairModel final : public PairModel {
 
  private:
   int32_t n_types_ = 0;
-  float cutoff_ = 0.f;
+  real_t cutoff_ = 0;
   bool shift_ = false;
-  std::vector<float> eps_, sig_;
+  std::vector<real_t> eps_, sig_;
   DeviceBuffer<LjCoeff> table_;
   DeviceBuffer<double> eps_d_, sig_d_;
 };
@@ -97,9 +97,9 @@ class HarmonicBondModel final : public ListedModel {
  public:
   HarmonicBondModel(const beads_spec_t& s, const ModelContext&) {
     for (int32_t t = 0; t < s.n_bond_types; ++t) {
-      require(finite(s.bond_k[t]) && s.bond_k[t] >= 0.f,
+      require(finite(s.bond_k[t]) && s.bond_k[t] >= 0,
               "Bond(\"harmonic\") bond_coeff.k must be non-negative.");
-      require(finite(s.bond_r0[t]) && s.bond_r0[t] >= 0.f,
+      require(finite(s.bond_r0[t]) && s.bond_r0[t] >= 0,
               "Bond(\"harmonic\") bond_coeff.r0 must be non-negative.");
     }
     n_ = s.n_bond_types;
@@ -109,10 +109,10 @@ class HarmonicBondModel final : public ListedModel {
   const char* style_name() const override { return "harmonic"; }
   int32_t type_count() const override { return n_; }
   void upload(hipStream_t stream) override {
-    std::vector<float2> c(std::max(1, n_), make_float2(0.f, 0.f));
+    std::vector<real2> c(std::max(1, n_), make_real2(0, 0));
     std::vector<double> k(std::max(1, n_), 0.0), r0(std::max(1, n_), 0.0);
     for (int t = 0; t < n_; ++t) {
-      c[t] = make_float2(k_[t], r0_[t]);
+      c[t] = make_real2(k_[t], r0_[t]);
       k[t] = k_[t];
       r0[t] = r0_[t];
     }
@@ -121,9 +121,9 @@ class HarmonicBondModel final : public ListedModel {
     r0_d_.upload(r0.data(), r0.size(), stream);
   }
   void bind(TopologyView& topo) const override { topo.bond_coeff = coeff_.data(); }
-  void set_parameters(const float* k, const float* r0) override {
+  void set_parameters(const real_t* k, const real_t* r0) override {
     for (int t = 0; t < n_; ++t) {
-      require(std::isfinite(k[t]) && k[t] >= 0.f && std::isfinite(r0[t]) && r0[t] >= 0.f,
+      require(std::isfinite(k[t]) && k[t] >= 0 && std::isfinite(r0[t]) && r0[t] >= 0,
               "bond k and r0 must be finite and non-negative.");
     }
     k_.assign(k, k + n_);
@@ -138,8 +138,8 @@ class HarmonicBondModel final : public ListedModel {
 
  private:
   int32_t n_ = 0;
-  std::vector<float> k_, r0_;
-  DeviceBuffer<float2> coeff_;
+  std::vector<real_t> k_, r0_;
+  DeviceBuffer<real2> coeff_;
   DeviceBuffer<double> k_d_, r0_d_;
 };
 
@@ -148,15 +148,16 @@ class HarmonicAngleModel final : public ListedModel {
  public:
   HarmonicAngleModel(const beads_spec_t& s, const ModelContext&) {
     for (int32_t t = 0; t < s.n_angle_types; ++t) {
-      require(finite(s.angle_k[t]) && s.angle_k[t] >= 0.f,
+      require(finite(s.angle_k[t]) && s.angle_k[t] >= 0,
               "Angle(\"harmonic\") angle_coeff.k must be non-negative.");
-      require(finite(s.angle_theta0_deg[t]) && s.angle_theta0_deg[t] >= 0.f &&
-                  s.angle_theta0_deg[t] <= 180.f,
+      require(finite(s.angle_theta0_deg[t]) && s.angle_theta0_deg[t] >= 0 &&
+                  s.angle_theta0_deg[t] <= 180,
               "Angle(\"harmonic\") angle_coeff.theta0 must be between 0 and 180 degrees.");
     }
     c_.resize(std::max(1, s.n_angle_types));
     for (int t = 0; t < s.n_angle_types; ++t) {
-      c_[t] = make_float2(s.angle_k[t], s.angle_theta0_deg[t] * static_cast<float>(kPiD) / 180.f);
+      c_[t] = make_real2(s.angle_k[t],
+                         s.angle_theta0_deg[t] * static_cast<real_t>(kPiD) / real_t(180));
     }
     n_ = s.n_angle_types;
   }
@@ -167,8 +168,8 @@ class HarmonicAngleModel final : public ListedModel {
 
  private:
   int32_t n_ = 0;
-  std::vector<float2> c_;
-  DeviceBuffer<float2> coeff_;
+  std::vector<real2> c_;
+  DeviceBuffer<real2> coeff_;
 };
 
 // dihedral_harmonic.cu: U = k [1 + d cos(n phi)], d = +-1, n >= 0.
@@ -176,7 +177,7 @@ class HarmonicDihedralModel final : public ListedModel {
  public:
   HarmonicDihedralModel(const beads_spec_t& s, const ModelContext&) {
     for (int32_t t = 0; t < s.n_dihedral_types; ++t) {
-      require(finite(s.dihedral_k[t]) && s.dihedral_k[t] >= 0.f,
+      require(finite(s.dihedral_k[t]) && s.dihedral_k[t] >= 0,
               "Dihedral(\"harmonic\") dihedral_coeff.k must be non-negative.");
       require(s.dihedral_d[t] == 1 || s.dihedral_d[t] == -1,
               "Dihedral(\"harmonic\") dihedral_coeff.d must be +1 or -1.");
@@ -184,8 +185,8 @@ class HarmonicDihedralModel final : public ListedModel {
     }
     c_.resize(std::max(1, s.n_dihedral_types));
     for (int t = 0; t < s.n_dihedral_types; ++t) {
-      c_[t] = make_float4(s.dihedral_k[t], static_cast<float>(s.dihedral_d[t]),
-                          static_cast<float>(s.dihedral_n[t]), 0.f);
+      c_[t] = make_real4(s.dihedral_k[t], static_cast<real_t>(s.dihedral_d[t]),
+                         static_cast<real_t>(s.dihedral_n[t]), 0);
     }
     n_ = s.n_dihedral_types;
   }
@@ -196,8 +197,8 @@ class HarmonicDihedralModel final : public ListedModel {
 
  private:
   int32_t n_ = 0;
-  std::vector<float4> c_;
-  DeviceBuffer<float4> coeff_;
+  std::vector<real4> c_;
+  DeviceBuffer<real4> coeff_;
 };
 
 // ------------------------------------------------------ dynamics ------------
@@ -206,17 +207,17 @@ class HarmonicDihedralModel final : public ListedModel {
 class VelocityVerletIntegrator final : public Integrator {
  public:
   VelocityVerletIntegrator(const beads_spec_t& s, const ModelContext&) : dt_(s.dt) {
-    require(finite(s.dt) && s.dt > 0.f, "VelocityVerletIntegrator dt must be finite and positive.");
+    require(finite(s.dt) && s.dt > 0, "VelocityVerletIntegrator dt must be finite and positive.");
   }
   const char* style_name() const override { return "velocity_verlet"; }
-  float dt() const override { return dt_; }
+  real_t dt() const override { return dt_; }
   void bind(IntegrateParams& in, ForceParams& f) const override {
     in.dt = dt_;
     f.dt = dt_;
   }
 
  private:
-  float dt_;
+  real_t dt_;
 };
 
 // berendsen_thermostat.cu:104-251: lambda from the kinetic energy after the
@@ -224,15 +225,15 @@ class VelocityVerletIntegrator final : public Integrator {
 class BerendsenThermostat final : public Thermostat {
  public:
   BerendsenThermostat(const beads_spec_t& s, const ModelContext& ctx) {
-    require(finite(s.berendsen_temperature) && s.berendsen_temperature >= 0.f,
+    require(finite(s.berendsen_temperature) && s.berendsen_temperature >= 0,
             "Thermostat(\"berendsen\").temperature must be non-negative.");
-    require(finite(s.berendsen_tau) && s.berendsen_tau > 0.f,
+    require(finite(s.berendsen_tau) && s.berendsen_tau > 0,
             "Thermostat(\"berendsen\").tau must be positive.");
     target_thermal_ = ctx.kb * static_cast<double>(s.berendsen_temperature);
     tau_ = s.berendsen_tau;
   }
   const char* style_name() const override { return "berendsen"; }
-  void bind(ForceParams& f, float dt) const override {
+  void bind(ForceParams& f, real_t dt) const override {
     f.berendsen = 1;
     f.target_thermal = target_thermal_;
     f.dt_over_tau = static_cast<double>(dt) / tau_;
@@ -240,7 +241,7 @@ class BerendsenThermostat final : public Thermostat {
 
  private:
   double target_thermal_ = 0.0;
-  float tau_ = 1.f;
+  real_t tau_ = 1;
 };
 
 template <class Model, class Impl>
@@ -265,7 +266,7 @@ class ListedRegistry : public ModelRegistry<ListedModel> {
 
 }  // namespace
 
-void ListedModel::set_parameters(const float*, const float*) {
+void ListedModel::set_parameters(const real_t*, const real_t*) {
   fail(BEADS_E_VALUE, std::string("the ") + style_name() +
                           " style has no parameters that frame mode can update.");
 }
@@ -331,10 +332,10 @@ ModelSet create_models(const beads_spec_t& s, const ModelContext& ctx) {
 }
 
 // pair_lj.cu:171-192: {sigma^2, 24 eps, 4 eps, shift} per ordered type pair in
-// FP64 from the binary32 parameters; shift = 4 eps ((sigma/rc)^12 - (sigma/rc)^6)
+// FP64 from the real_t parameters; shift = 4 eps ((sigma/rc)^12 - (sigma/rc)^6)
 // when the energy is shifted.
-std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                   float cutoff, bool shift_energy) {
+std::vector<LjCoeff> pack_lj_table(int32_t n_types, const real_t* epsilon, const real_t* sigma,
+                                   real_t cutoff, bool shift_energy) {
   const int T = n_types;
   std::vector<LjCoeff> lj(static_cast<size_t>(T) * T);
   const double rc = cutoff;

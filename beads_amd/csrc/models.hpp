@@ -39,13 +39,13 @@ class PairModel {
  public:
   virtual ~PairModel() = default;
   virtual const char* style_name() const = 0;
-  virtual float max_cutoff() const = 0;
+  virtual real_t max_cutoff() const = 0;
   virtual int32_t type_count() const = 0;
   virtual bool shift_energy() const = 0;
   virtual void upload(hipStream_t stream) = 0;
   virtual void bind(ForceParams& f) const = 0;
-  // Frame mode: new per-type-pair parameters (binary32, [T][T]).
-  virtual void set_type_pair_parameters(const float* a, const float* b) = 0;
+  // Frame mode: new per-type-pair parameters (real_t, [T][T]).
+  virtual void set_type_pair_parameters(const real_t* a, const real_t* b) = 0;
   // The adjoint's view of the parameters; throws if the adjoint cannot
   // differentiate this model in its current state.
   virtual void bind_adjoint(AdjointParams& p) const = 0;
@@ -60,9 +60,9 @@ class ListedModel {
   virtual int32_t type_count() const = 0;
   virtual void upload(hipStream_t stream) = 0;
   virtual void bind(TopologyView& topo) const = 0;
-  // Frame mode / adjoint (bonds only): new binary32 parameters and the
+  // Frame mode / adjoint (bonds only): new real_t parameters and the
   // adjoint's FP64 view of them.
-  virtual void set_parameters(const float* a, const float* b);
+  virtual void set_parameters(const real_t* a, const real_t* b);
   virtual void bind_adjoint(AdjointParams& p) const;
 };
 
@@ -72,7 +72,7 @@ class Integrator {
  public:
   virtual ~Integrator() = default;
   virtual const char* style_name() const = 0;
-  virtual float dt() const = 0;
+  virtual real_t dt() const = 0;
   virtual void bind(IntegrateParams& in, ForceParams& f) const = 0;
 };
 
@@ -82,7 +82,7 @@ class Thermostat {
  public:
   virtual ~Thermostat() = default;
   virtual const char* style_name() const = 0;
-  virtual void bind(ForceParams& f, float dt) const = 0;
+  virtual void bind(ForceParams& f, real_t dt) const = 0;
 };
 
 template <class Model>
@@ -137,8 +137,8 @@ const char* integrator_style_of(const beads_spec_t& spec);
 const char* thermostat_style_of(const beads_spec_t& spec);  // nullptr: none
 
 // pair_lj.cu:171-192 coefficient packing, shared with the batched replicas.
-std::vector<LjCoeff> pack_lj_table(int32_t n_types, const float* epsilon, const float* sigma,
-                                   float cutoff, bool shift_energy);
+std::vector<LjCoeff> pack_lj_table(int32_t n_types, const real_t* epsilon, const real_t* sigma,
+                                   real_t cutoff, bool shift_energy);
 std::vector<LjCoeff> pack_lj_table(const beads_spec_t& spec);
 
 }  // namespace beads_amd

@@ -1,4 +1,6 @@
-// pybind11 module beads_amd._native: build_info() and execute_simulation(spec).
+// pybind11 module beads_amd._native (beads_amd._native_real64 when compiled
+// with BEADS_REAL64 against libbeads_amd_real64.so): build_info() and
+// execute_simulation(spec). Real arrays must have the build's real dtype.
 //
 // Counterpart of native/src/bindings/module.cpp:15-55 and the dict parser
 // native/src/bindings/native_spec_parser.cpp:163-791 (array checks as in
@@ -22,6 +24,8 @@
 namespace py = pybind11;
 
 namespace {
+
+constexpr const char* kRealDtypeName = sizeof(beads_real_t) == 8 ? "float64" : "float32";
 
 struct TypeErr : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -70,7 +74,9 @@ double require_real(const py::handle& v, const std::string& ctx) {
   }
   const double x = py::cast<double>(v);
   if (!std::isfinite(x)) value_error(ctx + " must be finite.");
-  if (!std::isfinite(static_cast<float>(x))) value_error(ctx + " must be representable as float32.");
+  if (!std::isfinite(static_cast<beads_real_t>(x))) {
+    value_error(ctx + " must be representable as " + kRealDtypeName + ".");
+  }
   return x;
 }
 
@@ -142,7 +148,7 @@ struct ParsedSpec {
   beads_spec_t c{};
   std::vector<py::object> keep;  // numpy arrays borrowed for the call
   std::vector<uint32_t> bonds, angles, dihedrals;
-  std::vector<float> lj_eps, lj_sig, bond_k, bond_r0, angle_k, angle_theta0, dih_k;
+  std::vector<beads_real_t> lj_eps, lj_sig, bond_k, bond_r0, angle_k, angle_theta0, dih_k;
   std::vector<int32_t> dih_d, dih_n;
   std::string thermo_path, trajectory_path, final_state_path, log_path;
 };
@@ -159,10 +165,10 @@ void parse_system(const py::dict& sys, ParsedSpec& out) {
   if (n == 0 || n > std::numeric_limits<uint32_t>::max()) value_error("system.n_particles must be in 1..2^32-1.");
   out.c.n_particles = static_cast<uint32_t>(n);
   const py::ssize_t N = static_cast<py::ssize_t>(n);
-  auto bb = require_array<float>(require_item(sys, "box_bound", "system"), {2, 3}, "system.box_bound");
-  auto pos = require_array<float>(require_item(sys, "positions", "system"), {N, 3}, "system.positions");
-  auto vel = require_array<float>(require_item(sys, "velocities", "system"), {N, 3}, "system.velocities");
-  auto mass = require_array<float>(require_item(sys, "masses", "system"), {N}, "system.masses");
+  auto bb = require_array<beads_real_t>(require_item(sys, "box_bound", "system"), {2, 3}, "system.box_bound");
+  auto pos = require_array<beads_real_t>(require_item(sys, "positions", "system"), {N, 3}, "system.positions");
+  auto vel = require_array<beads_real_t>(require_item(sys, "velocities", "system"), {N, 3}, "system.velocities");
+  auto mass = require_array<beads_real_t>(require_item(sys, "masses", "system"), {N}, "system.masses");
   auto types = require_array<int32_t>(require_item(sys, "types", "system"), {N}, "system.types");
   auto tags = require_array<uint32_t>(require_item(sys, "tags", "system"), {N}, "system.tags");
   auto mol = require_array<uint32_t>(require_item(sys, "molecule_ids", "system"), {N}, "system.molecule_ids");
@@ -248,7 +254,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
   const Params pp = read_params(require_dict(require_item(ps, "params", "pair_style"), "pair_style.params"),
                                 "pair_style");
   require_param_keys(pp, {"cutoff"}, {"shift"}, "pair_style");
-  out.c.lj_cutoff = static_cast<float>(positive_param(pp, "cutoff", "pair_style"));
+  out.c.lj_cutoff = static_cast<beads_real_t>(positive_param(pp, "cutoff", "pair_style"));
   out.c.lj_shift = 0;
   if (pp.values.count("shift")) {
     if (!is_bool(pp.values.at("shift"))) type_error("pair_style.shift must be a bool.");
@@ -273,8 +279,8 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     const Params p = read_params(require_dict(require_item(d, "params", "pair_coeff"), "pair_coeff.params"),
                                  "pair_coeff");
     require_param_keys(p, {"epsilon", "sigma"}, {}, "pair_coeff");
-    const float eps = static_cast<float>(positive_param(p, "epsilon", "pair_coeff"));
-    const float sig = static_cast<float>(positive_param(p, "sigma", "pair_coeff"));
+    const beads_real_t eps = static_cast<beads_real_t>(positive_param(p, "epsilon", "pair_coeff"));
+    const beads_real_t sig = static_cast<beads_real_t>(positive_param(p, "sigma", "pair_coeff"));
     const size_t ij = (a - 1) * T + (b - 1), ji = (b - 1) * T + (a - 1);
     if (filled[ij]) value_error("Pair(\"lj\") pair_coeff duplicate coeff for type pair.");
     filled[ij] = filled[ji] = 1;
@@ -292,8 +298,8 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     const auto m = typed_coeffs(require_item(ff, "bond_coeffs", "forcefield"), "harmonic", "bond_coeffs");
     for (const auto& kv : m) {
       require_param_keys(kv.second, {"k", "r0"}, {}, "bond_coeff");
-      out.bond_k.push_back(static_cast<float>(nonnegative_param(kv.second, "k", "bond_coeff")));
-      out.bond_r0.push_back(static_cast<float>(nonnegative_param(kv.second, "r0", "bond_coeff")));
+      out.bond_k.push_back(static_cast<beads_real_t>(nonnegative_param(kv.second, "k", "bond_coeff")));
+      out.bond_r0.push_back(static_cast<beads_real_t>(nonnegative_param(kv.second, "r0", "bond_coeff")));
     }
   }
   if (!optional_style(ff, "angle_style", "harmonic").empty()) {
@@ -301,10 +307,10 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
     const auto m = typed_coeffs(require_item(ff, "angle_coeffs", "forcefield"), "harmonic", "angle_coeffs");
     for (const auto& kv : m) {
       require_param_keys(kv.second, {"k", "theta0"}, {}, "angle_coeff");
-      out.angle_k.push_back(static_cast<float>(nonnegative_param(kv.second, "k", "angle_coeff")));
+      out.angle_k.push_back(static_cast<beads_real_t>(nonnegative_param(kv.second, "k", "angle_coeff")));
       const double th = nonnegative_param(kv.second, "theta0", "angle_coeff");
       if (th > 180.0) value_error("Angle(\"harmonic\") angle_coeff.theta0 must be between 0 and 180 degrees.");
-      out.angle_theta0.push_back(static_cast<float>(th));
+      out.angle_theta0.push_back(static_cast<beads_real_t>(th));
     }
   }
   if (!optional_style(ff, "dihedral_style", "harmonic").empty()) {
@@ -313,7 +319,7 @@ void parse_forcefield(const py::dict& ff, ParsedSpec& out, int32_t n_types) {
                                 "dihedral_coeffs");
     for (const auto& kv : m) {
       require_param_keys(kv.second, {"k", "d", "n"}, {}, "dihedral_coeff");
-      out.dih_k.push_back(static_cast<float>(nonnegative_param(kv.second, "k", "dihedral_coeff")));
+      out.dih_k.push_back(static_cast<beads_real_t>(nonnegative_param(kv.second, "k", "dihedral_coeff")));
       const int64_t d = require_int(kv.second.values.at("d"), "dihedral_coeff.d");
       if (d != 1 && d != -1) value_error("Dihedral(\"harmonic\") dihedral_coeff.d must be +1 or -1.");
       const int64_t nn = require_int(kv.second.values.at("n"), "dihedral_coeff.n");
@@ -377,7 +383,7 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
   const Params p = read_params(require_dict(require_item(dyn, "params", "dynamics"), "dynamics.params"),
                                "dynamics");
   require_param_keys(p, {"dt"}, {}, "Dynamics(\"velocity_verlet\")");
-  out.c.dt = static_cast<float>(positive_param(p, "dt", "dynamics"));
+  out.c.dt = static_cast<beads_real_t>(positive_param(p, "dt", "dynamics"));
   out.c.integrator_style = "velocity_verlet";
   out.c.berendsen = 0;
   if (dyn.contains("thermostat") && !dyn["thermostat"].is_none()) {
@@ -390,8 +396,8 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
     out.c.berendsen = 1;
     out.c.thermostat_style = "berendsen";
     out.c.berendsen_temperature =
-        static_cast<float>(nonnegative_param(tp, "temperature", "Thermostat(\"berendsen\")"));
-    out.c.berendsen_tau = static_cast<float>(positive_param(tp, "tau", "Thermostat(\"berendsen\")"));
+        static_cast<beads_real_t>(nonnegative_param(tp, "temperature", "Thermostat(\"berendsen\")"));
+    out.c.berendsen_tau = static_cast<beads_real_t>(positive_param(tp, "tau", "Thermostat(\"berendsen\")"));
   }
 }
 
@@ -408,7 +414,7 @@ void parse_neighbor(const py::dict& nb, ParsedSpec& out) {
   }
   const double buf = require_real(require_item(nb, "cutoff_buffer", "neighbor"), "neighbor.cutoff_buffer");
   if (buf < 0.0) value_error("neighbor cutoff_buffer must be finite and non-negative.");
-  out.c.cutoff_buffer = static_cast<float>(buf);
+  out.c.cutoff_buffer = static_cast<beads_real_t>(buf);
   out.c.rebuild_check_every =
       require_uint64(require_item(nb, "rebuild_check_every", "neighbor"), "neighbor.rebuild_check_every");
   out.c.sort_every_rebuild =
@@ -573,7 +579,11 @@ void execute_simulation(const py::object& spec_obj) {
 
 }  // namespace
 
+#ifdef BEADS_REAL64
+PYBIND11_MODULE(_native_real64, m) {
+#else
 PYBIND11_MODULE(_native, m) {
+#endif
   m.doc() = "beads_amd native extension (MI355X engine)";
   m.def("build_info", &build_info);
   m.def("execute_simulation", &execute_simulation);

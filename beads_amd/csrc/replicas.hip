@@ -69,15 +69,15 @@ template <int BLOCK>
 __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, ReplicaState* st,
                                 ReplicaShared<BLOCK>& sh) {
   const uint32_t n = d.n, tid = threadIdx.x;
-  float4* pos = A.pos + d.atom0;
-  float4* vel = A.vel + d.atom0;
+  real4* pos = A.pos + d.atom0;
+  real4* vel = A.vel + d.atom0;
   int4* image = A.image + d.atom0;
   uint32_t* tag = A.tag + d.atom0;
   uint32_t* cell_of = A.cell_of + d.atom0;
   uint32_t* cell_atoms = A.cell_atoms + d.atom0;
   uint32_t* nbr_count = A.nbr_count + d.atom0;
   uint32_t* nbr = A.nbr + d.nbr0;
-  float4* ref_pos = A.ref_pos + d.atom0;
+  real4* ref_pos = A.ref_pos + d.atom0;
   int4* ref_image = A.ref_image + d.atom0;
   const uint32_t nc = d.grid.n_cells;
   uint32_t* count = A.cell_count + d.cell0;
@@ -120,8 +120,8 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
     }
     __syncthreads();
     if (reorder) {
-      float4* s_pos = A.s_pos + d.atom0;
-      float4* s_vel = A.s_vel + d.atom0;
+      real4* s_pos = A.s_pos + d.atom0;
+      real4* s_vel = A.s_vel + d.atom0;
       int4* s_img = A.s_image + d.atom0;
       uint32_t* s_tag = A.s_tag + d.atom0;
       uint32_t* s_cell = A.s_cell_of + d.atom0;
@@ -145,17 +145,17 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
       __syncthreads();
     }
   }
-  const float L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
-  const float h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
+  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
+  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
   uint32_t overflow = 0;
   for (uint32_t i = tid; i < n; i += BLOCK) {
-    const float4 xi = pos[i];
+    const real4 xi = pos[i];
     uint32_t cnt = 0;
     auto visit = [&](uint32_t j) {
-      const float4 xj = pos[j];
-      const float dx = mic(xi.x - xj.x, L0, h0);
-      const float dy = mic(xi.y - xj.y, L1, h1);
-      const float dz = mic(xi.z - xj.z, L2, h2);
+      const real4 xj = pos[j];
+      const real_t dx = mic(xi.x - xj.x, L0, h0);
+      const real_t dy = mic(xi.y - xj.y, L1, h1);
+      const real_t dz = mic(xi.z - xj.z, L2, h2);
       if (r2_strict(dx, dy, dz) > d.rs2 || j == i) return;
       if (cnt < d.max_neighbors) {
         nbr[static_cast<size_t>(cnt) * n + i] = j;
@@ -198,13 +198,13 @@ template <int BLOCK>
 __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool kick, bool single,
                               const LjCoeff* lj, double& ke, double& pe, double& vir) {
   const uint32_t n = d.n;
-  const float4* pos = A.pos + d.atom0;
-  float4* vel = A.vel + d.atom0;
-  float4* force = A.force + d.atom0;
+  const real4* pos = A.pos + d.atom0;
+  real4* vel = A.vel + d.atom0;
+  real4* force = A.force + d.atom0;
   const uint32_t* nbr_count = A.nbr_count + d.atom0;
   const uint32_t* nbr = A.nbr + d.nbr0;
-  const float L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
-  const float h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
+  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
+  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
   const int T = d.n_types;
   const LjCoeff c0 = lj[0];
   // A team of kReplicaTeam adjacent lanes per atom: lane t takes list slots
@@ -212,8 +212,8 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
   // team's FP64 partial forces meet in a butterfly and lane 0 stores.
   const uint32_t t = threadIdx.x & (kReplicaTeam - 1);
   for (uint32_t i = threadIdx.x / kReplicaTeam; i < n; i += BLOCK / kReplicaTeam) {
-    const float4 xi = pos[i];
-    const int ti = __float_as_int(xi.w);
+    const real4 xi = pos[i];
+    const int ti = static_cast<int>(id_of(xi.w));
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
     const uint32_t cnt = nbr_count[i];
     for (uint32_t base = 0; base < cnt; base += kReplicaTeam * kReplicaSlots) {
@@ -223,18 +223,18 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
         const uint32_t s = base + u * kReplicaTeam + t;
         jj[u] = s < cnt ? nbr[static_cast<size_t>(s) * n + i] : i;
       }
-      float4 xj[kReplicaSlots];
+      real4 xj[kReplicaSlots];
 #pragma unroll
       for (int u = 0; u < kReplicaSlots; ++u) {
         xj[u] = pos[jj[u]];
       }
 #pragma unroll
       for (int u = 0; u < kReplicaSlots; ++u) {
-        const float dx = mic_diff(xi.x, xj[u].x, L0, h0);
-        const float dy = mic_diff(xi.y, xj[u].y, L1, h1);
-        const float dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
+        const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
+        const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
         if (base + u * kReplicaTeam + t < cnt && r2_strict(dx, dy, dz) < d.rc2) {
-          const LjCoeff c = single ? c0 : lj[ti * T + __float_as_int(xj[u].w)];
+          const LjCoeff c = single ? c0 : lj[ti * T + static_cast<int>(id_of(xj[u].w))];
           double rr, uu;
           const double s = lj_pair(dx, dy, dz, c, rr, uu);
           fx += s * dx;
@@ -253,12 +253,12 @@ __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool
     if (t != 0) {
       continue;
     }
-    const float ffx = static_cast<float>(fx), ffy = static_cast<float>(fy),
-                ffz = static_cast<float>(fz);
-    force[i] = make_float4(ffx, ffy, ffz, 0.f);
-    float4 v = vel[i];
+    const real_t ffx = static_cast<real_t>(fx), ffy = static_cast<real_t>(fy),
+                ffz = static_cast<real_t>(fz);
+    force[i] = make_real4(ffx, ffy, ffz, 0);
+    real4 v = vel[i];
     if (kick) {
-      const float h = 0.5f * d.dt / v.w;
+      const real_t h = real_t(0.5) * d.dt / v.w;
       v.x += h * ffx;
       v.y += h * ffy;
       v.z += h * ffz;
@@ -302,41 +302,41 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
     __syncthreads();
   }
 
-  float4* pos = A.pos + d.atom0;
-  float4* vel = A.vel + d.atom0;
-  const float4* force = A.force + d.atom0;
+  real4* pos = A.pos + d.atom0;
+  real4* vel = A.vel + d.atom0;
+  const real4* force = A.force + d.atom0;
   int4* image = A.image + d.atom0;
-  const float4* ref_pos = A.ref_pos + d.atom0;
+  const real4* ref_pos = A.ref_pos + d.atom0;
   const int4* ref_image = A.ref_image + d.atom0;
   uint64_t step = st->step;
   for (uint64_t s = 0; s < n_steps; ++s) {
     const uint64_t physical = step + 1;
-    const bool every = d.cutoff_buffer == 0.f;
+    const bool every = d.cutoff_buffer == real_t(0);
     const bool check = !every && physical % d.check_every == 0;
-    const float lam = static_cast<float>(sh.lambda);
+    const real_t lam = static_cast<real_t>(sh.lambda);
     int exceeded = 0;
     for (uint32_t a = tid; a < n; a += BLOCK) {
-      float4 v = vel[a];
-      const float4 f = force[a];
-      float4 x = pos[a];
+      real4 v = vel[a];
+      const real4 f = force[a];
+      real4 x = pos[a];
       if (d.berendsen) {
         v.x *= lam;
         v.y *= lam;
         v.z *= lam;
       }
-      const float h = 0.5f * d.dt / v.w;
+      const real_t h = real_t(0.5) * d.dt / v.w;
       v.x += h * f.x;
       v.y += h * f.y;
       v.z += h * f.z;
       x.x += d.dt * v.x;
       x.y += d.dt * v.y;
       x.z += d.dt * v.z;
-      float c[3];
+      real_t c[3];
       {
 #pragma clang fp contract(off)
-        c[0] = floorf((x.x - d.box.lo[0]) * d.box.invL[0]);
-        c[1] = floorf((x.y - d.box.lo[1]) * d.box.invL[1]);
-        c[2] = floorf((x.z - d.box.lo[2]) * d.box.invL[2]);
+        c[0] = floor((x.x - d.box.lo[0]) * d.box.invL[0]);
+        c[1] = floor((x.y - d.box.lo[1]) * d.box.invL[1]);
+        c[2] = floor((x.z - d.box.lo[2]) * d.box.invL[2]);
         x.x -= c[0] * d.box.L[0];
         x.y -= c[1] * d.box.L[1];
         x.z -= c[2] * d.box.L[2];
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
       pos[a] = x;
       vel[a] = v;
       int4 im = image[a];
-      if (c[0] != 0.f || c[1] != 0.f || c[2] != 0.f) {
+      if (c[0] != real_t(0) || c[1] != real_t(0) || c[2] != real_t(0)) {
         im.x += static_cast<int>(c[0]);
         im.y += static_cast<int>(c[1]);
         im.z += static_cast<int>(c[2]);
@@ -352,11 +352,11 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
       }
       if (check) {
 #pragma clang fp contract(off)
-        const float4 r = ref_pos[a];
+        const real4 r = ref_pos[a];
         const int4 ri = ref_image[a];
-        const float dx = (x.x - r.x) + static_cast<float>(im.x - ri.x) * d.box.L[0];
-        const float dy = (x.y - r.y) + static_cast<float>(im.y - ri.y) * d.box.L[1];
-        const float dz = (x.z - r.z) + static_cast<float>(im.z - ri.z) * d.box.L[2];
+        const real_t dx = (x.x - r.x) + static_cast<real_t>(im.x - ri.x) * d.box.L[0];
+        const real_t dy = (x.y - r.y) + static_cast<real_t>(im.y - ri.y) * d.box.L[1];
+        const real_t dz = (x.z - r.z) + static_cast<real_t>(im.z - ri.z) * d.box.L[2];
         if (dx * dx + dy * dy + dz * dz > d.thr2) exceeded = 1;
       }
     }

@@ -18,8 +18,8 @@ struct ReplicaDesc {
   uint64_t nbr0;          // first neighbour-index entry (max_neighbors * n, slot-major)
   BoxView box;
   CellGridView grid;
-  float rs2, rc2, thr2, dt;
-  float cutoff_buffer;
+  real_t rs2, rc2, thr2, dt;
+  real_t cutoff_buffer;
   int32_t berendsen;
   int32_t n_types;
   uint32_t lj0;           // first LJ table entry (n_types^2 entries)
@@ -47,12 +47,12 @@ struct ReplicaState {
 };
 
 struct ReplicaArrays {
-  float4* pos;
-  float4* vel;
-  float4* force;
+  real4* pos;
+  real4* vel;
+  real4* force;
   int4* image;
   uint32_t* tag;
-  float4* ref_pos;
+  real4* ref_pos;
   int4* ref_image;
   uint32_t* cell_of;
   uint32_t* cell_atoms;
@@ -61,8 +61,8 @@ struct ReplicaArrays {
   uint32_t* cell_count;
   uint32_t* cell_start;
   uint32_t* cell_cursor;
-  float4* s_pos;
-  float4* s_vel;
+  real4* s_pos;
+  real4* s_vel;
   int4* s_image;
   uint32_t* s_tag;
   uint32_t* s_cell_of;

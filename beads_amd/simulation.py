@@ -101,14 +101,17 @@ def _log_settings(echo: object, prefix: object | None) -> dict:
 
 
 def load_native():
-    """The compiled engine binding; fails loudly when it is not built."""
+    """The compiled engine binding of the selected precision (BEADS_AMD_PRECISION);
+    fails loudly when it is not built."""
+    from ._build_config import native_module_name
     from ._runtime import share_hip_runtime
 
     share_hip_runtime()
+    name = native_module_name()
     try:
-        return import_module("beads_amd._native")
+        return import_module(name)
     except ModuleNotFoundError as exc:
-        if exc.name != "beads_amd._native":
+        if exc.name != name:
             raise
         raise RuntimeError(
             "beads_amd native extension is not built. Run __graft_entry__.build() "

@@ -1,7 +1,8 @@
 """Seeded synthetic workloads for the benchmark configurations and tests.
 
 A ``Workload`` holds every array and parameter of one run in the engine's
-dtypes. It can become a ``beads_amd.Simulation`` (the public API), an engine
+dtypes: reals in ``real`` ("float32", or "float64" for the BEADS_REAL64
+build's runs; generators make binary32 values, which widen exactly). It can become a ``beads_amd.Simulation`` (the public API), an engine
 handle through the C ABI (``beads_amd.capi.Engine``), or an oracle system.
 
 Generators (sizes are arguments so tests can shrink them):
@@ -59,16 +60,20 @@ class Workload:
     half_list: bool = False
     steps: int = 100
     seed: int = 0
+    real: str = "float32"
 
     def __post_init__(self) -> None:
         n = self.positions.shape[0]
-        self.positions = np.ascontiguousarray(self.positions, np.float32)
-        self.velocities = np.ascontiguousarray(self.velocities, np.float32)
-        self.masses = np.ascontiguousarray(self.masses, np.float32)
+        real = np.dtype(self.real)
+        if real not in (np.dtype(np.float32), np.dtype(np.float64)):
+            raise ValueError("Workload.real must be float32 or float64")
+        self.positions = np.ascontiguousarray(self.positions, real)
+        self.velocities = np.ascontiguousarray(self.velocities, real)
+        self.masses = np.ascontiguousarray(self.masses, real)
         self.types = np.ascontiguousarray(self.types, np.int32)
-        self.box_bound = np.ascontiguousarray(self.box_bound, np.float32)
-        self.lj_epsilon = np.ascontiguousarray(self.lj_epsilon, np.float32)
-        self.lj_sigma = np.ascontiguousarray(self.lj_sigma, np.float32)
+        self.box_bound = np.ascontiguousarray(self.box_bound, real)
+        self.lj_epsilon = np.ascontiguousarray(self.lj_epsilon, real)
+        self.lj_sigma = np.ascontiguousarray(self.lj_sigma, real)
         if self.tags is None:
             self.tags = np.arange(1, n + 1, dtype=np.uint32)
         if self.molecule_ids is None:
@@ -79,8 +84,8 @@ class Workload:
         self.molecule_ids = np.ascontiguousarray(self.molecule_ids, np.uint32)
         self.images = np.ascontiguousarray(self.images, np.int32)
         for name, dt in (("bonds", np.uint32), ("angles", np.uint32), ("dihedrals", np.uint32),
-                         ("bond_k", np.float32), ("bond_r0", np.float32), ("angle_k", np.float32),
-                         ("angle_theta0", np.float32), ("dihedral_k", np.float32),
+                         ("bond_k", real), ("bond_r0", real), ("angle_k", real),
+                         ("angle_theta0", real), ("dihedral_k", real),
                          ("dihedral_d", np.int32), ("dihedral_n", np.int32)):
             setattr(self, name, np.ascontiguousarray(getattr(self, name), dt))
 
@@ -98,6 +103,20 @@ class Workload:
 
     def with_(self, **changes) -> "Workload":
         return replace(self, **changes)
+
+    def as_float64(self, jitter: float = 0.0, seed: int = 0) -> "Workload":
+        """The same run with binary64 reals; jitter > 0 adds uniform
+        [-jitter, jitter) offsets to the positions in binary64 (kept inside the
+        box), so that they are no longer binary32 values."""
+        pos = self.positions.astype(np.float64)
+        if jitter > 0.0:
+            rng = np.random.default_rng(seed)
+            lo = self.box_bound[0].astype(np.float64)
+            hi = self.box_bound[1].astype(np.float64)
+            pos = pos + rng.uniform(-jitter, jitter, pos.shape)
+            pos = lo + np.mod(pos - lo, hi - lo)
+            pos = np.where(pos >= hi, lo, pos)
+        return replace(self, real="float64", positions=pos)
 
     def to_simulation(self):
         """The same run expressed through the public builders."""

@@ -13,6 +13,15 @@
  *                              advance_dynamics_step (simulation_run.cpp:60-133, 212-235),
  *                              split so tests and the bench can drive and inspect a run
  *   beads_spec_t            <- input::SimulationSpec (native/src/input/native_spec.hpp:16-176)
+ *   beads_real_t            <- beads::real_t (native/include/beads/core/types.hpp:12-16)
+ *
+ * Precision is chosen at build time, like the reference's BEADS_REAL64 switch
+ * (CMakeLists.txt:29-31): libbeads_amd.so stores positions, velocities,
+ * forces and every real-valued parameter in binary32; libbeads_amd_real64.so
+ * is built with BEADS_REAL64 defined and stores them in binary64. A program
+ * that links or loads the binary64 library compiles this header with
+ * BEADS_REAL64 defined; beads_build_info reports which precision a library
+ * was built with.
  */
 #ifndef BEADS_AMD_H
 #define BEADS_AMD_H
@@ -25,6 +34,12 @@ extern "C" {
 #endif
 
 #define BEADS_AMD_ABI_VERSION 3
+
+#ifdef BEADS_REAL64
+typedef double beads_real_t;
+#else
+typedef float beads_real_t;
+#endif
 
 typedef enum beads_status {
   BEADS_OK = 0,
@@ -75,10 +90,10 @@ typedef struct beads_spec {
   /* system */
   int32_t units;                 /* beads_units_t */
   uint32_t n_particles;
-  const float* box_bound;        /* [2][3] lower row, upper row */
-  const float* positions;        /* [n][3] */
-  const float* velocities;       /* [n][3] */
-  const float* masses;           /* [n] */
+  const beads_real_t* box_bound; /* [2][3] lower row, upper row */
+  const beads_real_t* positions; /* [n][3] */
+  const beads_real_t* velocities;/* [n][3] */
+  const beads_real_t* masses;    /* [n] */
   const int32_t* types;          /* [n], dense 1-based */
   const uint32_t* tags;          /* [n], permutation of 1..n */
   const uint32_t* molecule_ids;  /* [n] */
@@ -106,33 +121,33 @@ typedef struct beads_spec {
 
   /* pair style "lj" */
   int32_t n_types;
-  float lj_cutoff;
+  beads_real_t lj_cutoff;
   int32_t lj_shift;
-  const float* lj_epsilon;       /* [n_types][n_types], symmetric */
-  const float* lj_sigma;         /* [n_types][n_types], symmetric */
+  const beads_real_t* lj_epsilon; /* [n_types][n_types], symmetric */
+  const beads_real_t* lj_sigma;  /* [n_types][n_types], symmetric */
 
   /* listed styles "harmonic"; a count of 0 means the style is absent */
   int32_t n_bond_types;
-  const float* bond_k;           /* [n_bond_types] */
-  const float* bond_r0;
+  const beads_real_t* bond_k;    /* [n_bond_types] */
+  const beads_real_t* bond_r0;
   int32_t n_angle_types;
-  const float* angle_k;
-  const float* angle_theta0_deg;
+  const beads_real_t* angle_k;
+  const beads_real_t* angle_theta0_deg;
   int32_t n_dihedral_types;
-  const float* dihedral_k;
+  const beads_real_t* dihedral_k;
   const int32_t* dihedral_d;     /* +1 or -1 */
   const int32_t* dihedral_n;     /* >= 0 */
   int32_t exclusion_distance;    /* 0 none, else 1..3 */
   int32_t exclusion_policy;      /* beads_exclusion_policy_t */
 
   /* dynamics "velocity_verlet" + optional thermostat "berendsen" */
-  float dt;
+  beads_real_t dt;
   int32_t berendsen;
-  float berendsen_temperature;
-  float berendsen_tau;
+  beads_real_t berendsen_temperature;
+  beads_real_t berendsen_tau;
 
   /* neighbor */
-  float cutoff_buffer;
+  beads_real_t cutoff_buffer;
   uint64_t rebuild_check_every;
   uint64_t sort_every_rebuild;
   uint32_t max_neighbors;
@@ -157,8 +172,8 @@ typedef struct beads_spec {
 } beads_spec_t;
 
 typedef struct beads_build_info {
-  const char* real_precision;  /* "single" */
-  const char* real_dtype;      /* "float32" */
+  const char* real_precision;  /* "single", or "double" in the BEADS_REAL64 build */
+  const char* real_dtype;      /* "float32" / "float64" */
   const char* type_dtype;      /* "int32" */
   const char* index_dtype;     /* "uint32" */
   const char* image_dtype;     /* "int32" */
@@ -196,14 +211,14 @@ typedef struct beads_engine_stats {
 
 /* Host-side views filled by beads_engine_read_state (each pointer may be NULL). */
 typedef struct beads_state_out {
-  float* positions;    /* [n][3] slot order */
-  float* velocities;   /* [n][3] */
-  float* forces;       /* [n][3] */
-  int32_t* images;     /* [n][3] */
-  uint32_t* tags;      /* [n] */
-  int32_t* types;      /* [n] */
-  uint32_t* molecule_ids; /* [n] */
-  float* masses;       /* [n] */
+  beads_real_t* positions;       /* [n][3] slot order */
+  beads_real_t* velocities;      /* [n][3] */
+  beads_real_t* forces;          /* [n][3] */
+  int32_t* images;               /* [n][3] */
+  uint32_t* tags;                /* [n] */
+  int32_t* types;                /* [n] */
+  uint32_t* molecule_ids;        /* [n] */
+  beads_real_t* masses;          /* [n] */
 } beads_state_out_t;
 
 beads_status_t beads_engine_create(const beads_spec_t* spec, beads_engine_t** out);
@@ -245,30 +260,31 @@ beads_status_t beads_engine_profile_advance(beads_engine_t* engine, uint64_t n_s
  *
  * Parameter updates take host arrays: epsilon/sigma [T][T] (symmetric for the
  * adjoint), bond k/r0 [n_bond_types]. */
-beads_status_t beads_engine_set_lj_params(beads_engine_t* engine, const float* epsilon,
-                                          const float* sigma);
-beads_status_t beads_engine_set_bond_params(beads_engine_t* engine, const float* k,
-                                            const float* r0);
+beads_status_t beads_engine_set_lj_params(beads_engine_t* engine, const beads_real_t* epsilon,
+                                          const beads_real_t* sigma);
+beads_status_t beads_engine_set_bond_params(beads_engine_t* engine, const beads_real_t* k,
+                                            const beads_real_t* r0);
 /* Wrap positions into the box, update the neighbour list, compute forces.
  * reuse_list = 0 rebuilds the list at every call; reuse_list = 1 keeps the
  * previous frame's list while every atom stays within half the skin of its
  * position at the last build (the dynamics loop's test,
  * neighbor_system.cpp:107-146). energies (host, may be NULL): pair, bond,
  * angle, dihedral. */
-beads_status_t beads_engine_frame_forces(beads_engine_t* engine, const float* positions,
-                                         float* forces, double* energies, int32_t reuse_list);
+beads_status_t beads_engine_frame_forces(beads_engine_t* engine, const beads_real_t* positions,
+                                         beads_real_t* forces, double* energies,
+                                         int32_t reuse_list);
 /* Load positions and update the neighbour list exactly as frame_forces does,
  * without evaluating forces: re-establishes a frame for frame_backward after
  * other frames were evaluated. */
-beads_status_t beads_engine_frame_load(beads_engine_t* engine, const float* positions,
+beads_status_t beads_engine_frame_load(beads_engine_t* engine, const beads_real_t* positions,
                                        int32_t reuse_list);
 /* Adjoint of the last frame_forces (or frame_load) call for L with dL/dF = grad_forces and
  * dL/dE = grad_energy (E = total potential energy). Writes dL/dx [n][3] and
  * FP64 parameter gradients (device) laid out as
  * [epsilon T*T | sigma T*T | bond k nb | bond r0 nb]. LJ pairs and harmonic
  * bonds only: systems with angle or dihedral terms are rejected. */
-beads_status_t beads_engine_frame_backward(beads_engine_t* engine, const float* grad_forces,
-                                           double grad_energy, float* grad_positions,
+beads_status_t beads_engine_frame_backward(beads_engine_t* engine, const beads_real_t* grad_forces,
+                                           double grad_energy, beads_real_t* grad_positions,
                                            double* grad_params);
 /* Device time of the last frame_forces / frame_backward (HIP events); after
  * the async variants this waits for their end events. */
@@ -284,28 +300,33 @@ beads_status_t beads_engine_frame_timing(beads_engine_t* engine, double* forward
  * device memory (4 doubles, written in stream order), and a neighbour-list
  * overflow is reported by the next beads_engine_synchronize. */
 beads_status_t beads_engine_set_stream(beads_engine_t* engine, void* stream);
-beads_status_t beads_engine_frame_forces_async(beads_engine_t* engine, const float* positions,
-                                               float* forces, double* energies,
+beads_status_t beads_engine_frame_forces_async(beads_engine_t* engine,
+                                               const beads_real_t* positions,
+                                               beads_real_t* forces, double* energies,
                                                int32_t reuse_list);
-beads_status_t beads_engine_frame_load_async(beads_engine_t* engine, const float* positions,
-                                             int32_t reuse_list);
+beads_status_t beads_engine_frame_load_async(beads_engine_t* engine,
+                                             const beads_real_t* positions, int32_t reuse_list);
 beads_status_t beads_engine_frame_backward_async(beads_engine_t* engine,
-                                                 const float* grad_forces, double grad_energy,
-                                                 float* grad_positions, double* grad_params);
+                                                 const beads_real_t* grad_forces,
+                                                 double grad_energy, beads_real_t* grad_positions,
+                                                 double* grad_params);
 /* Allocation-free variant of beads_engine_frame_backward_async. No counterpart
  * in the reference; it is for callers that cannot allocate device memory once
  * set up: code under HIP graph capture (a captured training step), tight
  * serving loops, harnesses that forbid hipMalloc. Same arguments, meaning and
  * stream order as the async call, but dL/dF is staged in the library's static
- * per-GPU scratch array (16 MiB, at most 1,048,576 particles; a larger engine
- * gets BEADS_E_VALUE) instead of an engine buffer allocated on first use,
- * grad_params is cleared by a kernel rather than a memset, and no timing
- * events are recorded (beads_engine_frame_timing does not cover these calls).
+ * per-GPU scratch array (16 MiB, 32 MiB in the BEADS_REAL64 build; at most
+ * 1,048,576 particles; a larger engine gets BEADS_E_VALUE) instead of an
+ * engine buffer allocated on first use, grad_params is cleared by a kernel
+ * rather than a memset, and no timing events are recorded
+ * (beads_engine_frame_timing does not cover these calls).
  * The scratch keeps the last call's contents and is shared by every engine on
  * that GPU: calls that overlap on one GPU (engines on concurrent streams) race. */
 beads_status_t beads_engine_frame_backward_noalloc(beads_engine_t* engine,
-                                                   const float* grad_forces, double grad_energy,
-                                                   float* grad_positions, double* grad_params);
+                                                   const beads_real_t* grad_forces,
+                                                   double grad_energy,
+                                                   beads_real_t* grad_positions,
+                                                   double* grad_params);
 void beads_engine_destroy(beads_engine_t* engine);
 
 /* ---- batched replicas ----

@@ -33,6 +33,13 @@ class Plan(C.Structure):
                 ("search_cutoff_sq", C.c_float), ("path", C.c_int32)]
 
 
+class PlanF64(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("n_cells", C.c_uint32),
+                ("cell_size", C.c_double * 3), ("inv_cell", C.c_double * 3),
+                ("force_cutoff", C.c_double), ("search_cutoff", C.c_double),
+                ("search_cutoff_sq", C.c_double), ("path", C.c_int32)]
+
+
 class System(C.Structure):
     _fields_ = [("n", C.c_uint32), ("pos", _f32p), ("vel", _f32p), ("mass", _f32p), ("type", _i32p),
                 ("tag", _u32p), ("image", _i32p), ("lo", C.c_float * 3), ("hi", C.c_float * 3),
@@ -106,6 +113,15 @@ def library() -> C.CDLL:
                                                  _f32p, C.c_float]
         lib.orc_run.argtypes = [C.POINTER(System), C.POINTER(RunParams), _f64p, _f64p, _i32p, _u32p,
                                 C.POINTER(RunStats)]
+        lib.orc_plan_f64.argtypes = [_f32p, _f32p, C.c_double, C.c_double, C.c_uint32,
+                                     C.POINTER(PlanF64)]
+        lib.orc_cell_ids_f64.argtypes = [C.c_uint32, _f64p, _f32p, C.POINTER(PlanF64), _u32p]
+        lib.orc_neighbor_build_f64.argtypes = [C.POINTER(System), _f64p, C.c_double, C.c_uint32,
+                                               C.c_int32, _u32p, _u32p, _i32p]
+        lib.orc_forces_f64_list.argtypes = [C.POINTER(System), _f64p, _u32p, _u32p, C.c_uint32,
+                                            _f64p, C.POINTER(Observables)]
+        lib.orc_run_f64.argtypes = [C.POINTER(System), C.POINTER(RunParams), _f64p, _f64p, _f64p,
+                                    _f64p, _i32p, _u32p, C.POINTER(RunStats)]
         _lib = lib
     return _lib
 
@@ -313,6 +329,14 @@ def displacement_exceeds(w, pos, images, ref_pos, ref_images) -> bool:
                                              w.cutoff_buffer))
 
 
+def _run_params(w, steps, threads) -> RunParams:
+    return RunParams(dt=w.dt, thermostat=1 if w.berendsen else 0, temperature=w.temperature,
+                     tau=w.tau, kb=w.kb, cutoff_buffer=w.cutoff_buffer,
+                     rebuild_check_every=w.rebuild_check_every,
+                     sort_every_rebuild=w.sort_every_rebuild, max_neighbors=w.max_neighbors,
+                     steps=w.steps if steps is None else steps, threads=threads)
+
+
 def run(w, steps: int | None = None, threads: int = 0):
     """Whole FP64 run; returns final state (oracle slot order) and stats."""
     lib = library()
@@ -327,6 +351,89 @@ def run(w, steps: int | None = None, threads: int = 0):
     st = RunStats()
     rc = lib.orc_run(C.byref(sys.s), C.byref(p), _p(pos, C.c_double), _p(vel, C.c_double),
                      _p(img, C.c_int32), _p(tags, C.c_uint32), C.byref(st))
+    if rc:
+        raise OracleError(lib.orc_last_error().decode())
+    return {"positions": pos, "velocities": vel, "images": img, "tags": tags}, st
+
+
+# ---- the BEADS_REAL64 build: binary64 positions, cell grid and decisions ----
+
+def plan_f64(w) -> PlanF64:
+    lib = library()
+    p = PlanF64()
+    lo = np.ascontiguousarray(w.box_bound[0], np.float32)
+    hi = np.ascontiguousarray(w.box_bound[1], np.float32)
+    if lib.orc_plan_f64(_p(lo, C.c_float), _p(hi, C.c_float), float(w.cutoff),
+                        float(w.cutoff_buffer), w.n, C.byref(p)):
+        raise OracleError(lib.orc_last_error().decode())
+    return p
+
+
+def cell_ids_f64(w, pos64: np.ndarray) -> np.ndarray:
+    lib = library()
+    p = plan_f64(w)
+    pos = np.ascontiguousarray(pos64, np.float64)
+    lo = np.ascontiguousarray(w.box_bound[0], np.float32)
+    out = np.zeros(w.n, np.uint32)
+    lib.orc_cell_ids_f64(w.n, _p(pos, C.c_double), _p(lo, C.c_float), C.byref(p),
+                         _p(out, C.c_uint32))
+    return out
+
+
+def neighbor_lists_f64(w, pos64: np.ndarray, tags=None, types=None, path: int = -1,
+                       max_neighbors: int | None = None):
+    """(counts[n], list[n][max]) atom-major, overflow flag; binary64 decisions."""
+    lib = library()
+    m = w.max_neighbors if max_neighbors is None else max_neighbors
+    sys = OracleSystem(w, tags=tags, types=types)
+    pos = np.ascontiguousarray(pos64, np.float64)
+    counts = np.zeros(w.n, np.uint32)
+    lst = np.zeros((w.n, m), np.uint32)
+    ovf = np.zeros(1, np.int32)
+    if lib.orc_neighbor_build_f64(C.byref(sys.s), _p(pos, C.c_double), float(w.cutoff_buffer), m,
+                                  path, _p(counts, C.c_uint32), _p(lst, C.c_uint32),
+                                  _p(ovf, C.c_int32)):
+        raise OracleError(lib.orc_last_error().decode())
+    return counts, lst, bool(ovf[0])
+
+
+def forces_real64(w, pos64: np.ndarray, tags=None, types=None, counts=None, lst=None):
+    """FP64 forces [n][3] and observables at binary64 positions with the
+    BEADS_REAL64 build's decisions: the lists (atom-major counts/lst, or built
+    here with the workload's skin and binary64 decisions) and r^2 < r_c^2 in
+    binary64."""
+    lib = library()
+    if counts is None:
+        counts, lst, ovf = neighbor_lists_f64(w, pos64, tags=tags, types=types)
+        if ovf:
+            raise OracleError("oracle neighbour list overflow; raise max_neighbors")
+    sys = OracleSystem(w, tags=tags, types=types)
+    pos = np.ascontiguousarray(pos64, np.float64)
+    c = np.ascontiguousarray(counts, np.uint32)
+    l = np.ascontiguousarray(lst, np.uint32)
+    f = np.zeros((w.n, 3), np.float64)
+    obs = Observables()
+    if lib.orc_forces_f64_list(C.byref(sys.s), _p(pos, C.c_double), _p(c, C.c_uint32),
+                               _p(l, C.c_uint32), l.shape[1], _p(f, C.c_double), C.byref(obs)):
+        raise OracleError(lib.orc_last_error().decode())
+    return f, obs.as_dict()
+
+
+def run_f64(w, pos64: np.ndarray, vel64: np.ndarray, steps: int | None = None, threads: int = 0):
+    """Whole FP64 run from binary64 positions and velocities with the binary64
+    cell grid and decisions; returns final state (oracle slot order) and stats."""
+    lib = library()
+    sys = OracleSystem(w)
+    p = _run_params(w, steps, threads)
+    n = w.n
+    x0 = np.ascontiguousarray(pos64, np.float64)
+    v0 = np.ascontiguousarray(vel64, np.float64)
+    pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); img = np.zeros((n, 3), np.int32)
+    tags = np.zeros(n, np.uint32)
+    st = RunStats()
+    rc = lib.orc_run_f64(C.byref(sys.s), C.byref(p), _p(x0, C.c_double), _p(v0, C.c_double),
+                         _p(pos, C.c_double), _p(vel, C.c_double), _p(img, C.c_int32),
+                         _p(tags, C.c_uint32), C.byref(st))
     if rc:
         raise OracleError(lib.orc_last_error().decode())
     return {"positions": pos, "velocities": vel, "images": img, "tags": tags}, st

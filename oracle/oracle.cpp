@@ -32,6 +32,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -44,24 +45,34 @@ thread_local std::string g_error;
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
-// box_geometry.cpp:9-17 (binary32 like the reference's real_t).
-struct BoxF {
-  float lo[3];
-  float L[3];
-  float hL[3];
-  float invL[3];
+// box_geometry.cpp:9-17 in the reference's real_t: binary32 (BoxF), or
+// binary64 (BoxD) for the BEADS_REAL64 build.
+template <typename Real>
+struct BoxT {
+  Real lo[3];
+  Real L[3];
+  Real hL[3];
+  Real invL[3];
 };
+using BoxF = BoxT<float>;
+using BoxD = BoxT<double>;
 
-BoxF make_box(const float lo[3], const float hi[3]) {
-  BoxF b;
+template <typename Real>
+BoxT<Real> make_box_t(const float lo[3], const float hi[3]) {
+  BoxT<Real> b;
   for (int a = 0; a < 3; ++a) {
     b.lo[a] = lo[a];
-    b.L[a] = hi[a] - lo[a];
-    b.hL[a] = 0.5f * b.L[a];
-    b.invL[a] = 1.0f / b.L[a];
+    b.L[a] = static_cast<Real>(hi[a]) - static_cast<Real>(lo[a]);
+    b.hL[a] = Real(0.5) * b.L[a];
+    b.invL[a] = Real(1) / b.L[a];
   }
   return b;
 }
+
+BoxF make_box(const float lo[3], const float hi[3]) { return make_box_t<float>(lo, hi); }
+
+// The neighbour plan in binary64 (same fields as orc_plan_t).
+using PlanD = orc_plan_f64_t;
 
 // box_geometry_view.cuh:50-62, minimum image for wrapped coordinates.
 inline float mic_f(float d, float L, float hL) {
@@ -105,25 +116,27 @@ inline Real mic_diff_any(Real xi, Real xj, Real L, Real hL) {
 }
 
 // neighbor_plan.cpp:29-49 (count computed in double), :133-166, :107-115,
-// :90-105 and the constructor checks :182-236.
-bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
-               uint32_t n, orc_plan_t* p) {
-  const BoxF b = make_box(lo, hi);
+// :90-105 and the constructor checks :182-236, in real_t = Real. Plan is
+// orc_plan_t (Real = float) or PlanD (Real = double).
+template <typename Real, typename Plan>
+bool make_plan_t(const float lo[3], const float hi[3], Real rc, Real buffer, uint32_t n,
+                 Plan* p) {
+  const BoxT<Real> b = make_box_t<Real>(lo, hi);
   if (n == 0) {
     g_error = "neighbor n_particles must be positive.";
     return false;
   }
-  if (!std::isfinite(rc) || rc <= 0.f) {
+  if (!std::isfinite(rc) || rc <= Real(0)) {
     g_error = "neighbor force_cutoff must be finite and positive.";
     return false;
   }
-  if (!std::isfinite(buffer) || buffer < 0.f) {
+  if (!std::isfinite(buffer) || buffer < Real(0)) {
     g_error = "neighbor cutoff_buffer must be finite and non-negative.";
     return false;
   }
-  const float rs = rc + buffer;
-  const float rs2 = rs * rs;
-  const float half_shortest = 0.5f * std::min(b.L[0], std::min(b.L[1], b.L[2]));
+  const Real rs = rc + buffer;
+  const Real rs2 = rs * rs;
+  const Real half_shortest = Real(0.5) * std::min(b.L[0], std::min(b.L[1], b.L[2]));
   if (rs > half_shortest) {
     g_error = "neighbor search_cutoff must not exceed half shortest box length.";
     return false;
@@ -132,10 +145,10 @@ bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
   for (int a = 0; a < 3; ++a) {
     const double raw = std::floor(static_cast<double>(b.L[a]) / static_cast<double>(rs));
     counts[a] = raw < 1.0 ? 1u : static_cast<uint32_t>(raw);
-    p->cell_size[a] = b.L[a] / static_cast<float>(counts[a]);
-    p->inv_cell[a] = 1.0f / p->cell_size[a];
+    p->cell_size[a] = b.L[a] / static_cast<Real>(counts[a]);
+    p->inv_cell[a] = Real(1) / p->cell_size[a];
     if (counts[a] >= 3) {
-      const float tol = std::max(1.0e-6f, rs * 1.0e-5f);
+      const Real tol = std::max(Real(1.0e-6), rs * Real(1.0e-5));
       if (p->cell_size[a] + tol < rs) {
         g_error = "neighbor cell size must be at least neighbor search_cutoff.";
         return false;
@@ -159,10 +172,16 @@ bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
   return true;
 }
 
-// cell_list_build.cu:110-123.
-inline uint32_t axis_cell(float x, float lo, float inv, uint32_t count) {
-  const float s = (x - lo) * inv;
-  uint32_t i = s <= 0.f ? 0u : static_cast<uint32_t>(s);
+bool make_plan(const float lo[3], const float hi[3], float rc, float buffer,
+               uint32_t n, orc_plan_t* p) {
+  return make_plan_t<float>(lo, hi, rc, buffer, n, p);
+}
+
+// cell_list_build.cu:110-123 in real_t = Real.
+template <typename Real>
+inline uint32_t axis_cell(Real x, Real lo, Real inv, uint32_t count) {
+  const Real s = (x - lo) * inv;
+  uint32_t i = s <= Real(0) ? 0u : static_cast<uint32_t>(s);
   if (i >= count) {
     i = count - 1;
   }
@@ -203,9 +222,10 @@ inline int unique_axis(uint32_t c, uint32_t count, uint32_t out[3]) {
 // leaves a cell's members in arbitrary order; the engine under test orders
 // them by (x, slot), which this restates. Any order inside a cell gives the
 // same neighbour sets; it fixes the candidate order of a build.
+template <typename Key = float>
 void stable_order(uint32_t n, const uint32_t* cell, uint32_t n_cells,
                   std::vector<uint32_t>& order, std::vector<uint32_t>& counts,
-                  std::vector<uint32_t>& offsets, const float* x = nullptr,
+                  std::vector<uint32_t>& offsets, const Key* x = nullptr,
                   size_t x_stride = 1) {
   counts.assign(n_cells, 0);
   for (uint32_t i = 0; i < n; ++i) {
@@ -319,23 +339,26 @@ SlotExclusions slot_exclusions(const orc_system_t* s, const uint32_t* tags) {
 // (regular_cell_neighbor_builder.cu:80-107, small_dimension_neighbor_builder.cu:
 // 100-131, direct_neighbor_builder.cu:33-47) with the append rule of
 // full_neighbor_path_common.cuh:49-94. Pos is any accessor giving x,y,z of a
-// slot; Real is the decision arithmetic.
-template <typename Real, typename Pos>
-void build_lists(uint32_t n, const Pos& pos, const BoxF& box, const orc_plan_t& plan,
+// slot; Real is the decision arithmetic; cells and their (x, slot) order are
+// evaluated in the plan's real type (binary32 for orc_plan_t, binary64 for
+// PlanD) on the box of the same type.
+template <typename Real, typename Pos, typename Box, typename Plan>
+void build_lists(uint32_t n, const Pos& pos, const Box& box, const Plan& plan,
                  int path, const SlotExclusions& excl, Real rs2,
                  std::vector<std::vector<uint32_t>>& lists) {
+  using CR = std::remove_cv_t<std::remove_reference_t<decltype(plan.inv_cell[0])>>;
   lists.assign(n, {});
   std::vector<uint32_t> cell(n), order, counts, offsets;
   if (path != ORC_PATH_DIRECT) {
     for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t ix = axis_cell(static_cast<float>(pos.x(i)), box.lo[0], plan.inv_cell[0], plan.nx);
-      const uint32_t iy = axis_cell(static_cast<float>(pos.y(i)), box.lo[1], plan.inv_cell[1], plan.ny);
-      const uint32_t iz = axis_cell(static_cast<float>(pos.z(i)), box.lo[2], plan.inv_cell[2], plan.nz);
+      const uint32_t ix = axis_cell<CR>(static_cast<CR>(pos.x(i)), box.lo[0], plan.inv_cell[0], plan.nx);
+      const uint32_t iy = axis_cell<CR>(static_cast<CR>(pos.y(i)), box.lo[1], plan.inv_cell[1], plan.ny);
+      const uint32_t iz = axis_cell<CR>(static_cast<CR>(pos.z(i)), box.lo[2], plan.inv_cell[2], plan.nz);
       cell[i] = ix + plan.nx * (iy + plan.ny * iz);
     }
-    std::vector<float> xs(n);
-    for (uint32_t i = 0; i < n; ++i) xs[i] = static_cast<float>(pos.x(i));
-    stable_order(n, cell.data(), plan.n_cells, order, counts, offsets, xs.data());
+    std::vector<CR> xs(n);
+    for (uint32_t i = 0; i < n; ++i) xs[i] = static_cast<CR>(pos.x(i));
+    stable_order<CR>(n, cell.data(), plan.n_cells, order, counts, offsets, xs.data());
   }
   const Real L[3] = {static_cast<Real>(box.L[0]), static_cast<Real>(box.L[1]), static_cast<Real>(box.L[2])};
   const Real hL[3] = {static_cast<Real>(box.hL[0]), static_cast<Real>(box.hL[1]), static_cast<Real>(box.hL[2])};
@@ -463,12 +486,11 @@ Vec mic_vec(const Pos& pos, uint32_t i, uint32_t j, const double L[3], const dou
 // (pair_lj.cu:54), and its value in binary64 (pair_lj.cu:56-66, :128-150).
 // Listed terms follow bond_harmonic.cu:41-79, angle_harmonic.cu:61-158 and
 // dihedral_harmonic.cu:83-206; their coordinates are minimum-imaged in binary64.
-template <typename Real, typename Pos>
-void compute_forces(const orc_system_t* s, const Pos& pos,
+template <typename Real, typename Pos, typename Box>
+void compute_forces(const orc_system_t* s, const Pos& pos, const Box& box,
                     const std::vector<std::vector<uint32_t>>& lists, double* f,
                     orc_observables_t* obs) {
   const uint32_t n = s->n;
-  const BoxF box = make_box(s->lo, s->hi);
   const double Ld[3] = {box.L[0], box.L[1], box.L[2]};
   const double hLd[3] = {box.hL[0], box.hL[1], box.hL[2]};
   const Real Lr[3] = {static_cast<Real>(box.L[0]), static_cast<Real>(box.L[1]), static_cast<Real>(box.L[2])};
@@ -722,7 +744,7 @@ int orc_forces(const orc_system_t* sys, const uint32_t* counts, const uint32_t* 
     build_lists<float>(sys->n, PosF{sys->pos}, box, plan, plan.path, excl,
                        plan.search_cutoff_sq, lists);
   }
-  compute_forces<float>(sys, PosF{sys->pos}, lists, forces, obs);
+  compute_forces<float>(sys, PosF{sys->pos}, make_box(sys->lo, sys->hi), lists, forces, obs);
   return 0;
 }
 
@@ -737,7 +759,67 @@ int orc_forces_f64(const orc_system_t* sys, const double* pos64, double* forces,
   std::vector<std::vector<uint32_t>> lists;
   const double rc = sys->lj_cutoff;
   build_lists<double>(sys->n, PosD{pos64}, box, plan, plan.path, excl, rc * rc, lists);
-  compute_forces<double>(sys, PosD{pos64}, lists, forces, obs);
+  compute_forces<double>(sys, PosD{pos64}, box, lists, forces, obs);
+  return 0;
+}
+
+int orc_plan_f64(const float lo[3], const float hi[3], double force_cutoff,
+                 double cutoff_buffer, uint32_t n, orc_plan_f64_t* out) {
+  return make_plan_t<double>(lo, hi, force_cutoff, cutoff_buffer, n, out) ? 0 : 1;
+}
+
+void orc_cell_ids_f64(uint32_t n, const double* pos, const float lo[3],
+                      const orc_plan_f64_t* plan, uint32_t* cell) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t ix = axis_cell<double>(pos[3 * i], lo[0], plan->inv_cell[0], plan->nx);
+    const uint32_t iy = axis_cell<double>(pos[3 * i + 1], lo[1], plan->inv_cell[1], plan->ny);
+    const uint32_t iz = axis_cell<double>(pos[3 * i + 2], lo[2], plan->inv_cell[2], plan->nz);
+    cell[i] = ix + plan->nx * (iy + plan->ny * iz);
+  }
+}
+
+int orc_neighbor_build_f64(const orc_system_t* sys, const double* pos64, double cutoff_buffer,
+                           uint32_t max_neighbors, int32_t path, uint32_t* counts,
+                           uint32_t* list, int32_t* overflow) {
+  PlanD plan;
+  if (!make_plan_t<double>(sys->lo, sys->hi, static_cast<double>(sys->lj_cutoff), cutoff_buffer,
+                           sys->n, &plan)) {
+    return 1;
+  }
+  if (path < 0) {
+    path = plan.path;
+  }
+  if (path == ORC_PATH_REGULAR && (plan.nx < 3 || plan.ny < 3 || plan.nz < 3)) {
+    g_error = "regular path requires at least 3 cells per axis.";
+    return 1;
+  }
+  const BoxD box = make_box_t<double>(sys->lo, sys->hi);
+  const SlotExclusions excl = slot_exclusions(sys, sys->tag);
+  std::vector<std::vector<uint32_t>> lists;
+  build_lists<double>(sys->n, PosD{pos64}, box, plan, path, excl, plan.search_cutoff_sq, lists);
+  *overflow = 0;
+  for (uint32_t i = 0; i < sys->n; ++i) {
+    const auto& v = lists[i];
+    const uint32_t m = static_cast<uint32_t>(std::min<size_t>(v.size(), max_neighbors));
+    if (v.size() > max_neighbors) {
+      *overflow = 1;
+    }
+    counts[i] = m;
+    std::copy(v.begin(), v.begin() + m, list + static_cast<size_t>(i) * max_neighbors);
+  }
+  return 0;
+}
+
+int orc_forces_f64_list(const orc_system_t* sys, const double* pos64, const uint32_t* counts,
+                        const uint32_t* list, uint32_t max_neighbors, double* forces,
+                        orc_observables_t* obs) {
+  std::vector<std::vector<uint32_t>> lists(sys->n);
+  for (uint32_t i = 0; i < sys->n; ++i) {
+    const uint32_t* row = list + static_cast<size_t>(i) * max_neighbors;
+    lists[i].assign(row, row + counts[i]);
+  }
+  compute_forces<double>(sys, PosD{pos64}, make_box_t<double>(sys->lo, sys->hi), lists, forces,
+                         obs);
   return 0;
 }
 
@@ -825,10 +907,17 @@ int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
 // A whole run in binary64 (simulation_run.cpp:189-235 step order; neighbour
 // policy neighbor_system.cpp:79-146, 198-216; displacement test
 // rebuild_tracker.cu:55-94; reorder cadence counted in rebuilds). Used as the
-// CPU baseline and for trajectory-level invariants.
-extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
-                       double* pos_out, double* vel_out, int32_t* image_out,
-                       uint32_t* tags_out, orc_run_stats_t* st) {
+// CPU baseline and for trajectory-level invariants. The cell grid and box are
+// the binary32 build's (D = false) or the BEADS_REAL64 build's (D = true);
+// the state, the neighbour test and the displacement test are binary64 in
+// both. x0, v0: initial positions and velocities [n][3].
+namespace {
+template <bool D>
+int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x0,
+             const double* v0, double* pos_out, double* vel_out, int32_t* image_out,
+             uint32_t* tags_out, orc_run_stats_t* st) {
+  using CR = std::conditional_t<D, double, float>;
+  using Plan = std::conditional_t<D, PlanD, orc_plan_t>;
 #ifdef _OPENMP
   if (p->threads > 0) {
     omp_set_num_threads(p->threads);
@@ -838,18 +927,19 @@ extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
   st->threads_used = 1;
 #endif
   const uint32_t n = sys->n;
-  orc_plan_t plan;
-  if (!make_plan(sys->lo, sys->hi, sys->lj_cutoff, p->cutoff_buffer, n, &plan)) {
+  Plan plan;
+  if (!make_plan_t<CR>(sys->lo, sys->hi, static_cast<CR>(sys->lj_cutoff),
+                       static_cast<CR>(p->cutoff_buffer), n, &plan)) {
     return 1;
   }
-  const BoxF box = make_box(sys->lo, sys->hi);
+  const BoxT<CR> box = make_box_t<CR>(sys->lo, sys->hi);
   std::vector<double> x(3 * static_cast<size_t>(n)), v(3 * static_cast<size_t>(n)), m(n);
   std::vector<int32_t> img(3 * static_cast<size_t>(n)), type(sys->type, sys->type + n);
   std::vector<uint32_t> tag(sys->tag, sys->tag + n);
   std::vector<float> mass_f(sys->mass, sys->mass + n);
   for (size_t k = 0; k < 3 * static_cast<size_t>(n); ++k) {
-    x[k] = sys->pos[k];
-    v[k] = sys->vel[k];
+    x[k] = x0[k];
+    v[k] = v0[k];
     img[k] = sys->image[k];
   }
   for (uint32_t i = 0; i < n; ++i) {
@@ -875,9 +965,9 @@ extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
     if (plan.path != ORC_PATH_DIRECT && build_count % p->sort_every_rebuild == 0) {
       std::vector<uint32_t> cell(n), order, counts, offsets;
       for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t ix = axis_cell(static_cast<float>(x[3 * i]), box.lo[0], plan.inv_cell[0], plan.nx);
-        const uint32_t iy = axis_cell(static_cast<float>(x[3 * i + 1]), box.lo[1], plan.inv_cell[1], plan.ny);
-        const uint32_t iz = axis_cell(static_cast<float>(x[3 * i + 2]), box.lo[2], plan.inv_cell[2], plan.nz);
+        const uint32_t ix = axis_cell<CR>(static_cast<CR>(x[3 * i]), box.lo[0], plan.inv_cell[0], plan.nx);
+        const uint32_t iy = axis_cell<CR>(static_cast<CR>(x[3 * i + 1]), box.lo[1], plan.inv_cell[1], plan.ny);
+        const uint32_t iz = axis_cell<CR>(static_cast<CR>(x[3 * i + 2]), box.lo[2], plan.inv_cell[2], plan.nz);
         cell[i] = ix + plan.nx * (iy + plan.ny * iz);
       }
       stable_order(n, cell.data(), plan.n_cells, order, counts, offsets);
@@ -932,7 +1022,7 @@ extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
   if (!rebuild()) {
     return 2;
   }
-  compute_forces<double>(&cur, PosD{x.data()}, lists, f.data(), &obs);
+  compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs);
 
   const auto t0 = std::chrono::steady_clock::now();
   for (uint64_t step = 1; step <= p->steps; ++step) {
@@ -977,7 +1067,7 @@ extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
     if (need && !rebuild()) {
       return 2;
     }
-    compute_forces<double>(&cur, PosD{x.data()}, lists, f.data(), &obs);
+    compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs);
     double ke = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : ke)
     for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
@@ -1015,4 +1105,21 @@ extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
   std::copy(img.begin(), img.end(), image_out);
   std::copy(tag.begin(), tag.end(), tags_out);
   return 0;
+}
+}  // namespace
+
+extern "C" int orc_run(const orc_system_t* sys, const orc_run_params_t* p,
+                       double* pos_out, double* vel_out, int32_t* image_out,
+                       uint32_t* tags_out, orc_run_stats_t* st) {
+  const size_t m = 3 * static_cast<size_t>(sys->n);
+  const std::vector<double> x0(sys->pos, sys->pos + m), v0(sys->vel, sys->vel + m);
+  return run_impl<false>(sys, p, x0.data(), v0.data(), pos_out, vel_out, image_out, tags_out,
+                         st);
+}
+
+extern "C" int orc_run_f64(const orc_system_t* sys, const orc_run_params_t* p,
+                           const double* pos_in, const double* vel_in, double* pos_out,
+                           double* vel_out, int32_t* image_out, uint32_t* tags_out,
+                           orc_run_stats_t* st) {
+  return run_impl<true>(sys, p, pos_in, vel_in, pos_out, vel_out, image_out, tags_out, st);
 }

@@ -9,7 +9,8 @@
  *  - discrete decisions (cell index, r^2 <= r_s^2 neighbour test, r^2 < r_c^2
  *    LJ test, wrap crossing count) are evaluated in IEEE binary32 with the
  *    reference's operation order and no FMA contraction, so neighbour sets and
- *    cell assignments can be compared bit-exactly with the device;
+ *    cell assignments can be compared bit-exactly with the device (the *_f64
+ *    entry points evaluate them in binary64, as the BEADS_REAL64 build does);
  *  - values (forces, energies, virials, kinetic energy, Berendsen lambda,
  *    integrator updates) are evaluated in binary64.
  */
@@ -33,6 +34,17 @@ typedef struct orc_plan {
   float search_cutoff_sq;
   int32_t path;
 } orc_plan_t;
+
+/* The plan of the BEADS_REAL64 build: cell sizes, search cutoff in binary64. */
+typedef struct orc_plan_f64 {
+  uint32_t nx, ny, nz, n_cells;
+  double cell_size[3];
+  double inv_cell[3];
+  double force_cutoff;
+  double search_cutoff;
+  double search_cutoff_sq;
+  int32_t path;
+} orc_plan_f64_t;
 
 /* A particle system in slot order. Positions/velocities are AoS [n][3]. */
 typedef struct orc_system {
@@ -155,6 +167,30 @@ int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
 int orc_run(const orc_system_t* sys, const orc_run_params_t* params,
             double* pos_out, double* vel_out, int32_t* image_out,
             uint32_t* tags_out, orc_run_stats_t* stats);
+
+/* The BEADS_REAL64 build (real_t = double), for its parity tests: binary64
+ * positions and velocities, and the box (hi - lo in binary64), cell grid,
+ * search cutoff (force cutoff + buffer in binary64) and every decision (cell
+ * index, r^2 <= r_s^2, r^2 < r_c^2, wrap, displacement test) in binary64 with
+ * the reference's operation order. Box bounds and force-field parameters are
+ * taken from sys (binary32 values, exact in binary64). */
+int orc_plan_f64(const float lo[3], const float hi[3], double force_cutoff,
+                 double cutoff_buffer, uint32_t n, orc_plan_f64_t* out);
+void orc_cell_ids_f64(uint32_t n, const double* pos, const float lo[3],
+                      const orc_plan_f64_t* plan, uint32_t* cell);
+int orc_neighbor_build_f64(const orc_system_t* sys, const double* pos64, double cutoff_buffer,
+                           uint32_t max_neighbors, int32_t path, uint32_t* counts,
+                           uint32_t* list, int32_t* overflow);
+/* Forces and observables at binary64 positions over the given lists
+ * (atom-major), with the r^2 < r_c^2 test in binary64 on the binary64 box. */
+int orc_forces_f64_list(const orc_system_t* sys, const double* pos64, const uint32_t* counts,
+                        const uint32_t* list, uint32_t max_neighbors, double* forces,
+                        orc_observables_t* obs);
+/* orc_run from binary64 initial positions and velocities (pos_in, vel_in). */
+int orc_run_f64(const orc_system_t* sys, const orc_run_params_t* params,
+                const double* pos_in, const double* vel_in, double* pos_out,
+                double* vel_out, int32_t* image_out, uint32_t* tags_out,
+                orc_run_stats_t* stats);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,9 @@ import pytest
 
 import beads_amd as beads
 from beads_amd import workloads as W
+from beads_amd._build_config import dtype_for
+
+REAL = dtype_for("real_dtype")  # float32, or float64 with BEADS_AMD_PRECISION=double
 
 
 def small_system(**kw):
@@ -36,7 +39,7 @@ def test_public_surface():
 def test_system_defaults_and_dtypes():
     s = small_system()
     assert s.n_particles == 3
-    assert s.positions.dtype == np.float32 and s.types.dtype == np.int32
+    assert s.positions.dtype == REAL and s.types.dtype == np.int32
     assert s.tags.dtype == np.uint32 and list(s.tags) == [1, 2, 3]
     assert (s.masses == 1).all() and (s.velocities == 0).all() and (s.images == 0).all()
     assert list(s.molecule_ids) == [1, 1, 1]
@@ -127,7 +130,7 @@ def test_native_spec_dict_layout():
     spec = sim._native_spec()
     assert set(spec) == {"system", "forcefield", "dynamics", "neighbor", "output", "runsteps"}
     sysd = spec["system"]
-    assert sysd["positions"].dtype == np.float32 and sysd["positions"].flags.c_contiguous
+    assert sysd["positions"].dtype == REAL and sysd["positions"].flags.c_contiguous
     assert sysd["tags"].dtype == np.uint32 and sysd["images"].dtype == np.int32
     assert len(sysd["topology"]["bonds"]) == 3 * 7 and len(sysd["topology"]["dihedrals"]) == 3 * 5
     ff = spec["forcefield"]
@@ -283,10 +286,10 @@ def test_native_parser_accepts_a_full_spec(tmp_path):
     sim.save_final_state(prefix=str(tmp_path / "run"))
     sim.set_logging(echo="none")
     sim.set_neighbor(half_list=True)
-    from beads_amd import _native
+    from beads_amd.simulation import load_native
 
     try:
-        _native.execute_simulation(sim._native_spec())
+        load_native().execute_simulation(sim._native_spec())
     except RuntimeError as exc:
         assert "HIP" in str(exc) or "device" in str(exc), exc
 

@@ -12,6 +12,9 @@ import numpy as np
 import pytest
 
 import beads_amd as beads
+from beads_amd._build_config import dtype_for
+
+REAL = dtype_for("real_dtype")
 
 BASE = """BEADS test data  # title line
 
@@ -100,13 +103,13 @@ def _write(tmp_path, text):
 def test_reads_molecular_file(tmp_path):
     s = beads.io.read_lammps_data(_write(tmp_path, _text()), units="nm_kjmol")
     assert s.n_particles == 12 and s.units == "nm_kjmol"
-    np.testing.assert_array_equal(s.box_bound, np.array([[0, -1, 0], [4, 3, 3.5]], np.float32))
+    np.testing.assert_array_equal(s.box_bound, np.array([[0, -1, 0], [4, 3, 3.5]], REAL))
     np.testing.assert_array_equal(s.tags, np.arange(1, 13))
     bead = np.arange(12) % 6
     chain = np.arange(12) // 6
     np.testing.assert_array_equal(s.types, 1 + bead % 2)
     np.testing.assert_array_equal(s.molecule_ids, chain + 1)
-    np.testing.assert_array_equal(s.masses, np.where(bead % 2 == 0, 39.948, 83.798).astype(np.float32))
+    np.testing.assert_array_equal(s.masses, np.where(bead % 2 == 0, 39.948, 83.798).astype(REAL))
     np.testing.assert_array_equal(s.images, np.stack([bead % 2, 0 * bead, -chain], axis=1))
     np.testing.assert_allclose(s.positions[:, 0], 0.5 + 0.35 * bead, rtol=1e-6)
     np.testing.assert_allclose(s.velocities[:, 1], -0.02 * np.arange(1, 13), rtol=1e-6)

@@ -472,3 +472,141 @@ def test_zero_skin_rebuilds_every_step():
     w = drifting_lattice(0.01, skin=0.0, steps=25)
     _, st = O.run(w)
     assert (st.build_count, st.dangerous_rebuild_count) == (26, 0)
+
+
+# ------------------------------------- BEADS_REAL64 entry points (binary64) --
+#
+# The *_f64 functions are the checker of the double-precision engine build:
+# binary64 positions, and the box, cell grid, search cutoff and every decision
+# in binary64. Each is pinned by a binary64 brute force or a closed form, and
+# by a knife-edge case that binary32 arithmetic decides the other way (so a
+# binary32 decision left in by mistake fails).
+
+def _brute_sets_f64(w: W.Workload, pos: np.ndarray, rs2: float, excluded: set) -> list[set]:
+    L = w.box_bound[1].astype(np.float64) - w.box_bound[0].astype(np.float64)
+    hL = 0.5 * L
+    out = []
+    for i in range(w.n):
+        d = pos[i] - pos
+        d = np.where(d > hL, d - L, np.where(d < -hL, d + L, d))
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        js = np.nonzero(r2 <= rs2)[0]
+        out.append({int(j) for j in js if j != i and
+                    (min(w.tags[i], w.tags[j]), max(w.tags[i], w.tags[j])) not in excluded})
+    return out
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: W.lj_box(1200, 0.8, 1.0, seed=4),
+    lambda: W.polymer_melt(n_chains=24, beads_per_chain=50, seed=2),
+])
+def test_f64_neighbor_paths_agree_with_binary64_brute_force(maker):
+    w = maker().as_float64(jitter=1e-3, seed=5)
+    pos = w.positions
+    assert (pos != pos.astype(np.float32)).mean() > 0.99  # not binary32 values
+    rs = np.float64(w.cutoff) + np.float64(w.cutoff_buffer)
+    p = O.plan_f64(w)
+    assert p.search_cutoff == rs and p.search_cutoff_sq == rs * rs
+    excluded = set()
+    if w.exclusion_distance:
+        excluded = {tuple(int(v) for v in q)
+                    for q in O.exclusion_pairs(w.n, w.bonds, w.exclusion_distance)}
+    brute = _brute_sets_f64(w, pos, rs * rs, excluded)
+    paths = [O.PATH_DIRECT, O.PATH_SMALL] + ([O.PATH_REGULAR] if p.path == O.PATH_REGULAR else [])
+    for path in paths:
+        c, lst, ovf = O.neighbor_lists_f64(w, pos, path=path)
+        assert not ovf
+        for i in range(w.n):
+            assert set(lst[i, : c[i]].tolist()) == brute[i], (path, i)
+
+
+def test_f64_neighbor_test_is_binary64_at_the_search_radius():
+    """Two beads r_s + 1e-12 apart: binary64 keeps them apart (r^2 > r_s^2);
+    binary32 rounds the position onto r_s exactly and keeps the pair."""
+    w = two_body(2.75).with_(cutoff=2.5, cutoff_buffer=0.25)  # r_s = 2.75 in both precisions
+    pos = w.positions.astype(np.float64)
+    pos[1, 0] = pos[0, 0] + 2.75 + 1e-12
+    c32, _, _ = O.neighbor_lists(w, positions=pos.astype(np.float32))
+    assert c32.tolist() == [1, 1]
+    c64, _, _ = O.neighbor_lists_f64(w, pos)
+    assert c64.tolist() == [0, 0]
+    pos[1, 0] = pos[0, 0] + 2.75 - 1e-12
+    c64, _, _ = O.neighbor_lists_f64(w, pos)
+    assert c64.tolist() == [1, 1]
+
+
+def test_f64_plan_and_cell_ids_closed_form():
+    w = W.lj_box(1200, 0.8, 1.0, seed=4).as_float64(jitter=1e-3, seed=1)
+    p = O.plan_f64(w)
+    L = w.box_bound[1] - w.box_bound[0]
+    rs = np.float64(w.cutoff) + np.float64(w.cutoff_buffer)
+    counts = np.maximum(1, np.floor(L / rs)).astype(int)
+    assert [p.nx, p.ny, p.nz] == counts.tolist()
+    np.testing.assert_array_equal(np.array(p.cell_size), L / counts)
+    np.testing.assert_array_equal(np.array(p.inv_cell), 1.0 / (L / counts))
+    inv = np.array(p.inv_cell)
+    s = (w.positions - w.box_bound[0]) * inv
+    idx = np.minimum(np.where(s <= 0, 0, s.astype(np.int64)), counts - 1)
+    want = idx[:, 0] + p.nx * (idx[:, 1] + p.ny * idx[:, 2])
+    np.testing.assert_array_equal(O.cell_ids_f64(w, w.positions), want)
+    # a bead 1e-12 below a cell boundary: binary64 keeps it in the lower
+    # cell, binary32 rounds it onto the boundary (upper cell)
+    x = np.array(w.positions)
+    x[0] = w.box_bound[0] + np.array(p.cell_size) * 2 - 1e-12
+    assert O.cell_ids_f64(w, x)[0] == 1 + p.nx * (1 + p.ny * 1)
+    w32 = W.lj_box(1200, 0.8, 1.0, seed=4)
+    assert O.cell_ids(w32, x.astype(np.float32))[0] != O.cell_ids_f64(w, x)[0]
+
+
+def test_forces_real64_two_body_closed_form():
+    """LJ force at a separation that binary32 cannot hold: the binary64 path
+    matches the closed form to rounding; the binary32 path misses by ~1e-7."""
+    r = 1.1 + 3e-9
+    w = two_body(1.1).with_(cutoff_buffer=0.25)
+    pos = w.positions.astype(np.float64)
+    pos[1, 0] = pos[0, 0] + r
+    f, obs = O.forces_real64(w, pos)
+    sr6 = r ** -6
+    want = 24.0 * (2.0 * sr6 * sr6 - sr6) / r
+    assert f[1, 0] == pytest.approx(want, rel=1e-13) and f[0, 0] == pytest.approx(-want, rel=1e-13)
+    assert obs["pair_pe"] == pytest.approx(4.0 * (sr6 * sr6 - sr6), rel=1e-13)
+    f32, _ = O.forces(w, positions=pos.astype(np.float32))
+    assert abs(f32[1, 0] - want) > 1e-9 * abs(want)
+
+
+def test_forces_real64_agree_with_forces_f64_and_energy_gradient():
+    w = W.polymer_melt(n_chains=24, beads_per_chain=50, seed=2).as_float64(jitter=1e-4, seed=3)
+    f, obs = O.forces_real64(w, w.positions)
+    f2, obs2 = O.forces_f64(w, w.positions)
+    scale = np.sqrt(np.mean(f2 ** 2))
+    assert np.max(np.abs(f - f2)) <= 1e-12 * scale
+    for k in obs:
+        assert obs[k] == pytest.approx(obs2[k], rel=1e-12, abs=1e-12)
+    # central difference of the energy on a few coordinates
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    for i in rng.choice(w.n, 5, replace=False):
+        for a in range(3):
+            xp = w.positions.copy(); xp[i, a] += h
+            xm = w.positions.copy(); xm[i, a] -= h
+            ep = energy(O.forces_real64(w, xp)[1])
+            em = energy(O.forces_real64(w, xm)[1])
+            assert -(ep - em) / (2 * h) == pytest.approx(f[i, a], rel=2e-6, abs=1e-5)
+
+
+def test_run_f64_equals_run_on_binary32_inputs_and_conserves_energy():
+    """From binary32-valued initial conditions orc_run and orc_run_f64 run the
+    same FP64 dynamics (cell grids differ only in precision); NVE energy is
+    conserved over 200 steps."""
+    w = W.lj_box(1200, 0.8, 1.0, seed=4).with_(steps=40, rebuild_check_every=1)
+    a, _ = O.run(w)
+    b, _ = O.run_f64(w, w.positions.astype(np.float64), w.velocities.astype(np.float64))
+    oa, ob = np.argsort(a["tags"]), np.argsort(b["tags"])
+    np.testing.assert_allclose(b["positions"][ob], a["positions"][oa], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(b["velocities"][ob], a["velocities"][oa], rtol=0, atol=1e-10)
+    w64 = w.as_float64(jitter=1e-4, seed=2).with_(steps=0)
+    _, s0 = O.run_f64(w64, w64.positions, w64.velocities, steps=0)
+    _, s1 = O.run_f64(w64, w64.positions, w64.velocities, steps=200)
+    e0 = s0.final_ke + energy(s0.final_obs.as_dict())
+    e1 = s1.final_ke + energy(s1.final_obs.as_dict())
+    assert abs(e1 - e0) < 2e-4 * w.n
