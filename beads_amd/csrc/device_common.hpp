@@ -275,7 +275,8 @@ __device__ __forceinline__ real3 bond_force(real3 df, real2 kc, double& energy,
 // atan2(|a x b|, a.b) and sin(theta) = |a x b| / (|a||b|) instead of acos(c)
 // and sqrt(1 - c^2), and the force directions as a x (a x b), b x (a x b)
 // instead of differences of nearly parallel vectors. The reference's floor
-// (sin theta replaced by 0.001 at or below 0.001) is kept.
+// (sin theta replaced by kSinFloor at or below it, angle_harmonic.cu:25, 126;
+// real_t{0.001}: the binary32 value in the binary32 build) is kept.
 __device__ __forceinline__ real3 angle_role_force(real3 xi, real3 xj, real3 xk, real2 kc,
                                                   int role, const BoxView& box, double& energy,
                                                   double& virial) {
@@ -300,7 +301,8 @@ __device__ __forceinline__ real3 angle_role_force(real3 xi, real3 xj, real3 xk, 
   const double dth = theta - theta0;
   energy = 0.5 * k * dth * dth;
   const double sn_true = nn * ia * ib;
-  const double sn = sn_true <= 0.001 ? 0.001 : sn_true;
+  constexpr double kSinFloor = static_cast<double>(static_cast<real_t>(0.001));
+  const double sn = sn_true <= kSinFloor ? kSinFloor : sn_true;
   const double A = -k * dth * rcp_f64(sn);
   const double aib = A * ia * ib;
   const d3 fi = dscale(dcross(a, nrm), aib * ia * ia);

@@ -120,6 +120,9 @@ def library() -> C.CDLL:
                                                C.c_int32, _u32p, _u32p, _i32p]
         lib.orc_forces_f64_list.argtypes = [C.POINTER(System), _f64p, _u32p, _u32p, C.c_uint32,
                                             _f64p, C.POINTER(Observables)]
+        lib.orc_run_tangent_f64.argtypes = [C.POINTER(System), C.POINTER(RunParams), _f64p, _f64p,
+                                            _f64p, _f64p, _f64p, _f64p, _i32p, _u32p, _f64p,
+                                            _f64p, C.POINTER(RunStats)]
         lib.orc_run_f64.argtypes = [C.POINTER(System), C.POINTER(RunParams), _f64p, _f64p, _f64p,
                                     _f64p, _i32p, _u32p, C.POINTER(RunStats)]
         _lib = lib
@@ -437,3 +440,26 @@ def run_f64(w, pos64: np.ndarray, vel64: np.ndarray, steps: int | None = None, t
     if rc:
         raise OracleError(lib.orc_last_error().decode())
     return {"positions": pos, "velocities": vel, "images": img, "tags": tags}, st
+
+
+def run_tangent_f64(w, pos64: np.ndarray, vel64: np.ndarray, dpos: np.ndarray, dvel: np.ndarray,
+                    steps: int | None = None, threads: int = 0):
+    """run_f64 with the tangent of the initial perturbation (dpos, dvel): returns
+    the final state and {"dpos", "dvel"} in the same (oracle) slot order."""
+    lib = library()
+    sys = OracleSystem(w)
+    p = _run_params(w, steps, threads)
+    n = w.n
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (pos64, vel64, dpos, dvel)]
+    pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); img = np.zeros((n, 3), np.int32)
+    tags = np.zeros(n, np.uint32)
+    dpo = np.zeros((n, 3)); dvo = np.zeros((n, 3))
+    st = RunStats()
+    rc = lib.orc_run_tangent_f64(C.byref(sys.s), C.byref(p), *[_p(a, C.c_double) for a in arrs],
+                                 _p(pos, C.c_double), _p(vel, C.c_double), _p(img, C.c_int32),
+                                 _p(tags, C.c_uint32), _p(dpo, C.c_double), _p(dvo, C.c_double),
+                                 C.byref(st))
+    if rc:
+        raise OracleError(lib.orc_last_error().decode())
+    return ({"positions": pos, "velocities": vel, "images": img, "tags": tags},
+            {"dpos": dpo, "dvel": dvo}, st)

@@ -43,7 +43,9 @@ namespace {
 
 thread_local std::string g_error;
 
-constexpr double kPi = 3.141592653589793238462643383279502884;
+// angle_harmonic.cu:25: real_t{0.001} of the binary32 and binary64 builds.
+constexpr double kSinFloor32 = static_cast<double>(0.001f);
+constexpr double kSinFloor64 = 0.001;
 
 // box_geometry.cpp:9-17 in the reference's real_t: binary32 (BoxF), or
 // binary64 (BoxD) for the BEADS_REAL64 build.
@@ -466,13 +468,7 @@ std::vector<LjCoeff> lj_table(const orc_system_t* s) {
 struct Vec {
   double x, y, z;
 };
-inline Vec vsub(Vec a, Vec b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline Vec vadd(Vec a, Vec b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline Vec vscale(Vec a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 inline double vdot(Vec a, Vec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline Vec vcross(Vec a, Vec b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
 
 template <typename Pos>
 Vec mic_vec(const Pos& pos, uint32_t i, uint32_t j, const double L[3], const double hL[3]) {
@@ -481,15 +477,42 @@ Vec mic_vec(const Pos& pos, uint32_t i, uint32_t j, const double L[3], const dou
           mic_d(static_cast<double>(pos.z(i)) - static_cast<double>(pos.z(j)), L[2], hL[2])};
 }
 
+// Extended precision (x87, 64-bit significand) for the listed terms: the
+// reference's acos-based angle and dihedral are ill-conditioned near 0 and pi,
+// where binary64 would leave the checker's own rounding above the error of
+// the engine's stable formulation.
+using X = long double;
+constexpr X kPiX = 3.141592653589793238462643383279502884L;
+struct XVec {
+  X x, y, z;
+};
+inline XVec xsub(XVec a, XVec b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline XVec xadd(XVec a, XVec b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline XVec xscale(XVec a, X s) { return {a.x * s, a.y * s, a.z * s}; }
+inline X xdot(XVec a, XVec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline XVec xcross(XVec a, XVec b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline X mic_x(X d, X L, X hL) { return d > hL ? d - L : (d < -hL ? d + L : d); }
+
+template <typename Pos>
+XVec mic_xvec(const Pos& pos, uint32_t i, uint32_t j, const X L[3], const X hL[3]) {
+  return {mic_x(X(pos.x(i)) - X(pos.x(j)), L[0], hL[0]),
+          mic_x(X(pos.y(i)) - X(pos.y(j)), L[1], hL[1]),
+          mic_x(X(pos.z(i)) - X(pos.z(j)), L[2], hL[2])};
+}
+
 // Forces + observables. lists[i] holds the stored neighbours of slot i;
 // the pair is evaluated if the decision arithmetic (Real) gives r2 < rc2
 // (pair_lj.cu:54), and its value in binary64 (pair_lj.cu:56-66, :128-150).
 // Listed terms follow bond_harmonic.cu:41-79, angle_harmonic.cu:61-158 and
-// dihedral_harmonic.cu:83-206; their coordinates are minimum-imaged in binary64.
+// dihedral_harmonic.cu:83-206 in extended precision, rounded to binary64 when
+// added; sin_floor is the angle's real_t{0.001} (angle_harmonic.cu:25) of the
+// build under test.
 template <typename Real, typename Pos, typename Box>
 void compute_forces(const orc_system_t* s, const Pos& pos, const Box& box,
                     const std::vector<std::vector<uint32_t>>& lists, double* f,
-                    orc_observables_t* obs) {
+                    orc_observables_t* obs, double sin_floor) {
   const uint32_t n = s->n;
   const double Ld[3] = {box.L[0], box.L[1], box.L[2]};
   const double hLd[3] = {box.hL[0], box.hL[1], box.hL[2]};
@@ -543,114 +566,116 @@ void compute_forces(const orc_system_t* s, const Pos& pos, const Box& box,
   for (uint32_t i = 0; i < n; ++i) {
     slot_of_tag[s->tag[i]] = i;
   }
-  auto add = [&](uint32_t slot, Vec v) {
-    f[3 * slot] += v.x;
-    f[3 * slot + 1] += v.y;
-    f[3 * slot + 2] += v.z;
+  auto add = [&](uint32_t slot, XVec v) {
+    f[3 * slot] += static_cast<double>(v.x);
+    f[3 * slot + 1] += static_cast<double>(v.y);
+    f[3 * slot + 2] += static_cast<double>(v.z);
   };
+  const X Lx[3] = {Ld[0], Ld[1], Ld[2]};
+  const X hLx[3] = {hLd[0], hLd[1], hLd[2]};
 
-  double bond_pe = 0, bond_vir = 0;
+  X bond_pe = 0, bond_vir = 0;
   for (uint32_t b = 0; b < s->n_bonds; ++b) {
     const uint32_t si = slot_of_tag[s->bonds[3 * b]];
     const uint32_t sj = slot_of_tag[s->bonds[3 * b + 1]];
     const int ty = static_cast<int>(s->bonds[3 * b + 2]) - 1;
-    const double k = s->bond_k[ty], r0 = s->bond_r0[ty];
-    const Vec d = mic_vec(pos, si, sj, Ld, hLd);
-    const double r = std::sqrt(vdot(d, d));
-    const double stretch = r - r0;
-    bond_pe += 0.5 * k * stretch * stretch;
-    if (r == 0.0) {
+    const X k = s->bond_k[ty], r0 = s->bond_r0[ty];
+    const XVec d = mic_xvec(pos, si, sj, Lx, hLx);
+    const X r = std::sqrt(xdot(d, d));
+    const X stretch = r - r0;
+    bond_pe += X(0.5) * k * stretch * stretch;
+    if (r == 0) {
       continue;
     }
-    const Vec fi = vscale(d, -k * stretch / r);
+    const XVec fi = xscale(d, -k * stretch / r);
     add(si, fi);
-    add(sj, vscale(fi, -1.0));
-    bond_vir += vdot(d, fi);
+    add(sj, xscale(fi, -1));
+    bond_vir += xdot(d, fi);
   }
 
-  const double sin_floor = static_cast<double>(0.001f);
-  double angle_pe = 0, angle_vir = 0;
+  const X sf = sin_floor;
+  X angle_pe = 0, angle_vir = 0;
   for (uint32_t a = 0; a < s->n_angles; ++a) {
     const uint32_t si = slot_of_tag[s->angles[4 * a]];
     const uint32_t sj = slot_of_tag[s->angles[4 * a + 1]];
     const uint32_t sk = slot_of_tag[s->angles[4 * a + 2]];
     const int ty = static_cast<int>(s->angles[4 * a + 3]) - 1;
-    const double k = s->angle_k[ty];
-    const double theta0 = static_cast<double>(s->angle_theta0_deg[ty]) * kPi / 180.0;
-    const Vec va = mic_vec(pos, si, sj, Ld, hLd);
-    const Vec vb = mic_vec(pos, sk, sj, Ld, hLd);
-    const double ra2 = vdot(va, va), rb2 = vdot(vb, vb);
-    if (ra2 == 0.0 || rb2 == 0.0) {
-      angle_pe += 0.5 * k * theta0 * theta0;
+    const X k = s->angle_k[ty];
+    const X theta0 = static_cast<X>(s->angle_theta0_deg[ty]) * kPiX / 180;
+    const XVec va = mic_xvec(pos, si, sj, Lx, hLx);
+    const XVec vb = mic_xvec(pos, sk, sj, Lx, hLx);
+    const X ra2 = xdot(va, va), rb2 = xdot(vb, vb);
+    if (ra2 == 0 || rb2 == 0) {
+      angle_pe += X(0.5) * k * theta0 * theta0;
       continue;
     }
-    const double ra = std::sqrt(ra2), rb = std::sqrt(rb2);
-    double c = vdot(va, vb) / (ra * rb);
-    c = std::max(-1.0, std::min(1.0, c));
-    const double theta = c == 1.0 ? 0.0 : (c == -1.0 ? kPi : std::acos(c));
-    const double dth = theta - theta0;
-    angle_pe += 0.5 * k * dth * dth;
-    const double sin_sq = 1.0 - c * c;
-    const double sn = sin_sq <= sin_floor * sin_floor ? sin_floor : std::sqrt(sin_sq);
-    const double A = -k * dth / sn;
-    const double a11 = A * c / ra2, a12 = -A / (ra * rb), a22 = A * c / rb2;
-    const Vec fi = vadd(vscale(va, a11), vscale(vb, a12));
-    const Vec fk = vadd(vscale(vb, a22), vscale(va, a12));
+    const X ra = std::sqrt(ra2), rb = std::sqrt(rb2);
+    X c = xdot(va, vb) / (ra * rb);
+    c = std::max(X(-1), std::min(X(1), c));
+    const X theta = c == 1 ? X(0) : (c == -1 ? kPiX : std::acos(c));
+    const X dth = theta - theta0;
+    angle_pe += X(0.5) * k * dth * dth;
+    const X sin_sq = 1 - c * c;
+    const X sn = sin_sq <= sf * sf ? sf : std::sqrt(sin_sq);
+    const X A = -k * dth / sn;
+    const X a11 = A * c / ra2, a12 = -A / (ra * rb), a22 = A * c / rb2;
+    const XVec fi = xadd(xscale(va, a11), xscale(vb, a12));
+    const XVec fk = xadd(xscale(vb, a22), xscale(va, a12));
     add(si, fi);
     add(sk, fk);
-    add(sj, vscale(vadd(fi, fk), -1.0));
-    angle_vir += vdot(va, fi) + vdot(vb, fk);
+    add(sj, xscale(xadd(fi, fk), -1));
+    angle_vir += xdot(va, fi) + xdot(vb, fk);
   }
 
-  double dih_pe = 0, dih_vir = 0;
+  X dih_pe = 0, dih_vir = 0;
   for (uint32_t q = 0; q < s->n_dihedrals; ++q) {
     const uint32_t si = slot_of_tag[s->dihedrals[5 * q]];
     const uint32_t sj = slot_of_tag[s->dihedrals[5 * q + 1]];
     const uint32_t sk = slot_of_tag[s->dihedrals[5 * q + 2]];
     const uint32_t sl = slot_of_tag[s->dihedrals[5 * q + 3]];
     const int ty = static_cast<int>(s->dihedrals[5 * q + 4]) - 1;
-    const double k = s->dihedral_k[ty];
+    const X k = s->dihedral_k[ty];
     const int dd = s->dihedral_d[ty], nn = s->dihedral_n[ty];
-    const Vec v0 = mic_vec(pos, si, sj, Ld, hLd);
-    const Vec v1 = mic_vec(pos, sk, sj, Ld, hLd);
-    const Vec v2 = mic_vec(pos, sk, sl, Ld, hLd);
-    const Vec c0 = vcross(v0, v1), c1 = vcross(v1, v2);
-    const double n1 = vdot(v1, v1), nc0 = vdot(c0, c0), nc1 = vdot(c1, c1);
-    if (n1 == 0.0 || nc0 == 0.0 || nc1 == 0.0) {
-      dih_pe += nn == 0 ? k * (1.0 + dd) : k;
+    const XVec v0 = mic_xvec(pos, si, sj, Lx, hLx);
+    const XVec v1 = mic_xvec(pos, sk, sj, Lx, hLx);
+    const XVec v2 = mic_xvec(pos, sk, sl, Lx, hLx);
+    const XVec c0 = xcross(v0, v1), c1 = xcross(v1, v2);
+    const X n1 = xdot(v1, v1), nc0 = xdot(c0, c0), nc1 = xdot(c1, c1);
+    if (n1 == 0 || nc0 == 0 || nc1 == 0) {
+      dih_pe += nn == 0 ? k * (1 + dd) : k;
       continue;
     }
-    double c = vdot(c0, c1) / std::sqrt(nc0 * nc1);
-    c = std::max(-1.0, std::min(1.0, c));
-    double phi = c == 1.0 ? 0.0 : (c == -1.0 ? kPi : std::acos(c));
-    if (vdot(v0, c1) < 0.0) {
+    X c = xdot(c0, c1) / std::sqrt(nc0 * nc1);
+    c = std::max(X(-1), std::min(X(1), c));
+    X phi = c == 1 ? X(0) : (c == -1 ? kPiX : std::acos(c));
+    if (xdot(v0, c1) < 0) {
       phi = -phi;
     }
-    const double arg = nn * phi;
-    dih_pe += k * (1.0 + dd * std::cos(arg));
-    const double dE = -k * dd * nn * std::sin(arg);
-    const double nb = std::sqrt(n1);
-    const Vec fi = vscale(c0, -dE * nb / nc0);
-    const Vec fl = vscale(c1, dE * nb / nc1);
-    const Vec sv = vsub(vscale(fi, vdot(v0, v1) / n1), vscale(fl, vdot(v2, v1) / n1));
-    const Vec fj = vsub(sv, fi);
-    const Vec fk = vscale(vadd(sv, fl), -1.0);
+    const X arg = nn * phi;
+    dih_pe += k * (1 + dd * std::cos(arg));
+    const X dE = -k * dd * nn * std::sin(arg);
+    const X nb = std::sqrt(n1);
+    const XVec fi = xscale(c0, -dE * nb / nc0);
+    const XVec fl = xscale(c1, dE * nb / nc1);
+    const XVec sv = xsub(xscale(fi, xdot(v0, v1) / n1), xscale(fl, xdot(v2, v1) / n1));
+    const XVec fj = xsub(sv, fi);
+    const XVec fk = xscale(xadd(sv, fl), -1);
     add(si, fi);
     add(sj, fj);
     add(sk, fk);
     add(sl, fl);
-    dih_vir += vdot(v0, fi) + vdot(v1, fk) + vdot(vsub(v1, v2), fl);
+    dih_vir += xdot(v0, fi) + xdot(v1, fk) + xdot(xsub(v1, v2), fl);
   }
 
   if (obs != nullptr) {
     obs->pair_pe = pair_pe;
     obs->pair_virial = pair_vir;
-    obs->bond_pe = bond_pe;
-    obs->bond_virial = bond_vir;
-    obs->angle_pe = angle_pe;
-    obs->angle_virial = angle_vir;
-    obs->dihedral_pe = dih_pe;
-    obs->dihedral_virial = dih_vir;
+    obs->bond_pe = static_cast<double>(bond_pe);
+    obs->bond_virial = static_cast<double>(bond_vir);
+    obs->angle_pe = static_cast<double>(angle_pe);
+    obs->angle_virial = static_cast<double>(angle_vir);
+    obs->dihedral_pe = static_cast<double>(dih_pe);
+    obs->dihedral_virial = static_cast<double>(dih_vir);
   }
 }
 
@@ -744,7 +769,8 @@ int orc_forces(const orc_system_t* sys, const uint32_t* counts, const uint32_t* 
     build_lists<float>(sys->n, PosF{sys->pos}, box, plan, plan.path, excl,
                        plan.search_cutoff_sq, lists);
   }
-  compute_forces<float>(sys, PosF{sys->pos}, make_box(sys->lo, sys->hi), lists, forces, obs);
+  compute_forces<float>(sys, PosF{sys->pos}, make_box(sys->lo, sys->hi), lists, forces, obs,
+                        kSinFloor32);
   return 0;
 }
 
@@ -759,7 +785,7 @@ int orc_forces_f64(const orc_system_t* sys, const double* pos64, double* forces,
   std::vector<std::vector<uint32_t>> lists;
   const double rc = sys->lj_cutoff;
   build_lists<double>(sys->n, PosD{pos64}, box, plan, plan.path, excl, rc * rc, lists);
-  compute_forces<double>(sys, PosD{pos64}, box, lists, forces, obs);
+  compute_forces<double>(sys, PosD{pos64}, box, lists, forces, obs, kSinFloor32);
   return 0;
 }
 
@@ -819,7 +845,7 @@ int orc_forces_f64_list(const orc_system_t* sys, const double* pos64, const uint
     lists[i].assign(row, row + counts[i]);
   }
   compute_forces<double>(sys, PosD{pos64}, make_box_t<double>(sys->lo, sys->hi), lists, forces,
-                         obs);
+                         obs, kSinFloor64);
   return 0;
 }
 
@@ -912,12 +938,102 @@ int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
 // the state, the neighbour test and the displacement test are binary64 in
 // both. x0, v0: initial positions and velocities [n][3].
 namespace {
+
+// Tangent of the forces, df = J dx with J = dF/dx (the Hessian of -U), for LJ
+// pairs over the lists (same r^2 < r_c^2 decision as compute_forces<double>)
+// and harmonic bonds: the linearisation of pair_lj.cu:56-66 and
+// bond_harmonic.cu:41-79. For a pair, d = x_i - x_j, u = r^2,
+// F_i = s(u) d, s = 24 eps / u (2 sr12 - sr6), ds/du = -24 eps / u^2 (14 sr12 -
+// 4 sr6): dF_i = s e + 2 ds/du (d . e) d with e = dx_i - dx_j. For a bond,
+// F_i = phi(r) d, phi = -k (r - r0) / r: dF_i = phi e - k r0 / r^3 (d . e) d.
+template <typename Pos, typename Box>
+void force_tangent(const orc_system_t* s, const Pos& pos, const Box& box,
+                   const std::vector<std::vector<uint32_t>>& lists, const double* dx,
+                   double* df) {
+  const uint32_t n = s->n;
+  const double Ld[3] = {static_cast<double>(box.L[0]), static_cast<double>(box.L[1]),
+                        static_cast<double>(box.L[2])};
+  const double hLd[3] = {static_cast<double>(box.hL[0]), static_cast<double>(box.hL[1]),
+                         static_cast<double>(box.hL[2])};
+  const double rc = static_cast<double>(s->lj_cutoff);
+  const double rc2 = rc * rc;
+  const auto table = lj_table(s);
+  const int T = s->n_types;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+    const uint32_t i = static_cast<uint32_t>(ii);
+    double gx = 0, gy = 0, gz = 0;
+    const int ti = s->type[i] - 1;
+    for (uint32_t j : lists[i]) {
+      const double dxr = mic_diff_any<double>(pos.x(i), pos.x(j), Ld[0], hLd[0]);
+      const double dyr = mic_diff_any<double>(pos.y(i), pos.y(j), Ld[1], hLd[1]);
+      const double dzr = mic_diff_any<double>(pos.z(i), pos.z(j), Ld[2], hLd[2]);
+      if (dxr * dxr + dyr * dyr + dzr * dzr >= rc2) {
+        continue;
+      }
+      const LjCoeff& c = table[static_cast<size_t>(ti) * T + (s->type[j] - 1)];
+      const Vec d = mic_vec(pos, i, j, Ld, hLd);
+      const double u = vdot(d, d);
+      const double sr2 = c.sigma2 / u;
+      const double sr6 = sr2 * sr2 * sr2;
+      const double sr12 = sr6 * sr6;
+      const double sc = c.eps24 / u * (2.0 * sr12 - sr6);
+      const double sp = -c.eps24 / (u * u) * (14.0 * sr12 - 4.0 * sr6);
+      const double ex = dx[3 * i] - dx[3 * j], ey = dx[3 * i + 1] - dx[3 * j + 1],
+                   ez = dx[3 * i + 2] - dx[3 * j + 2];
+      const double de = d.x * ex + d.y * ey + d.z * ez;
+      gx += sc * ex + 2.0 * sp * de * d.x;
+      gy += sc * ey + 2.0 * sp * de * d.y;
+      gz += sc * ez + 2.0 * sp * de * d.z;
+    }
+    df[3 * i] = gx;
+    df[3 * i + 1] = gy;
+    df[3 * i + 2] = gz;
+  }
+  std::vector<uint32_t> slot_of_tag(static_cast<size_t>(n) + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    slot_of_tag[s->tag[i]] = i;
+  }
+  for (uint32_t b = 0; b < s->n_bonds; ++b) {
+    const uint32_t si = slot_of_tag[s->bonds[3 * b]];
+    const uint32_t sj = slot_of_tag[s->bonds[3 * b + 1]];
+    const int ty = static_cast<int>(s->bonds[3 * b + 2]) - 1;
+    const double k = s->bond_k[ty], r0 = s->bond_r0[ty];
+    const Vec d = mic_vec(pos, si, sj, Ld, hLd);
+    const double r = std::sqrt(vdot(d, d));
+    if (r == 0.0) {
+      continue;
+    }
+    const double phi = -k * (r - r0) / r;
+    const double dphi_r = -k * r0 / (r * r * r);
+    const double ex = dx[3 * si] - dx[3 * sj], ey = dx[3 * si + 1] - dx[3 * sj + 1],
+                 ez = dx[3 * si + 2] - dx[3 * sj + 2];
+    const double de = d.x * ex + d.y * ey + d.z * ez;
+    const double t[3] = {phi * ex + dphi_r * de * d.x, phi * ey + dphi_r * de * d.y,
+                         phi * ez + dphi_r * de * d.z};
+    for (int a = 0; a < 3; ++a) {
+      df[3 * si + a] += t[a];
+      df[3 * sj + a] -= t[a];
+    }
+  }
+}
+
+// tan (optional): the tangent run (forward mode) beside the state, NVE only:
+// dv += h dF, dx += dt dv, dF = J(x) dx, permuted with the state by reorders.
+struct Tangent {
+  const double* dx0;
+  const double* dv0;
+  double* dx_out;
+  double* dv_out;
+};
+
 template <bool D>
 int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x0,
              const double* v0, double* pos_out, double* vel_out, int32_t* image_out,
-             uint32_t* tags_out, orc_run_stats_t* st) {
+             uint32_t* tags_out, orc_run_stats_t* st, const Tangent* tan = nullptr) {
   using CR = std::conditional_t<D, double, float>;
   using Plan = std::conditional_t<D, PlanD, orc_plan_t>;
+  const double sin_floor = D ? kSinFloor64 : kSinFloor32;
 #ifdef _OPENMP
   if (p->threads > 0) {
     omp_set_num_threads(p->threads);
@@ -944,6 +1060,12 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
   }
   for (uint32_t i = 0; i < n; ++i) {
     m[i] = sys->mass[i];
+  }
+  std::vector<double> dx, dv, df;
+  if (tan != nullptr) {
+    dx.assign(tan->dx0, tan->dx0 + 3 * static_cast<size_t>(n));
+    dv.assign(tan->dv0, tan->dv0 + 3 * static_cast<size_t>(n));
+    df.assign(3 * static_cast<size_t>(n), 0.0);
   }
   orc_system_t cur = *sys;
   cur.type = type.data();
@@ -988,6 +1110,10 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
       permute3(x);
       permute3(v);
       permute3(img);
+      if (tan != nullptr) {
+        permute3(dx);
+        permute3(dv);
+      }
       permute1(m);
       permute1(type);
       permute1(tag);
@@ -1022,7 +1148,10 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
   if (!rebuild()) {
     return 2;
   }
-  compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs);
+  compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs, sin_floor);
+  if (tan != nullptr) {
+    force_tangent(&cur, PosD{x.data()}, box, lists, dx.data(), df.data());
+  }
 
   const auto t0 = std::chrono::steady_clock::now();
   for (uint64_t step = 1; step <= p->steps; ++step) {
@@ -1037,6 +1166,10 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
         xa -= c * static_cast<double>(box.L[a]);
         x[3 * i + a] = xa;
         img[3 * i + a] += static_cast<int32_t>(c);
+        if (tan != nullptr) {
+          dv[3 * i + a] += h * df[3 * i + a];
+          dx[3 * i + a] += dt * dv[3 * i + a];
+        }
       }
     }
     bool need = false;
@@ -1067,7 +1200,16 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
     if (need && !rebuild()) {
       return 2;
     }
-    compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs);
+    compute_forces<double>(&cur, PosD{x.data()}, box, lists, f.data(), &obs, sin_floor);
+    if (tan != nullptr) {
+      force_tangent(&cur, PosD{x.data()}, box, lists, dx.data(), df.data());
+#pragma omp parallel for schedule(static)
+      for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
+        const size_t i = static_cast<size_t>(ii);
+        const double h = 0.5 * dt / m[i];
+        for (int a = 0; a < 3; ++a) dv[3 * i + a] += h * df[3 * i + a];
+      }
+    }
     double ke = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : ke)
     for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
@@ -1104,6 +1246,10 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
   std::copy(v.begin(), v.end(), vel_out);
   std::copy(img.begin(), img.end(), image_out);
   std::copy(tag.begin(), tag.end(), tags_out);
+  if (tan != nullptr) {
+    std::copy(dx.begin(), dx.end(), tan->dx_out);
+    std::copy(dv.begin(), dv.end(), tan->dv_out);
+  }
   return 0;
 }
 }  // namespace
@@ -1122,4 +1268,18 @@ extern "C" int orc_run_f64(const orc_system_t* sys, const orc_run_params_t* p,
                            double* vel_out, int32_t* image_out, uint32_t* tags_out,
                            orc_run_stats_t* st) {
   return run_impl<true>(sys, p, pos_in, vel_in, pos_out, vel_out, image_out, tags_out, st);
+}
+
+extern "C" int orc_run_tangent_f64(const orc_system_t* sys, const orc_run_params_t* p,
+                                   const double* pos_in, const double* vel_in,
+                                   const double* dpos_in, const double* dvel_in,
+                                   double* pos_out, double* vel_out, int32_t* image_out,
+                                   uint32_t* tags_out, double* dpos_out, double* dvel_out,
+                                   orc_run_stats_t* st) {
+  if (sys->n_angles != 0 || sys->n_dihedrals != 0 || p->thermostat != 0) {
+    g_error = "the tangent run covers LJ pairs and harmonic bonds under NVE only.";
+    return 1;
+  }
+  const Tangent tan{dpos_in, dvel_in, dpos_out, dvel_out};
+  return run_impl<true>(sys, p, pos_in, vel_in, pos_out, vel_out, image_out, tags_out, st, &tan);
 }

@@ -191,6 +191,15 @@ int orc_run_f64(const orc_system_t* sys, const orc_run_params_t* params,
                 const double* pos_in, const double* vel_in, double* pos_out,
                 double* vel_out, int32_t* image_out, uint32_t* tags_out,
                 orc_run_stats_t* stats);
+/* orc_run_f64 with its tangent (forward-mode derivative) beside the state:
+ * (dpos_in, dvel_in) is a perturbation of the initial state and
+ * (dpos_out, dvel_out) its image under the linearised dynamics, in the slot
+ * order of pos_out. LJ pairs and harmonic bonds under NVE only (else 1). */
+int orc_run_tangent_f64(const orc_system_t* sys, const orc_run_params_t* params,
+                        const double* pos_in, const double* vel_in, const double* dpos_in,
+                        const double* dvel_in, double* pos_out, double* vel_out,
+                        int32_t* image_out, uint32_t* tags_out, double* dpos_out,
+                        double* dvel_out, orc_run_stats_t* stats);
 
 #ifdef __cplusplus
 }

@@ -610,3 +610,35 @@ def test_run_f64_equals_run_on_binary32_inputs_and_conserves_energy():
     e0 = s0.final_ke + energy(s0.final_obs.as_dict())
     e1 = s1.final_ke + energy(s1.final_obs.as_dict())
     assert abs(e1 - e0) < 2e-4 * w.n
+
+
+def test_tangent_run_matches_central_differences_of_the_run():
+    """orc_run_tangent_f64 (forward-mode derivative of the binary64 run, the
+    reference dynamics linearised by hand) against central differences of
+    orc_run_f64 over a few steps, where the difference quotient is accurate;
+    its steps are sums of the one-step differentials, so a wrong term in
+    J dx (a dropped 2 ds/du term, a wrong sign of the bond's r0 term) or in
+    the kick order fails here."""
+    w = W.polymer_solvent(n_chains=4, beads_per_chain=10, n_solvent=200, seed=3, jitter=0.03,
+                          shift=True).as_float64(jitter=1e-4, seed=1)
+    w = w.with_(berendsen=False, cutoff=float(np.float32(w.cutoff)), cutoff_buffer=0.25,
+                rebuild_check_every=1, sort_every_rebuild=2)
+    rng = np.random.default_rng(4)
+    dpos = rng.standard_normal((w.n, 3))
+    dvel = rng.standard_normal((w.n, 3))
+    L = w.box_bound[1] - w.box_bound[0]
+    for steps in (1, 5):
+        ref, tan, _ = O.run_tangent_f64(w, w.positions, w.velocities, dpos, dvel, steps=steps)
+        order = np.argsort(ref["tags"])
+        h = 1e-7
+        outs = []
+        for sgn in (1, -1):
+            r, _ = O.run_f64(w, w.positions + sgn * h * dpos, w.velocities + sgn * h * dvel,
+                             steps=steps)
+            o = np.argsort(r["tags"])
+            outs.append((r["positions"][o] + r["images"][o] * L, r["velocities"][o]))
+        fd_x = (outs[0][0] - outs[1][0]) / (2 * h)
+        fd_v = (outs[0][1] - outs[1][1]) / (2 * h)
+        tx, tv = tan["dpos"][order], tan["dvel"][order]
+        assert np.max(np.abs(tx - fd_x)) <= 1e-5 * np.max(np.abs(fd_x)), steps
+        assert np.max(np.abs(tv - fd_v)) <= 1e-5 * np.max(np.abs(fd_v)), steps
