@@ -55,7 +55,8 @@ def _spec_fields(real):
         ("dt", real), ("berendsen", C.c_int32), ("berendsen_temperature", real),
         ("berendsen_tau", real),
         ("cutoff_buffer", real), ("rebuild_check_every", C.c_uint64),
-        ("sort_every_rebuild", C.c_uint64), ("max_neighbors", C.c_uint32),
+        ("sort_every_rebuild", C.c_uint64), ("sort_every_steps", C.c_uint64),
+        ("max_neighbors", C.c_uint32),
         ("half_list", C.c_int32),
         ("thermo_path", C.c_char_p), ("thermo_every", C.c_uint64),
         ("trajectory_path", C.c_char_p), ("trajectory_every", C.c_uint64),
@@ -265,6 +266,7 @@ def make_spec(w: Workload, precision: str | None = None):
     s.cutoff_buffer = w.cutoff_buffer
     s.rebuild_check_every = w.rebuild_check_every
     s.sort_every_rebuild = w.sort_every_rebuild
+    s.sort_every_steps = w.sort_every_steps
     s.max_neighbors = w.max_neighbors
     s.half_list = 1 if w.half_list else 0
     s.log_echo = 3

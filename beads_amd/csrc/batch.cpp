@@ -71,6 +71,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     d.dof = std::max(0.0, 3.0 * s.n_particles - 3.0);
     d.check_every = p.check_every;
     d.sort_every = p.sort_every;
+    d.sort_every_steps = s.sort_every_steps;
     desc_host_[k] = d;
     atoms += s.n_particles;
     cells += static_cast<uint64_t>(p.n_cells) + 1;

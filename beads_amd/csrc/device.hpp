@@ -79,7 +79,8 @@ struct alignas(256) Ctrl {
   uint64_t dangerous;
   uint64_t reorder_count;
   uint64_t neighbor_total;  // sum of neighbour counts of the current list
-  uint32_t pad0[4];
+  uint64_t clock;           // steps advanced (counted by the control kernel)
+  uint64_t last_sort;       // clock at the last reorder
   double lambda;            // Berendsen factor still to be applied to vel
   double ke_pre;            // kinetic energy after the post-force half-kick
   double ke;                // kinetic energy of the velocities as stored
@@ -90,7 +91,9 @@ struct alignas(256) Ctrl {
 
 struct NeighborControl {
   uint64_t check_every;
-  uint64_t sort_every;
+  uint64_t sort_every;       // reorder every sort_every-th build ...
+  uint64_t sort_every_steps; // ... or, when nonzero, at the first build at least this
+                             // many steps after the last reorder
   int32_t path;
 };
 

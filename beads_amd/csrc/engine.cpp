@@ -774,13 +774,14 @@ void Engine::set_up_params() {
 
   control_.check_every = plan_.check_every;
   control_.sort_every = plan_.sort_every;
+  control_.sort_every_steps = spec_.sort_every_steps;
   control_.path = plan_.path;
 }
 
 // NeighborSystem::rebuild_now (neighbor_system.cpp:107-146) as a predicated
 // launch sequence.
-void Engine::rebuild_sequence(bool forced) {
-  launch_control(ctrl_.data(), control_, forced, stream_);
+void Engine::rebuild_sequence(bool forced, uint64_t steps) {
+  launch_control(ctrl_.data(), control_, forced, steps, stream_);
   if (plan_.path != 0) {
     launch_bin(bin_params_, stream_);
     launch_cell_scan(scan_params_, stream_);
@@ -797,7 +798,7 @@ void Engine::force_pass(bool kick, bool observables) {
 
 void Engine::initialize(bool observables) {
   require(!frame_mode_, "an engine used in frame mode cannot run dynamics.", BEADS_E_RUNTIME);
-  rebuild_sequence(true);
+  rebuild_sequence(true, 0);
   force_pass(false, observables);
   BEADS_HIP(hipGetLastError());
   ke_from_scale_ = false;
@@ -813,7 +814,7 @@ void Engine::step_once(bool observables) {
   const bool check = !every && physical % plan_.check_every == 0;
   launch_integrate(integrate_params_, thermostat_, check, false, stream_);
   if (every || check) {
-    rebuild_sequence(every);
+    rebuild_sequence(every, every ? uint64_t{1} : plan_.check_every);
   }
   force_pass(true, observables);
   ke_from_scale_ = false;
@@ -1015,7 +1016,9 @@ void Engine::profile_advance(uint64_t n_steps, bool per_phase, beads_phase_times
     BEADS_HIP(hipEventRecord(ev[4 * k], stream_));
     launch_integrate(integrate_params_, thermostat_, check, false, stream_);
     BEADS_HIP(hipEventRecord(ev[4 * k + 1], stream_));
-    if (every || check) rebuild_sequence(every);
+    if (every || check) {
+      rebuild_sequence(every, every ? uint64_t{1} : plan_.check_every);
+    }
     BEADS_HIP(hipEventRecord(ev[4 * k + 2], stream_));
     force_pass(true, false);
     BEADS_HIP(hipEventRecord(ev[4 * k + 3], stream_));
@@ -1098,7 +1101,7 @@ void Engine::load_frame(const real_t* positions_dev, bool reuse_list) {
   lp.threshold_sq = integrate_params_.threshold_sq;
   lp.ctrl = ctrl_.data();
   launch_frame_load(lp, check, stream_);
-  rebuild_sequence(!check);
+  rebuild_sequence(!check, 0);
 }
 
 void Engine::frame_forces(const real_t* positions_dev, real_t* forces_dev, double* energies_out,
@@ -1554,8 +1557,11 @@ void Engine::log_run_start() {
       << label_value("path", plan_.path == 0 ? "direct"
                                             : (plan_.path == 1 ? "regular_cell" : "small_dimension"))
       << '\n' << label_value("rebuild_check_every", spec_.rebuild_check_every) << '\n'
-      << label_value("sort_every_rebuild", spec_.sort_every_rebuild) << '\n'
-      << label_value("max_neighbors", spec_.max_neighbors);
+      << label_value("sort_every_rebuild", spec_.sort_every_rebuild) << '\n';
+    if (spec_.sort_every_steps != 0) {
+      m << label_value("sort_every_steps", spec_.sort_every_steps) << '\n';
+    }
+    m << label_value("max_neighbors", spec_.max_neighbors);
     log("INFO", m.str());
   }
   {

@@ -130,7 +130,8 @@ class Engine {
   void load_frame(const real_t* positions_dev, bool reuse_list);
   void upload_state();
   void set_up_params();
-  void rebuild_sequence(bool forced);
+  // steps: steps advanced since the previous rebuild sequence (0 at set-up).
+  void rebuild_sequence(bool forced, uint64_t steps);
   void force_pass(bool kick, bool observables);
   void step_once(bool observables);
   uint64_t graph_steps() const;

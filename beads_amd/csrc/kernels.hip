@@ -41,11 +41,15 @@ using namespace dev;
 // ---------------------------------------------------------------- rebuild --
 
 // Rebuild bookkeeping of NeighborSystem::update_after_position_change /
-// rebuild_now (neighbor_system.cpp:79-146, 198-207), on the device.
-__global__ void control_kernel(Ctrl* ctrl, NeighborControl nc, int forced) {
+// rebuild_now (neighbor_system.cpp:79-146, 198-207), on the device. The sort
+// cadence counts builds (neighbor_system.cpp:198-202) or, with
+// sort_every_steps, steps: a build sorts when at least that many steps have
+// passed since the last sort (the sort needs the rebuild that follows it).
+__global__ void control_kernel(Ctrl* ctrl, NeighborControl nc, int forced, uint64_t steps) {
   if (threadIdx.x != 0 || blockIdx.x != 0) {
     return;
   }
+  ctrl->clock += steps;
   if (forced) {
     ctrl->rebuild = 1;
   } else {
@@ -57,9 +61,13 @@ __global__ void control_kernel(Ctrl* ctrl, NeighborControl nc, int forced) {
   }
   if (ctrl->rebuild) {
     ctrl->build_count += 1;
-    const bool reorder = nc.path != 0 && (ctrl->build_count % nc.sort_every) == 0;
+    const bool due = nc.sort_every_steps != 0
+                         ? ctrl->clock - ctrl->last_sort >= nc.sort_every_steps
+                         : (ctrl->build_count % nc.sort_every) == 0;
+    const bool reorder = nc.path != 0 && due;
     ctrl->reorder = reorder ? 1u : 0u;
     ctrl->reorder_count += reorder ? 1u : 0u;
+    if (reorder) ctrl->last_sort = ctrl->clock;
     ctrl->checked = 0;
     ctrl->neighbor_total = 0;
   }
@@ -1228,8 +1236,9 @@ __global__ __launch_bounds__(kBlock) void integrate_kernel(IntegrateParams p) {
 
 }  // namespace
 
-void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, hipStream_t stream) {
-  control_kernel<<<1, 64, 0, stream>>>(ctrl, control, forced ? 1 : 0);
+void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, uint64_t steps,
+                    hipStream_t stream) {
+  control_kernel<<<1, 64, 0, stream>>>(ctrl, control, forced ? 1 : 0, steps);
 }
 
 void launch_bin(const BinParams& p, hipStream_t stream) {

@@ -63,7 +63,9 @@ constexpr uint32_t kCandidatePad = 16;
 
 // Neighbour-rebuild sequence; every kernel is predicated on ctrl->rebuild so
 // the host never waits for the displacement check.
-void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, hipStream_t stream);
+// steps: steps advanced since the previous control launch (0 at set-up).
+void launch_control(Ctrl* ctrl, NeighborControl control, bool forced, uint64_t steps,
+                    hipStream_t stream);
 void launch_bin(const BinParams& p, hipStream_t stream);
 void launch_cell_scan(const CellScanParams& p, hipStream_t stream);
 void launch_cell_scatter(const CellScanParams& p, hipStream_t stream);

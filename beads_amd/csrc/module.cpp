@@ -403,8 +403,15 @@ void parse_dynamics(const py::dict& dyn, ParsedSpec& out) {
 
 void parse_neighbor(const py::dict& nb, ParsedSpec& out) {
   require_only_keys(nb, {"cutoff_buffer", "rebuild_check_every", "sort_every_rebuild", "max_neighbors",
-                         "half_list"},
+                         "half_list", "sort_every_steps"},
                     "neighbor");
+  // Optional, no reference counterpart: a step-based sort cadence.
+  out.c.sort_every_steps = 0;
+  if (nb.contains("sort_every_steps")) {
+    out.c.sort_every_steps =
+        require_uint64(nb["sort_every_steps"], "neighbor.sort_every_steps");
+    if (out.c.sort_every_steps == 0) value_error("neighbor.sort_every_steps must be positive.");
+  }
   // Optional, no reference counterpart: store each LJ pair once.
   out.c.half_list = 0;
   if (nb.contains("half_list")) {

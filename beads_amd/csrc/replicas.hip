@@ -67,7 +67,7 @@ __device__ void block_scan(const uint32_t* count, uint32_t* start, uint32_t* cur
 // workgroup): bin, counting sort, per-cell slot order, optional reorder, build.
 template <int BLOCK>
 __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, ReplicaState* st,
-                                ReplicaShared<BLOCK>& sh) {
+                                ReplicaShared<BLOCK>& sh, uint64_t now) {
   const uint32_t n = d.n, tid = threadIdx.x;
   real4* pos = A.pos + d.atom0;
   real4* vel = A.vel + d.atom0;
@@ -85,11 +85,16 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
   uint32_t* cursor = A.cell_cursor + d.cell0;
 
   uint64_t builds = st->build_count + 1;
-  const bool reorder = d.grid.path != 0 && builds % d.sort_every == 0;
+  const bool due = d.sort_every_steps != 0 ? now - st->last_sort_step >= d.sort_every_steps
+                                           : builds % d.sort_every == 0;
+  const bool reorder = d.grid.path != 0 && due;
   __syncthreads();
   if (tid == 0) {
     st->build_count = builds;
-    if (reorder) st->reorder_count += 1;
+    if (reorder) {
+      st->reorder_count += 1;
+      st->last_sort_step = now;
+    }
     st->checked = 0;
   }
   if (d.grid.path != 0) {
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
   double ke = 0.0, pe = 0.0, vir = 0.0;
   if (st->initialized == 0) {
     // Initial build and step-0 forces (simulation_run.cpp:113-119).
-    rebuild_replica<BLOCK>(d, A, st, sh);
+    rebuild_replica<BLOCK>(d, A, st, sh, 0);
     force_replica<BLOCK>(d, A, false, single, sh.lj, ke, pe, vir);
     ke = block_sum<BLOCK>(ke, sh.red);
     pe = block_sum<BLOCK>(pe, sh.red);
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
       }
     }
     if (rebuild) {
-      rebuild_replica<BLOCK>(d, A, st, sh);
+      rebuild_replica<BLOCK>(d, A, st, sh, physical);
     }
     const bool obs = observables_on_last && s + 1 == n_steps;
     ke = pe = vir = 0.0;

@@ -28,6 +28,7 @@ struct ReplicaDesc {
   double dof;
   uint64_t check_every;
   uint64_t sort_every;
+  uint64_t sort_every_steps;  // nonzero: step-based sort cadence (see NeighborControl)
 };
 
 // Per-replica run state carried across launches.
@@ -44,6 +45,7 @@ struct ReplicaState {
   double ke;      // kinetic energy after the last second half-kick
   double pe;      // pair energy (last observable step)
   double virial;
+  uint64_t last_sort_step;  // step of the last reorder (step-based cadence)
 };
 
 struct ReplicaArrays {

@@ -30,10 +30,16 @@ class NeighborSettings:
     # No reference counterpart: store each LJ pair once (j > i) and add the
     # reaction with atomics instead of keeping the reference's full list.
     half_list: bool = False
+    # No reference counterpart: sort at the first rebuild at least this many
+    # steps after the last sort, instead of every sort_every_rebuild-th build.
+    sort_every_steps: int | None = None
 
     def __post_init__(self) -> None:
         if not isinstance(self.half_list, bool):
             raise TypeError("neighbor half_list must be a bool")
+        if self.sort_every_steps is not None:
+            object.__setattr__(self, "sort_every_steps",
+                               positive_uint64("neighbor sort_every_steps", self.sort_every_steps))
         buf = nonnegative_real("neighbor cutoff_buffer", self.cutoff_buffer)
         check = positive_uint64("neighbor rebuild_check_every", self.rebuild_check_every)
         sort = positive_uint64("neighbor sort_every_rebuild", self.sort_every_rebuild)
@@ -157,10 +163,12 @@ class Simulation:
 
     def set_neighbor(self, *, cutoff_buffer: object = 0.3, rebuild_check_every: object = 1,
                      sort_every_rebuild: object = 10, max_neighbors: object = 512,
-                     half_list: bool = False) -> "Simulation":
+                     half_list: bool = False,
+                     sort_every_steps: object = None) -> "Simulation":
         self._require_not_executed()
         self._neighbor = NeighborSettings(cutoff_buffer, rebuild_check_every,
-                                          sort_every_rebuild, max_neighbors, half_list)
+                                          sort_every_rebuild, max_neighbors, half_list,
+                                          sort_every_steps)
         return self
 
     def set_thermo(self, *, every: object, prefix: object) -> "Simulation":
@@ -244,6 +252,8 @@ class Simulation:
                 "sort_every_rebuild": self._neighbor.sort_every_rebuild,
                 "max_neighbors": self._neighbor.max_neighbors,
                 **({"half_list": True} if self._neighbor.half_list else {}),
+                **({"sort_every_steps": self._neighbor.sort_every_steps}
+                   if self._neighbor.sort_every_steps is not None else {}),
             },
             "output": output,
             "runsteps": self._runsteps,

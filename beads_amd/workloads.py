@@ -56,6 +56,7 @@ class Workload:
     cutoff_buffer: float = 0.3
     rebuild_check_every: int = 1
     sort_every_rebuild: int = 10
+    sort_every_steps: int = 0  # nonzero: step-based sort cadence (replaces the rebuild count)
     max_neighbors: int = 512
     half_list: bool = False
     steps: int = 100
@@ -159,6 +160,7 @@ class Workload:
         sim.set_neighbor(cutoff_buffer=float(self.cutoff_buffer),
                          rebuild_check_every=int(self.rebuild_check_every),
                          sort_every_rebuild=int(self.sort_every_rebuild),
+                         sort_every_steps=int(self.sort_every_steps) or None,
                          max_neighbors=int(self.max_neighbors), half_list=bool(self.half_list))
         sim.set_runsteps(int(self.steps))
         return sim
@@ -324,8 +326,9 @@ def lj_mixture(n: int = 4_000_000, n_types: int = 4, density: float = 8.5, seed:
                steps: int = 200, temperature: float = 300.0, mass: float = 72.0) -> Workload:
     """Multi-type LJ mixture in nm_kjmol: types uniform at random, eps_ab ~ U[1,4]
     kJ/mol, sigma_ab ~ U[0.43,0.53] nm, r_c 1.1, skin 0.1, max_neighbors 128,
-    dt 0.01 ps. Positions: n distinct sites drawn from the smallest cubic lattice
-    with at least n sites, jittered by a tenth of the spacing."""
+    spatial sort every 100 steps (the step-based cadence), dt 0.01 ps.
+    Positions: n distinct sites drawn from the smallest cubic lattice with at
+    least n sites, jittered by a tenth of the spacing."""
     rng = np.random.default_rng(seed)
     L = (n / density) ** (1.0 / 3.0)
     side = int(math.ceil(n ** (1.0 / 3.0)))
@@ -346,7 +349,7 @@ def lj_mixture(n: int = 4_000_000, n_types: int = 4, density: float = 8.5, seed:
         types=rng.integers(1, n_types + 1, n).astype(np.int32),
         box_bound=np.stack([lo, hi]).astype(np.float32), lj_epsilon=eps, lj_sigma=sig,
         cutoff=1.1, shift=False, dt=0.01, cutoff_buffer=0.1, rebuild_check_every=1,
-        sort_every_rebuild=10, max_neighbors=128, steps=steps, seed=seed)
+        sort_every_rebuild=10, sort_every_steps=100, max_neighbors=128, steps=steps, seed=seed)
 
 
 def lj_box(n: int, density: float, temperature: float, seed: int, cutoff: float = 2.5,

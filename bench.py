@@ -15,8 +15,10 @@ roofline: the fused force kernel (the dominant kernel) against HBM peak.
   algorithmic bytes per atom = 60 + 4 K + A_bonded (read x+type 16, count 4,
   indices 4 K, velocity 12, mass 4; write velocity 12, force 12; listed terms
   12 per bond + 16 per angle + 20 per dihedral, per atom), K the mean stored
-  neighbour count; achieved = that x N / mean force-kernel duration, the
-  duration measured with HIP events on the engine's stream.
+  neighbour count; every real is twice as wide with BEADS_AMD_PRECISION=double
+  (the BEADS_REAL64 library): 116 + 4 K + 2 A_bonded. achieved = that x N /
+  mean force-kernel duration, the duration measured with HIP events on the
+  engine's stream.
 cpu_baseline: the FP64 OpenMP host oracle on a bounded sample of the same
   workload (same system, fewer steps), on this host's cores.
 
@@ -44,6 +46,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Bytes per real of the engine library the run uses (BEADS_AMD_PRECISION).
+REAL_SIZE = 8 if os.environ.get("BEADS_AMD_PRECISION", "single").strip().lower() == "double" else 4
 
 
 def make_workload(name: str):
@@ -58,9 +62,10 @@ def make_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def force_bytes_per_atom(w, mean_neighbors: float) -> float:
+def force_bytes_per_atom(w, mean_neighbors: float, real_size: int = 4) -> float:
+    scale = real_size / 4.0
     bonded = (12.0 * len(w.bonds) + 16.0 * len(w.angles) + 20.0 * len(w.dihedrals)) / w.n
-    return 60.0 + 4.0 * mean_neighbors + bonded
+    return 56.0 * scale + 4.0 + 4.0 * mean_neighbors + bonded * scale
 
 
 def cpu_model() -> str:
@@ -237,6 +242,8 @@ def measured_traffic(workload: str, mean_neighbors: float):
     path = ROOT / "profiles" / f"force_kernel_{workload}_hbm_traffic.md"
     if not path.exists():
         return None, None
+    if REAL_SIZE != 4:
+        return None, {"file": str(path.relative_to(ROOT)), "note": "profiled on the binary32 build"}
     traffic, k = None, None
     for line in path.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
@@ -367,7 +374,7 @@ def main() -> None:
         pt = eng.profile(psteps, per_phase=True)
         k_mean = eng.stats().mean_neighbors
         force_avg = pt.force_s / max(1, pt.force_launches)
-        per_atom = force_bytes_per_atom(w, k_mean)
+        per_atom = force_bytes_per_atom(w, k_mean, REAL_SIZE)
         achieved = per_atom * w.n / force_avg / 1e9
         traffic, traffic_src = measured_traffic(args.workload, k_mean)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -411,7 +418,7 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if REAL_SIZE == 8 else "f32",
         "data": "synthetic",
         "config": {"workload": description, "n_beads": w.n, "steps": args.steps,
                    "neighbor_builds_in_timed_steps": builds,

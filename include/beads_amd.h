@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BEADS_AMD_ABI_VERSION 3
+#define BEADS_AMD_ABI_VERSION 4
 
 #ifdef BEADS_REAL64
 typedef double beads_real_t;
@@ -150,6 +150,9 @@ typedef struct beads_spec {
   beads_real_t cutoff_buffer;
   uint64_t rebuild_check_every;
   uint64_t sort_every_rebuild;
+  uint64_t sort_every_steps;     /* 0: sort every sort_every_rebuild-th build (the reference's
+                                    rule); else sort at the first build at least this many
+                                    steps after the last sort (no reference counterpart) */
   uint32_t max_neighbors;
   int32_t half_list;             /* 0: full list (the reference's); 1: each pair stored once,
                                     j > i, reactions added with FP64 atomics (no reference

@@ -65,6 +65,7 @@ class RunParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("thermostat", C.c_int32), ("temperature", C.c_double),
                 ("tau", C.c_double), ("kb", C.c_double), ("cutoff_buffer", C.c_float),
                 ("rebuild_check_every", C.c_uint64), ("sort_every_rebuild", C.c_uint64),
+                ("sort_every_steps", C.c_uint64),
                 ("max_neighbors", C.c_uint32), ("steps", C.c_uint64), ("threads", C.c_int32)]
 
 
@@ -72,7 +73,7 @@ class RunStats(C.Structure):
     _fields_ = [("build_count", C.c_uint64), ("dangerous_rebuild_count", C.c_uint64),
                 ("wall_seconds", C.c_double), ("mean_neighbors", C.c_double),
                 ("threads_used", C.c_int32), ("final_ke", C.c_double),
-                ("final_obs", Observables)]
+                ("final_obs", Observables), ("reorder_count", C.c_uint64)]
 
 
 _lib = None
@@ -336,7 +337,8 @@ def _run_params(w, steps, threads) -> RunParams:
     return RunParams(dt=w.dt, thermostat=1 if w.berendsen else 0, temperature=w.temperature,
                      tau=w.tau, kb=w.kb, cutoff_buffer=w.cutoff_buffer,
                      rebuild_check_every=w.rebuild_check_every,
-                     sort_every_rebuild=w.sort_every_rebuild, max_neighbors=w.max_neighbors,
+                     sort_every_rebuild=w.sort_every_rebuild,
+                     sort_every_steps=w.sort_every_steps, max_neighbors=w.max_neighbors,
                      steps=w.steps if steps is None else steps, threads=threads)
 
 
@@ -344,10 +346,7 @@ def run(w, steps: int | None = None, threads: int = 0):
     """Whole FP64 run; returns final state (oracle slot order) and stats."""
     lib = library()
     sys = OracleSystem(w)
-    p = RunParams(dt=w.dt, thermostat=1 if w.berendsen else 0, temperature=w.temperature, tau=w.tau,
-                  kb=w.kb, cutoff_buffer=w.cutoff_buffer, rebuild_check_every=w.rebuild_check_every,
-                  sort_every_rebuild=w.sort_every_rebuild, max_neighbors=w.max_neighbors,
-                  steps=w.steps if steps is None else steps, threads=threads)
+    p = _run_params(w, steps, threads)
     n = w.n
     pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); img = np.zeros((n, 3), np.int32)
     tags = np.zeros(n, np.uint32)

@@ -932,7 +932,8 @@ int orc_displacement_exceeds(uint32_t n, const float* pos, const int32_t* image,
 
 // A whole run in binary64 (simulation_run.cpp:189-235 step order; neighbour
 // policy neighbor_system.cpp:79-146, 198-216; displacement test
-// rebuild_tracker.cu:55-94; reorder cadence counted in rebuilds). Used as the
+// rebuild_tracker.cu:55-94; reorder cadence counted in rebuilds, or in steps
+// when sort_every_steps is set, the engine's step-based option). Used as the
 // CPU baseline and for trajectory-level invariants. The cell grid and box are
 // the binary32 build's (D = false) or the BEADS_REAL64 build's (D = true);
 // the state, the neighbour test and the displacement test are binary64 in
@@ -1077,14 +1078,18 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
   std::vector<int32_t> imgref;
   SlotExclusions excl;
   bool excl_valid = false;
-  uint64_t build_count = 0, dangerous = 0;
+  uint64_t build_count = 0, dangerous = 0, reorders = 0, now = 0, last_sort = 0;
   bool checked_since_build = false;
   double neighbor_sum = 0.0;
   orc_observables_t obs{};
 
   auto rebuild = [&]() {
     ++build_count;
-    if (plan.path != ORC_PATH_DIRECT && build_count % p->sort_every_rebuild == 0) {
+    const bool due = p->sort_every_steps != 0 ? now - last_sort >= p->sort_every_steps
+                                              : build_count % p->sort_every_rebuild == 0;
+    if (plan.path != ORC_PATH_DIRECT && due) {
+      ++reorders;
+      last_sort = now;
       std::vector<uint32_t> cell(n), order, counts, offsets;
       for (uint32_t i = 0; i < n; ++i) {
         const uint32_t ix = axis_cell<CR>(static_cast<CR>(x[3 * i]), box.lo[0], plan.inv_cell[0], plan.nx);
@@ -1155,6 +1160,7 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
 
   const auto t0 = std::chrono::steady_clock::now();
   for (uint64_t step = 1; step <= p->steps; ++step) {
+    now = step;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii) {
       const size_t i = static_cast<size_t>(ii);
@@ -1234,6 +1240,7 @@ int run_impl(const orc_system_t* sys, const orc_run_params_t* p, const double* x
   st->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
   st->build_count = build_count;
   st->dangerous_rebuild_count = dangerous;
+  st->reorder_count = reorders;
   st->mean_neighbors = build_count ? neighbor_sum / static_cast<double>(build_count) : 0.0;
   double ke = 0.0;
   for (uint32_t i = 0; i < n; ++i) {

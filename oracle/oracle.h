@@ -91,6 +91,7 @@ typedef struct orc_run_params {
   float cutoff_buffer;
   uint64_t rebuild_check_every;
   uint64_t sort_every_rebuild;
+  uint64_t sort_every_steps; /* nonzero: sort at the first build >= this many steps after the last */
   uint32_t max_neighbors;
   uint64_t steps;
   int32_t threads;        /* 0 = OpenMP default */
@@ -104,6 +105,7 @@ typedef struct orc_run_stats {
   int32_t threads_used;
   double final_ke;
   orc_observables_t final_obs;
+  uint64_t reorder_count;
 } orc_run_stats_t;
 
 /* neighbor_plan.cpp:182-236. Returns 0, or 1 with a message for an invalid plan. */

@@ -207,26 +207,49 @@ def drifting_lattice(v_dt: float, check_every: int = 1, steps: int = 60, skin: f
         sort_every_rebuild=10, max_neighbors=64, steps=steps)
 
 
-def expected_builds(steps_to_cross: float, check_every: int, steps: int,
-                    skin: float) -> tuple[int, int]:
-    """(build_count, dangerous_rebuild_count) after `steps` steps when the
-    largest displacement since the last build is k / steps_to_cross times half
-    the skin after k steps. The rule of neighbor_system.cpp:79-105, 198-216:
-    skin 0 rebuilds after every step; otherwise the displacement is checked
-    when step % check_every == 0, a rebuild follows when it exceeds half the
-    skin (strictly), and a rebuild at the first check after a build counts as
-    dangerous when check_every > 1. The initial build counts once."""
+def expected_build_steps(steps_to_cross: float, check_every: int, steps: int,
+                         skin: float) -> tuple[list[int], int]:
+    """(steps at which a build happens, 0 being the initial build; dangerous
+    rebuild count) when the largest displacement since the last build is
+    k / steps_to_cross times half the skin after k steps. The rule of
+    neighbor_system.cpp:79-105, 198-216: skin 0 rebuilds after every step;
+    otherwise the displacement is checked when step % check_every == 0, a
+    rebuild follows when it exceeds half the skin (strictly), and a rebuild at
+    the first check after a build counts as dangerous when check_every > 1."""
     if skin == 0.0:
-        return steps + 1, 0
-    builds, dangerous, last, seen = 1, 0, 0, False
+        return list(range(steps + 1)), 0
+    builds, dangerous, last, seen = [0], 0, 0, False
     for step in range(1, steps + 1):
         if step % check_every:
             continue
         first, seen = not seen, True
         if step - last > steps_to_cross:
             dangerous += int(check_every > 1 and first)
-            builds, last, seen = builds + 1, step, False
+            builds.append(step)
+            last, seen = step, False
     return builds, dangerous
+
+
+def expected_builds(steps_to_cross: float, check_every: int, steps: int,
+                    skin: float) -> tuple[int, int]:
+    """(build_count, dangerous_rebuild_count) of expected_build_steps; the
+    initial build counts once."""
+    builds, dangerous = expected_build_steps(steps_to_cross, check_every, steps, skin)
+    return len(builds), dangerous
+
+
+def expected_reorders(build_steps: list[int], sort_every_rebuild: int,
+                      sort_every_steps: int = 0) -> int:
+    """Reorders of a run whose builds happen at build_steps: every
+    sort_every_rebuild-th build (neighbor_system.cpp:198-202), or, with
+    sort_every_steps, the first build at least that many steps after the last
+    sort (the start of the run counting as a sort at step 0)."""
+    count, last = 0, 0
+    for k, step in enumerate(build_steps, start=1):
+        due = (step - last >= sort_every_steps) if sort_every_steps else k % sort_every_rebuild == 0
+        if due:
+            count, last = count + 1, step
+    return count
 
 
 def cases() -> dict[str, W.Workload]:

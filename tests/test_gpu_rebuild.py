@@ -22,8 +22,8 @@ import pytest
 
 import oracle as O
 from beads_amd import workloads as W
-from beads_amd.capi import Engine
-from gpu_cases import drifting_lattice, expected_builds
+from beads_amd.capi import Batch, Engine
+from gpu_cases import drifting_lattice, expected_build_steps, expected_builds, expected_reorders
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,57 @@ def test_rebuild_cadence_closed_form(v_dt, check_every, steps):
         st = e.stats()
     assert st.step == steps
     assert (st.build_count, st.dangerous_rebuild_count) == want
+
+
+@pytest.mark.parametrize("sort_every_rebuild,sort_every_steps", [(3, 0), (10, 1), (10, 12),
+                                                                  (10, 25)])
+def test_reorder_cadence_closed_form(sort_every_rebuild, sort_every_steps):
+    """Sorting rebuilds: every sort_every_rebuild-th build (neighbor_system.cpp:
+    198-202), or with the step-based option the first build at least
+    sort_every_steps steps after the last sort; the plain engine, the batched
+    replica kernel and the oracle run all follow the closed form."""
+    v_dt = 0.15 / 4.5
+    steps = 60
+    w = drifting_lattice(v_dt, check_every=1, steps=steps).with_(
+        sort_every_rebuild=sort_every_rebuild, sort_every_steps=sort_every_steps)
+    build_steps, _ = expected_build_steps(0.15 / v_dt, 1, steps, 0.3)
+    want = expected_reorders(build_steps, sort_every_rebuild, sort_every_steps)
+    _, ost = O.run(w)
+    assert (ost.build_count, ost.reorder_count) == (len(build_steps), want)
+    with Engine(w) as e:
+        e.initialize(observables=False)
+        e.advance(steps)
+        st = e.stats()
+    assert (st.build_count, st.reorder_count) == (len(build_steps), want)
+    with Batch([w]) as b:
+        b.run(steps)
+        bs = b.stats(0)
+    assert (bs.build_count, bs.reorder_count) == (len(build_steps), want)
+
+
+def test_step_based_sort_cadence_through_the_public_api(tmp_path):
+    """neighbor sort_every_steps travels from set_neighbor through the spec dict
+    and the C ABI: the run-start log names it, and on an interacting system
+    (the mixture, skin 0.1, rebuilding every few steps) the engine sorts
+    exactly at the builds the rule selects from its own build steps."""
+    w = W.lj_mixture(n=20000, seed=2).with_(steps=40, sort_every_steps=15)
+    sim = w.to_simulation()
+    sim.set_logging(echo="log", prefix=str(tmp_path / "run"))
+    sim.execute()
+    log = (tmp_path / "run.log").read_text()
+    line = next(ln for ln in log.splitlines() if "sort_every_steps" in ln)
+    assert line.rstrip().endswith(": 15"), line
+    with Engine(w) as e:
+        e.initialize(observables=False)
+        build_steps, builds = [0], 1
+        for step in range(1, 41):
+            e.advance(1)
+            st = e.stats()
+            if st.build_count > builds:
+                build_steps.append(step)
+                builds = st.build_count
+    assert len(build_steps) > 4
+    assert st.reorder_count == expected_reorders(build_steps, w.sort_every_rebuild, 15) >= 2
 
 
 def test_zero_skin_rebuilds_after_every_step():

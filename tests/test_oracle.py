@@ -642,3 +642,21 @@ def test_tangent_run_matches_central_differences_of_the_run():
         tx, tv = tan["dpos"][order], tan["dvel"][order]
         assert np.max(np.abs(tx - fd_x)) <= 1e-5 * np.max(np.abs(fd_x)), steps
         assert np.max(np.abs(tv - fd_v)) <= 1e-5 * np.max(np.abs(fd_v)), steps
+
+
+@pytest.mark.parametrize("sort_every_rebuild,sort_every_steps", [(1, 0), (3, 0), (10, 0),
+                                                                  (10, 1), (10, 12), (10, 25)])
+def test_reorder_cadence_closed_form(sort_every_rebuild, sort_every_steps):
+    """Reorders of the oracle run on the drifting bead: every k-th build, or
+    the first build at least S steps after the last sort (step-based option)."""
+    from gpu_cases import expected_build_steps, expected_reorders
+
+    v_dt = 0.15 / 4.5
+    w = drifting_lattice(v_dt, check_every=1, steps=60).with_(
+        sort_every_rebuild=sort_every_rebuild, sort_every_steps=sort_every_steps)
+    build_steps, _ = expected_build_steps(0.15 / v_dt, 1, 60, 0.3)
+    _, st = O.run(w)
+    assert st.build_count == len(build_steps)
+    assert st.reorder_count == expected_reorders(build_steps, sort_every_rebuild, sort_every_steps)
+    if sort_every_steps == 1:
+        assert st.reorder_count == len(build_steps) - 1  # every build after the first
