@@ -444,6 +444,131 @@ constexpr int kBuildTeam = 4;
 constexpr int kBuildSlots = 4;
 constexpr uint32_t kBuildAtoms = kBlock / kBuildTeam;
 
+// How the build forms a candidate's minimum image (build_kernel's accept).
+constexpr int kShift = 0, kShiftYZ = 1, kGeneric = 2;
+using kShiftMode = std::integral_constant<int, kShift>;
+using kShiftYZMode = std::integral_constant<int, kShiftYZ>;
+using kGenericMode = std::integral_constant<int, kGeneric>;
+
+// Rows of three x-adjacent cells scanned per atom on the rows path (3 along
+// z times 3 along y).
+constexpr int kRows = 9;
+// A row descriptor's end index (below n < 2^28) carries the row's shift codes
+// in its top bits.
+constexpr int kRowCodeShift = 28;
+constexpr uint32_t kRowEndMask = (1u << kRowCodeShift) - 1u;
+// Candidate entries a build workgroup stages in LDS: the union of its atoms'
+// row runs, when it fits.
+constexpr uint32_t kStageCap = 20480 / sizeof(real4);
+
+// Neighbour k (0, 1, 2) along an axis of count cells, in the order of
+// unique_axis, and its image shift code (0: none, 1: +L, 2: -L): with at
+// least 5 cells per axis a candidate in a wrapped neighbour cell lies more
+// than L/2 away (mic adds +L or -L), and in any other neighbour cell less
+// than L/2 away (mic adds 0).
+__device__ inline uint32_t neighbour_cell(uint32_t cc, uint32_t count, int k, uint32_t& code) {
+  code = 0;
+  if (count <= 1) return 0u;
+  if (count == 2) return k == 0 ? (cc == 0 ? 1u : 0u) : cc;
+  if (k == 0) {
+    if (cc == 0) {
+      code = 1;
+      return count - 1;
+    }
+    return cc - 1;
+  }
+  if (k == 2) {
+    if (cc + 1 == count) {
+      code = 2;
+      return 0u;
+    }
+    return cc + 1;
+  }
+  return cc;
+}
+
+__device__ inline real_t shift_of(uint32_t code, real_t L) {
+  return code == 0 ? static_cast<real_t>(0) : (code == 1 ? L : -L);
+}
+
+// Members of cell cc with x-bin below m (cell_xbins; m in 0..kXBins).
+__device__ inline uint32_t members_below(const uint4* xbins, uint32_t cc, int m) {
+  if (m <= 0) return 0u;
+  const uint4 q = xbins[cc];
+  const int f = m - 1;
+  const uint32_t word = (f >> 1) == 0 ? q.x : ((f >> 1) == 1 ? q.y : ((f >> 1) == 2 ? q.z : q.w));
+  return (word >> (16 * (f & 1))) & 0xffffu;
+}
+
+// Rows path: regular grid, at least 5 cells per axis, atom i not in an
+// x-boundary cell. Each row of three x-adjacent cells is one run of the
+// candidate stream, ascending in x (members are in (x, slot) order). A row
+// whose cells lie at least gy along y and gz along z from the atom needs only
+// members within ext = sqrt(r_s^2 - gy^2 - gz^2) along x, and none when
+// gy^2 + gz^2 > r_s^2; the run starts and ends at the x-bins within ext of
+// the atom, one bin of margin on each side (a bin is an eighth of a cell).
+// The gaps are taken kGapSlack of a cell short, far above the rounding of the
+// cell coordinates, so a skipped member is farther than ext along x, or
+// farther than the gaps along y and z, by a margin that no rounding of r^2
+// closes: its r^2 exceeds r_s^2 and the build would reject it. The accepted
+// candidates and their order are those of the full scan. Writes row r (z
+// neighbour r / 3, y neighbour r % 3) as {run begin, run end | (y shift code
+// | z shift code << 2) << kRowCodeShift} for r = t, t + kBuildTeam, ... (the
+// team's lanes share the rows), widens the block's span of row r, [lo[r],
+// hi[r]), to cover a nonempty run, and returns false off the rows path.
+__device__ inline bool row_runs(const BuildParams& p, uint32_t i, uint32_t t, uint2* rows,
+                                uint32_t* lo_r, uint32_t* hi_r) {
+  const CellGridView& g = p.grid;
+  if (g.path != 1 || g.nx < 5 || g.ny < 5 || g.nz < 5) return false;
+  const uint32_t c = p.cell_of[i];
+  const uint32_t cx = c % g.nx, cyz = c / g.nx, cy = cyz % g.ny, cz = cyz / g.ny;
+  if (cx < 1 || cx + 1 >= g.nx) return false;
+  constexpr real_t kGapSlack = static_cast<real_t>(1.0 / 64.0);
+  const real4 xi = p.pos[i];
+  const real_t lo = p.box.lo[0], inv = g.inv_cell[0];
+  const real_t fy = (xi.y - p.box.lo[1]) * g.inv_cell[1] - static_cast<real_t>(cy);
+  const real_t fz = (xi.z - p.box.lo[2]) * g.inv_cell[2] - static_cast<real_t>(cz);
+  // Gap to the row below (k = 0) and above (k = 2) along one axis.
+  auto gap = [](real_t f, int k, real_t inv_cell) -> real_t {
+    const real_t d = k == 0 ? f - kGapSlack : (static_cast<real_t>(1) - f) - kGapSlack;
+    return k == 1 || d < 0 ? static_cast<real_t>(0) : d / inv_cell;
+  };
+#pragma unroll 1
+  for (uint32_t r = t; r < static_cast<uint32_t>(kRows); r += kBuildTeam) {
+    const int a = static_cast<int>(r / 3), b = static_cast<int>(r % 3);
+    const real_t gz = gap(fz, a, g.inv_cell[2]);
+    const real_t gy = gap(fy, b, g.inv_cell[1]);
+    const real_t ext2 = p.rs2 - (gy * gy + gz * gz);
+    if (ext2 < 0) {
+      rows[r] = make_uint2(0u, 0u);
+      continue;
+    }
+    const real_t ext = sqrt(ext2);
+    const real_t fl = ((xi.x - ext) - lo) * inv - static_cast<real_t>(cx - 1);
+    const real_t fr = ((xi.x + ext) - lo) * inv - static_cast<real_t>(cx + 1);
+    const int m_left = static_cast<int>(floor(fl * static_cast<real_t>(kXBins))) - 1;
+    const int m_right = static_cast<int>(floor(fr * static_cast<real_t>(kXBins))) + 2;
+    const int ml = m_left < 0 ? 0 : (m_left > kXBins ? kXBins : m_left);
+    const int mr = m_right < 0 ? 0 : (m_right > kXBins ? kXBins : m_right);
+    uint32_t code_y, code_z;
+    const uint32_t zc = neighbour_cell(cz, g.nz, a, code_z);
+    const uint32_t rowc = g.nx * (neighbour_cell(cy, g.ny, b, code_y) + g.ny * zc);
+    const uint32_t cl = rowc + cx - 1, cr = rowc + cx + 1;
+    const uint32_t s_l = p.cell_start[cl], s_m = p.cell_start[cl + 1];
+    const uint32_t s_r = p.cell_start[cr], e_r = p.cell_start[cr + 1];
+    const uint32_t n_l = s_m - s_l, n_r = e_r - s_r;
+    const uint32_t skip_l = n_l <= kXBinsMaxCount ? members_below(p.cell_xbins, cl, ml) : 0u;
+    const uint32_t keep_r = n_r <= kXBinsMaxCount ? members_below(p.cell_xbins, cr, mr) : n_r;
+    const uint32_t b_run = s_l + skip_l, e_run = s_r + keep_r;
+    rows[r] = make_uint2(b_run, e_run | ((code_y | (code_z << 2)) << kRowCodeShift));
+    if (b_run < e_run) {
+      atomicMin(lo_r + r, b_run);
+      atomicMax(hi_r + r, e_run);
+    }
+  }
+  return true;
+}
+
 // Full-list build with the append rule of full_neighbor_path_common.cuh:49-94:
 // skip self, skip excluded, keep r2 <= r_s^2 (reference minimum image and
 // operation order), overflow past max_neighbors. Each atom has a team of
@@ -469,9 +594,58 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
   const uint32_t t = threadIdx.x & (kBuildTeam - 1);
   const uint32_t team_lane0 = (threadIdx.x & 63u) & ~static_cast<uint32_t>(kBuildTeam - 1);
   const uint32_t below = (1u << t) - 1u;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBuildAtoms + threadIdx.x / kBuildTeam;
+  // The workgroup's atoms: kBuildAtoms consecutive entries of the candidate
+  // stream (cell order) on the cell paths, consecutive slots on the direct
+  // path.
+  const uint32_t q = xcd_block(blockIdx.x, gridDim.x) * kBuildAtoms + threadIdx.x / kBuildTeam;
+  const uint32_t i = q < p.n && p.grid.path != 0 ? id_of(p.cell_pos[q].w) : q;
   uint32_t count = 0;  // the same in every lane of a team
-  if (i < p.n) {       // uniform over a team
+  // Rows path: each team's row runs, and the workgroup's span of each row,
+  // staged in LDS (stage_off[r] - lo[r] is row r's stream-to-LDS offset)
+  // when the spans fit; the teams then read their candidates from LDS.
+  __shared__ uint2 row_lds[kBuildAtoms * kRows];
+  __shared__ real4 stage[kStageCap + kCandidatePad];
+  __shared__ uint32_t row_lo[kRows], row_hi[kRows], stage_off[kRows + 1];
+  if (threadIdx.x < kRows) {
+    row_lo[threadIdx.x] = 0xffffffffu;
+    row_hi[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  uint2* const team_rows = row_lds + (threadIdx.x / kBuildTeam) * kRows;
+  const bool rows_path = i < p.n && row_runs(p, i, t, team_rows, row_lo, row_hi);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t off = 0;
+    for (int r = 0; r < kRows; ++r) {
+      stage_off[r] = off;
+      off += row_hi[r] > row_lo[r] ? row_hi[r] - row_lo[r] : 0u;
+    }
+    stage_off[kRows] = off;
+  }
+  __syncthreads();
+  const bool staged = stage_off[kRows] <= kStageCap;  // uniform over the workgroup
+  if (staged) {
+    for (int r = 0; r < kRows; ++r) {
+      const uint32_t len = stage_off[r + 1] - stage_off[r];
+      for (uint32_t k = threadIdx.x; k < len; k += kBlock) {
+        stage[stage_off[r] + k] = p.cell_pos[row_lo[r] + k];
+      }
+    }
+    if (rows_path) {
+      // Re-base this lane's rows onto the stage (nonempty runs lie inside
+      // their row's span).
+      for (uint32_t r = t; r < static_cast<uint32_t>(kRows); r += kBuildTeam) {
+        const uint2 d = team_rows[r];
+        const uint32_t b_run = d.x, e_run = d.y & kRowEndMask;
+        if (b_run < e_run) {
+          const uint32_t shift = stage_off[r] - row_lo[r];
+          team_rows[r] = make_uint2(b_run + shift, ((e_run + shift) & kRowEndMask) | (d.y & ~kRowEndMask));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (i < p.n) {  // uniform over a team
     const real4 xi = p.pos[i];
     const uint32_t n = p.n;
     uint32_t ex[EXCL > 0 ? EXCL : 1];
@@ -498,29 +672,34 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     const uint32_t maxn = p.max_neighbors;
     // sh: the image shift of the candidate's cell, with mic(xi - xj) equal to
     // (xi - xj) + sh on every axis (cells of a regular grid with at least 5
-    // cells per axis); generic: mic() per candidate.
-    auto accept = [&](uint32_t j, real4 xj, auto generic, real3 sh) -> bool {
+    // cells per axis). mode: kShift adds sh on every axis, kShiftYZ only on y
+    // and z (a row of x-adjacent cells has no x shift; adding a zero would
+    // change nothing but the sign of a zero difference, which r^2 squares
+    // away), kGeneric takes mic() per candidate.
+    // Returns the wave's lanes that accept their candidate, as a lane mask:
+    // every test is a compare whose ballot is its lane mask, so the tests
+    // combine as scalar ANDs without branches.
+    auto accept = [&](uint32_t j, real4 xj, auto mode, real3 sh) -> uint64_t {
       real_t dx, dy, dz;
-      if (decltype(generic)::value) {
+      if (decltype(mode)::value == kGeneric) {
         dx = mic(xi.x - xj.x, L0, h0);
         dy = mic(xi.y - xj.y, L1, h1);
         dz = mic(xi.z - xj.z, L2, h2);
       } else {
-        dx = (xi.x - xj.x) + sh.x;
+        dx = decltype(mode)::value == kShift ? (xi.x - xj.x) + sh.x : xi.x - xj.x;
         dy = (xi.y - xj.y) + sh.y;
         dz = (xi.z - xj.z) + sh.z;
       }
       const real_t r2 = r2_strict(dx, dy, dz);
-      // Non-short-circuit tests: selects, no branches.
-      bool ok = !(r2 > p.rs2) & (HALF ? j > i : j != i);
+      uint64_t m = __builtin_amdgcn_ballot_w64(!(r2 > p.rs2)) &
+                   __builtin_amdgcn_ballot_w64(HALF ? j > i : j != i);
       if (EXCL > 0) {
-        bool hit = false;
 #pragma unroll
         for (int k = 0; k < EXCL; ++k) {
-          hit |= ex[k] == j;
+          m &= __builtin_amdgcn_ballot_w64(ex[k] != j);
         }
-        ok &= !hit;
       } else if (EXCL < 0) {
+        bool ok = __builtin_amdgcn_inverse_ballot_w64(m);
         if (ok) {
           for (uint32_t k = 0; k < ne; ++k) {
             if (p.excl[static_cast<size_t>(k) * n + i] == j) ok = false;
@@ -529,149 +708,134 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
             if (p.spill_excl[spill_off + k] == j) ok = false;
           }
         }
+        m = __builtin_amdgcn_ballot_w64(ok);
       }
-      return ok;
+      return m;
     };
     // Candidates [b, e) of a run (b < e) whose entry s is entry(s) = {x, y,
     // z, slot bits}, kBuildTeam * kBuildSlots per round.
-    auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto generic, real3 sh) {
-      for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
-        // Entries past the run (at most kCandidatePad - 1 past the stream's
-        // last atom) are loaded and then ignored.
-        static_assert(kBuildTeam * kBuildSlots <= kCandidatePad, "candidate padding");
-        real4 x[kBuildSlots];
+    // Atom i's entries of chunk c start at c * chunk_words + row_word.
+    const uint32_t row_word = i * kNbrChunk;
+    const uint32_t chunk_words = n * kNbrChunk;
+    // One round: candidates k + u * kBuildTeam + t (u < kBuildSlots) of the
+    // run ending at e (k < e).
+    auto round = [&](uint32_t k, uint32_t e, auto entry, auto mode, real3 sh) {
+      // Entries past the run (at most kCandidatePad - 1 past the stream's
+      // last atom) are loaded and then ignored.
+      static_assert(kBuildTeam * kBuildSlots <= kCandidatePad, "candidate padding");
+      real4 x[kBuildSlots];
 #pragma unroll
-        for (int u = 0; u < kBuildSlots; ++u) {
-          x[u] = entry(k + u * kBuildTeam + t);
-        }
+      for (int u = 0; u < kBuildSlots; ++u) {
+        x[u] = entry(k + t, u);
+      }
+      // Slot u of this lane is in the run while u * kBuildTeam < left.
+      const int left = static_cast<int>(e - k) - static_cast<int>(t);
 #pragma unroll
-        for (int u = 0; u < kBuildSlots; ++u) {
-          const uint32_t j = id_of(x[u].w);
-          const bool ok = (k + u * kBuildTeam + t < e) & accept(j, x[u], generic, sh);
-          const uint32_t bits =
-              static_cast<uint32_t>(__ballot(ok) >> team_lane0) & ((1u << kBuildTeam) - 1u);
-          if (ok) {
-            const uint32_t slot = count + __builtin_popcount(bits & below);
-            if (slot < maxn) {
-              p.nbr[nbr_chunk_base(slot / kNbrChunk, i, n) + (slot & (kNbrChunk - 1))] = j;
-            }
+      for (int u = 0; u < kBuildSlots; ++u) {
+        const uint32_t j = id_of(x[u].w);
+        const uint64_t m =
+            __builtin_amdgcn_ballot_w64(u * kBuildTeam < left) & accept(j, x[u], mode, sh);
+        const uint32_t bits = static_cast<uint32_t>(m >> team_lane0) & ((1u << kBuildTeam) - 1u);
+        if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+          const uint32_t slot = count + __builtin_popcount(bits & below);
+          if (slot < maxn) {
+            // Entry (chunk slot / kNbrChunk, atom i, slot % kNbrChunk);
+            // a chunk holds n * kNbrChunk < 2^32 entries.
+            const uint64_t w = static_cast<uint64_t>(slot / kNbrChunk) * chunk_words +
+                               (row_word | (slot & (kNbrChunk - 1)));
+            p.nbr[w] = j;
           }
-          count += __builtin_popcount(bits);
         }
+        count += __builtin_popcount(bits);
+      }
+    };
+    auto scan_range = [&](uint32_t b, uint32_t e, auto entry, auto mode, real3 sh) {
+      for (uint32_t k = b; k < e; k += kBuildTeam * kBuildSlots) {
+        round(k, e, entry, mode, sh);
       }
     };
     const real3 no_shift = make_real3(0, 0, 0);
+    // entry(k0, u): the stream's entry k0 + u * kBuildTeam, addressed by a
+    // 32-bit byte offset from the stream's base ((n + kCandidatePad) *
+    // sizeof(real4) < 2^32) plus u's constant, so that the slots of a round
+    // are immediate offsets of one address.
+    constexpr uint32_t kEntryBytes = sizeof(real4);
+    constexpr size_t kSlotBytes = kBuildTeam * sizeof(real4);
     if (p.grid.path == 0) {
-      const real4* pos = p.pos;
-      const auto entry = [pos](uint32_t k) {
-        const real4 q = pos[k];
-        return make_real4(q.x, q.y, q.z, real_of_id(k));
+      const char* pos_bytes = reinterpret_cast<const char*>(p.pos);
+      const auto entry = [pos_bytes](uint32_t k0, int u) {
+        const real4 q = *reinterpret_cast<const real4*>(
+            pos_bytes + static_cast<size_t>(k0 * kEntryBytes) + u * kSlotBytes);
+        return make_real4(q.x, q.y, q.z, real_of_id(k0 + u * kBuildTeam));
       };
-      scan_range(0, n, entry, std::true_type{}, no_shift);
+      scan_range(0, n, entry, kGenericMode{}, no_shift);
     } else {
       const uint32_t c = p.cell_of[i];
       const CellGridView& g = p.grid;
       const uint32_t cx = c % g.nx, cyz = c / g.nx, cy = cyz % g.ny, cz = cyz / g.ny;
-      const real4* cp = p.cell_pos;
-      const auto entry = [cp](uint32_t k) { return cp[k]; };
-      // Neighbour k (0, 1, 2) along an axis, in the order of unique_axis, and
-      // its image shift: with at least 5 cells per axis a candidate in a
-      // wrapped neighbour cell lies more than L/2 away (mic adds +L or -L),
-      // and in any other neighbour cell less than L/2 away (mic adds 0).
-      auto axis = [](uint32_t cc, uint32_t count, int k, real_t L, real_t& shift) -> uint32_t {
-        shift = 0;
-        if (count <= 1) return 0u;
-        if (count == 2) return k == 0 ? (cc == 0 ? 1u : 0u) : cc;
-        if (k == 0) {
-          if (cc == 0) {
-            shift = L;
-            return count - 1;
-          }
-          return cc - 1;
-        }
-        if (k == 2) {
-          if (cc + 1 == count) {
-            shift = -L;
-            return 0u;
-          }
-          return cc + 1;
-        }
-        return cc;
+      const char* cp_bytes = reinterpret_cast<const char*>(p.cell_pos);
+      const auto entry = [cp_bytes](uint32_t k0, int u) {
+        return *reinterpret_cast<const real4*>(
+            cp_bytes + static_cast<size_t>(k0 * kEntryBytes) + u * kSlotBytes);
       };
       const int mx = g.nx < 3 ? static_cast<int>(g.nx) : 3;
       const int my = g.ny < 3 ? static_cast<int>(g.ny) : 3;
       const int mz = g.nz < 3 ? static_cast<int>(g.nz) : 3;
-      // Members of cell cc with x-bin below m (cell_xbins; m in 0..kXBins).
-      auto below = [&](uint32_t cc, int m) -> uint32_t {
-        if (m <= 0) return 0u;
-        const uint4 q = p.cell_xbins[cc];
-        const int f = m - 1;
-        const uint32_t word = (f >> 1) == 0 ? q.x : ((f >> 1) == 1 ? q.y : ((f >> 1) == 2 ? q.z : q.w));
-        return (word >> (16 * (f & 1))) & 0xffffu;
-      };
-      // Regular grid, at least 5 cells per axis, atom not in an x-boundary
-      // cell: each row of three x-adjacent cells is one run of the candidate
-      // stream, ascending in x (members are in (x, slot) order), and the run
-      // starts and ends at the x-bins within r_s of the atom, one bin of
-      // margin on each side (a bin is an eighth of a cell, far above any
-      // rounding). The members skipped at the ends lie farther than r_s along
-      // x alone, so their r^2 exceeds r_s^2 and the build would reject them:
-      // the accepted candidates and their order are those of the full scan.
-      auto rows = [&]() {
-        const real_t rs = sqrt(p.rs2);
-        const real_t lo = p.box.lo[0], inv = g.inv_cell[0];
-        const real_t fl = ((xi.x - rs) - lo) * inv - static_cast<real_t>(cx - 1);
-        const real_t fr = ((xi.x + rs) - lo) * inv - static_cast<real_t>(cx + 1);
-        const int m_left = static_cast<int>(floor(fl * static_cast<real_t>(kXBins))) - 1;
-        const int m_right = static_cast<int>(floor(fr * static_cast<real_t>(kXBins))) + 2;
-        const int ml = m_left < 0 ? 0 : (m_left > kXBins ? kXBins : m_left);
-        const int mr = m_right < 0 ? 0 : (m_right > kXBins ? kXBins : m_right);
-#pragma unroll 1
-        for (int a = 0; a < 3; ++a) {
-          real_t sz;
-          const uint32_t zc = axis(cz, g.nz, a, L2, sz);
-#pragma unroll 1
-          for (int b = 0; b < 3; ++b) {
-            real_t sy;
-            const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
-            const uint32_t cl = rowc + cx - 1, cr = rowc + cx + 1;
-            const uint32_t s_l = p.cell_start[cl], s_m = p.cell_start[cl + 1];
-            const uint32_t s_r = p.cell_start[cr], e_r = p.cell_start[cr + 1];
-            const uint32_t n_l = s_m - s_l, n_r = e_r - s_r;
-            const uint32_t skip_l = n_l <= kXBinsMaxCount ? below(cl, ml) : 0u;
-            const uint32_t keep_r = n_r <= kXBinsMaxCount ? below(cr, mr) : n_r;
-            scan_range(s_l + skip_l, s_r + keep_r, entry, std::false_type{},
-                       make_real3(0, sy, sz));
+      // The team's rows in order, one run after another: a team moves to its
+      // next row as soon as its run ends, so that the wave's lanes wait only
+      // for the team with the most rounds in all, not in each row.
+      auto scan_rows = [&](auto row_entry) {
+        uint32_t r = 0, k = 0, e = 0;
+        real3 sh = make_real3(0, 0, 0);
+        for (;;) {
+          if (k >= e) {
+            bool more = true;
+            do {
+              if (r == static_cast<uint32_t>(kRows)) {
+                more = false;
+                break;
+              }
+              const uint2 d = team_rows[r++];
+              k = d.x;
+              e = d.y & kRowEndMask;
+              const uint32_t code = d.y >> kRowCodeShift;
+              sh = make_real3(0, shift_of(code & 3u, L1), shift_of(code >> 2, L2));
+            } while (k >= e);
+            if (!more) break;
           }
+          round(k, e, row_entry, kShiftYZMode{}, sh);
+          k += kBuildTeam * kBuildSlots;
         }
       };
-      auto cells = [&](auto generic) {
+      auto cells = [&](auto mode) {
 #pragma unroll 1
         for (int a = 0; a < mz; ++a) {
-          real_t sz;
-          const uint32_t zc = axis(cz, g.nz, a, L2, sz);
+          uint32_t code_z;
+          const uint32_t zc = neighbour_cell(cz, g.nz, a, code_z);
+          const real_t sz = shift_of(code_z, L2);
 #pragma unroll 1
           for (int b = 0; b < my; ++b) {
-            real_t sy;
-            const uint32_t rowc = g.nx * (axis(cy, g.ny, b, L1, sy) + g.ny * zc);
+            uint32_t code_y;
+            const uint32_t rowc = g.nx * (neighbour_cell(cy, g.ny, b, code_y) + g.ny * zc);
+            const real_t sy = shift_of(code_y, L1);
 #pragma unroll 1
             for (int d = 0; d < mx; ++d) {
-              real_t sx;
-              const uint32_t nc = axis(cx, g.nx, d, L0, sx) + rowc;
-              scan_range(p.cell_start[nc], p.cell_start[nc + 1], entry, generic,
-                         make_real3(sx, sy, sz));
+              uint32_t code_x;
+              const uint32_t nc = neighbour_cell(cx, g.nx, d, code_x) + rowc;
+              scan_range(p.cell_start[nc], p.cell_start[nc + 1], entry, mode,
+                         make_real3(shift_of(code_x, L0), sy, sz));
             }
           }
         }
       };
-      if (g.path == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {
-        if (cx >= 1 && cx + 1 < g.nx) {
-          rows();
-        } else {
-          cells(std::false_type{});
-        }
+      if (rows_path && staged) {
+        scan_rows([](uint32_t k0, int u) { return stage[k0 + u * kBuildTeam]; });
+      } else if (rows_path) {
+        scan_rows(entry);
+      } else if (g.path == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {
+        cells(kShiftMode{});
       } else {
-        cells(std::true_type{});
+        cells(kGenericMode{});
       }
     }
     if (t == 0) {
