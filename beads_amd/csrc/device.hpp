@@ -193,6 +193,7 @@ struct BuildParams {
   uint32_t* nbr;
   Ctrl* ctrl;
   int32_t half;  // keep only partners with a higher slot
+  uint32_t stage_cap;  // candidate entries a workgroup may stage in LDS (set by launch_build)
 };
 
 struct ReorderParams {

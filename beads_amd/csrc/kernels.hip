@@ -457,9 +457,9 @@ constexpr int kRows = 9;
 // in its top bits.
 constexpr int kRowCodeShift = 28;
 constexpr uint32_t kRowEndMask = (1u << kRowCodeShift) - 1u;
-// Candidate entries a build workgroup stages in LDS: the union of its atoms'
-// row runs, when it fits.
-constexpr uint32_t kStageCap = 20480 / sizeof(real4);
+// LDS bytes a build workgroup may use to stage the union of its atoms' row
+// runs (more LDS per workgroup costs occupancy).
+constexpr uint32_t kStageBytesMax = 24576;
 
 // Neighbour k (0, 1, 2) along an axis of count cells, in the order of
 // unique_axis, and its image shift code (0: none, 1: +L, 2: -L): with at
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
   // staged in LDS (stage_off[r] - lo[r] is row r's stream-to-LDS offset)
   // when the spans fit; the teams then read their candidates from LDS.
   __shared__ uint2 row_lds[kBuildAtoms * kRows];
-  __shared__ real4 stage[kStageCap + kCandidatePad];
+  extern __shared__ real4 stage[];  // p.stage_cap + kCandidatePad entries
   __shared__ uint32_t row_lo[kRows], row_hi[kRows], stage_off[kRows + 1];
   if (threadIdx.x < kRows) {
     row_lo[threadIdx.x] = 0xffffffffu;
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void build_kernel(BuildParams p) {
     stage_off[kRows] = off;
   }
   __syncthreads();
-  const bool staged = stage_off[kRows] <= kStageCap;  // uniform over the workgroup
+  const bool staged = stage_off[kRows] <= p.stage_cap;  // uniform over the workgroup
   if (staged) {
     for (int r = 0; r < kRows; ++r) {
       const uint32_t len = stage_off[r + 1] - stage_off[r];
@@ -1432,15 +1432,25 @@ void launch_pack_records(const PackParams& p, hipStream_t stream) {
   pack_records_kernel<<<grid_for(p.n), kBlock, 0, stream>>>(p);
 }
 
-void launch_build(const BuildParams& p, hipStream_t stream) {
-  const unsigned g = grid_for(p.n, kBuildAtoms);
+void launch_build(const BuildParams& params, hipStream_t stream) {
+  const unsigned g = grid_for(params.n, kBuildAtoms);
+  // Stage capacity from the mean cell occupancy m: a workgroup's kBuildAtoms
+  // atoms span about kBuildAtoms / m cells of one row of cells, so each of the
+  // kRows rows of the stencil holds about kBuildAtoms + 2m members; 15 %
+  // margin, whole multiples of 64 entries, at most kStageBytesMax.
+  BuildParams p = params;
+  const double m = p.grid.n_cells > 0 ? static_cast<double>(p.n) / p.grid.n_cells : 0.0;
+  const uint32_t want = static_cast<uint32_t>(1.15 * kRows * (kBuildAtoms + 2.0 * m) + 63.0) & ~63u;
+  const uint32_t most = kStageBytesMax / sizeof(real4) - kCandidatePad;
+  p.stage_cap = p.grid.path == 1 ? (want < most ? want : most) : 0u;
+  const size_t lds = p.grid.path == 1 ? (p.stage_cap + kCandidatePad) * sizeof(real4) : 0;
   // Exclusion registers: the fewest of 4 / 8 that hold every slot's
   // exclusions (ecap is the largest count), else read from memory.
   const int ex = p.spill_index != nullptr
                      ? -1
                      : (p.ecap == 0 ? 0 : (p.ecap <= 4 ? 4 : (p.ecap <= 8 ? 8 : -1)));
 #define BEADS_BUILD_CASE(E, H) \
-  if (ex == E && (p.half != 0) == H) build_kernel<E, H><<<g, kBlock, 0, stream>>>(p);
+  if (ex == E && (p.half != 0) == H) build_kernel<E, H><<<g, kBlock, lds, stream>>>(p);
   BEADS_BUILD_CASE(0, false)
   BEADS_BUILD_CASE(4, false)
   BEADS_BUILD_CASE(8, false)
