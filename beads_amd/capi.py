@@ -77,6 +77,20 @@ class Spec64(C.Structure):
     _fields_ = _spec_fields(C.c_double)
 
 
+BATCH_CONFIGS = 9  # BEADS_BATCH_CONFIGS
+
+
+class BatchConfig(C.Structure):
+    """beads_batch_config_t: mode 0 persistent (one workgroup per replica),
+    1 phased (per-step batched kernels over 64-atom chunks)."""
+    _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("team", C.c_int32),
+                ("slots", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"mode": "persistent" if self.mode == 0 else "phased", "block": self.block,
+                "team": self.team, "slots": self.slots}
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("step", C.c_uint64), ("build_count", C.c_uint64), ("dangerous_rebuild_count", C.c_uint64),
@@ -157,8 +171,11 @@ def signatures(precision: str = "single") -> dict:
                                                           C.c_void_p, C.c_void_p]),
         "beads_engine_destroy": (None, [C.c_void_p]),
         "beads_batch_create": (C.c_int, [C.POINTER(Spec), C.c_uint32, C.POINTER(C.c_void_p)]),
+        "beads_batch_config_info": (C.c_int, [C.c_uint32, C.POINTER(BatchConfig)]),
         "beads_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32),
                                            C.POINTER(C.c_double)]),
+        "beads_batch_set_config": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "beads_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
         "beads_batch_set_block": (C.c_int, [C.c_void_p, C.c_int32]),
         "beads_batch_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32]),
         "beads_batch_profile": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
@@ -434,7 +451,9 @@ def _state_buffers(n: int, precision: str):
 
 
 class Batch:
-    """Many pair-only systems advanced together by one persistent launch."""
+    """Many pair-only systems advanced together, either by one persistent
+    launch (one workgroup per replica) or by per-step batched kernels over
+    64-atom chunks of every replica (phased)."""
 
     def __init__(self, workloads: list[Workload], precision: str | None = None):
         self.precision = precision or selected_precision()
@@ -471,11 +490,29 @@ class Batch:
         except Exception:
             pass
 
+    def config_info(self, index: int) -> dict:
+        c = BatchConfig()
+        self._check(self.lib.beads_batch_config_info(index, C.byref(c)))
+        return c.as_dict()
+
+    def configs(self) -> list[dict]:
+        return [self.config_info(k) for k in range(BATCH_CONFIGS)]
+
     def autotune(self, trial_steps: int = 20) -> tuple[int, list[float]]:
-        block = C.c_int32()
-        secs = (C.c_double * 3)()
-        self._check(self.lib.beads_batch_autotune(self.handle, trial_steps, C.byref(block), secs))
-        return block.value, list(secs)
+        """Times every configuration from the initial state, keeps the fastest
+        (returned as its index) and restores the initial state."""
+        chosen = C.c_int32()
+        secs = (C.c_double * BATCH_CONFIGS)()
+        self._check(self.lib.beads_batch_autotune(self.handle, trial_steps, C.byref(chosen), secs))
+        return chosen.value, list(secs)
+
+    def set_config(self, index: int) -> None:
+        self._check(self.lib.beads_batch_set_config(self.handle, index))
+
+    def config(self) -> int:
+        k = C.c_uint32()
+        self._check(self.lib.beads_batch_get_config(self.handle, C.byref(k)))
+        return k.value
 
     def set_block(self, block: int) -> None:
         self._check(self.lib.beads_batch_set_block(self.handle, block))

@@ -1,6 +1,7 @@
-// Host side of the batched-replica mode: validates and packs many LJ systems
-// into one set of device arrays, picks the workgroup size with a small
-// autotuner, and advances every replica with one persistent launch that is
+// Host side of the batched replicas: validates and packs many LJ systems into
+// one set of device arrays, picks the execution configuration with a small
+// autotuner, and advances every replica either with one persistent launch per
+// run or with one graph of batched kernels per step (phased mode), each
 // captured once into a hipGraph and replayed.
 #include "batch.hpp"
 
@@ -38,7 +39,8 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   for (uint32_t k = 0; k < count; ++k) position_of_[order_[k]] = k;
 
   desc_host_.resize(count);
-  uint64_t atoms = 0, cells = 0, nbr = 0, lj = 0;
+  uint64_t atoms = 0, cells = 0, nbr = 0, lj = 0, chunks = 0;
+  std::vector<uint32_t> chunk_replica;
   for (uint32_t k = 0; k < count; ++k) {
     const uint32_t r = order_[k];
     const beads_spec_t& s = specs[r];
@@ -72,6 +74,10 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     d.check_every = p.check_every;
     d.sort_every = p.sort_every;
     d.sort_every_steps = s.sort_every_steps;
+    d.chunk0 = static_cast<uint32_t>(chunks);
+    d.n_chunks = (s.n_particles + kReplicaChunk - 1) / kReplicaChunk;
+    chunks += d.n_chunks;
+    chunk_replica.insert(chunk_replica.end(), d.n_chunks, k);
     desc_host_[k] = d;
     atoms += s.n_particles;
     cells += static_cast<uint64_t>(p.n_cells) + 1;
@@ -81,6 +87,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   require(atoms <= 0xffffffffull && cells <= 0xffffffffull,
           "batch exceeds 2^32 atoms or cells.", BEADS_E_OVERFLOW);
   total_atoms_ = atoms;
+  total_chunks_ = static_cast<uint32_t>(chunks);
 
   // Host images of the initial state, kept to restore after autotuning.
   init_pos_.resize(atoms);
@@ -125,6 +132,13 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   s_image_.resize(n);
   s_tag_.resize(n);
   s_cell_of_.resize(n);
+  cell_tmp_.resize(n);
+  cell_pos_.resize(n);
+  chunk_replica_.upload(chunk_replica.data(), chunk_replica.size(), stream_);
+  work_.resize(chunks);
+  work_reorder_.resize(chunks);
+  work_count_.resize(2);
+  partials_.resize(3 * chunks);
   state_.resize(count);
   reset_state();
 
@@ -148,6 +162,15 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   A.s_image = s_image_.data();
   A.s_tag = s_tag_.data();
   A.s_cell_of = s_cell_of_.data();
+  A.cell_tmp = cell_tmp_.data();
+  A.cell_pos = cell_pos_.data();
+  params_.count = count;
+  params_.total_chunks = total_chunks_;
+  params_.chunk_replica = chunk_replica_.data();
+  params_.partials = partials_.data();
+  params_.work = work_.data();
+  params_.work_reorder = work_reorder_.data();
+  params_.work_count = work_count_.data();
   params_.desc = desc_.data();
   params_.state = state_.data();
   params_.lj = lj_.data();
@@ -176,6 +199,8 @@ void Batch::reset_state() {
   cell_count_.zero(stream_);
   cell_start_.zero(stream_);
   cell_of_.zero(stream_);
+  work_count_.zero(stream_);
+  initialized_ = false;
   std::vector<ReplicaState> st(count_);
   for (auto& s : st) {
     std::memset(&s, 0, sizeof(s));
@@ -184,25 +209,58 @@ void Batch::reset_state() {
   state_.upload(st.data(), count_, stream_);
 }
 
-hipGraphExec_t Batch::graph_for(int block, uint64_t steps, bool obs) {
+static_assert(Batch::kConfigs == BEADS_BATCH_CONFIGS, "configuration table size");
+
+beads_batch_config_t Batch::config_info(uint32_t index) {
+  static const beads_batch_config_t table[kConfigs] = {
+      {0, 256, 4, 4}, {0, 512, 4, 4}, {0, 1024, 4, 4},  // persistent
+      {1, 256, 4, 4}, {1, 128, 2, 8}, {1, 512, 8, 2},   // phased
+      {1, 256, 4, 8}, {1, 512, 8, 4}, {1, 128, 2, 4}};
+  require(index < kConfigs, "batch configuration index out of range.", BEADS_E_INDEX);
+  return table[index];
+}
+
+void Batch::set_config(uint32_t index) {
+  (void)config_info(index);
+  config_ = index;
+}
+
+hipGraphExec_t Batch::graph_for(uint32_t config, uint64_t steps, bool obs) {
   for (const auto& g : graphs_) {
-    if (g.block == block && g.steps == steps && g.obs == obs) return g.exec;
+    if (g.config == config && g.steps == steps && g.obs == obs) return g.exec;
   }
+  const beads_batch_config_t c = config_info(config);
   hipGraph_t graph;
   BEADS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  launch_replicas(params_, count_, block, steps, obs, stream_);
+  if (c.mode == 0) {
+    launch_replicas(params_, count_, c.block, steps, obs, stream_);
+  } else {
+    launch_phased_step(params_, PhasedShape{c.team, c.slots}, steps == 0, obs, stream_);
+  }
   BEADS_HIP(hipStreamEndCapture(stream_, &graph));
   hipGraphExec_t exec;
   BEADS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   BEADS_HIP(hipGraphDestroy(graph));
-  graphs_.push_back({block, steps, obs, exec});
+  graphs_.push_back({config, steps, obs, exec});
   return exec;
 }
 
 void Batch::run(uint64_t steps, bool observables_on_last) {
-  // steps == 0 still launches: the first launch performs every replica's
-  // initial build and step-0 forces.
-  BEADS_HIP(hipGraphLaunch(graph_for(block_, steps, observables_on_last), stream_));
+  const beads_batch_config_t c = config_info(config_);
+  if (c.mode == 0) {
+    // steps == 0 still launches: the first launch performs every replica's
+    // initial build and step-0 forces.
+    BEADS_HIP(hipGraphLaunch(graph_for(config_, steps, observables_on_last), stream_));
+  } else {
+    if (!initialized_) {
+      BEADS_HIP(hipGraphLaunch(graph_for(config_, 0, false), stream_));
+    }
+    for (uint64_t s = 0; s < steps; ++s) {
+      const bool obs = observables_on_last && s + 1 == steps;
+      BEADS_HIP(hipGraphLaunch(graph_for(config_, 1, obs), stream_));
+    }
+  }
+  initialized_ = true;
   steps_done_ += steps;
 }
 
@@ -223,28 +281,27 @@ double Batch::profile(uint64_t steps) {
   return ms * 1e-3;
 }
 
-// Autotuner: time each candidate workgroup size on the real batch from the
-// initial state, keep the fastest, then restore the initial state.
+// Autotuner: time each configuration on the real batch from the initial
+// state, keep the fastest, then restore the initial state.
 int Batch::autotune(uint32_t trial_steps, double* seconds_out) {
-  const int candidates[] = {256, 512, 1024};
   double best = 0.0;
-  int best_block = block_;
-  for (int k = 0; k < 3; ++k) {
+  uint32_t best_config = config_;
+  for (uint32_t k = 0; k < kConfigs; ++k) {
     reset_state();
-    block_ = candidates[k];
-    run(1, false);  // initial build + forces, outside the timing
+    config_ = k;
+    run(1, false);  // initial build + forces and the graphs' first replay, outside the timing
     const double t = profile(trial_steps);
     if (seconds_out != nullptr) seconds_out[k] = t;
     if (k == 0 || t < best) {
       best = t;
-      best_block = candidates[k];
+      best_config = k;
     }
   }
-  block_ = best_block;
+  config_ = best_config;
   reset_state();
   steps_done_ = 0;
   synchronize();
-  return block_;
+  return static_cast<int>(config_);
 }
 
 void Batch::replica_stats(uint32_t r, beads_engine_stats_t* out) {

@@ -266,11 +266,29 @@ beads_status_t beads_batch_create(const beads_spec_t* specs, uint32_t count, bea
   });
 }
 
+beads_status_t beads_batch_config_info(uint32_t index, beads_batch_config_t* out) {
+  return guarded([&] {
+    beads_amd::require(out != nullptr, "config output is null");
+    *out = beads_amd::Batch::config_info(index);
+  });
+}
+
 beads_status_t beads_batch_autotune(beads_batch_t* b, uint32_t trial_steps, int32_t* chosen,
                                     double* seconds) {
   return guarded([&] {
-    const int block = batch_of(b).autotune(trial_steps, seconds);
-    if (chosen != nullptr) *chosen = block;
+    const int config = batch_of(b).autotune(trial_steps, seconds);
+    if (chosen != nullptr) *chosen = config;
+  });
+}
+
+beads_status_t beads_batch_set_config(beads_batch_t* b, uint32_t index) {
+  return guarded([&] { batch_of(b).set_config(index); });
+}
+
+beads_status_t beads_batch_get_config(beads_batch_t* b, uint32_t* index) {
+  return guarded([&] {
+    beads_amd::require(index != nullptr, "config output is null");
+    *index = batch_of(b).config();
   });
 }
 
@@ -278,7 +296,10 @@ beads_status_t beads_batch_set_block(beads_batch_t* b, int32_t block) {
   return guarded([&] {
     beads_amd::require(block == 256 || block == 512 || block == 1024,
                        "batch workgroup size must be 256, 512 or 1024.");
-    batch_of(b).set_block(block);
+    for (uint32_t k = 0; k < beads_amd::Batch::kConfigs; ++k) {
+      const beads_batch_config_t c = beads_amd::Batch::config_info(k);
+      if (c.mode == 0 && c.block == block) batch_of(b).set_config(k);
+    }
   });
 }
 

@@ -1,11 +1,25 @@
-// Batched replicas: many independent LJ systems advanced by ONE persistent
-// launch. Each workgroup owns one replica and runs its whole step loop --
+// Batched replicas: many independent LJ systems advanced together, in one of
+// two modes.
+//
+// Persistent mode: ONE launch for all steps. Each workgroup owns one replica
+// and runs its whole step loop --
 // integrate + displacement check, the cell-list rebuild with stable sort and
 // reorder, the full-list LJ force with the second half-kick and the Berendsen
 // factor -- with workgroup barriers between phases and no grid-wide sync.
 // Replicas are independent, so the launch needs no inter-workgroup
 // communication; uneven sizes are balanced by dispatching the largest replicas
 // first (the host orders the descriptors) and by the autotuned workgroup size.
+//
+// Phased mode: each step is a sequence of batched kernels whose boundaries
+// stand for the persistent kernel's workgroup barriers. Per-atom work goes by
+// chunks of kReplicaChunk atoms of one replica (one wave each), so a replica
+// spreads over as many waves as its size needs and no replica sets a critical
+// path; the rebuild kernels walk a device-built list of the chunks of the
+// replicas that rebuild in this step, so a step without rebuilds costs them a
+// launch each and no more. Both modes share the per-atom code below (integrate
+// and check, list build, team force), so they make the same decisions with
+// the same arithmetic; the phased sort ranks a cell's members by slot instead
+// of insertion-sorting them (the same order).
 //
 // Per replica the arithmetic is the single-system path's (same decisions, same
 // FP64 pair values and reductions): velocity_verlet.cu:59-111,
@@ -14,6 +28,8 @@
 // pair_lj.cu:21-167, neighbor_system.cpp:79-146,198-216 (paths under the
 // reference's native/src).
 #include "replicas.hpp"
+
+#include <algorithm>
 
 #include "device_common.hpp"
 
@@ -63,6 +79,205 @@ __device__ void block_scan(const uint32_t* count, uint32_t* start, uint32_t* cur
   __syncthreads();
 }
 
+// Velocity-Verlet first half-kick (with the pending Berendsen factor), drift,
+// wrap and image update of atom a of a replica (arrays offset to the replica),
+// and, when check, the displacement test against the reference positions.
+// Returns 1 when the atom moved farther than the half skin.
+__device__ __forceinline__ int integrate_atom(const ReplicaDesc& d, real4* pos, real4* vel,
+                                              const real4* force, int4* image,
+                                              const real4* ref_pos, const int4* ref_image,
+                                              uint32_t a, real_t lam, bool check) {
+  real4 v = vel[a];
+  const real4 f = force[a];
+  real4 x = pos[a];
+  if (d.berendsen) {
+    v.x *= lam;
+    v.y *= lam;
+    v.z *= lam;
+  }
+  const real_t h = real_t(0.5) * d.dt / v.w;
+  v.x += h * f.x;
+  v.y += h * f.y;
+  v.z += h * f.z;
+  x.x += d.dt * v.x;
+  x.y += d.dt * v.y;
+  x.z += d.dt * v.z;
+  real_t c[3];
+  {
+#pragma clang fp contract(off)
+    c[0] = floor((x.x - d.box.lo[0]) * d.box.invL[0]);
+    c[1] = floor((x.y - d.box.lo[1]) * d.box.invL[1]);
+    c[2] = floor((x.z - d.box.lo[2]) * d.box.invL[2]);
+    x.x -= c[0] * d.box.L[0];
+    x.y -= c[1] * d.box.L[1];
+    x.z -= c[2] * d.box.L[2];
+  }
+  pos[a] = x;
+  vel[a] = v;
+  int4 im = image[a];
+  if (c[0] != real_t(0) || c[1] != real_t(0) || c[2] != real_t(0)) {
+    im.x += static_cast<int>(c[0]);
+    im.y += static_cast<int>(c[1]);
+    im.z += static_cast<int>(c[2]);
+    image[a] = im;
+  }
+  int exceeded = 0;
+  if (check) {
+#pragma clang fp contract(off)
+    const real4 r = ref_pos[a];
+    const int4 ri = ref_image[a];
+    const real_t dx = (x.x - r.x) + static_cast<real_t>(im.x - ri.x) * d.box.L[0];
+    const real_t dy = (x.y - r.y) + static_cast<real_t>(im.y - ri.y) * d.box.L[1];
+    const real_t dz = (x.z - r.z) + static_cast<real_t>(im.z - ri.z) * d.box.L[2];
+    if (dx * dx + dy * dy + dz * dz > d.thr2) exceeded = 1;
+  }
+  return exceeded;
+}
+
+// Full neighbour list of atom i of a replica (arrays offset to the replica):
+// candidates in the 27 (or per-axis unique) neighbour cells in neighbourhood
+// order, members of a cell in slot order, or every atom on the direct path.
+// Captures the reference position. Returns 1 on overflow.
+__device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4* pos,
+                                               const int4* image, const uint32_t* cell_of,
+                                               const uint32_t* cell_atoms, const uint32_t* start,
+                                               uint32_t* nbr, uint32_t* nbr_count,
+                                               real4* ref_pos, int4* ref_image, uint32_t i) {
+  const uint32_t n = d.n;
+  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
+  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
+  uint32_t overflow = 0;
+  const real4 xi = pos[i];
+  uint32_t cnt = 0;
+  auto visit = [&](uint32_t j) {
+    const real4 xj = pos[j];
+    const real_t dx = mic(xi.x - xj.x, L0, h0);
+    const real_t dy = mic(xi.y - xj.y, L1, h1);
+    const real_t dz = mic(xi.z - xj.z, L2, h2);
+    if (r2_strict(dx, dy, dz) > d.rs2 || j == i) return;
+    if (cnt < d.max_neighbors) {
+      nbr[static_cast<size_t>(cnt) * n + i] = j;
+      ++cnt;
+    } else {
+      overflow = 1;
+    }
+  };
+  if (d.grid.path == 0) {
+    for (uint32_t j = 0; j < n; ++j) visit(j);
+  } else {
+    const CellNeighborhood h = neighborhood(cell_of[i], d.grid);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a >= h.nz) break;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (b >= h.ny) break;
+        const uint32_t row = d.grid.nx * (h.ys[b] + d.grid.ny * h.zs[a]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (c >= h.nx) break;
+          const uint32_t cell = h.xs[c] + row;
+          for (uint32_t k = start[cell]; k < start[cell + 1]; ++k) visit(cell_atoms[k]);
+        }
+      }
+    }
+  }
+  nbr_count[i] = cnt;
+  ref_pos[i] = xi;
+  ref_image[i] = image[i];
+  return overflow;
+}
+
+// Full-list LJ force of atom i of a replica (arrays offset to the replica) by
+// a team of TEAM adjacent lanes (lane t of the team): lane t takes list slots
+// base + u*TEAM + t, SLOTS of them in flight per round; the team's FP64 partial
+// forces meet in a butterfly and lane 0 stores the force and, when kick, the
+// second half-kick. Adds this lane's share of the kinetic energy (lane 0),
+// pair energy and virial.
+template <int TEAM, int SLOTS>
+__device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* pos, real4* vel,
+                                           real4* force, const uint32_t* nbr_count,
+                                           const uint32_t* nbr, const LjCoeff* lj, bool kick,
+                                           bool single, uint32_t i, uint32_t t, double& ke,
+                                           double& pe, double& vir) {
+  const uint32_t n = d.n;
+  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
+  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
+  const int T = d.n_types;
+  const LjCoeff c0 = lj[0];
+  const real4 xi = pos[i];
+  const int ti = static_cast<int>(id_of(xi.w));
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
+  const uint32_t cnt = nbr_count[i];
+  for (uint32_t base = 0; base < cnt; base += TEAM * SLOTS) {
+    uint32_t jj[SLOTS];
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+      const uint32_t s = base + u * TEAM + t;
+      jj[u] = s < cnt ? nbr[static_cast<size_t>(s) * n + i] : i;
+    }
+    real4 xj[SLOTS];
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+      xj[u] = pos[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+      const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
+      const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
+      const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
+      if (base + u * TEAM + t < cnt && r2_strict(dx, dy, dz) < d.rc2) {
+        const LjCoeff c = single ? c0 : lj[ti * T + static_cast<int>(id_of(xj[u].w))];
+        double rr, uu;
+        const double s = lj_pair(dx, dy, dz, c, rr, uu);
+        fx += s * dx;
+        fy += s * dy;
+        fz += s * dz;
+        e += uu;
+        w += s * rr;
+      }
+    }
+  }
+  fx = team_sum<TEAM>(fx);
+  fy = team_sum<TEAM>(fy);
+  fz = team_sum<TEAM>(fz);
+  pe += 0.5 * e;
+  vir += 0.5 * w;
+  if (t != 0) {
+    return;
+  }
+  const real_t ffx = static_cast<real_t>(fx), ffy = static_cast<real_t>(fy),
+              ffz = static_cast<real_t>(fz);
+  force[i] = make_real4(ffx, ffy, ffz, 0);
+  real4 v = vel[i];
+  if (kick) {
+    const real_t h = real_t(0.5) * d.dt / v.w;
+    v.x += h * ffx;
+    v.y += h * ffy;
+    v.z += h * ffz;
+    vel[i] = v;
+  }
+  const double vx = v.x, vy = v.y, vz = v.z;
+  ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
+}
+
+// Rebuild decisions of one replica at step now (NeighborSystem::rebuild_now's
+// counters): returns whether the rebuild also reorders.
+__device__ __forceinline__ bool begin_rebuild(const ReplicaDesc& d, ReplicaState* st,
+                                              uint64_t now) {
+  const uint64_t builds = st->build_count + 1;
+  const bool due = d.sort_every_steps != 0 ? now - st->last_sort_step >= d.sort_every_steps
+                                           : builds % d.sort_every == 0;
+  const bool reorder = d.grid.path != 0 && due;
+  st->build_count = builds;
+  if (reorder) {
+    st->reorder_count += 1;
+    st->last_sort_step = now;
+  }
+  st->checked = 0;
+  return reorder;
+}
+
 // Neighbour rebuild of one replica (NeighborSystem::rebuild_now restated for a
 // workgroup): bin, counting sort, per-cell slot order, optional reorder, build.
 template <int BLOCK>
@@ -84,18 +299,13 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
   uint32_t* start = A.cell_start + d.cell0;
   uint32_t* cursor = A.cell_cursor + d.cell0;
 
-  uint64_t builds = st->build_count + 1;
+  const uint64_t builds = st->build_count + 1;
   const bool due = d.sort_every_steps != 0 ? now - st->last_sort_step >= d.sort_every_steps
                                            : builds % d.sort_every == 0;
   const bool reorder = d.grid.path != 0 && due;
   __syncthreads();
   if (tid == 0) {
-    st->build_count = builds;
-    if (reorder) {
-      st->reorder_count += 1;
-      st->last_sort_step = now;
-    }
-    st->checked = 0;
+    (void)begin_rebuild(d, st, now);
   }
   if (d.grid.path != 0) {
     for (uint32_t c = tid; c < nc; c += BLOCK) count[c] = 0;
@@ -107,6 +317,8 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
     }
     __syncthreads();
     block_scan<BLOCK>(count, start, cursor, nc, sh.scan);
+    // Counts left at zero, as the phased mode's scan leaves them.
+    for (uint32_t c = tid; c < nc; c += BLOCK) count[c] = 0;
     for (uint32_t a = tid; a < n; a += BLOCK) {
       cell_atoms[atomicAdd(&cursor[cell_of[a]], 1u)] = a;
     }
@@ -150,48 +362,10 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
       __syncthreads();
     }
   }
-  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
-  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
   uint32_t overflow = 0;
   for (uint32_t i = tid; i < n; i += BLOCK) {
-    const real4 xi = pos[i];
-    uint32_t cnt = 0;
-    auto visit = [&](uint32_t j) {
-      const real4 xj = pos[j];
-      const real_t dx = mic(xi.x - xj.x, L0, h0);
-      const real_t dy = mic(xi.y - xj.y, L1, h1);
-      const real_t dz = mic(xi.z - xj.z, L2, h2);
-      if (r2_strict(dx, dy, dz) > d.rs2 || j == i) return;
-      if (cnt < d.max_neighbors) {
-        nbr[static_cast<size_t>(cnt) * n + i] = j;
-        ++cnt;
-      } else {
-        overflow = 1;
-      }
-    };
-    if (d.grid.path == 0) {
-      for (uint32_t j = 0; j < n; ++j) visit(j);
-    } else {
-      const CellNeighborhood h = neighborhood(cell_of[i], d.grid);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (a >= h.nz) break;
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          if (b >= h.ny) break;
-          const uint32_t row = d.grid.nx * (h.ys[b] + d.grid.ny * h.zs[a]);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            if (c >= h.nx) break;
-            const uint32_t cell = h.xs[c] + row;
-            for (uint32_t k = start[cell]; k < start[cell + 1]; ++k) visit(cell_atoms[k]);
-          }
-        }
-      }
-    }
-    nbr_count[i] = cnt;
-    ref_pos[i] = xi;
-    ref_image[i] = image[i];
+    overflow |= build_atom(d, pos, image, cell_of, cell_atoms, start, nbr, nbr_count, ref_pos,
+                           ref_image, i);
   }
   if (overflow) atomicOr(&st->overflow, 1u);
   __syncthreads();
@@ -202,75 +376,11 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
 template <int BLOCK>
 __device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool kick, bool single,
                               const LjCoeff* lj, double& ke, double& pe, double& vir) {
-  const uint32_t n = d.n;
-  const real4* pos = A.pos + d.atom0;
-  real4* vel = A.vel + d.atom0;
-  real4* force = A.force + d.atom0;
-  const uint32_t* nbr_count = A.nbr_count + d.atom0;
-  const uint32_t* nbr = A.nbr + d.nbr0;
-  const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
-  const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
-  const int T = d.n_types;
-  const LjCoeff c0 = lj[0];
-  // A team of kReplicaTeam adjacent lanes per atom: lane t takes list slots
-  // base + u*kReplicaTeam + t, kReplicaSlots of them in flight per round; the
-  // team's FP64 partial forces meet in a butterfly and lane 0 stores.
   const uint32_t t = threadIdx.x & (kReplicaTeam - 1);
-  for (uint32_t i = threadIdx.x / kReplicaTeam; i < n; i += BLOCK / kReplicaTeam) {
-    const real4 xi = pos[i];
-    const int ti = static_cast<int>(id_of(xi.w));
-    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
-    const uint32_t cnt = nbr_count[i];
-    for (uint32_t base = 0; base < cnt; base += kReplicaTeam * kReplicaSlots) {
-      uint32_t jj[kReplicaSlots];
-#pragma unroll
-      for (int u = 0; u < kReplicaSlots; ++u) {
-        const uint32_t s = base + u * kReplicaTeam + t;
-        jj[u] = s < cnt ? nbr[static_cast<size_t>(s) * n + i] : i;
-      }
-      real4 xj[kReplicaSlots];
-#pragma unroll
-      for (int u = 0; u < kReplicaSlots; ++u) {
-        xj[u] = pos[jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < kReplicaSlots; ++u) {
-        const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
-        const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
-        const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
-        if (base + u * kReplicaTeam + t < cnt && r2_strict(dx, dy, dz) < d.rc2) {
-          const LjCoeff c = single ? c0 : lj[ti * T + static_cast<int>(id_of(xj[u].w))];
-          double rr, uu;
-          const double s = lj_pair(dx, dy, dz, c, rr, uu);
-          fx += s * dx;
-          fy += s * dy;
-          fz += s * dz;
-          e += uu;
-          w += s * rr;
-        }
-      }
-    }
-    fx = team_sum<kReplicaTeam>(fx);
-    fy = team_sum<kReplicaTeam>(fy);
-    fz = team_sum<kReplicaTeam>(fz);
-    pe += 0.5 * e;
-    vir += 0.5 * w;
-    if (t != 0) {
-      continue;
-    }
-    const real_t ffx = static_cast<real_t>(fx), ffy = static_cast<real_t>(fy),
-                ffz = static_cast<real_t>(fz);
-    force[i] = make_real4(ffx, ffy, ffz, 0);
-    real4 v = vel[i];
-    if (kick) {
-      const real_t h = real_t(0.5) * d.dt / v.w;
-      v.x += h * ffx;
-      v.y += h * ffy;
-      v.z += h * ffz;
-      vel[i] = v;
-    }
-    const double vx = v.x, vy = v.y, vz = v.z;
-    ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
+  for (uint32_t i = threadIdx.x / kReplicaTeam; i < d.n; i += BLOCK / kReplicaTeam) {
+    force_atom<kReplicaTeam, kReplicaSlots>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                                            A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick,
+                                            single, i, t, ke, pe, vir);
   }
 }
 
@@ -321,49 +431,7 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
     const real_t lam = static_cast<real_t>(sh.lambda);
     int exceeded = 0;
     for (uint32_t a = tid; a < n; a += BLOCK) {
-      real4 v = vel[a];
-      const real4 f = force[a];
-      real4 x = pos[a];
-      if (d.berendsen) {
-        v.x *= lam;
-        v.y *= lam;
-        v.z *= lam;
-      }
-      const real_t h = real_t(0.5) * d.dt / v.w;
-      v.x += h * f.x;
-      v.y += h * f.y;
-      v.z += h * f.z;
-      x.x += d.dt * v.x;
-      x.y += d.dt * v.y;
-      x.z += d.dt * v.z;
-      real_t c[3];
-      {
-#pragma clang fp contract(off)
-        c[0] = floor((x.x - d.box.lo[0]) * d.box.invL[0]);
-        c[1] = floor((x.y - d.box.lo[1]) * d.box.invL[1]);
-        c[2] = floor((x.z - d.box.lo[2]) * d.box.invL[2]);
-        x.x -= c[0] * d.box.L[0];
-        x.y -= c[1] * d.box.L[1];
-        x.z -= c[2] * d.box.L[2];
-      }
-      pos[a] = x;
-      vel[a] = v;
-      int4 im = image[a];
-      if (c[0] != real_t(0) || c[1] != real_t(0) || c[2] != real_t(0)) {
-        im.x += static_cast<int>(c[0]);
-        im.y += static_cast<int>(c[1]);
-        im.z += static_cast<int>(c[2]);
-        image[a] = im;
-      }
-      if (check) {
-#pragma clang fp contract(off)
-        const real4 r = ref_pos[a];
-        const int4 ri = ref_image[a];
-        const real_t dx = (x.x - r.x) + static_cast<real_t>(im.x - ri.x) * d.box.L[0];
-        const real_t dy = (x.y - r.y) + static_cast<real_t>(im.y - ri.y) * d.box.L[1];
-        const real_t dz = (x.z - r.z) + static_cast<real_t>(im.z - ri.z) * d.box.L[2];
-        if (dx * dx + dy * dy + dz * dz > d.thr2) exceeded = 1;
-      }
+      exceeded |= integrate_atom(d, pos, vel, force, image, ref_pos, ref_image, a, lam, check);
     }
     const bool any = __syncthreads_or(exceeded) != 0;
     bool rebuild = every;
@@ -410,6 +478,360 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
   }
 }
 
+// ---------------------------------------------------------------- phased ---
+
+constexpr int kPhasedBlock = 256;
+constexpr uint32_t kPhasedWaves = kPhasedBlock / 64;  // chunks per workgroup
+constexpr uint32_t kWorkBlocks = 1024;                // grid of the work-list kernels
+
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Local index range of chunk c within its replica: atom a0 + lane.
+struct ChunkAtoms {
+  uint32_t r;   // replica
+  uint32_t a0;  // first atom of the chunk, local to the replica
+};
+__device__ __forceinline__ ChunkAtoms chunk_atoms(const ReplicaParams& P, uint32_t c) {
+  const uint32_t r = wave_uniform(P.chunk_replica[c]);
+  return {r, (c - P.desc[r].chunk0) * kReplicaChunk};
+}
+
+// Step kernel 1: first half-kick, drift, wrap and (on check steps) the
+// displacement test of every atom; a replica's test result is the OR over its
+// chunks.
+__global__ __launch_bounds__(kPhasedBlock) void phased_integrate_kernel(ReplicaParams P) {
+  const uint32_t c = wave_uniform(blockIdx.x * kPhasedWaves + threadIdx.x / 64);
+  if (c >= P.total_chunks) return;
+  const ChunkAtoms ca = chunk_atoms(P, c);
+  const ReplicaDesc& d = P.desc[ca.r];
+  ReplicaState* st = P.state + ca.r;
+  const ReplicaArrays& A = P.arrays;
+  const uint64_t physical = st->step + 1;
+  const bool every = d.cutoff_buffer == real_t(0);
+  const bool check = !every && physical % d.check_every == 0;
+  const real_t lam = static_cast<real_t>(st->lambda);
+  const uint32_t a = ca.a0 + (threadIdx.x & 63u);
+  int exceeded = 0;
+  if (a < d.n) {
+    exceeded = integrate_atom(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                              A.image + d.atom0, A.ref_pos + d.atom0, A.ref_image + d.atom0, a,
+                              lam, check);
+  }
+  if (__builtin_amdgcn_ballot_w64(exceeded != 0) != 0 && (threadIdx.x & 63u) == 0) {
+    atomicOr(&st->exceeded, 1u);
+  }
+}
+
+// Step kernel 2, one wave per replica: the check and rebuild decisions of the
+// persistent step loop (initial: the first build at step 0), and the
+// replica's entries of the rebuild kernels' work lists.
+__global__ __launch_bounds__(64) void phased_decide_kernel(ReplicaParams P, int initial) {
+  __shared__ uint32_t base[2];
+  const uint32_t r = blockIdx.x;
+  const ReplicaDesc& d = P.desc[r];
+  ReplicaState* st = P.state + r;
+  if (threadIdx.x == 0) {
+    bool rebuild = true, reorder = false;
+    uint64_t now = 0;
+    if (!initial) {
+      now = st->step + 1;
+      const bool every = d.cutoff_buffer == real_t(0);
+      const bool check = !every && now % d.check_every == 0;
+      rebuild = every;
+      if (check) {
+        const bool any = st->exceeded != 0;
+        const bool first = st->checked == 0;
+        rebuild = any;
+        st->checked = 1;
+        if (any && d.check_every > 1 && first) st->dangerous += 1;
+      }
+    }
+    st->exceeded = 0;
+    if (rebuild) reorder = begin_rebuild(d, st, now);
+    st->rebuild = rebuild ? 1u : 0u;
+    st->reorder = reorder ? 1u : 0u;
+    base[0] = rebuild ? atomicAdd(&P.work_count[0], d.n_chunks) : 0xffffffffu;
+    base[1] = reorder ? atomicAdd(&P.work_count[1], d.n_chunks) : 0xffffffffu;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < d.n_chunks; k += 64) {
+    if (base[0] != 0xffffffffu) P.work[base[0] + k] = d.chunk0 + k;
+    if (base[1] != 0xffffffffu) P.work_reorder[base[1] + k] = d.chunk0 + k;
+  }
+}
+
+// Runs body(chunk, replica, local atom) for every atom of the chunks on a
+// work list, one wave per chunk.
+template <typename Body>
+__device__ __forceinline__ void for_work(const ReplicaParams& P, const uint32_t* list,
+                                         uint32_t entries, Body body) {
+  for (uint32_t w = wave_uniform(blockIdx.x * kPhasedWaves + threadIdx.x / 64); w < entries;
+       w += gridDim.x * kPhasedWaves) {
+    const ChunkAtoms ca = chunk_atoms(P, wave_uniform(list[w]));
+    body(ca.r, ca.a0 + (threadIdx.x & 63u));
+  }
+}
+
+// Rebuild kernel: cell of every atom and per-cell counts (counts start at
+// zero: the scan clears them after use).
+__global__ __launch_bounds__(kPhasedBlock) void phased_bin_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  for_work(P, P.work, P.work_count[0], [&](uint32_t r, uint32_t a) {
+    const ReplicaDesc& d = P.desc[r];
+    if (d.grid.path == 0 || a >= d.n) return;
+    const uint32_t c = cell_of_position(A.pos[d.atom0 + a], d.box, d.grid);
+    A.cell_of[d.atom0 + a] = c;
+    atomicAdd(&A.cell_count[d.cell0 + c], 1u);
+  });
+}
+
+// Rebuild kernel, one workgroup per replica: exclusive scan of the counts into
+// cell starts and scatter cursors; clears the counts.
+__global__ __launch_bounds__(kPhasedBlock) void phased_scan_kernel(ReplicaParams P) {
+  __shared__ uint32_t sums[kPhasedBlock];
+  const ReplicaDesc& d = P.desc[blockIdx.x];
+  if (P.state[blockIdx.x].rebuild == 0 || d.grid.path == 0) return;
+  const ReplicaArrays& A = P.arrays;
+  uint32_t* count = A.cell_count + d.cell0;
+  block_scan<kPhasedBlock>(count, A.cell_start + d.cell0, A.cell_cursor + d.cell0,
+                           d.grid.n_cells, sums);
+  for (uint32_t c = threadIdx.x; c < d.grid.n_cells; c += kPhasedBlock) count[c] = 0;
+}
+
+// Rebuild kernel: counting-sort scatter of the atoms into their cells.
+__global__ __launch_bounds__(kPhasedBlock) void phased_scatter_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  for_work(P, P.work, P.work_count[0], [&](uint32_t r, uint32_t a) {
+    const ReplicaDesc& d = P.desc[r];
+    if (d.grid.path == 0 || a >= d.n) return;
+    const uint32_t c = A.cell_of[d.atom0 + a];
+    A.cell_tmp[d.atom0 + atomicAdd(&A.cell_cursor[d.cell0 + c], 1u)] = a;
+  });
+}
+
+// Rebuild kernel: each atom's place in its cell's slot order (the number of
+// members with a lower slot), giving the sorted member lists.
+__global__ __launch_bounds__(kPhasedBlock) void phased_rank_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  for_work(P, P.work, P.work_count[0], [&](uint32_t r, uint32_t a) {
+    const ReplicaDesc& d = P.desc[r];
+    if (d.grid.path == 0 || a >= d.n) return;
+    const uint32_t c = A.cell_of[d.atom0 + a];
+    const uint32_t b = A.cell_start[d.cell0 + c], e = A.cell_start[d.cell0 + c + 1];
+    const uint32_t* members = A.cell_tmp + d.atom0;
+    uint32_t rank = 0;
+    for (uint32_t k = b; k < e; ++k) rank += members[k] < a ? 1u : 0u;
+    A.cell_atoms[d.atom0 + b + rank] = a;
+    const real4 x = A.pos[d.atom0 + a];
+    A.cell_pos[d.atom0 + b + rank] = make_real4(x.x, x.y, x.z, real_of_id(a));
+  });
+}
+
+// Reorder kernels: gather the per-atom state into cell order, then commit it.
+__global__ __launch_bounds__(kPhasedBlock) void phased_reorder_gather_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  for_work(P, P.work_reorder, P.work_count[1], [&](uint32_t r, uint32_t s) {
+    const ReplicaDesc& d = P.desc[r];
+    if (s >= d.n) return;
+    const uint32_t o = d.atom0 + A.cell_atoms[d.atom0 + s];
+    const uint32_t g = d.atom0 + s;
+    A.s_pos[g] = A.pos[o];
+    A.s_vel[g] = A.vel[o];
+    A.s_image[g] = A.image[o];
+    A.s_tag[g] = A.tag[o];
+    A.s_cell_of[g] = A.cell_of[o];
+  });
+}
+
+__global__ __launch_bounds__(kPhasedBlock) void phased_reorder_commit_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  for_work(P, P.work_reorder, P.work_count[1], [&](uint32_t r, uint32_t s) {
+    const ReplicaDesc& d = P.desc[r];
+    if (s >= d.n) return;
+    const uint32_t g = d.atom0 + s;
+    A.pos[g] = A.s_pos[g];
+    A.vel[g] = A.s_vel[g];
+    A.image[g] = A.s_image[g];
+    A.tag[g] = A.s_tag[g];
+    A.cell_of[g] = A.s_cell_of[g];
+    A.cell_atoms[g] = s;
+    const real4 x = A.s_pos[g];
+    A.cell_pos[g] = make_real4(x.x, x.y, x.z, real_of_id(s));
+  });
+}
+
+// Rebuild kernel, one chunk per workgroup: the neighbour lists, with a team
+// of kPhasedBuildTeam adjacent lanes per atom. The candidates are build_atom's
+// in its order (neighbour cells in neighbourhood order, a cell's members in
+// slot order, read from the cell-ordered positions; every atom on the direct
+// path): lane t tests candidates k + u*kPhasedBuildTeam + t of each round and
+// the team appends the accepted ones in candidate order through a ballot, so
+// the lists are build_atom's. A replica's overflow is the OR over its chunks.
+constexpr int kPhasedBuildTeam = 4;
+constexpr int kPhasedBuildSlots = 4;
+static_assert(kPhasedBlock == kReplicaChunk * kPhasedBuildTeam, "one chunk per workgroup");
+
+__global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParams P) {
+  const ReplicaArrays& A = P.arrays;
+  const uint32_t t = threadIdx.x & (kPhasedBuildTeam - 1);
+  const uint32_t team_lane0 = (threadIdx.x & 63u) & ~static_cast<uint32_t>(kPhasedBuildTeam - 1);
+  const uint32_t below = (1u << t) - 1u;
+  const uint32_t entries = P.work_count[0];
+  for (uint32_t w = blockIdx.x; w < entries; w += gridDim.x) {
+    const ChunkAtoms ca = chunk_atoms(P, wave_uniform(P.work[w]));
+    const ReplicaDesc& d = P.desc[ca.r];
+    const uint32_t n = d.n;
+    const uint32_t i = ca.a0 + threadIdx.x / kPhasedBuildTeam;
+    uint32_t count = 0;
+    if (i < n) {  // uniform over a team
+      const real4* pos = A.pos + d.atom0;
+      const real4* cpos = A.cell_pos + d.atom0;
+      uint32_t* nbr = A.nbr + d.nbr0;
+      const uint32_t maxn = d.max_neighbors;
+      const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
+      const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
+      const real4 xi = pos[i];
+      // Candidates [b, e) of a run (b < e); entry(k) = {x, y, z, slot bits}.
+      auto scan = [&](uint32_t b, uint32_t e, auto entry) {
+        for (uint32_t k = b; k < e; k += kPhasedBuildTeam * kPhasedBuildSlots) {
+          real4 x[kPhasedBuildSlots];
+#pragma unroll
+          for (int u = 0; u < kPhasedBuildSlots; ++u) {
+            const uint32_t q = k + u * kPhasedBuildTeam + t;
+            x[u] = entry(q < e ? q : e - 1);
+          }
+#pragma unroll
+          for (int u = 0; u < kPhasedBuildSlots; ++u) {
+            const uint32_t j = id_of(x[u].w);
+            const real_t dx = mic(xi.x - x[u].x, L0, h0);
+            const real_t dy = mic(xi.y - x[u].y, L1, h1);
+            const real_t dz = mic(xi.z - x[u].z, L2, h2);
+            const uint64_t m =
+                __builtin_amdgcn_ballot_w64(k + u * kPhasedBuildTeam + t < e) &
+                __builtin_amdgcn_ballot_w64(!(r2_strict(dx, dy, dz) > d.rs2)) &
+                __builtin_amdgcn_ballot_w64(j != i);
+            const uint32_t bits =
+                static_cast<uint32_t>(m >> team_lane0) & ((1u << kPhasedBuildTeam) - 1u);
+            if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+              const uint32_t slot = count + __builtin_popcount(bits & below);
+              if (slot < maxn) nbr[static_cast<size_t>(slot) * n + i] = j;
+            }
+            count += __builtin_popcount(bits);
+          }
+        }
+      };
+      if (d.grid.path == 0) {
+        scan(0u, n, [pos](uint32_t k) {
+          const real4 q = pos[k];
+          return make_real4(q.x, q.y, q.z, real_of_id(k));
+        });
+      } else {
+        const uint32_t* start = A.cell_start + d.cell0;
+        const auto entry = [cpos](uint32_t k) { return cpos[k]; };
+        const CellNeighborhood h = neighborhood(A.cell_of[d.atom0 + i], d.grid);
+        for (int a = 0; a < h.nz; ++a) {
+          for (int b = 0; b < h.ny; ++b) {
+            const uint32_t row = d.grid.nx * (h.ys[b] + d.grid.ny * h.zs[a]);
+            for (int c = 0; c < h.nx; ++c) {
+              const uint32_t cell = h.xs[c] + row;
+              const uint32_t b0 = start[cell], e0 = start[cell + 1];
+              if (b0 < e0) scan(b0, e0, entry);
+            }
+          }
+        }
+      }
+      if (t == 0) {
+        A.nbr_count[d.atom0 + i] = min(count, maxn);
+        A.ref_pos[d.atom0 + i] = xi;
+        A.ref_image[d.atom0 + i] = A.image[d.atom0 + i];
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(count > d.max_neighbors && t == 0) != 0 &&
+        (threadIdx.x & 63u) == 0) {
+      atomicOr(&P.state[ca.r].overflow, 1u);
+    }
+  }
+}
+
+// Step kernel: full-list LJ force and the second half-kick (not on the
+// initial step) of one chunk per workgroup, TEAM lanes per atom; the chunk's
+// kinetic energy, pair energy and virial go to its partials.
+template <int TEAM, int SLOTS>
+__global__ __launch_bounds__(kReplicaChunk * TEAM) void phased_force_kernel(ReplicaParams P,
+                                                                            int kick) {
+  constexpr int BLOCK = kReplicaChunk * TEAM;
+  __shared__ LjCoeff lj[kMaxLdsTypes * kMaxLdsTypes];
+  __shared__ double red[BLOCK / 64];
+  const uint32_t c = blockIdx.x;
+  const uint32_t r = wave_uniform(P.chunk_replica[c]);
+  const ReplicaDesc& d = P.desc[r];
+  for (int k = threadIdx.x; k < d.n_types * d.n_types; k += BLOCK) lj[k] = P.lj[d.lj0 + k];
+  __syncthreads();
+  const ReplicaArrays& A = P.arrays;
+  const uint32_t i = (c - d.chunk0) * kReplicaChunk + threadIdx.x / TEAM;
+  const uint32_t t = threadIdx.x & (TEAM - 1);
+  double ke = 0.0, pe = 0.0, vir = 0.0;
+  if (i < d.n) {
+    force_atom<TEAM, SLOTS>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                            A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick != 0,
+                            d.n_types == 1, i, t, ke, pe, vir);
+  }
+  ke = block_sum<BLOCK>(ke, red);
+  pe = block_sum<BLOCK>(pe, red);
+  vir = block_sum<BLOCK>(vir, red);
+  if (threadIdx.x == 0) {
+    P.partials[c] = ke;
+    P.partials[P.total_chunks + c] = pe;
+    P.partials[2 * static_cast<size_t>(P.total_chunks) + c] = vir;
+  }
+}
+
+// Step kernel, one workgroup per replica: totals of the chunk partials in
+// chunk order, the Berendsen factor for the next step, the step counter
+// (initial: the step-0 observables). Workgroup 0 also empties the work lists.
+__global__ __launch_bounds__(kPhasedBlock) void phased_reduce_kernel(ReplicaParams P, int initial,
+                                                                     int observables) {
+  __shared__ double red[kPhasedBlock / 64];
+  const uint32_t r = blockIdx.x;
+  const ReplicaDesc& d = P.desc[r];
+  ReplicaState* st = P.state + r;
+  const int nq = observables || initial ? 3 : 1;
+  double tot[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < nq; ++q) {
+    double v = 0.0;
+    for (uint32_t k = threadIdx.x; k < d.n_chunks; k += kPhasedBlock) {
+      v += P.partials[static_cast<size_t>(q) * P.total_chunks + d.chunk0 + k];
+    }
+    tot[q] = block_sum<kPhasedBlock>(v, red);
+  }
+  if (threadIdx.x == 0) {
+    const double ke = tot[0];
+    double lambda = 1.0;
+    if (!initial && d.berendsen && ke > 0.0 && d.dof > 0.0) {
+      const double current = 2.0 * ke / d.dof;
+      const double arg = 1.0 + d.dt_over_tau * (d.target_thermal / current - 1.0);
+      lambda = sqrt(arg > 0.0 ? arg : 0.0);
+    }
+    st->lambda = lambda;
+    st->ke = ke;
+    if (observables || initial) {
+      st->pe = tot[1];
+      st->virial = tot[2];
+    }
+    if (initial) {
+      st->initialized = 1;
+    } else {
+      st->step += 1;
+    }
+    if (r == 0) {
+      P.work_count[0] = 0;
+      P.work_count[1] = 0;
+    }
+  }
+}
+
 }  // namespace
 
 void launch_replicas(const ReplicaParams& p, uint32_t count, int block, uint64_t n_steps,
@@ -426,6 +848,51 @@ void launch_replicas(const ReplicaParams& p, uint32_t count, int block, uint64_t
       replica_kernel<1024><<<count, 1024, 0, stream>>>(p, n_steps, obs);
       break;
   }
+}
+
+bool phased_shape_supported(PhasedShape shape) {
+  const int code = shape.team * 100 + shape.slots;
+  return code == 204 || code == 208 || code == 404 || code == 408 || code == 802 || code == 804;
+}
+
+void launch_phased_step(const ReplicaParams& p, PhasedShape shape, bool initial, bool observables,
+                        hipStream_t stream) {
+  const unsigned per_atom = (p.total_chunks + kPhasedWaves - 1) / kPhasedWaves;
+  if (!initial) phased_integrate_kernel<<<per_atom, kPhasedBlock, 0, stream>>>(p);
+  phased_decide_kernel<<<p.count, 64, 0, stream>>>(p, initial ? 1 : 0);
+  // Work-list kernels loop over the listed chunks; their grid is fixed at
+  // capture time, so a step without rebuilds costs each one empty launch.
+  const unsigned work = std::min<unsigned>(kWorkBlocks, per_atom);
+  phased_bin_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  phased_scan_kernel<<<p.count, kPhasedBlock, 0, stream>>>(p);
+  phased_scatter_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  phased_rank_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  phased_reorder_gather_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  phased_reorder_commit_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  phased_build_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  const int kick = initial ? 0 : 1;
+  switch (shape.team * 100 + shape.slots) {
+    case 204:
+      phased_force_kernel<2, 4><<<p.total_chunks, kReplicaChunk * 2, 0, stream>>>(p, kick);
+      break;
+    case 208:
+      phased_force_kernel<2, 8><<<p.total_chunks, kReplicaChunk * 2, 0, stream>>>(p, kick);
+      break;
+    case 408:
+      phased_force_kernel<4, 8><<<p.total_chunks, kReplicaChunk * 4, 0, stream>>>(p, kick);
+      break;
+    case 802:
+      phased_force_kernel<8, 2><<<p.total_chunks, kReplicaChunk * 8, 0, stream>>>(p, kick);
+      break;
+    case 804:
+      phased_force_kernel<8, 4><<<p.total_chunks, kReplicaChunk * 8, 0, stream>>>(p, kick);
+      break;
+    default:
+      phased_force_kernel<4, 4><<<p.total_chunks, kReplicaChunk * 4, 0, stream>>>(p, kick);
+      break;
+  }
+  phased_reduce_kernel<<<p.count, kPhasedBlock, 0, stream>>>(p, initial ? 1 : 0,
+                                                             observables ? 1 : 0);
 }
 
 }  // namespace beads_amd

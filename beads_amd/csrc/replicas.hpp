@@ -29,7 +29,13 @@ struct ReplicaDesc {
   uint64_t check_every;
   uint64_t sort_every;
   uint64_t sort_every_steps;  // nonzero: step-based sort cadence (see NeighborControl)
+  uint32_t chunk0;            // first chunk of the replica's atoms (phased mode)
+  uint32_t n_chunks;          // ceil(n / kReplicaChunk)
 };
+
+// Phased mode: the replicas' atoms in chunks of kReplicaChunk (one wave's
+// worth), each inside one replica.
+constexpr uint32_t kReplicaChunk = 64;
 
 // Per-replica run state carried across launches.
 struct ReplicaState {
@@ -46,6 +52,12 @@ struct ReplicaState {
   double pe;      // pair energy (last observable step)
   double virial;
   uint64_t last_sort_step;  // step of the last reorder (step-based cadence)
+  // Phased mode, within a step: displacement check result, and the decisions
+  // the rebuild kernels follow.
+  uint32_t exceeded;
+  uint32_t rebuild;
+  uint32_t reorder;
+  uint32_t pad2;
 };
 
 struct ReplicaArrays {
@@ -68,6 +80,8 @@ struct ReplicaArrays {
   int4* s_image;
   uint32_t* s_tag;
   uint32_t* s_cell_of;
+  uint32_t* cell_tmp;  // phased mode: members of each cell before the slot sort
+  real4* cell_pos;     // phased mode: positions in cell order, .w = the slot's bits
 };
 
 struct ReplicaParams {
@@ -75,10 +89,32 @@ struct ReplicaParams {
   ReplicaState* state;
   const LjCoeff* lj;
   ReplicaArrays arrays;
+  // Phased mode.
+  uint32_t count;                // replicas
+  uint32_t total_chunks;
+  const uint32_t* chunk_replica; // replica of each chunk
+  double* partials;              // 3 x total_chunks: kinetic energy, pair energy, virial
+  uint32_t* work;                // chunks of the replicas that rebuild this step
+  uint32_t* work_reorder;        // chunks of the replicas that also reorder
+  uint32_t* work_count;          // [0]: entries of work, [1]: of work_reorder
+};
+
+// Phased mode force layout: kReplicaChunk atoms per workgroup, team lanes per
+// atom, list slots per lane per round.
+struct PhasedShape {
+  int team;
+  int slots;
 };
 
 // One workgroup per replica, all n_steps inside the launch.
 void launch_replicas(const ReplicaParams& p, uint32_t count, int block, uint64_t n_steps,
                      bool observables_on_last, hipStream_t stream);
+
+// Phased mode: one step of every replica as a sequence of batched kernels
+// (initial: the first build and step-0 forces instead). Captured into a graph
+// by the caller.
+bool phased_shape_supported(PhasedShape shape);
+void launch_phased_step(const ReplicaParams& p, PhasedShape shape, bool initial, bool observables,
+                        hipStream_t stream);
 
 }  // namespace beads_amd

@@ -104,7 +104,9 @@ def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
     atoms = sum(w.n for w in ws)
     print(f"bench: batched leg, {count} replicas, {atoms} atoms", file=sys.stderr, flush=True)
     with Batch(ws) as b:
-        block, trial = b.autotune(trial_steps=20)
+        chosen, trial = b.autotune(trial_steps=20)
+        config = b.config_info(chosen)
+        configs = b.configs()
         b.run(0)
         b.synchronize()
         t0 = time.perf_counter()
@@ -128,7 +130,8 @@ def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
     serial = atoms * serial_steps / serial_wall
     return {"workload": f"{count} LJ replicas, N log-uniform [500, 20000], {atoms} atoms, "
                         f"reduced units, Berendsen",
-            "steps": steps, "block": block, "autotune_s": [round(x, 5) for x in trial],
+            "steps": steps, "config": config,
+            "autotune_s": [dict(c, seconds=round(x, 5)) for c, x in zip(configs, trial)],
             "batched_atom_steps_per_s": batched, "batched_device_s": dev,
             "batched_replica_steps_per_s": count * steps / wall,
             "serial_atom_steps_per_s": serial, "serial_steps_per_replica": serial_steps,

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BEADS_AMD_ABI_VERSION 4
+#define BEADS_AMD_ABI_VERSION 5
 
 #ifdef BEADS_REAL64
 typedef double beads_real_t;
@@ -335,19 +335,39 @@ void beads_engine_destroy(beads_engine_t* engine);
 /* ---- batched replicas ----
  * No reference counterpart: the reference runs one system per process
  * (simulation_run.cpp). Many independent pair-only (LJ) systems, each a full
- * beads_spec_t (outputs ignored), advance together in ONE persistent launch:
- * one workgroup per replica runs that replica's whole step loop (integrate,
- * rebuild with stable sort, full-list LJ + kick + Berendsen) per launch. The
- * launch is captured once into a hipGraph and replayed. */
+ * beads_spec_t (outputs ignored), advance together in one of two modes:
+ *   persistent (mode 0): ONE launch per run, one workgroup of `block` lanes
+ *     per replica running that replica's whole step loop (integrate, rebuild
+ *     with stable sort, full-list LJ + kick + Berendsen);
+ *   phased (mode 1): per step, a sequence of batched kernels over 64-atom
+ *     chunks of every replica, so a large replica spreads over many
+ *     workgroups; the force kernel uses `team` lanes per atom and `slots` list
+ *     slots per lane per round (`block` = 64 * team).
+ * Both make the same decisions with the same arithmetic. Each launch sequence
+ * is captured once into a hipGraph and replayed. */
 typedef struct beads_batch beads_batch_t;
+
+typedef struct {
+  int32_t mode;  /* 0 persistent, 1 phased */
+  int32_t block; /* lanes per workgroup (phased: of the force kernel) */
+  int32_t team;  /* force kernel: lanes per atom */
+  int32_t slots; /* force kernel: list slots per lane per round */
+} beads_batch_config_t;
+
+#define BEADS_BATCH_CONFIGS 9
 
 beads_status_t beads_batch_create(const beads_spec_t* specs, uint32_t count,
                                   beads_batch_t** out);
-/* Times each candidate workgroup size (256, 512, 1024) for trial_steps from the
- * initial state, keeps the fastest and restores the initial state. seconds may
- * be NULL or hold 3 doubles. */
+/* Configuration index (0 .. BEADS_BATCH_CONFIGS - 1) described. */
+beads_status_t beads_batch_config_info(uint32_t index, beads_batch_config_t* out);
+/* Times each configuration for trial_steps from the initial state, keeps the
+ * fastest (its index in chosen_config) and restores the initial state.
+ * seconds may be NULL or hold BEADS_BATCH_CONFIGS doubles. */
 beads_status_t beads_batch_autotune(beads_batch_t* batch, uint32_t trial_steps,
-                                    int32_t* chosen_block, double* seconds);
+                                    int32_t* chosen_config, double* seconds);
+beads_status_t beads_batch_set_config(beads_batch_t* batch, uint32_t index);
+beads_status_t beads_batch_get_config(beads_batch_t* batch, uint32_t* index);
+/* The persistent configuration with this workgroup size (256, 512 or 1024). */
 beads_status_t beads_batch_set_block(beads_batch_t* batch, int32_t block);
 /* The first call also performs every replica's initial build and forces. */
 beads_status_t beads_batch_run(beads_batch_t* batch, uint64_t n_steps,
