@@ -81,7 +81,9 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     desc_host_[k] = d;
     atoms += s.n_particles;
     cells += static_cast<uint64_t>(p.n_cells) + 1;
-    nbr += static_cast<uint64_t>(p.max_neighbors) * s.n_particles;
+    // Lists in chunks of kNbrChunk entries per atom.
+    nbr += static_cast<uint64_t>((p.max_neighbors + kNbrChunk - 1) / kNbrChunk) * kNbrChunk *
+           s.n_particles;
     lj += static_cast<uint64_t>(s.n_types) * s.n_types;
   }
   require(atoms <= 0xffffffffull && cells <= 0xffffffffull,

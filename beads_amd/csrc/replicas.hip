@@ -79,6 +79,13 @@ __device__ void block_scan(const uint32_t* count, uint32_t* start, uint32_t* cur
   __syncthreads();
 }
 
+// A replica's lists are stored in chunks of kNbrChunk entries (the plain
+// path's layout): entry s of atom i at nbr_chunk_base(s / kNbrChunk, i, n) +
+// s % kNbrChunk, so a team's kNbrChunk consecutive slots are one 64-byte run.
+__device__ __forceinline__ size_t replica_nbr_index(uint32_t s, uint32_t i, uint32_t n) {
+  return nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1));
+}
+
 // Velocity-Verlet first half-kick (with the pending Berendsen factor), drift,
 // wrap and image update of atom a of a replica (arrays offset to the replica),
 // and, when check, the displacement test against the reference positions.
@@ -156,7 +163,7 @@ __device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4
     const real_t dz = mic(xi.z - xj.z, L2, h2);
     if (r2_strict(dx, dy, dz) > d.rs2 || j == i) return;
     if (cnt < d.max_neighbors) {
-      nbr[static_cast<size_t>(cnt) * n + i] = j;
+      nbr[replica_nbr_index(cnt, i, n)] = j;
       ++cnt;
     } else {
       overflow = 1;
@@ -214,7 +221,7 @@ __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* po
 #pragma unroll
     for (int u = 0; u < SLOTS; ++u) {
       const uint32_t s = base + u * TEAM + t;
-      jj[u] = s < cnt ? nbr[static_cast<size_t>(s) * n + i] : i;
+      jj[u] = s < cnt ? nbr[replica_nbr_index(s, i, n)] : i;
     }
     real4 xj[SLOTS];
 #pragma unroll
@@ -716,7 +723,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
                 static_cast<uint32_t>(m >> team_lane0) & ((1u << kPhasedBuildTeam) - 1u);
             if (__builtin_amdgcn_inverse_ballot_w64(m)) {
               const uint32_t slot = count + __builtin_popcount(bits & below);
-              if (slot < maxn) nbr[static_cast<size_t>(slot) * n + i] = j;
+              if (slot < maxn) nbr[replica_nbr_index(slot, i, n)] = j;
             }
             count += __builtin_popcount(bits);
           }

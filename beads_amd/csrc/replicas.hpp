@@ -15,7 +15,7 @@ struct ReplicaDesc {
   uint32_t atom0;         // first atom in the concatenated per-atom arrays
   uint32_t cell0;         // first cell entry (each replica owns n_cells + 1)
   uint32_t max_neighbors;
-  uint64_t nbr0;          // first neighbour-index entry (max_neighbors * n, slot-major)
+  uint64_t nbr0;          // first neighbour-index entry (chunks of kNbrChunk, see replica_nbr_index)
   BoxView box;
   CellGridView grid;
   real_t rs2, rc2, thr2, dt;
