@@ -199,9 +199,9 @@ __device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4
 // a team of TEAM adjacent lanes (lane t of the team): lane t takes list slots
 // base + u*TEAM + t, SLOTS of them in flight per round; the team's FP64 partial
 // forces meet in a butterfly and lane 0 stores the force and, when kick, the
-// second half-kick. Adds this lane's share of the kinetic energy (lane 0),
-// pair energy and virial.
-template <int TEAM, int SLOTS>
+// second half-kick. Adds this lane's share of the kinetic energy (lane 0)
+// and, when OBS, of the pair energy and virial.
+template <int TEAM, int SLOTS, bool OBS = true>
 __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* pos, real4* vel,
                                            real4* force, const uint32_t* nbr_count,
                                            const uint32_t* nbr, const LjCoeff* lj, bool kick,
@@ -240,16 +240,20 @@ __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* po
         fx += s * dx;
         fy += s * dy;
         fz += s * dz;
-        e += uu;
-        w += s * rr;
+        if (OBS) {
+          e += uu;
+          w += s * rr;
+        }
       }
     }
   }
   fx = team_sum<TEAM>(fx);
   fy = team_sum<TEAM>(fy);
   fz = team_sum<TEAM>(fz);
-  pe += 0.5 * e;
-  vir += 0.5 * w;
+  if (OBS) {
+    pe += 0.5 * e;
+    vir += 0.5 * w;
+  }
   if (t != 0) {
     return;
   }
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_bin_kernel(ReplicaParams 
     if (d.grid.path == 0 || a >= d.n) return;
     const uint32_t c = cell_of_position(A.pos[d.atom0 + a], d.box, d.grid);
     A.cell_of[d.atom0 + a] = c;
-    atomicAdd(&A.cell_count[d.cell0 + c], 1u);
+    (void)wave_increment(A.cell_count + d.cell0, c);  // one atomic per distinct cell
   });
 }
 
@@ -614,7 +618,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_scatter_kernel(ReplicaPar
     const ReplicaDesc& d = P.desc[r];
     if (d.grid.path == 0 || a >= d.n) return;
     const uint32_t c = A.cell_of[d.atom0 + a];
-    A.cell_tmp[d.atom0 + atomicAdd(&A.cell_cursor[d.cell0 + c], 1u)] = a;
+    A.cell_tmp[d.atom0 + wave_increment(A.cell_cursor + d.cell0, c)] = a;
   });
 }
 
@@ -764,8 +768,8 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
 
 // Step kernel: full-list LJ force and the second half-kick (not on the
 // initial step) of one chunk per workgroup, TEAM lanes per atom; the chunk's
-// kinetic energy, pair energy and virial go to its partials.
-template <int TEAM, int SLOTS>
+// kinetic energy and, when OBS, pair energy and virial go to its partials.
+template <int TEAM, int SLOTS, bool OBS>
 __global__ __launch_bounds__(kReplicaChunk * TEAM) void phased_force_kernel(ReplicaParams P,
                                                                             int kick) {
   constexpr int BLOCK = kReplicaChunk * TEAM;
@@ -781,17 +785,21 @@ __global__ __launch_bounds__(kReplicaChunk * TEAM) void phased_force_kernel(Repl
   const uint32_t t = threadIdx.x & (TEAM - 1);
   double ke = 0.0, pe = 0.0, vir = 0.0;
   if (i < d.n) {
-    force_atom<TEAM, SLOTS>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
-                            A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick != 0,
-                            d.n_types == 1, i, t, ke, pe, vir);
+    force_atom<TEAM, SLOTS, OBS>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                                 A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick != 0,
+                                 d.n_types == 1, i, t, ke, pe, vir);
   }
   ke = block_sum<BLOCK>(ke, red);
-  pe = block_sum<BLOCK>(pe, red);
-  vir = block_sum<BLOCK>(vir, red);
+  if (OBS) {
+    pe = block_sum<BLOCK>(pe, red);
+    vir = block_sum<BLOCK>(vir, red);
+  }
   if (threadIdx.x == 0) {
     P.partials[c] = ke;
-    P.partials[P.total_chunks + c] = pe;
-    P.partials[2 * static_cast<size_t>(P.total_chunks) + c] = vir;
+    if (OBS) {
+      P.partials[P.total_chunks + c] = pe;
+      P.partials[2 * static_cast<size_t>(P.total_chunks) + c] = vir;
+    }
   }
 }
 
@@ -878,26 +886,33 @@ void launch_phased_step(const ReplicaParams& p, PhasedShape shape, bool initial,
   phased_reorder_commit_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   phased_build_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   const int kick = initial ? 0 : 1;
+  const bool obs = observables || initial;
+#define BEADS_PHASED_FORCE(TEAM, SLOTS)                                                       \
+  (obs ? phased_force_kernel<TEAM, SLOTS, true><<<p.total_chunks, kReplicaChunk * TEAM, 0, stream>>>( \
+             p, kick)                                                                         \
+       : phased_force_kernel<TEAM, SLOTS, false><<<p.total_chunks, kReplicaChunk * TEAM, 0,   \
+                                                   stream>>>(p, kick))
   switch (shape.team * 100 + shape.slots) {
     case 204:
-      phased_force_kernel<2, 4><<<p.total_chunks, kReplicaChunk * 2, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(2, 4);
       break;
     case 208:
-      phased_force_kernel<2, 8><<<p.total_chunks, kReplicaChunk * 2, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(2, 8);
       break;
     case 408:
-      phased_force_kernel<4, 8><<<p.total_chunks, kReplicaChunk * 4, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(4, 8);
       break;
     case 802:
-      phased_force_kernel<8, 2><<<p.total_chunks, kReplicaChunk * 8, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(8, 2);
       break;
     case 804:
-      phased_force_kernel<8, 4><<<p.total_chunks, kReplicaChunk * 8, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(8, 4);
       break;
     default:
-      phased_force_kernel<4, 4><<<p.total_chunks, kReplicaChunk * 4, 0, stream>>>(p, kick);
+      BEADS_PHASED_FORCE(4, 4);
       break;
   }
+#undef BEADS_PHASED_FORCE
   phased_reduce_kernel<<<p.count, kPhasedBlock, 0, stream>>>(p, initial ? 1 : 0,
                                                              observables ? 1 : 0);
 }
