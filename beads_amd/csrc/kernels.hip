@@ -970,9 +970,18 @@ __device__ __forceinline__ void lj_team(const ForceParams& p, uint32_t i, uint32
     }
 #pragma unroll
     for (int u = 0; u < kTeamSlots; ++u) {
-      const real_t dx = mic_diff_sel(xi4.x, xm0, xj[u].x, L0, h0);
-      const real_t dy = mic_diff_sel(xi4.y, xm1, xj[u].y, L1, h1);
-      const real_t dz = mic_diff_sel(xi4.z, xm2, xj[u].z, L2, h2);
+      // mic_diff_sel returns the raw difference unless it exceeds half the
+      // box; the folded forms are formed only when some lane of the wave
+      // needs them (pairs across a periodic boundary), the same values.
+      real_t dx = xi4.x - xj[u].x;
+      real_t dy = xi4.y - xj[u].y;
+      real_t dz = xi4.z - xj[u].z;
+      if ((__builtin_amdgcn_ballot_w64(fabs(dx) > h0) | __builtin_amdgcn_ballot_w64(fabs(dy) > h1) |
+           __builtin_amdgcn_ballot_w64(fabs(dz) > h2)) != 0) {
+        dx = mic_diff_sel(xi4.x, xm0, xj[u].x, L0, h0);
+        dy = mic_diff_sel(xi4.y, xm1, xj[u].y, L1, h1);
+        dz = mic_diff_sel(xi4.z, xm2, xj[u].z, L2, h2);
+      }
       const real_t r2 = r2_strict(dx, dy, dz);
       if (u * kTeam < rem && r2 < p.rc2) {
         LjCoeff c;
