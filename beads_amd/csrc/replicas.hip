@@ -230,9 +230,17 @@ __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* po
     }
 #pragma unroll
     for (int u = 0; u < SLOTS; ++u) {
-      const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
-      const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
-      const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
+      // mic_diff returns the raw difference unless it exceeds half the box;
+      // the folded forms only when some lane of the wave needs them.
+      real_t dx = xi.x - xj[u].x;
+      real_t dy = xi.y - xj[u].y;
+      real_t dz = xi.z - xj[u].z;
+      if ((__builtin_amdgcn_ballot_w64(fabs(dx) > h0) | __builtin_amdgcn_ballot_w64(fabs(dy) > h1) |
+           __builtin_amdgcn_ballot_w64(fabs(dz) > h2)) != 0) {
+        dx = mic_diff(xi.x, xj[u].x, L0, h0);
+        dy = mic_diff(xi.y, xj[u].y, L1, h1);
+        dz = mic_diff(xi.z, xj[u].z, L2, h2);
+      }
       if (base + u * TEAM + t < cnt && r2_strict(dx, dy, dz) < d.rc2) {
         const LjCoeff c = single ? c0 : lj[ti * T + static_cast<int>(id_of(xj[u].w))];
         double rr, uu;
