@@ -8,8 +8,9 @@ gradients. This module puts the engine's frame-mode entry points
 
 * ``FrameModel.forces(x, params)`` returns forces and the total potential
   energy; gradients of any loss of them flow to the positions (J^T g - gE F,
-  the Hessian-vector product of the adjoint kernel) and to the per-type-pair LJ
-  epsilon/sigma and per-type bond k/r0 (FP64 accumulation on the device);
+  the Hessian-vector product of the adjoint kernels) and to the per-type-pair
+  LJ epsilon/sigma, the per-type bond k/r0, angle k/theta0 (degrees) and
+  dihedral k (FP64 accumulation on the device);
 * ``FrameModel.trajectory(x0, v0, params, steps, checkpoint_every)`` runs
   velocity Verlet (NVE) with engine forces and backpropagates through it by a
   hand-written reverse sweep that recomputes each segment of ``checkpoint_every``
@@ -37,15 +38,25 @@ from .workloads import Workload
 @dataclass
 class Params:
     """Differentiable force-field parameters: epsilon/sigma [T, T] (symmetric),
-    bond k/r0 [n_bond_types]."""
+    bond k/r0 [n_bond_types], angle k and theta0 in degrees [n_angle_types],
+    dihedral k [n_dihedral_types] (None where the system has no such term)."""
 
     epsilon: torch.Tensor
     sigma: torch.Tensor
     bond_k: torch.Tensor
     bond_r0: torch.Tensor
+    angle_k: torch.Tensor | None = None
+    angle_theta0: torch.Tensor | None = None
+    dihedral_k: torch.Tensor | None = None
+
+    def slots(self) -> tuple[torch.Tensor | None, ...]:
+        """All seven parameter slots in the adjoint's order, None where absent."""
+        return (self.epsilon, self.sigma, self.bond_k, self.bond_r0, self.angle_k,
+                self.angle_theta0, self.dihedral_k)
 
     def tensors(self) -> tuple[torch.Tensor, ...]:
-        return self.epsilon, self.sigma, self.bond_k, self.bond_r0
+        """The parameters the system has, in the same order."""
+        return tuple(t for t in self.slots() if t is not None)
 
 
 class FrameModel:
@@ -55,9 +66,6 @@ class FrameModel:
 
     def __init__(self, workload: Workload, device: str | torch.device = "cuda",
                  precision: str | None = None):
-        if workload.angles.shape[0] or workload.dihedrals.shape[0]:
-            raise ValueError("the adjoint covers LJ pairs and harmonic bonds; remove angle and "
-                             "dihedral terms")
         self.workload = workload
         self.device = torch.device(device)
         self.engine = Engine(workload, precision)
@@ -65,6 +73,10 @@ class FrameModel:
         self.n = workload.n
         self.n_types = workload.n_types
         self.n_bond_types = int(workload.bond_k.shape[0])
+        self.n_angle_types = int(workload.angle_k.shape[0]) if workload.angles.shape[0] else 0
+        self.n_dihedral_types = (int(workload.dihedral_k.shape[0]) if workload.dihedrals.shape[0]
+                                 else 0)
+        self.n_params = self.engine.adjoint_param_count()
         self.masses = torch.as_tensor(workload.masses, dtype=self.real,
                                       device=self.device).reshape(-1, 1)
         self._params_key: bytes | None = None
@@ -94,22 +106,31 @@ class FrameModel:
             return torch.tensor(np.asarray(a), dtype=self.real,
                                 device=self.device, requires_grad=requires_grad)
 
-        return Params(t(w.lj_epsilon), t(w.lj_sigma), t(w.bond_k), t(w.bond_r0))
+        return Params(t(w.lj_epsilon), t(w.lj_sigma), t(w.bond_k), t(w.bond_r0),
+                      t(w.angle_k) if self.n_angle_types else None,
+                      t(w.angle_theta0) if self.n_angle_types else None,
+                      t(w.dihedral_k) if self.n_dihedral_types else None)
 
-    def _set_params(self, eps, sigma, k, r0) -> None:
+    def _set_params(self, *params) -> None:
+        """Upload (eps, sigma, bond k, r0, angle k, theta0 [deg], dihedral k)."""
+        params = tuple(params) + (None,) * (7 - len(params))
+        present = [p for p in params if p is not None]
         # Same tensors, unmodified since the last upload: nothing to do (no
         # device-to-host copy).
-        ident = tuple((p.data_ptr(), p._version, str(p.device), tuple(p.shape))
-                      for p in (eps, sigma, k, r0))
+        ident = tuple((p.data_ptr(), p._version, str(p.device), tuple(p.shape)) for p in present)
         if ident == self._params_ident:
             return
-        arrs = [np.ascontiguousarray(p.detach().to("cpu", self.real).numpy())
-                for p in (eps, sigma, k, r0)]
-        key = b"".join(a.tobytes() for a in arrs)
+        arrs = [None if p is None else np.ascontiguousarray(p.detach().to("cpu", self.real).numpy())
+                for p in params]
+        key = b"".join(a.tobytes() for a in arrs if a is not None)
         if key != self._params_key:
             self.engine.set_lj_params(arrs[0], arrs[1])
             if self.n_bond_types:
                 self.engine.set_bond_params(arrs[2], arrs[3])
+            if self.n_angle_types and arrs[4] is not None and arrs[5] is not None:
+                self.engine.set_angle_params(arrs[4], arrs[5])
+            if self.n_dihedral_types and arrs[6] is not None:
+                self.engine.set_dihedral_params(arrs[6])
             self._params_key = key
         self._params_ident = ident
 
@@ -147,20 +168,28 @@ class FrameModel:
 
     def _backward_raw(self, g_forces: torch.Tensor, g_energy: float):
         self._on_current_stream()
-        n_p = 2 * self.n_types * self.n_types + 2 * self.n_bond_types
         gx = torch.empty((self.n, 3), dtype=self.real, device=self.device)
-        gp = torch.empty(max(n_p, 1), dtype=torch.float64, device=self.device)
+        gp = torch.empty(max(self.n_params, 1), dtype=torch.float64, device=self.device)
         self.engine.frame_backward_async(g_forces.data_ptr(), g_energy, gx.data_ptr(),
                                          gp.data_ptr())
         self.backward_calls += 1
         return gx, gp
 
-    def _split(self, gp: torch.Tensor, like: Params | tuple) -> tuple[torch.Tensor, ...]:
-        T, nb = self.n_types, self.n_bond_types
-        eps, sig, k, r0 = like
-        parts = (gp[: T * T].reshape(T, T), gp[T * T: 2 * T * T].reshape(T, T),
-                 gp[2 * T * T: 2 * T * T + nb], gp[2 * T * T + nb: 2 * T * T + 2 * nb])
-        return tuple(p.to(q.dtype).to(q.device) for p, q in zip(parts, (eps, sig, k, r0)))
+    def _split(self, gp: torch.Tensor, like: Params | tuple) -> tuple[torch.Tensor | None, ...]:
+        """The device gradient vector as (eps, sigma, bond k, r0, angle k,
+        theta0 per degree, dihedral k), in the dtypes and devices of `like`
+        (None where `like` has None)."""
+        T, nb, na, nd = self.n_types, self.n_bond_types, self.n_angle_types, self.n_dihedral_types
+        like = tuple(like.slots() if isinstance(like, Params) else like)
+        like = like + (None,) * (7 - len(like))
+        sizes = (T * T, T * T, nb, nb, na, na, nd)
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        parts = [gp[offs[q]: offs[q + 1]] for q in range(7)]
+        parts[0] = parts[0].reshape(T, T)
+        parts[1] = parts[1].reshape(T, T)
+        parts[5] = parts[5] * (np.pi / 180.0)  # the kernel's gradient is per radian
+        return tuple(None if q is None else p.to(q.dtype).to(q.device)
+                     for p, q in zip(parts, like))
 
     def _positions(self, x: torch.Tensor) -> torch.Tensor:
         if x.shape != (self.n, 3):
@@ -172,7 +201,7 @@ class FrameModel:
     def forces(self, x: torch.Tensor, params: Params) -> tuple[torch.Tensor, torch.Tensor]:
         """Forces [n, 3] (the engine's real dtype) and total potential energy (FP64
         scalar)."""
-        return _FrameForces.apply(self, x, *params.tensors())
+        return _FrameForces.apply(self, x, *params.slots())
 
     def trajectory(self, x0: torch.Tensor, v0: torch.Tensor, params: Params, steps: int,
                    checkpoint_every: int = 20) -> tuple[torch.Tensor, torch.Tensor]:
@@ -181,7 +210,7 @@ class FrameModel:
         if steps < 1 or checkpoint_every < 1:
             raise ValueError("steps and checkpoint_every must be positive")
         return _VerletTrajectory.apply(self, int(steps), int(checkpoint_every), x0, v0,
-                                       *params.tensors())
+                                       *params.slots())
 
 
 def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
@@ -194,19 +223,18 @@ def force_matching_value_and_grad(model: FrameModel, frames: list[torch.Tensor],
     parameter gradients are summed in FP64 and returned in the parameters'
     dtypes. For training loops that need only dL/dtheta."""
     _check_frames(frames, targets)
-    eps, sigma, k, r0 = (t.detach() for t in params.tensors())
-    model._set_params(eps, sigma, k, r0)
+    ps = tuple(None if t is None else t.detach() for t in params.slots())
+    model._set_params(*ps)
     scale = 1.0 / (len(frames) * model.n * 3)
-    n_p = 2 * model.n_types * model.n_types + 2 * model.n_bond_types
     total = torch.zeros((), dtype=torch.float64, device=model.device)
-    g_sum = torch.zeros(max(n_p, 1), dtype=torch.float64, device=model.device)
+    g_sum = torch.zeros(max(model.n_params, 1), dtype=torch.float64, device=model.device)
     for x, ft in zip(frames, targets):
         f, _, _ = model._forward_raw(model._positions(x))
         diff = f.double() - ft.to(model.device).double()
         total += (diff * diff).sum()
         _, gp = model._backward_raw((2.0 * scale * diff).to(model.real).contiguous(), 0.0)
         g_sum += gp
-    return total * scale, Params(*model._split(g_sum, (eps, sigma, k, r0)))
+    return total * scale, Params(*model._split(g_sum, ps))
 
 
 def _check_frames(frames, targets) -> None:
@@ -229,8 +257,8 @@ def force_matching_loss(model: FrameModel, frames: list[torch.Tensor],
 
 class _FrameForces(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, model: FrameModel, x, eps, sigma, k, r0):
-        model._set_params(eps, sigma, k, r0)
+    def forward(ctx, model: FrameModel, x, *params):
+        model._set_params(*params)
         pos = model._positions(x)
         f, e, token = model._forward_raw(pos)
         # An output that the loss does not use arrives as None, not as a zero
@@ -239,21 +267,20 @@ class _FrameForces(torch.autograd.Function):
         ctx.model = model
         ctx.token = token
         ctx.x_dtype = x.dtype
-        ctx.save_for_backward(pos, eps, sigma, k, r0)
+        ctx.save_for_backward(pos, *params)
         return f, e.sum()
 
     @staticmethod
     def backward(ctx, g_forces, g_energy):
         model: FrameModel = ctx.model
-        pos, eps, sigma, k, r0 = ctx.saved_tensors
-        model._set_params(eps, sigma, k, r0)
+        pos, *params = ctx.saved_tensors
+        model._set_params(*params)
         model._load_raw(pos, ctx.token)
         gF = (torch.zeros_like(pos) if g_forces is None
               else g_forces.detach().to(model.device, model.real).contiguous())
         gE = 0.0 if g_energy is None else float(g_energy)
         gx, gp = model._backward_raw(gF, gE)
-        g_eps, g_sig, g_k, g_r0 = model._split(gp, (eps, sigma, k, r0))
-        return None, gx.to(ctx.x_dtype), g_eps, g_sig, g_k, g_r0
+        return (None, gx.to(ctx.x_dtype)) + model._split(gp, params)
 
 
 class _VerletTrajectory(torch.autograd.Function):
@@ -269,8 +296,8 @@ class _VerletTrajectory(torch.autograd.Function):
     recomputed from its checkpoint before its reverse pass."""
 
     @staticmethod
-    def forward(ctx, model: FrameModel, steps, every, x0, v0, eps, sigma, k, r0):
-        model._set_params(eps, sigma, k, r0)
+    def forward(ctx, model: FrameModel, steps, every, x0, v0, *params):
+        model._set_params(*params)
         dt = float(model.workload.dt)
         a = (0.5 * dt) / model.masses
         x = model._positions(x0)
@@ -288,14 +315,14 @@ class _VerletTrajectory(torch.autograd.Function):
         ctx.steps, ctx.every = steps, every
         ctx.checkpoints = checkpoints
         ctx.dtypes = (x0.dtype, v0.dtype)
-        ctx.save_for_backward(eps, sigma, k, r0)
+        ctx.save_for_backward(*params)
         return x.to(x0.dtype), v.to(v0.dtype)
 
     @staticmethod
     def backward(ctx, gx_T, gv_T):
         model: FrameModel = ctx.model
-        eps, sigma, k, r0 = ctx.saved_tensors
-        model._set_params(eps, sigma, k, r0)
+        params = ctx.saved_tensors
+        model._set_params(*params)
         dt = float(model.workload.dt)
         a = (0.5 * dt) / model.masses
         steps, every = ctx.steps, ctx.every
@@ -305,8 +332,7 @@ class _VerletTrajectory(torch.autograd.Function):
         vbar = (torch.zeros((model.n, 3), dtype=model.real, device=dev) if gv_T is None
                 else gv_T.detach().to(dev, model.real).clone())
         carry = torch.zeros_like(xbar)
-        n_p = 2 * model.n_types * model.n_types + 2 * model.n_bond_types
-        pbar = torch.zeros(max(n_p, 1), dtype=torch.float64, device=dev)
+        pbar = torch.zeros(max(model.n_params, 1), dtype=torch.float64, device=dev)
         for seg in range(len(ctx.checkpoints) - 1, -1, -1):
             start = seg * every
             end = min(start + every, steps)
@@ -335,6 +361,5 @@ class _VerletTrajectory(torch.autograd.Function):
                 gx, gp = model._backward_raw(carry.contiguous(), 0.0)
                 xbar = xbar + gx
                 pbar = pbar + gp
-        g_eps, g_sig, g_k, g_r0 = model._split(pbar, (eps, sigma, k, r0))
-        return (None, None, None, xbar.to(ctx.dtypes[0]), vbar.to(ctx.dtypes[1]),
-                g_eps, g_sig, g_k, g_r0)
+        return ((None, None, None, xbar.to(ctx.dtypes[0]), vbar.to(ctx.dtypes[1]))
+                + model._split(pbar, params))

@@ -154,6 +154,9 @@ def signatures(precision: str = "single") -> dict:
                                                    C.POINTER(PhaseTimes)]),
         "beads_engine_set_lj_params": (C.c_int, [C.c_void_p, _rp, _rp]),
         "beads_engine_set_bond_params": (C.c_int, [C.c_void_p, _rp, _rp]),
+        "beads_engine_set_angle_params": (C.c_int, [C.c_void_p, _rp, _rp]),
+        "beads_engine_set_dihedral_params": (C.c_int, [C.c_void_p, _rp]),
+        "beads_engine_adjoint_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
         "beads_engine_frame_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
                                                 C.POINTER(C.c_double), C.c_int32]),
         "beads_engine_frame_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
@@ -380,6 +383,22 @@ class Engine:
         rr = np.ascontiguousarray(r0, dtype=self.real)
         self._check(self.lib.beads_engine_set_bond_params(self.handle, _ptr(kk, self._real_ct),
                                                           _ptr(rr, self._real_ct)))
+
+    def set_angle_params(self, k: np.ndarray, theta0_deg: np.ndarray) -> None:
+        kk = np.ascontiguousarray(k, dtype=self.real)
+        tt = np.ascontiguousarray(theta0_deg, dtype=self.real)
+        self._check(self.lib.beads_engine_set_angle_params(self.handle, _ptr(kk, self._real_ct),
+                                                           _ptr(tt, self._real_ct)))
+
+    def set_dihedral_params(self, k: np.ndarray) -> None:
+        kk = np.ascontiguousarray(k, dtype=self.real)
+        self._check(self.lib.beads_engine_set_dihedral_params(self.handle,
+                                                              _ptr(kk, self._real_ct)))
+
+    def adjoint_param_count(self) -> int:
+        c = C.c_uint64()
+        self._check(self.lib.beads_engine_adjoint_param_count(self.handle, C.byref(c)))
+        return c.value
 
     def frame_forces(self, positions_ptr: int, forces_ptr: int,
                      reuse_list: bool = False) -> np.ndarray:

@@ -21,6 +21,7 @@
 //   dL/dx_self += phi (g_self - g_p) + (phi'/r) d (d . (g_self - g_p)), phi' = -k r0 / r^2
 //   dL/dk  += g_self . (-(r - r0) / r) d,   dL/dr0 += g_self . (k / r) d
 // Every bond is seen from both atoms, so each side adds half of dE/dk, dE/dr0.
+// Harmonic angles and dihedrals: listed_backward_kernel below.
 #include "adjoint.hpp"
 
 #include <mutex>
@@ -272,6 +273,266 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
   }
 }
 
+// ------------------------------------------------ angles and dihedrals ----
+//
+// The listed terms' part of dL/dx = J^T g - gE F and of the parameter
+// gradients, in a pass of its own (after fm_backward_kernel, adding to its
+// dL/dx). For a term with atoms a and the force F_b on atom b,
+// (J^T g)_b = sum_a (dF_a/dx_b)^T g_a = sum_a (dF_b/dx_a) g_a, since the
+// blocks dF_a/dx_b = -d2U/dx_a dx_b are each other's transposes: the
+// directional derivative of F_b when every atom a of the term moves along
+// g_a. Each role evaluates the term's force on its atom in forward-mode FP64
+// arithmetic (value, derivative along g) following angle_role_force and
+// dihedral_role_force (device_common.hpp), with theta and phi from FP64
+// atan2. Parameters: F is linear in k, and for angles F = A G with
+// A = -k (theta - theta0) / sin, so dF/dtheta0 = (k / sin) G; the role-0
+// record of each term adds dE/dk and dE/dtheta0 once.
+
+struct dual {
+  double v, d;  // value and derivative along the atoms' g
+};
+__device__ __forceinline__ dual operator+(dual a, dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ dual operator-(dual a, dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ dual operator*(dual a, dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ dual operator*(dual a, double s) { return {a.v * s, a.d * s}; }
+__device__ __forceinline__ dual dual_rsq(dual x) {
+  const double r = rsq_f64(x.v);
+  return {r, -0.5 * r * r * r * x.d};
+}
+__device__ __forceinline__ dual dual_rcp(dual x) {
+  const double r = rcp_f64(x.v);
+  return {r, -r * r * x.d};
+}
+__device__ __forceinline__ dual dual_atan2(dual y, dual x) {
+  const double q = x.v * x.v + y.v * y.v;
+  return {atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / q};
+}
+struct dv3 {
+  dual x, y, z;
+};
+__device__ __forceinline__ dv3 operator+(dv3 a, dv3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ dv3 operator-(dv3 a, dv3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ dv3 operator*(dv3 a, dual s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ dv3 operator*(dv3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ dual ddot(dv3 a, dv3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ dv3 dcross(dv3 a, dv3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Minimum-image displacement p - q (the forward's mic3) with derivative gp - gq.
+__device__ __forceinline__ dv3 dual_disp(real3 p, real3 q, real4 gp, real4 gq, const BoxView& box) {
+  const real3 d = mic3(p, q, box);
+  return {{d.x, static_cast<double>(gp.x) - gq.x},
+          {d.y, static_cast<double>(gp.y) - gq.y},
+          {d.z, static_cast<double>(gp.z) - gq.z}};
+}
+__device__ __forceinline__ double gdot(real4 g, double x, double y, double z) {
+  return static_cast<double>(g.x) * x + static_cast<double>(g.y) * y + static_cast<double>(g.z) * z;
+}
+
+// One listed role's contribution: force on the role's atom (value and
+// derivative), and the parameter gradient terms it adds.
+struct RoleAdjoint {
+  dv3 f;
+  double dk = 0.0, dtheta0 = 0.0;  // g_self . dF/dparam, plus the term's gE dE/dparam on role 0
+};
+
+// angle_role_force's mathematics; atoms (i, j, k) with j the vertex.
+__device__ RoleAdjoint angle_adjoint(real3 pi, real3 pj, real3 pk, real4 gi, real4 gj, real4 gk,
+                                     double k, double theta0, int role, real4 gself, double gE,
+                                     const BoxView& box) {
+  RoleAdjoint out;
+  out.f = {{0, 0}, {0, 0}, {0, 0}};
+  const dv3 a = dual_disp(pi, pj, gi, gj, box);
+  const dv3 b = dual_disp(pk, pj, gk, gj, box);
+  const dual ra2 = ddot(a, a), rb2 = ddot(b, b);
+  if (ra2.v == 0.0 || rb2.v == 0.0) {
+    if (role == 0) out.dk = gE * 0.5 * theta0 * theta0, out.dtheta0 = gE * k * theta0;
+    return out;
+  }
+  const dual ia = dual_rsq(ra2), ib = dual_rsq(rb2);
+  const dv3 nrm = dcross(a, b);
+  const dual nn2 = ddot(nrm, nrm);
+  const dual nn = nn2.v > 0.0 ? nn2 * dual_rsq(nn2) : dual{0.0, 0.0};
+  const dual theta = dual_atan2(nn, ddot(a, b));
+  const dual dth = theta - dual{theta0, 0.0};
+  const dual sn_true = nn * ia * ib;
+  constexpr double kSinFloor = static_cast<double>(static_cast<real_t>(0.001));
+  const dual sn = sn_true.v <= kSinFloor ? dual{kSinFloor, 0.0} : sn_true;
+  const dual A = dth * dual_rcp(sn) * (-k);
+  const dual aib = A * ia * ib;
+  const dv3 ai = dcross(a, nrm), bk = dcross(b, nrm);
+  const dv3 fi = ai * (aib * ia * ia);
+  const dv3 fk = bk * (aib * ib * ib * -1.0);
+  // G: the force with A = 1 (value only).
+  const double gi_s = ia.v * ib.v * ia.v * ia.v, gk_s = -ia.v * ib.v * ib.v * ib.v;
+  double Gx, Gy, Gz;
+  if (role == 0) {
+    out.f = fi;
+    Gx = ai.x.v * gi_s, Gy = ai.y.v * gi_s, Gz = ai.z.v * gi_s;
+  } else if (role == 2) {
+    out.f = fk;
+    Gx = bk.x.v * gk_s, Gy = bk.y.v * gk_s, Gz = bk.z.v * gk_s;
+  } else {
+    out.f = (fi + fk) * -1.0;
+    Gx = -(ai.x.v * gi_s + bk.x.v * gk_s);
+    Gy = -(ai.y.v * gi_s + bk.y.v * gk_s);
+    Gz = -(ai.z.v * gi_s + bk.z.v * gk_s);
+  }
+  const double isn = 1.0 / sn.v;
+  const double wg = gdot(gself, Gx, Gy, Gz);
+  out.dk = wg * (-dth.v * isn);
+  out.dtheta0 = wg * (k * isn);
+  if (role == 0) {
+    out.dk += gE * 0.5 * dth.v * dth.v;
+    out.dtheta0 += gE * (-k * dth.v);
+  }
+  return out;
+}
+
+// dihedral_role_force's mathematics; atoms (0, 1, 2, 3) along the chain.
+__device__ RoleAdjoint dihedral_adjoint(real3 q0, real3 q1, real3 q2, real3 q3, real4 g0, real4 g1,
+                                        real4 g2, real4 g3, double k, double dsign, double nmul,
+                                        int role, real4 gself, double gE, const BoxView& box) {
+  RoleAdjoint out;
+  out.f = {{0, 0}, {0, 0}, {0, 0}};
+  const dv3 v0 = dual_disp(q0, q1, g0, g1, box);
+  const dv3 v1 = dual_disp(q2, q1, g2, g1, box);
+  const dv3 v2 = dual_disp(q2, q3, g2, g3, box);
+  const dv3 c0 = dcross(v0, v1), c1 = dcross(v1, v2);
+  const dual n1 = ddot(v1, v1), nc0 = ddot(c0, c0), nc1 = ddot(c1, c1);
+  if (n1.v == 0.0 || nc0.v == 0.0 || nc1.v == 0.0) {
+    if (role == 0) out.dk = gE * (nmul == 0.0 ? 1.0 + dsign : 1.0);
+    return out;
+  }
+  const dual inb = dual_rsq(n1);
+  const dual nb = n1 * inb;
+  const dual ic0 = dual_rsq(nc0), ic1 = dual_rsq(nc1);
+  const dual inv_c = ic0 * ic1;
+  const dual c1phi = ddot(c0, c1) * inv_c;
+  const dual s1phi = nb * ddot(v0, c1) * inv_c;
+  dual cs{1.0, 0.0}, sn{0.0, 0.0};
+  for (int m = 0; m < static_cast<int>(nmul); ++m) {
+    const dual c2 = cs * c1phi - sn * s1phi;
+    sn = sn * c1phi + cs * s1phi;
+    cs = c2;
+  }
+  // F is linear in k: evaluated at k = 1 (dF/dk), then scaled.
+  const dual dE1 = sn * (-dsign * nmul);
+  const dv3 fi = c0 * (dE1 * nb * ic0 * ic0 * -1.0);
+  const dv3 fl = c1 * (dE1 * nb * ic1 * ic1);
+  const dual inv_n1 = inb * inb;
+  const dv3 sv = fi * (ddot(v0, v1) * inv_n1) - fl * (ddot(v2, v1) * inv_n1);
+  dv3 f1;
+  if (role == 0) {
+    f1 = fi;
+  } else if (role == 1) {
+    f1 = sv - fi;
+  } else if (role == 2) {
+    f1 = (sv + fl) * -1.0;
+  } else {
+    f1 = fl;
+  }
+  out.f = f1 * k;
+  out.dk = gdot(gself, f1.x.v, f1.y.v, f1.z.v);
+  if (role == 0) out.dk += gE * (1.0 + dsign * cs.v);
+  return out;
+}
+
+constexpr int kListTeam = 4;
+constexpr uint32_t kListAtoms = kBlock / kListTeam;
+
+__global__ __launch_bounds__(kBlock) void listed_backward_kernel(AdjointParams p) {
+  __shared__ double acc[2 * kMaxAngleTypesAdjoint + kMaxDihedralTypesAdjoint];
+  const int na = p.n_angle_types, nd = p.n_dihedral_types;
+  const int n_acc = 2 * na + nd;
+  for (int k = threadIdx.x; k < n_acc; k += kBlock) acc[k] = 0.0;
+  __syncthreads();
+  const uint32_t n = p.n;
+  const uint32_t t = threadIdx.x & (kListTeam - 1);
+  const uint32_t i = blockIdx.x * kListAtoms + threadIdx.x / kListTeam;
+  const double gE = p.grad_energy;
+  if (i < n) {  // uniform over a team
+    const real3 xi = f3(p.pos[i]);
+    const real4 gi = p.g[i];
+    const uint32_t cw = p.topo_count[i];
+    const uint32_t ang_inline = p.acap > 0 ? (cw >> 8) & 255u : 0u;
+    const uint32_t dih_inline = p.dcap > 0 ? (cw >> 16) & 255u : 0u;
+    uint32_t nang = ang_inline, ndih = dih_inline, off_a = 0, off_d = 0;
+    if (p.spill_index != nullptr) {
+      const uint32_t si = p.spill_index[i];
+      if (si != kNoSpill) {
+        nang += p.spill[si].cnt[1];
+        off_a = p.spill[si].off[1];
+        ndih += p.spill[si].cnt[2];
+        off_d = p.spill[si].off[2];
+      }
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    auto add = [&](const RoleAdjoint& r) {
+      gx += r.f.x.d;
+      gy += r.f.y.d;
+      gz += r.f.z.d;
+      fx += r.f.x.v;
+      fy += r.f.y.v;
+      fz += r.f.z.v;
+    };
+    for (uint32_t k = t; k < nang; k += kListTeam) {
+      const uint4 r = k < ang_inline ? p.angle[static_cast<size_t>(k) * n + i]
+                                     : p.spill_angle[off_a + (k - ang_inline)];
+      const int role = static_cast<int>(r.w);
+      const real3 a = f3(p.pos[r.x]), b = f3(p.pos[r.y]);
+      const real4 ga = p.g[r.x], gb = p.g[r.y];
+      const real3 pi = role == 0 ? xi : a;
+      const real3 pj = role == 0 ? a : (role == 1 ? xi : b);
+      const real3 pk = role == 2 ? xi : b;
+      const real4 qi = role == 0 ? gi : ga;
+      const real4 qj = role == 0 ? ga : (role == 1 ? gi : gb);
+      const real4 qk = role == 2 ? gi : gb;
+      const RoleAdjoint ra = angle_adjoint(pi, pj, pk, qi, qj, qk, p.angle_k[r.z],
+                                           p.angle_theta0[r.z], role, gi, gE, p.box);
+      add(ra);
+      atomicAdd(&acc[r.z], ra.dk);
+      atomicAdd(&acc[na + r.z], ra.dtheta0);
+    }
+    for (uint32_t k = t; k < ndih; k += kListTeam) {
+      const uint4 r = k < dih_inline ? p.dih[static_cast<size_t>(k) * n + i]
+                                     : p.spill_dih[off_d + (k - dih_inline)];
+      const int role = static_cast<int>(r.w >> 30);
+      const uint32_t ty = r.w & 0x3fffffffu;
+      const real3 a = f3(p.pos[r.x]), b = f3(p.pos[r.y]), c = f3(p.pos[r.z]);
+      const real4 ga = p.g[r.x], gb = p.g[r.y], gc = p.g[r.z];
+      real3 q0, q1, q2, q3;
+      real4 h0, h1, h2, h3;
+      if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; h0 = gi; h1 = ga; h2 = gb; h3 = gc; }
+      else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; h0 = ga; h1 = gi; h2 = gb; h3 = gc; }
+      else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; h0 = ga; h1 = gb; h2 = gi; h3 = gc; }
+      else { q0 = a; q1 = b; q2 = c; q3 = xi; h0 = ga; h1 = gb; h2 = gc; h3 = gi; }
+      const RoleAdjoint rd = dihedral_adjoint(q0, q1, q2, q3, h0, h1, h2, h3, p.dih_k[ty],
+                                              p.dih_d[ty], p.dih_n[ty], role, gi, gE, p.box);
+      add(rd);
+      atomicAdd(&acc[2 * na + ty], rd.dk);
+    }
+    gx = team_sum<kListTeam>(gx);
+    gy = team_sum<kListTeam>(gy);
+    gz = team_sum<kListTeam>(gz);
+    fx = team_sum<kListTeam>(fx);
+    fy = team_sum<kListTeam>(fy);
+    fz = team_sum<kListTeam>(fz);
+    if (t == 0 && (nang | ndih) != 0) {
+      // fm_backward_kernel stored the pair and bond part of dL/dx.
+      real_t* o = p.grad_pos + 3 * static_cast<size_t>(i);
+      o[0] = static_cast<real_t>(static_cast<double>(o[0]) + (gx - gE * fx));
+      o[1] = static_cast<real_t>(static_cast<double>(o[1]) + (gy - gE * fy));
+      o[2] = static_cast<real_t>(static_cast<double>(o[2]) + (gz - gE * fz));
+    }
+  }
+  __syncthreads();
+  const size_t base = 2 * static_cast<size_t>(p.n_types) * p.n_types + 2 * p.n_bond_types;
+  for (int k = threadIdx.x; k < n_acc; k += kBlock) {
+    if (acc[k] != 0.0) atomicAdd(&p.grad_params[base + k], acc[k]);
+  }
+}
+
 }  // namespace
 
 void launch_frame_load(const FrameLoadParams& p, bool check, hipStream_t stream) {
@@ -321,6 +582,9 @@ void launch_fm_backward(const AdjointParams& p, hipStream_t stream) {
     fm_backward_kernel<4><<<g, kBlock, 0, stream>>>(p);
   } else {
     fm_backward_kernel<0><<<g, kBlock, 0, stream>>>(p);
+  }
+  if (p.acap > 0 || p.dcap > 0) {
+    listed_backward_kernel<<<grid_for(p.n, kListAtoms), kBlock, 0, stream>>>(p);
   }
 }
 

@@ -9,6 +9,8 @@
 namespace beads_amd {
 
 constexpr int kMaxBondTypesAdjoint = 64;
+constexpr int kMaxAngleTypesAdjoint = 64;
+constexpr int kMaxDihedralTypesAdjoint = 64;
 
 // Largest system frame_backward's allocation-free variant serves: its dL/dF
 // staging lives in a static device array of this many real4 (16 MiB per GPU,
@@ -37,9 +39,25 @@ struct AdjointParams {
   int32_t n_bond_types;
   const double* bond_k;
   const double* bond_r0;
+  // Harmonic angles and dihedrals (per-slot role records as in the force
+  // pass; acap/dcap 0 when the system has none).
+  const uint4* angle;
+  const uint4* dih;
+  uint32_t acap, dcap;
+  const uint4* spill_angle;
+  const uint4* spill_dih;
+  int32_t n_angle_types;
+  const double* angle_k;
+  const double* angle_theta0;  // radians
+  int32_t n_dihedral_types;
+  const double* dih_k;
+  const double* dih_d;
+  const double* dih_n;
   double grad_energy;       // dL/dE for the total potential energy
   real_t* grad_pos;          // [n][3]
-  double* grad_params;      // eps[T*T], sigma[T*T], k[nb], r0[nb]; zeroed by the caller
+  // eps[T*T], sigma[T*T], bond k[nb], bond r0[nb], angle k[na], angle
+  // theta0[na] (per radian), dihedral k[nd]; zeroed by the caller
+  double* grad_params;
 };
 
 // Caller positions [n][3] (unwrapped allowed) -> wrapped pos (type kept in .w)

@@ -174,6 +174,28 @@ beads_status_t beads_engine_set_bond_params(beads_engine_t* e, const beads_real_
   });
 }
 
+beads_status_t beads_engine_set_angle_params(beads_engine_t* e, const beads_real_t* k,
+                                             const beads_real_t* theta0_deg) {
+  return guarded([&] {
+    beads_amd::require(k && theta0_deg, "angle parameter arrays are null", BEADS_E_TYPE);
+    engine_of(e).set_angle_params(k, theta0_deg);
+  });
+}
+
+beads_status_t beads_engine_set_dihedral_params(beads_engine_t* e, const beads_real_t* k) {
+  return guarded([&] {
+    beads_amd::require(k != nullptr, "dihedral parameter array is null", BEADS_E_TYPE);
+    engine_of(e).set_dihedral_params(k);
+  });
+}
+
+beads_status_t beads_engine_adjoint_param_count(beads_engine_t* e, uint64_t* count) {
+  return guarded([&] {
+    beads_amd::require(count != nullptr, "count output is null", BEADS_E_TYPE);
+    *count = engine_of(e).adjoint_param_count();
+  });
+}
+
 beads_status_t beads_engine_frame_forces(beads_engine_t* e, const beads_real_t* positions, beads_real_t* forces,
                                          double* energies, int32_t reuse_list) {
   return guarded([&] {

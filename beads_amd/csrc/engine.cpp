@@ -1069,6 +1069,25 @@ void Engine::set_bond_params(const real_t* k, const real_t* r0) {
   upload_parameters();
 }
 
+void Engine::set_angle_params(const real_t* k, const real_t* theta0_deg) {
+  require(models_.angle != nullptr, "this engine has no angle style.", BEADS_E_VALUE);
+  models_.angle->set_parameters(k, theta0_deg);
+  upload_parameters();
+}
+
+void Engine::set_dihedral_params(const real_t* k) {
+  require(models_.dihedral != nullptr, "this engine has no dihedral style.", BEADS_E_VALUE);
+  models_.dihedral->set_parameters(k, nullptr);
+  upload_parameters();
+}
+
+size_t Engine::adjoint_param_count() const {
+  const size_t na = models_.angle ? static_cast<size_t>(models_.angle->type_count()) : 0;
+  const size_t nd = models_.dihedral ? static_cast<size_t>(models_.dihedral->type_count()) : 0;
+  return 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * static_cast<size_t>(n_bond_types_) +
+         2 * na + nd;
+}
+
 // One force evaluation at caller-supplied positions ([n][3] real_t in slot
 // order, device memory): wrap into the box, neighbour rebuild without
 // reordering (forced, or with reuse_list only when some atom moved more than
@@ -1167,14 +1186,12 @@ void Engine::frame_backward(const real_t* grad_forces_dev, double grad_energy,
   require(frame_loaded_, "frame_backward needs a preceding frame_forces call.", BEADS_E_RUNTIME);
   require(!half_list_, "the adjoint reads the full neighbour list; create the engine with "
           "half_list = 0.", BEADS_E_VALUE);
-  require(acap_ == 0 && dcap_ == 0,
-          "the adjoint differentiates LJ pairs and harmonic bonds; systems with angle or "
-          "dihedral terms are not supported.",
-          BEADS_E_VALUE);
   AdjointParams p{};
   models_.pair->bind_adjoint(p);
   if (models_.bond) models_.bond->bind_adjoint(p);
-  const size_t n_params = 2 * static_cast<size_t>(n_types_) * n_types_ + 2 * n_bond_types_;
+  if (models_.angle) models_.angle->bind_adjoint(p);
+  if (models_.dihedral) models_.dihedral->bind_adjoint(p);
+  const size_t n_params = adjoint_param_count();
   real4* staging = nullptr;
   if (noalloc) {
     require(n_ <= kNoAllocMaxAtoms,
@@ -1207,6 +1224,12 @@ void Engine::frame_backward(const real_t* grad_forces_dev, double grad_energy,
   p.spill_index = has_spill_ ? spill_index_.data() : nullptr;
   p.spill = spill_.data();
   p.spill_bond = spill_bond_.data();
+  p.angle = angle_.data();
+  p.dih = dih_.data();
+  p.acap = acap_;
+  p.dcap = dcap_;
+  p.spill_angle = spill_angle_.data();
+  p.spill_dih = spill_dih_.data();
   p.grad_energy = grad_energy;
   p.grad_pos = grad_positions_dev;
   p.grad_params = grad_params_dev;

@@ -107,6 +107,11 @@ class Engine {
   // caller in slot order, so particles are never reordered.
   void set_lj_params(const real_t* eps, const real_t* sigma);
   void set_bond_params(const real_t* k, const real_t* r0);
+  void set_angle_params(const real_t* k, const real_t* theta0_deg);
+  void set_dihedral_params(const real_t* k);
+  // Entries of frame_backward's parameter gradient: eps, sigma [T*T] each,
+  // bond k, r0 [nb] each, angle k, theta0 [na] each, dihedral k [nd].
+  size_t adjoint_param_count() const;
   // async: enqueue and return without waiting (energies_out is then device
   // memory, and a list overflow is reported by the next synchronize()).
   void frame_forces(const real_t* positions_dev, real_t* forces_dev, double* energies_out,

@@ -154,22 +154,50 @@ class HarmonicAngleModel final : public ListedModel {
                   s.angle_theta0_deg[t] <= 180,
               "Angle(\"harmonic\") angle_coeff.theta0 must be between 0 and 180 degrees.");
     }
-    c_.resize(std::max(1, s.n_angle_types));
-    for (int t = 0; t < s.n_angle_types; ++t) {
-      c_[t] = make_real2(s.angle_k[t],
-                         s.angle_theta0_deg[t] * static_cast<real_t>(kPiD) / real_t(180));
-    }
     n_ = s.n_angle_types;
+    assign(s.angle_k, s.angle_theta0_deg);
   }
   const char* style_name() const override { return "harmonic"; }
   int32_t type_count() const override { return n_; }
-  void upload(hipStream_t stream) override { coeff_.upload(c_.data(), c_.size(), stream); }
+  void upload(hipStream_t stream) override {
+    coeff_.upload(c_.data(), c_.size(), stream);
+    std::vector<double> k(std::max(1, n_), 0.0), th(std::max(1, n_), 0.0);
+    for (int t = 0; t < n_; ++t) {
+      k[t] = c_[t].x;
+      th[t] = c_[t].y;
+    }
+    k_d_.upload(k.data(), k.size(), stream);
+    theta0_d_.upload(th.data(), th.size(), stream);
+  }
   void bind(TopologyView& topo) const override { topo.angle_coeff = coeff_.data(); }
+  // Frame mode: k and theta0 in degrees, as in the spec.
+  void set_parameters(const real_t* k, const real_t* theta0_deg) override {
+    for (int t = 0; t < n_; ++t) {
+      require(std::isfinite(k[t]) && k[t] >= 0 && std::isfinite(theta0_deg[t]) &&
+                  theta0_deg[t] >= 0 && theta0_deg[t] <= 180,
+              "angle k must be finite and non-negative and theta0 between 0 and 180 degrees.");
+    }
+    assign(k, theta0_deg);
+  }
+  void bind_adjoint(AdjointParams& p) const override {
+    require(n_ <= kMaxAngleTypesAdjoint, "too many angle types for the adjoint kernels.",
+            BEADS_E_VALUE);
+    p.n_angle_types = n_;
+    p.angle_k = k_d_.data();
+    p.angle_theta0 = theta0_d_.data();
+  }
 
  private:
+  void assign(const real_t* k, const real_t* theta0_deg) {
+    c_.assign(std::max(1, n_), make_real2(0, 0));
+    for (int t = 0; t < n_; ++t) {
+      c_[t] = make_real2(k[t], theta0_deg[t] * static_cast<real_t>(kPiD) / real_t(180));
+    }
+  }
   int32_t n_ = 0;
   std::vector<real2> c_;
   DeviceBuffer<real2> coeff_;
+  DeviceBuffer<double> k_d_, theta0_d_;
 };
 
 // dihedral_harmonic.cu: U = k [1 + d cos(n phi)], d = +-1, n >= 0.
@@ -192,13 +220,41 @@ class HarmonicDihedralModel final : public ListedModel {
   }
   const char* style_name() const override { return "harmonic"; }
   int32_t type_count() const override { return n_; }
-  void upload(hipStream_t stream) override { coeff_.upload(c_.data(), c_.size(), stream); }
+  void upload(hipStream_t stream) override {
+    coeff_.upload(c_.data(), c_.size(), stream);
+    std::vector<double> k(std::max(1, n_), 0.0), d(std::max(1, n_), 0.0),
+        m(std::max(1, n_), 0.0);
+    for (int t = 0; t < n_; ++t) {
+      k[t] = c_[t].x;
+      d[t] = c_[t].y;
+      m[t] = c_[t].z;
+    }
+    k_d_.upload(k.data(), k.size(), stream);
+    d_d_.upload(d.data(), d.size(), stream);
+    n_d_.upload(m.data(), m.size(), stream);
+  }
   void bind(TopologyView& topo) const override { topo.dih_coeff = coeff_.data(); }
+  // Frame mode: k only (d and n are integers of the topology's style).
+  void set_parameters(const real_t* k, const real_t*) override {
+    for (int t = 0; t < n_; ++t) {
+      require(std::isfinite(k[t]) && k[t] >= 0, "dihedral k must be finite and non-negative.");
+    }
+    for (int t = 0; t < n_; ++t) c_[t].x = k[t];
+  }
+  void bind_adjoint(AdjointParams& p) const override {
+    require(n_ <= kMaxDihedralTypesAdjoint, "too many dihedral types for the adjoint kernels.",
+            BEADS_E_VALUE);
+    p.n_dihedral_types = n_;
+    p.dih_k = k_d_.data();
+    p.dih_d = d_d_.data();
+    p.dih_n = n_d_.data();
+  }
 
  private:
   int32_t n_ = 0;
   std::vector<real4> c_;
   DeviceBuffer<real4> coeff_;
+  DeviceBuffer<double> k_d_, d_d_, n_d_;
 };
 
 // ------------------------------------------------------ dynamics ------------

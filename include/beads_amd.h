@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BEADS_AMD_ABI_VERSION 5
+#define BEADS_AMD_ABI_VERSION 6
 
 #ifdef BEADS_REAL64
 typedef double beads_real_t;
@@ -262,11 +262,18 @@ beads_status_t beads_engine_profile_advance(beads_engine_t* engine, uint64_t n_s
  * device memory; every call synchronises the engine's stream before returning.
  *
  * Parameter updates take host arrays: epsilon/sigma [T][T] (symmetric for the
- * adjoint), bond k/r0 [n_bond_types]. */
+ * adjoint), bond k/r0 [n_bond_types], angle k and theta0 in degrees
+ * [n_angle_types], dihedral k [n_dihedral_types] (its d and n stay the spec's). */
 beads_status_t beads_engine_set_lj_params(beads_engine_t* engine, const beads_real_t* epsilon,
                                           const beads_real_t* sigma);
 beads_status_t beads_engine_set_bond_params(beads_engine_t* engine, const beads_real_t* k,
                                             const beads_real_t* r0);
+beads_status_t beads_engine_set_angle_params(beads_engine_t* engine, const beads_real_t* k,
+                                             const beads_real_t* theta0_deg);
+beads_status_t beads_engine_set_dihedral_params(beads_engine_t* engine, const beads_real_t* k);
+/* Number of doubles frame_backward writes to grad_params:
+ * 2*T*T + 2*n_bond_types + 2*n_angle_types + n_dihedral_types. */
+beads_status_t beads_engine_adjoint_param_count(beads_engine_t* engine, uint64_t* count);
 /* Wrap positions into the box, update the neighbour list, compute forces.
  * reuse_list = 0 rebuilds the list at every call; reuse_list = 1 keeps the
  * previous frame's list while every atom stays within half the skin of its
@@ -284,8 +291,10 @@ beads_status_t beads_engine_frame_load(beads_engine_t* engine, const beads_real_
 /* Adjoint of the last frame_forces (or frame_load) call for L with dL/dF = grad_forces and
  * dL/dE = grad_energy (E = total potential energy). Writes dL/dx [n][3] and
  * FP64 parameter gradients (device) laid out as
- * [epsilon T*T | sigma T*T | bond k nb | bond r0 nb]. LJ pairs and harmonic
- * bonds only: systems with angle or dihedral terms are rejected. */
+ * [epsilon T*T | sigma T*T | bond k nb | bond r0 nb | angle k na |
+ *  angle theta0 na (per radian) | dihedral k nd]
+ * (beads_engine_adjoint_param_count entries). LJ pairs, harmonic bonds,
+ * harmonic angles and harmonic dihedrals. */
 beads_status_t beads_engine_frame_backward(beads_engine_t* engine, const beads_real_t* grad_forces,
                                            double grad_energy, beads_real_t* grad_positions,
                                            double* grad_params);

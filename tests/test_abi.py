@@ -45,7 +45,7 @@ def test_library_exports_every_declared_symbol(precision):
 @pytest.mark.parametrize("precision,dtype", [("single", b"float32"), ("double", b"float64")])
 def test_abi_version_and_build_info(precision, dtype):
     lib = capi.library(precision)
-    assert lib.beads_abi_version() == 5
+    assert lib.beads_abi_version() == 6
     info = capi.BuildInfo()
     assert lib.beads_build_info(ctypes.byref(info)) == 0
     assert info.real_precision == precision.encode()

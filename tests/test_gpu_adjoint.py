@@ -420,10 +420,7 @@ def test_trajectory_backprop_vs_fd():
     assert _rel_err(gpu_v, fd_v) <= TRAJ_RTOL, (gpu_v, fd_v)
 
 
-def test_frame_mode_rejects_angles_and_dynamics_mixing():
-    w = W.polymer_melt(n_chains=4, beads_per_chain=10, density=8.5)
-    with pytest.raises(ValueError):
-        FrameModel(w)
+def test_frame_mode_rejects_dynamics_mixing():
     ws = small_system()
     with FrameModel(ws) as m:
         x = torch.tensor(ws.positions, device="cuda")
@@ -543,3 +540,149 @@ def test_full_size_trajectory_gradient_independent_of_checkpoint_interval():
         np.testing.assert_array_equal(got[0], ref[0])  # the same forward trajectory
         for g, r in zip(got[1:], ref[1:]):
             assert _rel_err(g, r) <= 1e-6, every
+
+
+# ------------------------------------------------ angles and dihedrals ----
+
+def melt_small(seed=3):
+    """480 beads of the melt's force field (bonds, 120-degree angles, n = 1
+    dihedrals, exclusions to distance 2), jittered off the lattice so that no
+    angle sits near 0 or 180 degrees."""
+    return W.polymer_melt(n_chains=24, beads_per_chain=20, density=8.5, seed=seed, jitter=0.06)
+
+
+def _with_listed(w, ak, at, dk):
+    return w.with_(angle_k=np.asarray(ak, np.float32), angle_theta0=np.asarray(at, np.float32),
+                   dihedral_k=np.asarray(dk, np.float32))
+
+
+def _listed_vector(w):
+    return [w.angle_k.astype(np.float32).copy(), w.angle_theta0.astype(np.float32).copy(),
+            w.dihedral_k.astype(np.float32).copy()]
+
+
+def test_listed_terms_force_matching_gradients_vs_fd():
+    """Positions and the angle k, theta0 (per degree) and dihedral k gradients
+    of a force-matching loss through harmonic angles and dihedrals, against
+    central differences of the FP64 oracle's loss, at the adjoint bar."""
+    w = melt_small()
+    x0 = _frames(w, 1, seed=4, amp=0.02)[0]
+    # Targets from parameters well away from the model's, so that dL/dF = 2 (F - F_target)
+    # / (3n) is large against the binary32 forward's own rounding (its angles come
+    # from a binary32 atan2, about 1e-7 rad).
+    wt = _with_listed(w, w.angle_k * 1.3, w.angle_theta0 * 0.9, w.dihedral_k * 1.5)
+    target = O.forces_f64(wt, x0.astype(np.float64))[0]
+
+    def loss64(pos64, listed):
+        f = O.forces_f64(_with_listed(w, *listed), pos64)[0]
+        return np.sum((f - target) ** 2) / (w.n * 3)
+
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        assert p.angle_k is not None and p.dihedral_k is not None
+        x = torch.tensor(x0, device="cuda", requires_grad=True)
+        loss = force_matching_loss(m, [x], [torch.tensor(target, device="cuda")], p)
+        loss.backward()
+        gx = x.grad.cpu().numpy().astype(np.float64)
+        g_par = [float(p.angle_k.grad[0]), float(p.angle_theta0.grad[0]),
+                 float(p.dihedral_k.grad[0])]
+    listed = _listed_vector(w)
+    base = x0.astype(np.float64)
+    # chain ends, second and middle beads of a few chains
+    atoms = [0, 1, 2, 9, 10, 19, 20, 21, 200, 239, 240, int(w.n - 1)]
+    g_fd, g_gpu = [], []
+    for a in atoms:
+        for c in range(3):
+            h = 1e-5
+            pp, pm = base.copy(), base.copy()
+            pp[a, c] += h
+            pm[a, c] -= h
+            g_fd.append((loss64(pp, listed) - loss64(pm, listed)) / (2 * h))
+            g_gpu.append(gx[a, c])
+    assert _rel_err(g_gpu, g_fd) <= GRAD_RTOL, (g_gpu, g_fd)
+    p_fd = [_fd_param(lambda q: loss64(base, q), listed, which, 0) for which in range(3)]
+    assert _rel_err(g_par, p_fd) <= GRAD_RTOL, (g_par, p_fd)
+
+
+def test_listed_terms_energy_gradients():
+    """dE/dx = -F with angles and dihedrals, and dE/d(angle k, theta0,
+    dihedral k) against central differences of the FP64 oracle's energy."""
+    w = melt_small(seed=5)
+    x0 = _frames(w, 1, seed=6, amp=0.02)[0]
+
+    def energy64(listed):
+        o = O.forces_f64(_with_listed(w, *listed), x0.astype(np.float64))[1]
+        return o["pair_pe"] + o["bond_pe"] + o["angle_pe"] + o["dihedral_pe"]
+
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        x = torch.tensor(x0, device="cuda", requires_grad=True)
+        f, e = m.forces(x, p)
+        e.backward()
+        gx = x.grad.cpu().numpy().astype(np.float64)
+        f = f.detach().cpu().numpy().astype(np.float64)
+        g_par = [float(p.angle_k.grad[0]), float(p.angle_theta0.grad[0]),
+                 float(p.dihedral_k.grad[0])]
+    assert _force_error(-gx, f) <= FORCE_RTOL
+    listed = _listed_vector(w)
+    p_fd = [_fd_param(energy64, listed, which, 0) for which in range(3)]
+    assert _rel_err(g_par, p_fd) <= GRAD_RTOL, (g_par, p_fd)
+
+
+def melt_two_types(seed=7):
+    """melt_small with angles and dihedrals alternating between two types of
+    different parameters; the second dihedral type has d = -1, n = 2."""
+    w = melt_small(seed=seed)
+    angles = w.angles.copy()
+    angles[1::2, 3] = 2
+    dihedrals = w.dihedrals.copy()
+    dihedrals[1::2, 4] = 2
+    return w.with_(angles=angles, dihedrals=dihedrals,
+                   angle_k=np.array([25.0, 40.0], np.float32),
+                   angle_theta0=np.array([120.0, 105.0], np.float32),
+                   dihedral_k=np.array([1.0, 2.5], np.float32),
+                   dihedral_d=np.array([1, -1], np.int32), dihedral_n=np.array([1, 2], np.int32))
+
+
+def test_listed_terms_gradients_per_type():
+    """Every angle k, theta0 and dihedral k gradient of two-type angles and
+    dihedrals (one with d = -1, n = 2), for the energy and for a force-matching
+    loss, against central differences of the FP64 oracle; dE/dx = -F."""
+    w = melt_two_types()
+    x0 = _frames(w, 1, seed=8, amp=0.02)[0]
+    pos64 = x0.astype(np.float64)
+    wt = _with_listed(w, w.angle_k * np.array([1.3, 0.8]), w.angle_theta0 * np.array([0.9, 1.1]),
+                      w.dihedral_k * np.array([1.5, 0.6]))
+    target = O.forces_f64(wt, pos64)[0]
+
+    def energy64(listed):
+        o = O.forces_f64(_with_listed(w, *listed), pos64)[1]
+        return o["pair_pe"] + o["bond_pe"] + o["angle_pe"] + o["dihedral_pe"]
+
+    def loss64(listed):
+        f = O.forces_f64(_with_listed(w, *listed), pos64)[0]
+        return np.sum((f - target) ** 2) / (w.n * 3)
+
+    def grads(p):
+        return [float(g) for t in (p.angle_k, p.angle_theta0, p.dihedral_k) for g in t.grad]
+
+    with FrameModel(w) as m:
+        assert m.n_angle_types == 2 and m.n_dihedral_types == 2
+        p = m.initial_params()
+        x = torch.tensor(x0, device="cuda", requires_grad=True)
+        f, e = m.forces(x, p)
+        e.backward()
+        f = f.detach().cpu().numpy().astype(np.float64)
+        assert _force_error(-x.grad.cpu().numpy().astype(np.float64), f) <= FORCE_RTOL
+        assert _force_error(f, O.forces_f64(w, pos64)[0]) <= FORCE_RTOL
+        g_energy = grads(p)
+        p = m.initial_params()
+        force_matching_loss(m, [torch.tensor(x0, device="cuda")],
+                            [torch.tensor(target, device="cuda")], p).backward()
+        g_loss = grads(p)
+    listed = _listed_vector(w)
+    slots = [(which, t) for which in range(3) for t in range(2)]
+    e_fd = [_fd_param(energy64, listed, which, t) for which, t in slots]
+    l_fd = [_fd_param(loss64, listed, which, t) for which, t in slots]
+    assert _rel_err(g_energy, e_fd) <= GRAD_RTOL, (g_energy, e_fd)
+    assert _rel_err(g_loss, l_fd) <= GRAD_RTOL, (g_loss, l_fd)
