@@ -470,7 +470,8 @@ def _state_buffers(n: int, precision: str):
 
 
 class Batch:
-    """Many pair-only systems advanced together, either by one persistent
+    """Many systems (LJ pairs, harmonic bonds, angles and dihedrals, bonded
+    exclusions) advanced together, either by one persistent
     launch (one workgroup per replica) or by per-step batched kernels over
     64-atom chunks of every replica (phased)."""
 

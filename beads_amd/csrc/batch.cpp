@@ -1,5 +1,6 @@
-// Host side of the batched replicas: validates and packs many LJ systems into
-// one set of device arrays, picks the execution configuration with a small
+// Host side of the batched replicas: validates and packs many systems (LJ
+// pairs, harmonic bonds, angles and dihedrals, bonded exclusions) into one
+// set of device arrays, picks the execution configuration with a small
 // autotuner, and advances every replica either with one persistent launch per
 // run or with one graph of batched kernels per step (phased mode), each
 // captured once into a hipGraph and replayed.
@@ -10,18 +11,19 @@
 #include <numeric>
 
 #include "engine.hpp"
+#include "models.hpp"
 
 namespace beads_amd {
 
 Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   require(specs != nullptr && count > 0, "batch needs at least one replica spec.");
+  register_builtin_models();
   std::vector<NeighborPlan> plans(count);
+  std::vector<ModelSet> models(count);
   for (uint32_t r = 0; r < count; ++r) {
     const beads_spec_t& s = specs[r];
     validate_spec(s);
-    require(s.n_bonds == 0 && s.n_angles == 0 && s.n_dihedrals == 0,
-            "batched replicas support pair-only (LJ) systems; replica " + std::to_string(r) +
-                " has listed terms.");
+    models[r] = create_models(s, ModelContext{unit_system_from(s.units).kb});
     require(s.n_types <= kMaxLdsTypes, "batched replicas support at most 16 particle types.");
     require(s.half_list == 0, "batched replicas use the full neighbour list.");
     plans[r] = make_neighbor_plan(make_box_view(s.box_bound), s.lj_cutoff, s.cutoff_buffer,
@@ -38,9 +40,26 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   position_of_.resize(count);
   for (uint32_t k = 0; k < count; ++k) position_of_[order_[k]] = k;
 
+  // Per-slot records of each replica (inline only) and the batch's
+  // capacities: the largest of any replica, so every replica's records keep
+  // their indices at cap * atom0 + k * n + i.
+  std::vector<HostTopology> topo(count);
+  HostTopology caps;
+  for (uint32_t k = 0; k < count; ++k) {
+    const beads_spec_t& s = specs[order_[k]];
+    if (s.n_bonds + s.n_angles + s.n_dihedrals == 0) continue;
+    topo[k] = build_host_topology(s, true);
+    caps.bcap = std::max(caps.bcap, topo[k].bcap);
+    caps.acap = std::max(caps.acap, topo[k].acap);
+    caps.dcap = std::max(caps.dcap, topo[k].dcap);
+    caps.ecap = std::max(caps.ecap, topo[k].ecap);
+  }
+
   desc_host_.resize(count);
   uint64_t atoms = 0, cells = 0, nbr = 0, lj = 0, chunks = 0;
   std::vector<uint32_t> chunk_replica;
+  std::vector<real2> bond_coeff, angle_coeff;
+  std::vector<real4> dih_coeff;
   for (uint32_t k = 0; k < count; ++k) {
     const uint32_t r = order_[k];
     const beads_spec_t& s = specs[r];
@@ -76,6 +95,21 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     d.sort_every_steps = s.sort_every_steps;
     d.chunk0 = static_cast<uint32_t>(chunks);
     d.n_chunks = (s.n_particles + kReplicaChunk - 1) / kReplicaChunk;
+    d.has_listed = s.n_bonds + s.n_angles + s.n_dihedrals > 0 ? 1 : 0;
+    d.has_excl = topo[k].excluded_pairs > 0 ? 1 : 0;
+    d.bond_t0 = static_cast<uint32_t>(bond_coeff.size());
+    d.angle_t0 = static_cast<uint32_t>(angle_coeff.size());
+    d.dih_t0 = static_cast<uint32_t>(dih_coeff.size());
+    const ModelSet& m = models[r];
+    if (m.bond) {
+      for (const real4& c : m.bond->table()) bond_coeff.push_back(make_real2(c.x, c.y));
+    }
+    if (m.angle) {
+      for (const real4& c : m.angle->table()) angle_coeff.push_back(make_real2(c.x, c.y));
+    }
+    if (m.dihedral) {
+      for (const real4& c : m.dihedral->table()) dih_coeff.push_back(c);
+    }
     chunks += d.n_chunks;
     chunk_replica.insert(chunk_replica.end(), d.n_chunks, k);
     desc_host_[k] = d;
@@ -115,10 +149,35 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
     const std::vector<LjCoeff> t = pack_lj_table(s);
     lj_all.insert(lj_all.end(), t.begin(), t.end());
   }
+  // Records in the batch layout (zeros past a replica's own capacities).
+  init_topo_count_.assign(atoms, 0u);
+  init_bond_.assign(caps.bcap * atoms, uint2{0, 0});
+  init_angle_.assign(caps.acap * atoms, uint4{0, 0, 0, 0});
+  init_dih_.assign(caps.dcap * atoms, uint4{0, 0, 0, 0});
+  init_excl_.assign(caps.ecap * atoms, 0u);
+  for (uint32_t k = 0; k < count; ++k) {
+    const HostTopology& h = topo[k];
+    if (h.count.empty()) continue;
+    const size_t a0 = desc_host_[k].atom0;
+    std::copy(h.count.begin(), h.count.end(), init_topo_count_.begin() + a0);
+    std::copy(h.bond.begin(), h.bond.end(), init_bond_.begin() + caps.bcap * a0);
+    std::copy(h.angle.begin(), h.angle.end(), init_angle_.begin() + caps.acap * a0);
+    std::copy(h.dih.begin(), h.dih.end(), init_dih_.begin() + caps.dcap * a0);
+    std::copy(h.excl.begin(), h.excl.end(), init_excl_.begin() + caps.ecap * a0);
+  }
 
   BEADS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   desc_.upload(desc_host_.data(), count, stream_);
   lj_.upload(lj_all.data(), lj_all.size(), stream_);
+  bond_coeff_.upload(bond_coeff.data(), bond_coeff.size(), stream_);
+  angle_coeff_.upload(angle_coeff.data(), angle_coeff.size(), stream_);
+  dih_coeff_.upload(dih_coeff.data(), dih_coeff.size(), stream_);
+  s_topo_count_.resize(atoms);
+  s_bond_.resize(init_bond_.size());
+  s_angle_.resize(init_angle_.size());
+  s_dih_.resize(init_dih_.size());
+  s_excl_.resize(init_excl_.size());
+  new_of_old_.resize(atoms);
   const size_t n = atoms;
   force_.resize(n);
   ref_pos_.resize(n);
@@ -140,7 +199,7 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   work_.resize(chunks);
   work_reorder_.resize(chunks);
   work_count_.resize(2);
-  partials_.resize(3 * chunks);
+  partials_.resize(kReplicaSums * chunks);
   state_.resize(count);
   reset_state();
 
@@ -166,6 +225,24 @@ Batch::Batch(const beads_spec_t* specs, uint32_t count) : count_(count) {
   A.s_cell_of = s_cell_of_.data();
   A.cell_tmp = cell_tmp_.data();
   A.cell_pos = cell_pos_.data();
+  A.topo_count = topo_count_.data();
+  A.bond = bond_.data();
+  A.angle = angle_.data();
+  A.dih = dih_.data();
+  A.excl = excl_.data();
+  A.s_topo_count = s_topo_count_.data();
+  A.s_bond = s_bond_.data();
+  A.s_angle = s_angle_.data();
+  A.s_dih = s_dih_.data();
+  A.s_excl = s_excl_.data();
+  A.new_of_old = new_of_old_.data();
+  params_.bcap = caps.bcap;
+  params_.acap = caps.acap;
+  params_.dcap = caps.dcap;
+  params_.ecap = caps.ecap;
+  params_.bond_coeff = bond_coeff_.data();
+  params_.angle_coeff = angle_coeff_.data();
+  params_.dih_coeff = dih_coeff_.data();
   params_.count = count;
   params_.total_chunks = total_chunks_;
   params_.chunk_replica = chunk_replica_.data();
@@ -196,6 +273,11 @@ void Batch::reset_state() {
   image_.upload(init_img_.data(), n, stream_);
   tag_.upload(init_tag_.data(), n, stream_);
   cell_atoms_.upload(init_cell_atoms_.data(), n, stream_);
+  topo_count_.upload(init_topo_count_.data(), init_topo_count_.size(), stream_);
+  bond_.upload(init_bond_.data(), init_bond_.size(), stream_);
+  angle_.upload(init_angle_.data(), init_angle_.size(), stream_);
+  dih_.upload(init_dih_.data(), init_dih_.size(), stream_);
+  excl_.upload(init_excl_.data(), init_excl_.size(), stream_);
   force_.zero(stream_);
   nbr_count_.zero(stream_);
   cell_count_.zero(stream_);
@@ -326,8 +408,10 @@ void Batch::replica_stats(uint32_t r, beads_engine_stats_t* out) {
   out->max_neighbors = d.max_neighbors;
   out->ke = st.ke;
   out->lambda = st.lambda;
-  out->pe[0] = st.pe;
-  out->virial[0] = st.virial;
+  for (int q = 0; q < 4; ++q) {
+    out->pe[q] = st.pe[q];
+    out->virial[q] = st.virial[q];
+  }
   out->temperature = d.dof > 0 ? 2.0 * st.ke / d.dof : 0.0;  // in kB units
 }
 

@@ -51,6 +51,9 @@ class Batch {
   std::vector<real4> init_pos_, init_vel_;
   std::vector<int4> init_img_;
   std::vector<uint32_t> init_tag_, init_cell_atoms_;
+  std::vector<uint32_t> init_topo_count_, init_excl_;
+  std::vector<uint2> init_bond_;
+  std::vector<uint4> init_angle_, init_dih_;
   std::vector<CachedGraph> graphs_;
   hipStream_t stream_ = nullptr;
 
@@ -63,6 +66,11 @@ class Batch {
   DeviceBuffer<uint32_t> cell_count_, cell_start_, cell_cursor_, cell_tmp_;
   DeviceBuffer<uint32_t> chunk_replica_, work_, work_reorder_, work_count_;
   DeviceBuffer<double> partials_;
+  DeviceBuffer<uint32_t> topo_count_, s_topo_count_, excl_, s_excl_, new_of_old_;
+  DeviceBuffer<uint2> bond_, s_bond_;
+  DeviceBuffer<uint4> angle_, s_angle_, dih_, s_dih_;
+  DeviceBuffer<real2> bond_coeff_, angle_coeff_;
+  DeviceBuffer<real4> dih_coeff_;
   ReplicaParams params_{};
 };
 

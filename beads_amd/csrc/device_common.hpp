@@ -361,6 +361,117 @@ __device__ __forceinline__ real3 dihedral_role_force(real3 xi, real3 xj, real3 x
   return flt(fl);
 }
 
+// Per-thread sums of one force pass.
+struct StepSums {
+  double ke = 0.0;
+  double e_pair = 0.0, w_pair = 0.0;
+  double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
+};
+
+// Listed terms, atom-centric: atom i evaluates every term it belongs to and
+// keeps its own role's force (no atomics, deterministic; the reference adds
+// each term's forces with float atomics, bond_harmonic.cu, angle_harmonic.cu,
+// dihedral_harmonic.cu). A term's energy and virial are counted once, by role
+// 0 (bonds: the flag bit of the coefficient word). Records hold partner slots
+// of the same position array; topo's per-slot arrays have stride n.
+template <bool OBS>
+__device__ __forceinline__ real3 bond_term(const real4* pos, const TopologyView& topo,
+                                            const BoxView& box, real3 xi, uint2 r,
+                                            StepSums& sums) {
+  double e, w;
+  const real3 f =
+      bond_force(mic3(xi, f3(pos[r.x]), box), topo.bond_coeff[r.y & 0x7fffffffu], e, w);
+  if (OBS && (r.y >> 31) != 0) {
+    sums.e_bond += e;
+    sums.w_bond += w;
+  }
+  return f;
+}
+
+template <bool OBS>
+__device__ __forceinline__ real3 angle_term(const real4* pos, const TopologyView& topo,
+                                             const BoxView& box, real3 xi, uint4 r,
+                                             StepSums& sums) {
+  const int role = static_cast<int>(r.w);
+  const real3 a = f3(pos[r.x]);
+  const real3 b = f3(pos[r.y]);
+  const real3 pi = role == 0 ? xi : a;
+  const real3 pj = role == 0 ? a : (role == 1 ? xi : b);
+  const real3 pk = role == 2 ? xi : b;
+  double e, w;
+  const real3 f = angle_role_force(pi, pj, pk, topo.angle_coeff[r.z], role, box, e, w);
+  if (OBS && role == 0) {
+    sums.e_angle += e;
+    sums.w_angle += w;
+  }
+  return f;
+}
+
+template <bool OBS>
+__device__ __forceinline__ real3 dihedral_term(const real4* pos, const TopologyView& topo,
+                                                const BoxView& box, real3 xi, uint4 r,
+                                                StepSums& sums) {
+  const int role = static_cast<int>(r.w >> 30);
+  const real3 a = f3(pos[r.x]);
+  const real3 b = f3(pos[r.y]);
+  const real3 c = f3(pos[r.z]);
+  real3 q0, q1, q2, q3;
+  if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
+  else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
+  else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
+  else { q0 = a; q1 = b; q2 = c; q3 = xi; }
+  double e, w;
+  const real3 f =
+      dihedral_role_force(q0, q1, q2, q3, topo.dih_coeff[r.w & 0x3fffffffu], role, box, e, w);
+  if (OBS && role == 0) {
+    sums.e_dih += e;
+    sums.w_dih += w;
+  }
+  return f;
+}
+
+// The listed terms of atom i shared over its team of TEAM lanes: lane t takes
+// terms t, t + TEAM, ... of each kind and adds their forces to its FP64
+// partial sums.
+template <bool OBS, bool SPILL, int TEAM>
+__device__ __forceinline__ void listed_team(const real4* pos, const TopologyView& topo,
+                                            const BoxView& box, size_t n, uint32_t i, uint32_t t,
+                                            real4 xi4, double& fxd, double& fyd, double& fzd,
+                                            StepSums& sums) {
+  const uint32_t cw = topo.count[i];
+  const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
+  const real3 xi = f3(xi4);
+  auto add = [&](real3 f) {
+    fxd += f.x;
+    fyd += f.y;
+    fzd += f.z;
+  };
+  for (uint32_t k = t; k < nbond; k += TEAM) {
+    add(bond_term<OBS>(pos, topo, box, xi, topo.bond[k * n + i], sums));
+  }
+  for (uint32_t k = t; k < nang; k += TEAM) {
+    add(angle_term<OBS>(pos, topo, box, xi, topo.angle[k * n + i], sums));
+  }
+  for (uint32_t k = t; k < ndih; k += TEAM) {
+    add(dihedral_term<OBS>(pos, topo, box, xi, topo.dih[k * n + i], sums));
+  }
+  if (SPILL) {  // some particle spills: its records past the inline capacities
+    const uint32_t si = topo.spill_index[i];
+    if (si != kNoSpill) {  // uniform over the team
+      const SpillEntry e = topo.spill[si];
+      for (uint32_t k = t; k < e.cnt[0]; k += TEAM) {
+        add(bond_term<OBS>(pos, topo, box, xi, topo.spill_bond[e.off[0] + k], sums));
+      }
+      for (uint32_t k = t; k < e.cnt[1]; k += TEAM) {
+        add(angle_term<OBS>(pos, topo, box, xi, topo.spill_angle[e.off[1] + k], sums));
+      }
+      for (uint32_t k = t; k < e.cnt[2]; k += TEAM) {
+        add(dihedral_term<OBS>(pos, topo, box, xi, topo.spill_dih[e.off[2] + k], sums));
+      }
+    }
+  }
+}
+
 // Neighbour storage: each atom's list is cut into chunks of kNbrChunk entries;
 // chunk c of atom i is kNbrChunk consecutive words at ((c * N) + i) * kNbrChunk.
 // A build appends into an LDS staging row and writes whole 64-byte chunks; the

@@ -232,7 +232,7 @@ uint32_t inline_capacity(const std::vector<uint32_t>& counts) {
 // (exclusions.cu:131-210, 337-445); here every kind has an inline capacity
 // (inline_capacity) and records past it go to the particle's spill segment,
 // so a particle may take part in any number of terms and exclusions.
-HostTopology build_host_topology(const beads_spec_t& s) {
+HostTopology build_host_topology(const beads_spec_t& s, bool inline_only) {
   HostTopology h;
   const uint32_t n = s.n_particles;
   std::vector<uint32_t> slot_of_tag(static_cast<size_t>(n) + 1, 0);
@@ -256,10 +256,17 @@ HostTopology build_host_topology(const beads_spec_t& s) {
     ++ne[slot_of_tag[p.first]];
     ++ne[slot_of_tag[p.second]];
   }
-  h.bcap = inline_capacity(nb);
-  h.acap = inline_capacity(na);
-  h.dcap = inline_capacity(nd);
-  h.ecap = inline_capacity(ne);
+  auto capacity = [&](const std::vector<uint32_t>& c, const char* kind) {
+    if (!inline_only) return inline_capacity(c);
+    const uint32_t mx = c.empty() ? 0u : *std::max_element(c.begin(), c.end());
+    require(mx <= 255u, std::string("batched replicas keep at most 255 ") + kind +
+                            " per particle; a particle has " + std::to_string(mx) + ".");
+    return mx;
+  };
+  h.bcap = capacity(nb, "bonds");
+  h.acap = capacity(na, "angles");
+  h.dcap = capacity(nd, "dihedrals");
+  h.ecap = capacity(ne, "exclusions");
   const uint32_t caps[4] = {h.bcap, h.acap, h.dcap, h.ecap};
   const std::vector<uint32_t>* all[4] = {&nb, &na, &nd, &ne};
   // Spill entries and segment offsets.

@@ -72,7 +72,9 @@ struct HostTopology {
 // particles exceed (those spill the rest), never above 255.
 uint32_t inline_capacity(const std::vector<uint32_t>& counts);
 
-HostTopology build_host_topology(const beads_spec_t& spec);
+// inline_only: every record inline (capacities = the largest counts, at most
+// 255 per kind and particle), for the batched replicas, which have no spill.
+HostTopology build_host_topology(const beads_spec_t& spec, bool inline_only = false);
 
 // Shell validation of one spec (throws Error with the matching status).
 void validate_spec(const beads_spec_t& spec);

@@ -1048,115 +1048,6 @@ __global__ __launch_bounds__(kBlock) void half_pair_kernel(ForceParams p) {
   }
 }
 
-// Per-thread sums of one force pass.
-struct StepSums {
-  double ke = 0.0;
-  double e_pair = 0.0, w_pair = 0.0;
-  double e_bond = 0.0, w_bond = 0.0, e_angle = 0.0, w_angle = 0.0, e_dih = 0.0, w_dih = 0.0;
-};
-
-// Listed terms, atom-centric: atom i evaluates every term it belongs to and
-// keeps its own role's force (no atomics, deterministic; the reference adds
-// each term's forces with float atomics, bond_harmonic.cu, angle_harmonic.cu,
-// dihedral_harmonic.cu). A term's energy and virial are counted once, by role
-// 0 (bonds: the flag bit of the coefficient word).
-template <bool OBS>
-__device__ __forceinline__ real3 bond_term(const ForceParams& p, real3 xi, uint2 r,
-                                            StepSums& sums) {
-  double e, w;
-  const real3 f =
-      bond_force(mic3(xi, f3(p.pos[r.x]), p.box), p.topo.bond_coeff[r.y & 0x7fffffffu], e, w);
-  if (OBS && (r.y >> 31) != 0) {
-    sums.e_bond += e;
-    sums.w_bond += w;
-  }
-  return f;
-}
-
-template <bool OBS>
-__device__ __forceinline__ real3 angle_term(const ForceParams& p, real3 xi, uint4 r,
-                                             StepSums& sums) {
-  const int role = static_cast<int>(r.w);
-  const real3 a = f3(p.pos[r.x]);
-  const real3 b = f3(p.pos[r.y]);
-  const real3 pi = role == 0 ? xi : a;
-  const real3 pj = role == 0 ? a : (role == 1 ? xi : b);
-  const real3 pk = role == 2 ? xi : b;
-  double e, w;
-  const real3 f = angle_role_force(pi, pj, pk, p.topo.angle_coeff[r.z], role, p.box, e, w);
-  if (OBS && role == 0) {
-    sums.e_angle += e;
-    sums.w_angle += w;
-  }
-  return f;
-}
-
-template <bool OBS>
-__device__ __forceinline__ real3 dihedral_term(const ForceParams& p, real3 xi, uint4 r,
-                                                StepSums& sums) {
-  const int role = static_cast<int>(r.w >> 30);
-  const real3 a = f3(p.pos[r.x]);
-  const real3 b = f3(p.pos[r.y]);
-  const real3 c = f3(p.pos[r.z]);
-  real3 q0, q1, q2, q3;
-  if (role == 0) { q0 = xi; q1 = a; q2 = b; q3 = c; }
-  else if (role == 1) { q0 = a; q1 = xi; q2 = b; q3 = c; }
-  else if (role == 2) { q0 = a; q1 = b; q2 = xi; q3 = c; }
-  else { q0 = a; q1 = b; q2 = c; q3 = xi; }
-  double e, w;
-  const real3 f = dihedral_role_force(q0, q1, q2, q3, p.topo.dih_coeff[r.w & 0x3fffffffu], role,
-                                       p.box, e, w);
-  if (OBS && role == 0) {
-    sums.e_dih += e;
-    sums.w_dih += w;
-  }
-  return f;
-}
-
-// The listed terms of atom i shared over its team: lane t takes terms t,
-// t + kTeam, ... of each kind and adds their forces to its FP64 partial sums.
-template <bool OBS, bool SPILL>
-__device__ __forceinline__ void listed_team(const ForceParams& p, uint32_t i, uint32_t t,
-                                            real4 xi4, double& fxd, double& fyd, double& fzd,
-                                            StepSums& sums) {
-  if (!p.topo.has_listed) {
-    return;
-  }
-  const uint32_t cw = p.topo.count[i];
-  const uint32_t nbond = cw & 255u, nang = (cw >> 8) & 255u, ndih = (cw >> 16) & 255u;
-  const real3 xi = f3(xi4);
-  const size_t n = p.n;
-  auto add = [&](real3 f) {
-    fxd += f.x;
-    fyd += f.y;
-    fzd += f.z;
-  };
-  for (uint32_t k = t; k < nbond; k += kTeam) {
-    add(bond_term<OBS>(p, xi, p.topo.bond[k * n + i], sums));
-  }
-  for (uint32_t k = t; k < nang; k += kTeam) {
-    add(angle_term<OBS>(p, xi, p.topo.angle[k * n + i], sums));
-  }
-  for (uint32_t k = t; k < ndih; k += kTeam) {
-    add(dihedral_term<OBS>(p, xi, p.topo.dih[k * n + i], sums));
-  }
-  if (SPILL) {  // some particle spills: its records past the inline capacities
-    const uint32_t si = p.topo.spill_index[i];
-    if (si != kNoSpill) {  // uniform over the team
-      const SpillEntry e = p.topo.spill[si];
-      for (uint32_t k = t; k < e.cnt[0]; k += kTeam) {
-        add(bond_term<OBS>(p, xi, p.topo.spill_bond[e.off[0] + k], sums));
-      }
-      for (uint32_t k = t; k < e.cnt[1]; k += kTeam) {
-        add(angle_term<OBS>(p, xi, p.topo.spill_angle[e.off[1] + k], sums));
-      }
-      for (uint32_t k = t; k < e.cnt[2]; k += kTeam) {
-        add(dihedral_term<OBS>(p, xi, p.topo.spill_dih[e.off[2] + k], sums));
-      }
-    }
-  }
-}
-
 // Atom i's total force (rounded once from FP64), the force store, optionally
 // the post-force half-kick (velocity_verlet.cu:97-111), and the atom's
 // kinetic energy. Run by one lane of the team.
@@ -1297,7 +1188,9 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceParams p) {
     } else {
       lj_team<OBS, TYPES>(p, i, t, xi4, lj_lds, c0, fxd, fyd, fzd, sums.e_pair, sums.w_pair);
     }
-    listed_team<OBS, SPILL>(p, i, t, xi4, fxd, fyd, fzd, sums);
+    if (p.topo.has_listed) {
+      listed_team<OBS, SPILL, kTeam>(p.pos, p.topo, p.box, p.n, i, t, xi4, fxd, fyd, fzd, sums);
+    }
     fxd = team_sum<kTeam>(fxd);
     fyd = team_sum<kTeam>(fyd);
     fzd = team_sum<kTeam>(fzd);

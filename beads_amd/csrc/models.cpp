@@ -121,6 +121,11 @@ class HarmonicBondModel final : public ListedModel {
     r0_d_.upload(r0.data(), r0.size(), stream);
   }
   void bind(TopologyView& topo) const override { topo.bond_coeff = coeff_.data(); }
+  std::vector<real4> table() const override {
+    std::vector<real4> t(n_);
+    for (int q = 0; q < n_; ++q) t[q] = make_real4(k_[q], r0_[q], 0, 0);
+    return t;
+  }
   void set_parameters(const real_t* k, const real_t* r0) override {
     for (int t = 0; t < n_; ++t) {
       require(std::isfinite(k[t]) && k[t] >= 0 && std::isfinite(r0[t]) && r0[t] >= 0,
@@ -170,6 +175,11 @@ class HarmonicAngleModel final : public ListedModel {
     theta0_d_.upload(th.data(), th.size(), stream);
   }
   void bind(TopologyView& topo) const override { topo.angle_coeff = coeff_.data(); }
+  std::vector<real4> table() const override {
+    std::vector<real4> t(n_);
+    for (int q = 0; q < n_; ++q) t[q] = make_real4(c_[q].x, c_[q].y, 0, 0);
+    return t;
+  }
   // Frame mode: k and theta0 in degrees, as in the spec.
   void set_parameters(const real_t* k, const real_t* theta0_deg) override {
     for (int t = 0; t < n_; ++t) {
@@ -234,6 +244,7 @@ class HarmonicDihedralModel final : public ListedModel {
     n_d_.upload(m.data(), m.size(), stream);
   }
   void bind(TopologyView& topo) const override { topo.dih_coeff = coeff_.data(); }
+  std::vector<real4> table() const override { return {c_.begin(), c_.begin() + n_}; }
   // Frame mode: k only (d and n are integers of the topology's style).
   void set_parameters(const real_t* k, const real_t*) override {
     for (int t = 0; t < n_; ++t) {

@@ -60,8 +60,12 @@ class ListedModel {
   virtual int32_t type_count() const = 0;
   virtual void upload(hipStream_t stream) = 0;
   virtual void bind(TopologyView& topo) const = 0;
-  // Frame mode / adjoint (bonds only): new real_t parameters and the
-  // adjoint's FP64 view of them.
+  // The packed coefficients of each type as the force pass reads them
+  // (bond {k, r0}, angle {k, theta0 rad}, dihedral {k, d, n}), zero-padded to
+  // four components; the batched replicas concatenate these tables.
+  virtual std::vector<real4> table() const = 0;
+  // Frame mode / adjoint: new real_t parameters and the adjoint's FP64 view
+  // of them.
   virtual void set_parameters(const real_t* a, const real_t* b);
   virtual void bind_adjoint(AdjointParams& p) const;
 };

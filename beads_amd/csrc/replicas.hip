@@ -86,6 +86,102 @@ __device__ __forceinline__ size_t replica_nbr_index(uint32_t s, uint32_t i, uint
   return nbr_chunk_base(s / kNbrChunk, i, n) + (s & (kNbrChunk - 1));
 }
 
+// A replica's listed-term records and exclusions as the single-system view
+// (per-slot arrays of stride n, partner slots local to the replica, no
+// spill).
+__device__ __forceinline__ TopologyView replica_topology(const ReplicaParams& P,
+                                                         const ReplicaDesc& d) {
+  const ReplicaArrays& A = P.arrays;
+  TopologyView v{};
+  v.count = A.topo_count + d.atom0;
+  v.bond = A.bond + static_cast<size_t>(P.bcap) * d.atom0;
+  v.angle = A.angle + static_cast<size_t>(P.acap) * d.atom0;
+  v.dih = A.dih + static_cast<size_t>(P.dcap) * d.atom0;
+  v.excl = A.excl + static_cast<size_t>(P.ecap) * d.atom0;
+  v.bcap = P.bcap;
+  v.acap = P.acap;
+  v.dcap = P.dcap;
+  v.ecap = P.ecap;
+  v.bond_coeff = P.bond_coeff + d.bond_t0;
+  v.angle_coeff = P.angle_coeff + d.angle_t0;
+  v.dih_coeff = P.dih_coeff + d.dih_t0;
+  v.has_listed = d.has_listed;
+  return v;
+}
+
+// Whether local slot j is one of the ne excluded partners of local slot i.
+__device__ __forceinline__ bool excluded_partner(const uint32_t* excl, uint32_t n, uint32_t i,
+                                                 uint32_t ne, uint32_t j) {
+  bool hit = false;
+  for (uint32_t k = 0; k < ne; ++k) hit |= excl[static_cast<size_t>(k) * n + i] == j;
+  return hit;
+}
+
+// Reorder of a replica's records: new slot s takes old slot o's records
+// (gather) and, once every gather of the replica is done, its partner slots
+// are translated through new_of_old (commit); cell_list_build.cu:183-219 for
+// the particle arrays, the single-system reorder kernels for the records.
+__device__ __forceinline__ void topology_gather(const ReplicaParams& P, const ReplicaDesc& d,
+                                                uint32_t s, uint32_t o) {
+  const ReplicaArrays& A = P.arrays;
+  const size_t n = d.n, a0 = d.atom0;
+  A.new_of_old[a0 + o] = s;
+  A.s_topo_count[a0 + s] = A.topo_count[a0 + o];
+  const uint2* bond = A.bond + P.bcap * a0;
+  const uint4* angle = A.angle + P.acap * a0;
+  const uint4* dih = A.dih + P.dcap * a0;
+  const uint32_t* excl = A.excl + P.ecap * a0;
+  for (uint32_t k = 0; k < P.bcap; ++k) A.s_bond[P.bcap * a0 + k * n + s] = bond[k * n + o];
+  for (uint32_t k = 0; k < P.acap; ++k) A.s_angle[P.acap * a0 + k * n + s] = angle[k * n + o];
+  for (uint32_t k = 0; k < P.dcap; ++k) A.s_dih[P.dcap * a0 + k * n + s] = dih[k * n + o];
+  for (uint32_t k = 0; k < P.ecap; ++k) A.s_excl[P.ecap * a0 + k * n + s] = excl[k * n + o];
+}
+
+__device__ __forceinline__ void topology_commit(const ReplicaParams& P, const ReplicaDesc& d,
+                                                uint32_t s) {
+  const ReplicaArrays& A = P.arrays;
+  const size_t n = d.n, a0 = d.atom0;
+  const uint32_t* m = A.new_of_old + a0;
+  const uint32_t cw = A.s_topo_count[a0 + s];
+  A.topo_count[a0 + s] = cw;
+  const uint32_t nb = cw & 255u, na = (cw >> 8) & 255u, nd = (cw >> 16) & 255u, ne = cw >> 24;
+  for (uint32_t k = 0; k < nb; ++k) {
+    uint2 r = A.s_bond[P.bcap * a0 + k * n + s];
+    r.x = m[r.x];
+    A.bond[P.bcap * a0 + k * n + s] = r;
+  }
+  for (uint32_t k = 0; k < na; ++k) {
+    uint4 r = A.s_angle[P.acap * a0 + k * n + s];
+    r.x = m[r.x];
+    r.y = m[r.y];
+    A.angle[P.acap * a0 + k * n + s] = r;
+  }
+  for (uint32_t k = 0; k < nd; ++k) {
+    uint4 r = A.s_dih[P.dcap * a0 + k * n + s];
+    r.x = m[r.x];
+    r.y = m[r.y];
+    r.z = m[r.z];
+    A.dih[P.dcap * a0 + k * n + s] = r;
+  }
+  for (uint32_t k = 0; k < ne; ++k) {
+    A.excl[P.ecap * a0 + k * n + s] = m[A.s_excl[P.ecap * a0 + k * n + s]];
+  }
+}
+
+// Block totals of a force pass's sums (kinetic energy; with obs also the
+// four energies and virials) into out[kReplicaSums].
+template <int BLOCK>
+__device__ __forceinline__ void step_totals(const StepSums& sums, bool obs, double* out,
+                                            double* red) {
+  out[0] = block_sum<BLOCK>(sums.ke, red);
+  if (obs) {
+    const double v[8] = {sums.e_pair, sums.e_bond, sums.e_angle, sums.e_dih,
+                         sums.w_pair, sums.w_bond, sums.w_angle, sums.w_dih};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) out[q + 1] = block_sum<BLOCK>(v[q], red);
+  }
+}
+
 // Velocity-Verlet first half-kick (with the pending Berendsen factor), drift,
 // wrap and image update of atom a of a replica (arrays offset to the replica),
 // and, when check, the displacement test against the reference positions.
@@ -143,13 +239,15 @@ __device__ __forceinline__ int integrate_atom(const ReplicaDesc& d, real4* pos, 
 
 // Full neighbour list of atom i of a replica (arrays offset to the replica):
 // candidates in the 27 (or per-axis unique) neighbour cells in neighbourhood
-// order, members of a cell in slot order, or every atom on the direct path.
+// order, members of a cell in slot order, or every atom on the direct path,
+// less i's ne excluded partners (excl: the replica's exclusion records).
 // Captures the reference position. Returns 1 on overflow.
 __device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4* pos,
                                                const int4* image, const uint32_t* cell_of,
                                                const uint32_t* cell_atoms, const uint32_t* start,
                                                uint32_t* nbr, uint32_t* nbr_count,
-                                               real4* ref_pos, int4* ref_image, uint32_t i) {
+                                               real4* ref_pos, int4* ref_image,
+                                               const uint32_t* excl, uint32_t ne, uint32_t i) {
   const uint32_t n = d.n;
   const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
   const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
@@ -162,6 +260,7 @@ __device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4
     const real_t dy = mic(xi.y - xj.y, L1, h1);
     const real_t dz = mic(xi.z - xj.z, L2, h2);
     if (r2_strict(dx, dy, dz) > d.rs2 || j == i) return;
+    if (ne != 0 && excluded_partner(excl, n, i, ne, j)) return;
     if (cnt < d.max_neighbors) {
       nbr[replica_nbr_index(cnt, i, n)] = j;
       ++cnt;
@@ -195,18 +294,19 @@ __device__ __forceinline__ uint32_t build_atom(const ReplicaDesc& d, const real4
   return overflow;
 }
 
-// Full-list LJ force of atom i of a replica (arrays offset to the replica) by
-// a team of TEAM adjacent lanes (lane t of the team): lane t takes list slots
-// base + u*TEAM + t, SLOTS of them in flight per round; the team's FP64 partial
+// Full-list LJ force plus the listed terms of atom i of a replica (arrays
+// offset to the replica) by a team of TEAM adjacent lanes (lane t of the
+// team): lane t takes list slots base + u*TEAM + t, SLOTS of them in flight
+// per round, then listed terms t, t + TEAM, ...; the team's FP64 partial
 // forces meet in a butterfly and lane 0 stores the force and, when kick, the
 // second half-kick. Adds this lane's share of the kinetic energy (lane 0)
-// and, when OBS, of the pair energy and virial.
-template <int TEAM, int SLOTS, bool OBS = true>
+// and, when OBS, of the energies and virials.
+template <int TEAM, int SLOTS, bool OBS, bool LISTED>
 __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* pos, real4* vel,
                                            real4* force, const uint32_t* nbr_count,
-                                           const uint32_t* nbr, const LjCoeff* lj, bool kick,
-                                           bool single, uint32_t i, uint32_t t, double& ke,
-                                           double& pe, double& vir) {
+                                           const uint32_t* nbr, const LjCoeff* lj,
+                                           const TopologyView& topo, bool kick, bool single,
+                                           uint32_t i, uint32_t t, StepSums& sums) {
   const uint32_t n = d.n;
   const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
   const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
@@ -255,12 +355,15 @@ __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* po
       }
     }
   }
+  if (LISTED && d.has_listed) {
+    listed_team<OBS, false, TEAM>(pos, topo, d.box, n, i, t, xi, fx, fy, fz, sums);
+  }
   fx = team_sum<TEAM>(fx);
   fy = team_sum<TEAM>(fy);
   fz = team_sum<TEAM>(fz);
   if (OBS) {
-    pe += 0.5 * e;
-    vir += 0.5 * w;
+    sums.e_pair += 0.5 * e;
+    sums.w_pair += 0.5 * w;
   }
   if (t != 0) {
     return;
@@ -277,7 +380,7 @@ __device__ __forceinline__ void force_atom(const ReplicaDesc& d, const real4* po
     vel[i] = v;
   }
   const double vx = v.x, vy = v.y, vz = v.z;
-  ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
+  sums.ke += 0.5 * static_cast<double>(v.w) * (vx * vx + vy * vy + vz * vz);
 }
 
 // Rebuild decisions of one replica at step now (NeighborSystem::rebuild_now's
@@ -299,9 +402,11 @@ __device__ __forceinline__ bool begin_rebuild(const ReplicaDesc& d, ReplicaState
 
 // Neighbour rebuild of one replica (NeighborSystem::rebuild_now restated for a
 // workgroup): bin, counting sort, per-cell slot order, optional reorder, build.
-template <int BLOCK>
-__device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, ReplicaState* st,
+template <int BLOCK, bool TOPO>
+__device__ void rebuild_replica(const ReplicaParams& P, const ReplicaDesc& d, ReplicaState* st,
                                 ReplicaShared<BLOCK>& sh, uint64_t now) {
+  const ReplicaArrays& A = P.arrays;
+  const bool topo = TOPO && (d.has_listed || d.has_excl);
   const uint32_t n = d.n, tid = threadIdx.x;
   real4* pos = A.pos + d.atom0;
   real4* vel = A.vel + d.atom0;
@@ -368,6 +473,7 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
         s_img[s] = image[o];
         s_tag[s] = tag[o];
         s_cell[s] = cell_of[o];
+        if (topo) topology_gather(P, d, s, o);
       }
       __syncthreads();
       for (uint32_t s = tid; s < n; s += BLOCK) {
@@ -377,33 +483,48 @@ __device__ void rebuild_replica(const ReplicaDesc& d, const ReplicaArrays& A, Re
         tag[s] = s_tag[s];
         cell_of[s] = s_cell[s];
         cell_atoms[s] = s;
+        if (topo) topology_commit(P, d, s);
       }
       __syncthreads();
     }
   }
+  const uint32_t* topo_count = A.topo_count + d.atom0;
+  const uint32_t* excl = A.excl + static_cast<size_t>(P.ecap) * d.atom0;
   uint32_t overflow = 0;
   for (uint32_t i = tid; i < n; i += BLOCK) {
+    const uint32_t ne = TOPO && d.has_excl ? topo_count[i] >> 24 : 0u;
     overflow |= build_atom(d, pos, image, cell_of, cell_atoms, start, nbr, nbr_count, ref_pos,
-                           ref_image, i);
+                           ref_image, excl, ne, i);
   }
   if (overflow) atomicOr(&st->overflow, 1u);
   __syncthreads();
 }
 
-// Full-list LJ + optional second half-kick; returns this thread's share of the
-// kinetic energy, pair energy and virial.
-template <int BLOCK>
-__device__ void force_replica(const ReplicaDesc& d, const ReplicaArrays& A, bool kick, bool single,
-                              const LjCoeff* lj, double& ke, double& pe, double& vir) {
+// Full-list LJ + listed terms + optional second half-kick; adds this thread's
+// share of the kinetic energy, energies and virials.
+template <int BLOCK, bool TOPO>
+__device__ void force_replica(const ReplicaParams& P, const ReplicaDesc& d, bool kick,
+                              bool single, const LjCoeff* lj, StepSums& sums) {
+  const ReplicaArrays& A = P.arrays;
+  const TopologyView topo = replica_topology(P, d);
   const uint32_t t = threadIdx.x & (kReplicaTeam - 1);
   for (uint32_t i = threadIdx.x / kReplicaTeam; i < d.n; i += BLOCK / kReplicaTeam) {
-    force_atom<kReplicaTeam, kReplicaSlots>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
-                                            A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick,
-                                            single, i, t, ke, pe, vir);
+    force_atom<kReplicaTeam, kReplicaSlots, true, TOPO>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                                            A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, topo, kick,
+                                            single, i, t, sums);
   }
 }
 
-template <int BLOCK>
+__device__ __forceinline__ void store_observables(ReplicaState* st, const double* tot) {
+  for (int q = 0; q < 4; ++q) {
+    st->pe[q] = tot[1 + q];
+    st->virial[q] = tot[5 + q];
+  }
+}
+
+// TOPO: the batch has listed terms or exclusions (a pair-only batch runs the
+// kernel without their code).
+template <int BLOCK, bool TOPO>
 __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_t n_steps,
                                                         int observables_on_last) {
   __shared__ ReplicaShared<BLOCK> sh;
@@ -418,19 +539,17 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
   }
   __syncthreads();
 
-  double ke = 0.0, pe = 0.0, vir = 0.0;
+  double tot[kReplicaSums];
   if (st->initialized == 0) {
     // Initial build and step-0 forces (simulation_run.cpp:113-119).
-    rebuild_replica<BLOCK>(d, A, st, sh, 0);
-    force_replica<BLOCK>(d, A, false, single, sh.lj, ke, pe, vir);
-    ke = block_sum<BLOCK>(ke, sh.red);
-    pe = block_sum<BLOCK>(pe, sh.red);
-    vir = block_sum<BLOCK>(vir, sh.red);
+    rebuild_replica<BLOCK, TOPO>(P, d, st, sh, 0);
+    StepSums sums;
+    force_replica<BLOCK, TOPO>(P, d, false, single, sh.lj, sums);
+    step_totals<BLOCK>(sums, true, tot, sh.red);
     if (tid == 0) {
       st->initialized = 1;
-      st->ke = ke;
-      st->pe = pe;
-      st->virial = vir;
+      st->ke = tot[0];
+      store_observables(st, tot);
       sh.lambda = 1.0;
     }
     __syncthreads();
@@ -464,16 +583,13 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
       }
     }
     if (rebuild) {
-      rebuild_replica<BLOCK>(d, A, st, sh, physical);
+      rebuild_replica<BLOCK, TOPO>(P, d, st, sh, physical);
     }
     const bool obs = observables_on_last && s + 1 == n_steps;
-    ke = pe = vir = 0.0;
-    force_replica<BLOCK>(d, A, true, single, sh.lj, ke, pe, vir);
-    ke = block_sum<BLOCK>(ke, sh.red);
-    if (obs) {
-      pe = block_sum<BLOCK>(pe, sh.red);
-      vir = block_sum<BLOCK>(vir, sh.red);
-    }
+    StepSums sums;
+    force_replica<BLOCK, TOPO>(P, d, true, single, sh.lj, sums);
+    step_totals<BLOCK>(sums, obs, tot, sh.red);
+    const double ke = tot[0];
     if (tid == 0) {
       double lambda = 1.0;
       if (d.berendsen && ke > 0.0 && d.dof > 0.0) {
@@ -483,10 +599,7 @@ __global__ __launch_bounds__(BLOCK) void replica_kernel(ReplicaParams P, uint64_
       }
       sh.lambda = lambda;
       st->ke = ke;
-      if (obs) {
-        st->pe = pe;
-        st->virial = vir;
-      }
+      if (obs) store_observables(st, tot);
     }
     __syncthreads();
     step = physical;
@@ -661,6 +774,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_reorder_gather_kernel(Rep
     A.s_image[g] = A.image[o];
     A.s_tag[g] = A.tag[o];
     A.s_cell_of[g] = A.cell_of[o];
+    if (d.has_listed || d.has_excl) topology_gather(P, d, s, o - d.atom0);
   });
 }
 
@@ -678,6 +792,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_reorder_commit_kernel(Rep
     A.cell_atoms[g] = s;
     const real4 x = A.s_pos[g];
     A.cell_pos[g] = make_real4(x.x, x.y, x.z, real_of_id(s));
+    if (d.has_listed || d.has_excl) topology_commit(P, d, s);
   });
 }
 
@@ -692,6 +807,7 @@ constexpr int kPhasedBuildTeam = 4;
 constexpr int kPhasedBuildSlots = 4;
 static_assert(kPhasedBlock == kReplicaChunk * kPhasedBuildTeam, "one chunk per workgroup");
 
+template <bool EXCL>
 __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParams P) {
   const ReplicaArrays& A = P.arrays;
   const uint32_t t = threadIdx.x & (kPhasedBuildTeam - 1);
@@ -712,6 +828,8 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
       const real_t L0 = d.box.L[0], L1 = d.box.L[1], L2 = d.box.L[2];
       const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
       const real4 xi = pos[i];
+      const uint32_t* excl = A.excl + static_cast<size_t>(P.ecap) * d.atom0;
+      const uint32_t ne = EXCL && d.has_excl ? A.topo_count[d.atom0 + i] >> 24 : 0u;
       // Candidates [b, e) of a run (b < e); entry(k) = {x, y, z, slot bits}.
       auto scan = [&](uint32_t b, uint32_t e, auto entry) {
         for (uint32_t k = b; k < e; k += kPhasedBuildTeam * kPhasedBuildSlots) {
@@ -730,7 +848,8 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
             const uint64_t m =
                 __builtin_amdgcn_ballot_w64(k + u * kPhasedBuildTeam + t < e) &
                 __builtin_amdgcn_ballot_w64(!(r2_strict(dx, dy, dz) > d.rs2)) &
-                __builtin_amdgcn_ballot_w64(j != i);
+                __builtin_amdgcn_ballot_w64(j != i &&
+                                            !(ne != 0 && excluded_partner(excl, n, i, ne, j)));
             const uint32_t bits =
                 static_cast<uint32_t>(m >> team_lane0) & ((1u << kPhasedBuildTeam) - 1u);
             if (__builtin_amdgcn_inverse_ballot_w64(m)) {
@@ -774,10 +893,11 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
   }
 }
 
-// Step kernel: full-list LJ force and the second half-kick (not on the
-// initial step) of one chunk per workgroup, TEAM lanes per atom; the chunk's
-// kinetic energy and, when OBS, pair energy and virial go to its partials.
-template <int TEAM, int SLOTS, bool OBS>
+// Step kernel: full-list LJ force, listed terms and the second half-kick (not
+// on the initial step) of one chunk per workgroup, TEAM lanes per atom; the
+// chunk's kinetic energy and, when OBS, energies and virials go to its
+// partials.
+template <int TEAM, int SLOTS, bool OBS, bool LISTED>
 __global__ __launch_bounds__(kReplicaChunk * TEAM) void phased_force_kernel(ReplicaParams P,
                                                                             int kick) {
   constexpr int BLOCK = kReplicaChunk * TEAM;
@@ -791,23 +911,17 @@ __global__ __launch_bounds__(kReplicaChunk * TEAM) void phased_force_kernel(Repl
   const ReplicaArrays& A = P.arrays;
   const uint32_t i = (c - d.chunk0) * kReplicaChunk + threadIdx.x / TEAM;
   const uint32_t t = threadIdx.x & (TEAM - 1);
-  double ke = 0.0, pe = 0.0, vir = 0.0;
+  StepSums sums;
   if (i < d.n) {
-    force_atom<TEAM, SLOTS, OBS>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
-                                 A.nbr_count + d.atom0, A.nbr + d.nbr0, lj, kick != 0,
-                                 d.n_types == 1, i, t, ke, pe, vir);
+    force_atom<TEAM, SLOTS, OBS, LISTED>(d, A.pos + d.atom0, A.vel + d.atom0, A.force + d.atom0,
+                                 A.nbr_count + d.atom0, A.nbr + d.nbr0, lj,
+                                 replica_topology(P, d), kick != 0, d.n_types == 1, i, t, sums);
   }
-  ke = block_sum<BLOCK>(ke, red);
-  if (OBS) {
-    pe = block_sum<BLOCK>(pe, red);
-    vir = block_sum<BLOCK>(vir, red);
-  }
+  double tot[kReplicaSums];
+  step_totals<BLOCK>(sums, OBS, tot, red);
   if (threadIdx.x == 0) {
-    P.partials[c] = ke;
-    if (OBS) {
-      P.partials[P.total_chunks + c] = pe;
-      P.partials[2 * static_cast<size_t>(P.total_chunks) + c] = vir;
-    }
+    const int nq = OBS ? kReplicaSums : 1;
+    for (int q = 0; q < nq; ++q) P.partials[static_cast<size_t>(q) * P.total_chunks + c] = tot[q];
   }
 }
 
@@ -820,8 +934,8 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_reduce_kernel(ReplicaPara
   const uint32_t r = blockIdx.x;
   const ReplicaDesc& d = P.desc[r];
   ReplicaState* st = P.state + r;
-  const int nq = observables || initial ? 3 : 1;
-  double tot[3] = {0.0, 0.0, 0.0};
+  const int nq = observables || initial ? kReplicaSums : 1;
+  double tot[kReplicaSums] = {};
   for (int q = 0; q < nq; ++q) {
     double v = 0.0;
     for (uint32_t k = threadIdx.x; k < d.n_chunks; k += kPhasedBlock) {
@@ -839,10 +953,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_reduce_kernel(ReplicaPara
     }
     st->lambda = lambda;
     st->ke = ke;
-    if (observables || initial) {
-      st->pe = tot[1];
-      st->virial = tot[2];
-    }
+    if (observables || initial) store_observables(st, tot);
     if (initial) {
       st->initialized = 1;
     } else {
@@ -855,21 +966,51 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_reduce_kernel(ReplicaPara
   }
 }
 
+template <bool TOPO>
+void launch_persistent(const ReplicaParams& p, uint32_t count, int block, uint64_t n_steps,
+                       int obs, hipStream_t stream) {
+  switch (block) {
+    case 256:
+      replica_kernel<256, TOPO><<<count, 256, 0, stream>>>(p, n_steps, obs);
+      break;
+    case 512:
+      replica_kernel<512, TOPO><<<count, 512, 0, stream>>>(p, n_steps, obs);
+      break;
+    default:
+      replica_kernel<1024, TOPO><<<count, 1024, 0, stream>>>(p, n_steps, obs);
+      break;
+  }
+}
+
+template <int TEAM, int SLOTS>
+void launch_phased_force(const ReplicaParams& p, bool obs, bool listed, int kick,
+                         hipStream_t stream) {
+  const dim3 grid(p.total_chunks), block(kReplicaChunk * TEAM);
+  if (listed) {
+    if (obs) {
+      phased_force_kernel<TEAM, SLOTS, true, true><<<grid, block, 0, stream>>>(p, kick);
+    } else {
+      phased_force_kernel<TEAM, SLOTS, false, true><<<grid, block, 0, stream>>>(p, kick);
+    }
+  } else if (obs) {
+    phased_force_kernel<TEAM, SLOTS, true, false><<<grid, block, 0, stream>>>(p, kick);
+  } else {
+    phased_force_kernel<TEAM, SLOTS, false, false><<<grid, block, 0, stream>>>(p, kick);
+  }
+}
+
+// Whether any replica of the batch has listed terms (records of any kind).
+bool batch_has_listed(const ReplicaParams& p) { return p.bcap + p.acap + p.dcap > 0; }
+
 }  // namespace
 
 void launch_replicas(const ReplicaParams& p, uint32_t count, int block, uint64_t n_steps,
                      bool observables_on_last, hipStream_t stream) {
   const int obs = observables_on_last ? 1 : 0;
-  switch (block) {
-    case 256:
-      replica_kernel<256><<<count, 256, 0, stream>>>(p, n_steps, obs);
-      break;
-    case 512:
-      replica_kernel<512><<<count, 512, 0, stream>>>(p, n_steps, obs);
-      break;
-    default:
-      replica_kernel<1024><<<count, 1024, 0, stream>>>(p, n_steps, obs);
-      break;
+  if (batch_has_listed(p) || p.ecap > 0) {
+    launch_persistent<true>(p, count, block, n_steps, obs, stream);
+  } else {
+    launch_persistent<false>(p, count, block, n_steps, obs, stream);
   }
 }
 
@@ -892,35 +1033,34 @@ void launch_phased_step(const ReplicaParams& p, PhasedShape shape, bool initial,
   phased_rank_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   phased_reorder_gather_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   phased_reorder_commit_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
-  phased_build_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
+  if (p.ecap > 0) {
+    phased_build_kernel<true><<<work, kPhasedBlock, 0, stream>>>(p);
+  } else {
+    phased_build_kernel<false><<<work, kPhasedBlock, 0, stream>>>(p);
+  }
   const int kick = initial ? 0 : 1;
   const bool obs = observables || initial;
-#define BEADS_PHASED_FORCE(TEAM, SLOTS)                                                       \
-  (obs ? phased_force_kernel<TEAM, SLOTS, true><<<p.total_chunks, kReplicaChunk * TEAM, 0, stream>>>( \
-             p, kick)                                                                         \
-       : phased_force_kernel<TEAM, SLOTS, false><<<p.total_chunks, kReplicaChunk * TEAM, 0,   \
-                                                   stream>>>(p, kick))
+  const bool listed = batch_has_listed(p);
   switch (shape.team * 100 + shape.slots) {
     case 204:
-      BEADS_PHASED_FORCE(2, 4);
+      launch_phased_force<2, 4>(p, obs, listed, kick, stream);
       break;
     case 208:
-      BEADS_PHASED_FORCE(2, 8);
+      launch_phased_force<2, 8>(p, obs, listed, kick, stream);
       break;
     case 408:
-      BEADS_PHASED_FORCE(4, 8);
+      launch_phased_force<4, 8>(p, obs, listed, kick, stream);
       break;
     case 802:
-      BEADS_PHASED_FORCE(8, 2);
+      launch_phased_force<8, 2>(p, obs, listed, kick, stream);
       break;
     case 804:
-      BEADS_PHASED_FORCE(8, 4);
+      launch_phased_force<8, 4>(p, obs, listed, kick, stream);
       break;
     default:
-      BEADS_PHASED_FORCE(4, 4);
+      launch_phased_force<4, 4>(p, obs, listed, kick, stream);
       break;
   }
-#undef BEADS_PHASED_FORCE
   phased_reduce_kernel<<<p.count, kPhasedBlock, 0, stream>>>(p, initial ? 1 : 0,
                                                              observables ? 1 : 0);
 }

@@ -31,11 +31,20 @@ struct ReplicaDesc {
   uint64_t sort_every_steps;  // nonzero: step-based sort cadence (see NeighborControl)
   uint32_t chunk0;            // first chunk of the replica's atoms (phased mode)
   uint32_t n_chunks;          // ceil(n / kReplicaChunk)
+  // Listed terms and exclusions: the replica's per-slot records start at
+  // cap * atom0 in each kind's array (stride n, partner slots local to the
+  // replica); its types index the coefficient tables from these offsets.
+  int32_t has_listed;         // some bond, angle or dihedral record
+  int32_t has_excl;           // some excluded pair
+  uint32_t bond_t0, angle_t0, dih_t0;
 };
 
 // Phased mode: the replicas' atoms in chunks of kReplicaChunk (one wave's
 // worth), each inside one replica.
 constexpr uint32_t kReplicaChunk = 64;
+// Per-chunk partial sums: kinetic energy, pair/bond/angle/dihedral energies
+// and virials.
+constexpr uint32_t kReplicaSums = 9;
 
 // Per-replica run state carried across launches.
 struct ReplicaState {
@@ -49,8 +58,8 @@ struct ReplicaState {
   uint32_t pad;
   double lambda;  // Berendsen factor still to be applied to the stored velocities
   double ke;      // kinetic energy after the last second half-kick
-  double pe;      // pair energy (last observable step)
-  double virial;
+  double pe[4];   // pair, bond, angle, dihedral energy (last observable step)
+  double virial[4];
   uint64_t last_sort_step;  // step of the last reorder (step-based cadence)
   // Phased mode, within a step: displacement check result, and the decisions
   // the rebuild kernels follow.
@@ -82,6 +91,19 @@ struct ReplicaArrays {
   uint32_t* s_cell_of;
   uint32_t* cell_tmp;  // phased mode: members of each cell before the slot sort
   real4* cell_pos;     // phased mode: positions in cell order, .w = the slot's bits
+  // Per-slot listed-term roles and exclusions (the single-system layout of
+  // device.hpp; inline only) and their reorder staging.
+  uint32_t* topo_count;
+  uint2* bond;
+  uint4* angle;
+  uint4* dih;
+  uint32_t* excl;
+  uint32_t* s_topo_count;
+  uint2* s_bond;
+  uint4* s_angle;
+  uint4* s_dih;
+  uint32_t* s_excl;
+  uint32_t* new_of_old;  // reorder: new local slot of each old local slot
 };
 
 struct ReplicaParams {
@@ -89,11 +111,16 @@ struct ReplicaParams {
   ReplicaState* state;
   const LjCoeff* lj;
   ReplicaArrays arrays;
+  uint32_t bcap, acap, dcap, ecap;  // records per slot of each kind
+  const real2* bond_coeff;          // k, r0
+  const real2* angle_coeff;         // k, theta0 (rad)
+  const real4* dih_coeff;           // k, d, n, unused
   // Phased mode.
   uint32_t count;                // replicas
   uint32_t total_chunks;
   const uint32_t* chunk_replica; // replica of each chunk
-  double* partials;              // 3 x total_chunks: kinetic energy, pair energy, virial
+  double* partials;              // kReplicaSums x total_chunks: kinetic energy, energies[4],
+                                 // virials[4]
   uint32_t* work;                // chunks of the replicas that rebuild this step
   uint32_t* work_reorder;        // chunks of the replicas that also reorder
   uint32_t* work_count;          // [0]: entries of work, [1]: of work_reorder

@@ -343,11 +343,12 @@ void beads_engine_destroy(beads_engine_t* engine);
 
 /* ---- batched replicas ----
  * No reference counterpart: the reference runs one system per process
- * (simulation_run.cpp). Many independent pair-only (LJ) systems, each a full
- * beads_spec_t (outputs ignored), advance together in one of two modes:
+ * (simulation_run.cpp). Many independent systems (LJ pairs, harmonic bonds,
+ * angles and dihedrals, bonded exclusions), each a full beads_spec_t (outputs
+ * ignored), advance together in one of two modes:
  *   persistent (mode 0): ONE launch per run, one workgroup of `block` lanes
  *     per replica running that replica's whole step loop (integrate, rebuild
- *     with stable sort, full-list LJ + kick + Berendsen);
+ *     with stable sort, full-list LJ + listed terms + kick + Berendsen);
  *   phased (mode 1): per step, a sequence of batched kernels over 64-atom
  *     chunks of every replica, so a large replica spreads over many
  *     workgroups; the force kernel uses `team` lanes per atom and `slots` list
@@ -384,8 +385,9 @@ beads_status_t beads_batch_run(beads_batch_t* batch, uint64_t n_steps,
 beads_status_t beads_batch_profile(beads_batch_t* batch, uint64_t n_steps, double* seconds);
 beads_status_t beads_batch_synchronize(beads_batch_t* batch);
 /* stats of replica r (index in the order given to create): build/reorder
- * counts, kinetic energy, pending lambda, pair energy and virial in pe[0] and
- * virial[0] (last observable step). */
+ * counts, kinetic energy, pending lambda, and the pair, bond, angle and
+ * dihedral energies and virials in pe[0..3] and virial[0..3] (last observable
+ * step), as the single engine reports them. */
 beads_status_t beads_batch_replica_stats(beads_batch_t* batch, uint32_t r,
                                          beads_engine_stats_t* out);
 beads_status_t beads_batch_read_state(beads_batch_t* batch, uint32_t r,

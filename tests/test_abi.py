@@ -159,3 +159,27 @@ def test_engine_rejects_unregistered_styles_before_any_device_call():
         assert status == 1, field  # BEADS_E_VALUE
         assert lib.beads_last_error().decode() == f'unsupported {kind} style "nope".'
         assert not handle.value
+
+
+def test_batch_rejects_more_than_255_records_per_particle_before_any_device_call():
+    """Batched replicas keep every listed-term record inline (no spill
+    segments), at most 255 of a kind per particle: a replica whose hub bead
+    has 300 bonds is BEADS_E_VALUE at create, raised while the specs are
+    packed (no device)."""
+    from gpu_cases import hub_lattice
+
+    from beads_amd import workloads as W
+
+    lib = capi.library()
+    ws = [W.lj_box(300, 0.8, 1.0, seed=1), hub_lattice(300, 1, 2, False)]
+    specs = (capi.Spec * len(ws))()
+    keep = []
+    for k, w in enumerate(ws):
+        specs[k], kk = capi.make_spec(w)
+        keep.append(kk)
+    handle = ctypes.c_void_p()
+    status = lib.beads_batch_create(specs, len(ws), ctypes.byref(handle))
+    assert status == 1  # BEADS_E_VALUE
+    assert lib.beads_last_error().decode() == \
+        "batched replicas keep at most 255 bonds per particle; a particle has 300."
+    assert not handle.value

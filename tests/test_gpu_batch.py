@@ -160,3 +160,96 @@ def test_full_replica_set_against_single_engines_and_oracle(config):
             assert np.max(np.abs(dx)) < 1e-4, r
             f_ref, _ = O.forces(w, positions=st["positions"], tags=st["tags"], types=st["types"])
             assert _force_error(st["forces"].astype(np.float64), f_ref) <= FORCE_RTOL, r
+
+
+def bonded_replicas():
+    """Polymer melts (harmonic bonds, angles, dihedrals, exclusions) on each
+    neighbour path beside a pair-only replica: 1,200 beads (cell path, a
+    reorder every 2nd build), 3,000 beads (regular cell path), 216 beads
+    (two cells per axis), and 480 beads with two angle and dihedral types (the second
+    d = -1, n = 2) and exclusions to bond distance 3."""
+    a = W.polymer_melt(n_chains=40, beads_per_chain=30, seed=11, jitter=0.05)
+    a = a.with_(sort_every_rebuild=2)
+    b = W.polymer_melt(n_chains=100, beads_per_chain=30, seed=12, jitter=0.05)
+    c = W.polymer_melt(n_chains=18, beads_per_chain=12, density=7.5, seed=13, jitter=0.05)
+    d = W.polymer_melt(n_chains=24, beads_per_chain=20, seed=14, jitter=0.05)
+    angles, dihedrals = d.angles.copy(), d.dihedrals.copy()
+    angles[1::2, 3] = 2
+    dihedrals[1::2, 4] = 2
+    d = d.with_(angles=angles, dihedrals=dihedrals, exclusion_distance=3,
+                angle_k=np.array([25.0, 40.0]), angle_theta0=np.array([120.0, 105.0]),
+                dihedral_k=np.array([1.0, 2.5]), dihedral_d=np.array([1, -1]),
+                dihedral_n=np.array([1, 2]))
+    lj = W.lj_box(700, 0.65, 1.5, seed=43, berendsen=True, tau=0.5)
+    return [a, b, c, d, lj]
+
+
+def _tag_sorted_dx(w, xa, xb):
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    oa, ob = np.argsort(xa["tags"]), np.argsort(xb["tags"])
+    dx = xa["positions"][oa].astype(np.float64) - xb["positions"][ob]
+    dx -= np.round(dx / L) * L
+    return float(np.max(np.abs(dx)))
+
+
+@pytest.mark.parametrize("config", MODES)
+def test_bonded_replicas_match_single_engine_and_oracle(config):
+    """Step-0 forces of melts with listed terms and exclusions equal the
+    single engine's bit for bit and the FP64 oracle's to the force bar; the
+    four energies and virials match the engine's; after 60 steps the build
+    and reorder counts equal the engine's and positions agree within 1e-4."""
+    ws = bonded_replicas()
+    steps = 60
+    with Batch(ws) as b:
+        b.set_config(config)
+        b.run(0)
+        initial = [(b.state(r), b.stats(r)) for r in range(len(ws))]
+        b.run(steps, observables_on_last=True)
+        final = [(b.state(r), b.stats(r)) for r in range(len(ws))]
+    for r, w in enumerate(ws):
+        with Engine(w) as e:
+            e.initialize(observables=True)
+            s0, ss0 = e.state(), e.stats()
+            e.advance(steps, observables_on_last=True)
+            s1, ss1 = e.state(), e.stats()
+        (b0, bs0), (b1, bs1) = initial[r], final[r]
+        np.testing.assert_array_equal(b0["tags"], s0["tags"])
+        np.testing.assert_array_equal(b0["forces"], s0["forces"])
+        for q in range(4):
+            assert bs0.pe[q] == pytest.approx(ss0.pe[q], rel=1e-6, abs=1e-9), (r, q)
+            assert bs0.virial[q] == pytest.approx(ss0.virial[q], rel=1e-6, abs=1e-9), (r, q)
+        f_ref, obs = O.forces(w, positions=b0["positions"], tags=b0["tags"], types=b0["types"])
+        assert _force_error(b0["forces"].astype(np.float64), f_ref) <= FORCE_RTOL, r
+        for q, key in enumerate(("pair_pe", "bond_pe", "angle_pe", "dihedral_pe")):
+            assert bs0.pe[q] == pytest.approx(obs[key], rel=1e-5, abs=1e-6), (r, key)
+        assert bs1.step == steps and bs1.overflow == 0
+        assert (bs1.build_count, bs1.reorder_count) == (ss1.build_count, ss1.reorder_count), r
+        assert _tag_sorted_dx(w, b1, s1) < 1e-4, r
+        for q in range(4):
+            assert bs1.pe[q] == pytest.approx(ss1.pe[q], rel=1e-4, abs=1e-6), (r, q)
+        f_ref, _ = O.forces(w, positions=b1["positions"], tags=b1["tags"], types=b1["types"])
+        assert _force_error(b1["forces"].astype(np.float64), f_ref) <= FORCE_RTOL, r
+
+
+def test_bonded_replicas_reorder_and_modes_agree():
+    """The 1,200-bead melt reorders every 2nd build; persistent and phased
+    runs of the bonded set, the phased one in pieces, end with the same
+    counters and positions within 1e-5; the reordered replica's reorder count
+    is positive, so its records went through the remap."""
+    ws = bonded_replicas()
+    out = {}
+    for config in (1, 3):
+        with Batch(ws) as b:
+            b.set_config(config)
+            b.run(0)
+            b.run(11)
+            b.run(29, observables_on_last=True)
+            out[config] = [(b.stats(r), b.state(r)) for r in range(len(ws))]
+    assert out[1][0][0].reorder_count > 0
+    for r, w in enumerate(ws):
+        (sa, xa), (sb, xb) = out[1][r], out[3][r]
+        assert (sa.step, sa.build_count, sa.reorder_count, sa.dangerous_rebuild_count) == \
+            (sb.step, sb.build_count, sb.reorder_count, sb.dangerous_rebuild_count), r
+        assert _tag_sorted_dx(w, xa, xb) < 1e-5, r
+        for q in range(4):
+            assert sa.pe[q] == pytest.approx(sb.pe[q], rel=1e-6, abs=1e-9), (r, q)
