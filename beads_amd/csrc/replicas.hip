@@ -807,7 +807,10 @@ constexpr int kPhasedBuildTeam = 4;
 constexpr int kPhasedBuildSlots = 4;
 static_assert(kPhasedBlock == kReplicaChunk * kPhasedBuildTeam, "one chunk per workgroup");
 
-template <bool EXCL>
+// EXCL: 0 no exclusions in the batch; 4 or 8: an atom's excluded partners
+// (at most EXCL) held in registers, unused entries never a slot; -1: read
+// from memory per candidate.
+template <int EXCL>
 __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParams P) {
   const ReplicaArrays& A = P.arrays;
   const uint32_t t = threadIdx.x & (kPhasedBuildTeam - 1);
@@ -829,7 +832,24 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
       const real_t h0 = d.box.hL[0], h1 = d.box.hL[1], h2 = d.box.hL[2];
       const real4 xi = pos[i];
       const uint32_t* excl = A.excl + static_cast<size_t>(P.ecap) * d.atom0;
-      const uint32_t ne = EXCL && d.has_excl ? A.topo_count[d.atom0 + i] >> 24 : 0u;
+      const uint32_t ne = EXCL != 0 && d.has_excl ? A.topo_count[d.atom0 + i] >> 24 : 0u;
+      uint32_t ex[EXCL > 0 ? EXCL : 1];
+#pragma unroll
+      for (int k = 0; k < (EXCL > 0 ? EXCL : 1); ++k) {
+        ex[k] = EXCL > 0 && static_cast<uint32_t>(k) < ne
+                    ? excl[static_cast<size_t>(k) * n + i]
+                    : 0xffffffffu;
+      }
+      const auto excluded = [&](uint32_t j) {
+        bool hit = false;
+        if (EXCL > 0) {
+#pragma unroll
+          for (int k = 0; k < (EXCL > 0 ? EXCL : 1); ++k) hit |= ex[k] == j;
+        } else if (EXCL < 0) {
+          hit = ne != 0 && excluded_partner(excl, n, i, ne, j);
+        }
+        return hit;
+      };
       // Candidates [b, e) of a run (b < e); entry(k) = {x, y, z, slot bits}.
       auto scan = [&](uint32_t b, uint32_t e, auto entry) {
         for (uint32_t k = b; k < e; k += kPhasedBuildTeam * kPhasedBuildSlots) {
@@ -848,8 +868,7 @@ __global__ __launch_bounds__(kPhasedBlock) void phased_build_kernel(ReplicaParam
             const uint64_t m =
                 __builtin_amdgcn_ballot_w64(k + u * kPhasedBuildTeam + t < e) &
                 __builtin_amdgcn_ballot_w64(!(r2_strict(dx, dy, dz) > d.rs2)) &
-                __builtin_amdgcn_ballot_w64(j != i &&
-                                            !(ne != 0 && excluded_partner(excl, n, i, ne, j)));
+                __builtin_amdgcn_ballot_w64(j != i && !excluded(j));
             const uint32_t bits =
                 static_cast<uint32_t>(m >> team_lane0) & ((1u << kPhasedBuildTeam) - 1u);
             if (__builtin_amdgcn_inverse_ballot_w64(m)) {
@@ -1033,10 +1052,14 @@ void launch_phased_step(const ReplicaParams& p, PhasedShape shape, bool initial,
   phased_rank_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   phased_reorder_gather_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
   phased_reorder_commit_kernel<<<work, kPhasedBlock, 0, stream>>>(p);
-  if (p.ecap > 0) {
-    phased_build_kernel<true><<<work, kPhasedBlock, 0, stream>>>(p);
+  if (p.ecap == 0) {
+    phased_build_kernel<0><<<work, kPhasedBlock, 0, stream>>>(p);
+  } else if (p.ecap <= 4) {
+    phased_build_kernel<4><<<work, kPhasedBlock, 0, stream>>>(p);
+  } else if (p.ecap <= 8) {
+    phased_build_kernel<8><<<work, kPhasedBlock, 0, stream>>>(p);
   } else {
-    phased_build_kernel<false><<<work, kPhasedBlock, 0, stream>>>(p);
+    phased_build_kernel<-1><<<work, kPhasedBlock, 0, stream>>>(p);
   }
   const int kick = initial ? 0 : 1;
   const bool obs = observables || initial;

@@ -376,6 +376,25 @@ def lj_box(n: int, density: float, temperature: float, seed: int, cutoff: float 
         seed=seed)
 
 
+def melt_replica_set(count: int = 128, side_min: int = 10, side_max: int = 27, seed: int = 9,
+                     steps: int = 1000) -> list[Workload]:
+    """Independent bead-spring melts of polymer_melt's force field and uneven
+    size: lattice side s log-uniform in [side_min, side_max] (s^3 sites),
+    chains of 20, 30, 40 or 50 beads filling 95 % of the sites, box 0.48 s nm
+    (spacing 0.48 nm, density about 8.6 nm^-3), 300 K."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        side = int(round(math.exp(rng.uniform(math.log(side_min), math.log(side_max)))))
+        per_chain = int(rng.choice([20, 30, 40, 50]))
+        chains = max(1, int(0.95 * side ** 3) // per_chain)
+        n = chains * per_chain
+        out.append(polymer_melt(n_chains=chains, beads_per_chain=per_chain,
+                                density=n / (0.48 * side) ** 3, seed=seed * 100003 + k,
+                                steps=steps))
+    return out
+
+
 def replica_set(count: int = 512, n_min: int = 500, n_max: int = 20000, seed: int = 5,
                 steps: int = 1000) -> list[Workload]:
     """Independent reduced-unit LJ systems: N log-uniform in [n_min, n_max],

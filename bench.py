@@ -95,14 +95,18 @@ def cpu_baseline(w, sample_steps: int) -> dict:
             "omp_places": os.environ["OMP_PLACES"]}
 
 
-def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
-    """Batched replicas vs serial plain runs of the same replicas."""
+def batched_leg(steps: int, serial_steps: int, count: int, melts: bool = False) -> dict:
+    """Batched replicas vs serial plain runs of the same replicas: the LJ
+    replica set, or (melts) bead-spring melts with bonds, angles, dihedrals
+    and exclusions."""
     from beads_amd import workloads as W
     from beads_amd.capi import Batch, Engine
 
-    ws = W.replica_set(count)
+    ws = W.melt_replica_set(count) if melts else W.replica_set(count)
     atoms = sum(w.n for w in ws)
-    print(f"bench: batched leg, {count} replicas, {atoms} atoms", file=sys.stderr, flush=True)
+    kind = "melt" if melts else "LJ"
+    print(f"bench: batched leg, {count} {kind} replicas, {atoms} atoms", file=sys.stderr,
+          flush=True)
     with Batch(ws) as b:
         chosen, trial = b.autotune(trial_steps=20)
         config = b.config_info(chosen)
@@ -128,8 +132,14 @@ def batched_leg(steps: int, serial_steps: int, count: int) -> dict:
             e.synchronize()
             serial_wall += time.perf_counter() - t0
     serial = atoms * serial_steps / serial_wall
-    return {"workload": f"{count} LJ replicas, N log-uniform [500, 20000], {atoms} atoms, "
-                        f"reduced units, Berendsen",
+    if melts:
+        sizes = [w.n for w in ws]
+        label = (f"{count} bead-spring melts (bonds, angles, dihedrals, exclusions to distance 2),"
+                 f" N {min(sizes)}-{max(sizes)}, {atoms} beads, nm_kjmol, Berendsen 300 K")
+    else:
+        label = (f"{count} LJ replicas, N log-uniform [500, 20000], {atoms} atoms, "
+                 f"reduced units, Berendsen")
+    return {"workload": label,
             "steps": steps, "config": config,
             "autotune_s": [dict(c, seconds=round(x, 5)) for c, x in zip(configs, trial)],
             "batched_atom_steps_per_s": batched, "batched_device_s": dev,
@@ -317,6 +327,8 @@ def main() -> None:
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serial-steps", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--melt-replicas", type=int, default=128,
+                    help="bead-spring melt replicas of the batched-melt leg (0: skip it)")
     ap.add_argument("--adjoint-frames", type=int, default=64)
     ap.add_argument("--traj-steps", type=int, default=200)
     ap.add_argument("--half-list", action="store_true",
@@ -395,6 +407,10 @@ def main() -> None:
     batched = None
     if not args.no_batch:
         batched = batched_leg(args.batch_steps, args.serial_steps, args.replicas)
+    batched_melts = None
+    if not args.no_batch and args.melt_replicas > 0:
+        batched_melts = batched_leg(args.batch_steps, args.serial_steps, args.melt_replicas,
+                                    melts=True)
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(args.adjoint_frames, args.traj_steps, 20)
@@ -432,6 +448,7 @@ def main() -> None:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_melts": batched_melts,
         "adjoint": adjoint,
         "other_workloads": others,
     }

@@ -253,3 +253,46 @@ def test_bonded_replicas_reorder_and_modes_agree():
         assert _tag_sorted_dx(w, xa, xb) < 1e-5, r
         for q in range(4):
             assert sa.pe[q] == pytest.approx(sb.pe[q], rel=1e-6, abs=1e-9), (r, q)
+
+
+def _exclusion_case(name):
+    if name == "hub":
+        from gpu_cases import hub_lattice
+
+        return [hub_lattice(12, 2, 3, True),
+                W.polymer_melt(n_chains=18, beads_per_chain=12, density=7.5, seed=22,
+                               jitter=0.05)]
+    distance = {"chains2": 2, "chains3": 3}[name]
+    ws = [W.polymer_melt(n_chains=40, beads_per_chain=30, seed=21, jitter=0.05),
+          W.polymer_melt(n_chains=18, beads_per_chain=12, density=7.5, seed=22, jitter=0.05)]
+    return [w.with_(exclusion_distance=distance) for w in ws]
+
+
+@pytest.mark.parametrize("case", ["chains2", "chains3", "hub"])
+def test_bonded_replica_exclusion_variants(case):
+    """Each exclusion variant of the phased build: up to 4 partners in
+    registers (chains, bond distance 2), up to 8 (chains, distance 3: 6
+    partners) and read from memory (a hub of 12 two-bead arms at distance 3,
+    beside a melt: the hub and its first beads have 24 excluded partners).
+    Step-0 forces equal the single engine's bit for bit in both modes, and
+    after 30 steps the build and reorder counts are the engine's and
+    positions agree within 1e-4."""
+    ws = _exclusion_case(case)
+    steps = 30
+    for config in (1, 3):
+        with Batch(ws) as b:
+            b.set_config(config)
+            b.run(0)
+            initial = [b.state(r) for r in range(len(ws))]
+            b.run(steps)
+            final = [(b.state(r), b.stats(r)) for r in range(len(ws))]
+        for r, w in enumerate(ws):
+            with Engine(w) as e:
+                e.initialize(observables=False)
+                s0 = e.state()
+                e.advance(steps)
+                s1, ss1 = e.state(), e.stats()
+            np.testing.assert_array_equal(initial[r]["forces"], s0["forces"])
+            (b1, bs1) = final[r]
+            assert (bs1.build_count, bs1.reorder_count) == (ss1.build_count, ss1.reorder_count)
+            assert _tag_sorted_dx(w, b1, s1) < 1e-4, (config, r)
