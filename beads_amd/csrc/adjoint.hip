@@ -107,18 +107,55 @@ constexpr int kAdjTeam = 4;
 constexpr int kAdjSlots = 4;
 constexpr uint32_t kAdjAtoms = kBlock / kAdjTeam;
 
+// Per type pair, the factors of the pair terms (d = x_i - x_j, u = 1/r^2,
+// t1 = 2 sr12 - sr6):
+//   s = e24 u t1,  ds/d(r^2) = -e24 u^2 (14 sr12 - 4 sr6),
+//   ds/deps = 24 u t1,  ds/dsigma = cs u (4 sr12 - sr6)   (cs = 144 eps / sigma),
+// and for dE/dtheta: 4 (sr12 - sr6) - sh_e and f4 (12 sr12 - 6 sr6) - sh_s
+// (f4 = 4 eps / sigma; sh_e, sh_s the shift's derivatives, 0 when unshifted).
+struct AdjLj {
+  double sig2, e24, cs, f4, sh_e, sh_s;
+};
+
+__device__ __forceinline__ AdjLj adj_lj(const AdjointParams& p, int k) {
+  const double eps = p.eps[k], sig = p.sigma[k];
+  AdjLj c;
+  c.sig2 = sig * sig;
+  c.e24 = 24.0 * eps;
+  c.cs = 144.0 * eps / sig;
+  c.f4 = 4.0 * eps / sig;
+  c.sh_e = 0.0;
+  c.sh_s = 0.0;
+  if (p.lj_shift) {
+    const double sc2 = c.sig2 / p.rc2_f64;
+    const double sc6 = sc2 * sc2 * sc2;
+    const double sc12 = sc6 * sc6;
+    c.sh_e = 4.0 * (sc12 - sc6);
+    c.sh_s = c.f4 * (12.0 * sc12 - 6.0 * sc6);
+  }
+  return c;
+}
+
 // TREG > 0: the atom's row of the type table (at most TREG types) is
 // accumulated in registers and flushed once per lane; TREG == 0: every
 // interaction adds into LDS directly (any type count up to kMaxLdsTypes).
-// Each atom's team sums its partial dL/dx in FP64 before the store.
-template <int TREG>
+// One type: the pair factors live in registers; more: in an LDS table.
+// GE: dL/dE is nonzero, so the pass also sums F for the -gE F term and the
+// energy's parameter derivatives. Each atom's team sums its partial dL/dx in
+// FP64 before the store.
+template <int TREG, bool GE>
 __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
   __shared__ double acc[2 * kMaxLdsTypes * kMaxLdsTypes + 2 * kMaxBondTypesAdjoint];
+  __shared__ AdjLj coef[TREG == 1 ? 1 : kMaxLdsTypes * kMaxLdsTypes];
   const int T = p.n_types;
   const int n_pair = T * T;
   const int n_acc = 2 * n_pair + 2 * p.n_bond_types;
   for (int k = threadIdx.x; k < n_acc; k += kBlock) acc[k] = 0.0;
+  if (TREG != 1) {
+    for (int k = threadIdx.x; k < n_pair; k += kBlock) coef[k] = adj_lj(p, k);
+  }
   __syncthreads();
+  const AdjLj c0 = TREG == 1 ? adj_lj(p, 0) : AdjLj{};
   const uint32_t n = p.n;
   const uint32_t t = threadIdx.x & (kAdjTeam - 1);
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kAdjAtoms + threadIdx.x / kAdjTeam;
@@ -130,7 +167,6 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
     const double gix = gi.x, giy = gi.y, giz = gi.z;
     const real_t L0 = p.box.L[0], L1 = p.box.L[1], L2 = p.box.L[2];
     const real_t h0 = p.box.hL[0], h1 = p.box.hL[1], h2 = p.box.hL[2];
-    const double inv_rc2 = rcp_f64(p.rc2_f64);
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double re[TREG > 0 ? TREG : 1], rs[TREG > 0 ? TREG : 1];
@@ -152,46 +188,46 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
       }
 #pragma unroll
       for (int u = 0; u < kAdjSlots; ++u) {
-        const real_t dx = mic_diff(xi.x, xj[u].x, L0, h0);
-        const real_t dy = mic_diff(xi.y, xj[u].y, L1, h1);
-        const real_t dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        // mic_diff returns the raw difference unless it exceeds half the box;
+        // the folded forms only when some lane of the wave needs them.
+        real_t dx = xi.x - xj[u].x;
+        real_t dy = xi.y - xj[u].y;
+        real_t dz = xi.z - xj[u].z;
+        if ((__builtin_amdgcn_ballot_w64(fabs(dx) > h0) |
+             __builtin_amdgcn_ballot_w64(fabs(dy) > h1) |
+             __builtin_amdgcn_ballot_w64(fabs(dz) > h2)) != 0) {
+          dx = mic_diff(xi.x, xj[u].x, L0, h0);
+          dy = mic_diff(xi.y, xj[u].y, L1, h1);
+          dz = mic_diff(xi.z, xj[u].z, L2, h2);
+        }
         if (base + u * kAdjTeam + t >= cnt || r2_strict(dx, dy, dz) >= p.rc2) continue;
         const int tj = static_cast<int>(id_of(xj[u].w));
-        const double eps = p.eps[ti * T + tj];
-        const double sig = p.sigma[ti * T + tj];
-        const double inv_sig = rcp_f64(sig);
+        const AdjLj c = TREG == 1 ? c0 : coef[ti * T + tj];
         const double ddx = dx, ddy = dy, ddz = dz;
         const double u2 = rcp_f64(ddx * ddx + ddy * ddy + ddz * ddz);
-        const double sig2 = sig * sig;
-        const double sr2 = sig2 * u2;
+        const double sr2 = c.sig2 * u2;
         const double sr6 = sr2 * sr2 * sr2;
         const double sr12 = sr6 * sr6;
-        const double s = 24.0 * eps * u2 * (2.0 * sr12 - sr6);
-        const double sp = -24.0 * eps * u2 * u2 * (14.0 * sr12 - 4.0 * sr6);
+        const double b = u2 * (2.0 * sr12 - sr6);
+        const double s = c.e24 * b;
+        const double sp2 = -2.0 * c.e24 * u2 * u2 * (14.0 * sr12 - 4.0 * sr6);
         const double ex = gix - gj[u].x, ey = giy - gj[u].y, ez = giz - gj[u].z;
-        const double dd = ddx * ex + ddy * ey + ddz * ez;
-        gx += s * ex + 2.0 * sp * ddx * dd;
-        gy += s * ey + 2.0 * sp * ddy * dd;
-        gz += s * ez + 2.0 * sp * ddz * dd;
-        fx += s * ddx;
-        fy += s * ddy;
-        fz += s * ddz;
+        const double q2 = sp2 * (ddx * ex + ddy * ey + ddz * ez);
+        gx += s * ex + q2 * ddx;
+        gy += s * ey + q2 * ddy;
+        gz += s * ez + q2 * ddz;
+        if (GE) {
+          fx += s * ddx;
+          fy += s * ddy;
+          fz += s * ddz;
+        }
         const double w = gix * ddx + giy * ddy + giz * ddz;
         // dF/deps and dF/dsigma terms, plus half of dE/deps and dE/dsigma.
-        double de = w * 24.0 * u2 * (2.0 * sr12 - sr6);
-        double ds = w * 24.0 * eps * u2 * inv_sig * (24.0 * sr12 - 6.0 * sr6);
-        if (gE != 0.0) {
-          double e_eps = 4.0 * (sr12 - sr6);
-          double e_sig = 4.0 * eps * (12.0 * sr12 - 6.0 * sr6) * inv_sig;
-          if (p.lj_shift) {
-            const double sc2 = sig2 * inv_rc2;
-            const double sc6 = sc2 * sc2 * sc2;
-            const double sc12 = sc6 * sc6;
-            e_eps -= 4.0 * (sc12 - sc6);
-            e_sig -= 4.0 * eps * (12.0 * sc12 - 6.0 * sc6) * inv_sig;
-          }
-          de += 0.5 * gE * e_eps;
-          ds += 0.5 * gE * e_sig;
+        double de = 24.0 * w * b;
+        double ds = w * c.cs * u2 * (4.0 * sr12 - sr6);
+        if (GE) {
+          de += 0.5 * gE * (4.0 * (sr12 - sr6) - c.sh_e);
+          ds += 0.5 * gE * (c.f4 * (12.0 * sr12 - 6.0 * sr6) - c.sh_s);
         }
         if constexpr (TREG > 0) {
 #pragma unroll
@@ -245,9 +281,11 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
         gx += phi * ex + dphi_r * ddx * dd;
         gy += phi * ey + dphi_r * ddy * dd;
         gz += phi * ez + dphi_r * ddz * dd;
-        fx += phi * ddx;
-        fy += phi * ddy;
-        fz += phi * ddz;
+        if (GE) {
+          fx += phi * ddx;
+          fy += phi * ddy;
+          fz += phi * ddz;
+        }
         const double w = gix * ddx + giy * ddy + giz * ddz;
         const double stretch = rr - r0;
         atomicAdd(&acc[2 * n_pair + bt], w * (-stretch * inv_r) + gE * 0.25 * stretch * stretch);
@@ -258,9 +296,11 @@ __global__ __launch_bounds__(kBlock) void fm_backward_kernel(AdjointParams p) {
     gx = team_sum<kAdjTeam>(gx);
     gy = team_sum<kAdjTeam>(gy);
     gz = team_sum<kAdjTeam>(gz);
-    fx = team_sum<kAdjTeam>(fx);
-    fy = team_sum<kAdjTeam>(fy);
-    fz = team_sum<kAdjTeam>(fz);
+    if (GE) {
+      fx = team_sum<kAdjTeam>(fx);
+      fy = team_sum<kAdjTeam>(fy);
+      fz = team_sum<kAdjTeam>(fz);
+    }
     if (t == 0) {
       p.grad_pos[3 * i] = static_cast<real_t>(gx - gE * fx);
       p.grad_pos[3 * i + 1] = static_cast<real_t>(gy - gE * fy);
@@ -576,12 +616,16 @@ void launch_pack_xyz(uint32_t n, const real_t* xyz, real4* out, hipStream_t stre
 
 void launch_fm_backward(const AdjointParams& p, hipStream_t stream) {
   const unsigned g = grid_for(p.n, kAdjAtoms);
+  const bool ge = p.grad_energy != 0.0;
   if (p.n_types <= 1) {
-    fm_backward_kernel<1><<<g, kBlock, 0, stream>>>(p);
+    if (ge) fm_backward_kernel<1, true><<<g, kBlock, 0, stream>>>(p);
+    else fm_backward_kernel<1, false><<<g, kBlock, 0, stream>>>(p);
   } else if (p.n_types <= 4) {
-    fm_backward_kernel<4><<<g, kBlock, 0, stream>>>(p);
+    if (ge) fm_backward_kernel<4, true><<<g, kBlock, 0, stream>>>(p);
+    else fm_backward_kernel<4, false><<<g, kBlock, 0, stream>>>(p);
   } else {
-    fm_backward_kernel<0><<<g, kBlock, 0, stream>>>(p);
+    if (ge) fm_backward_kernel<0, true><<<g, kBlock, 0, stream>>>(p);
+    else fm_backward_kernel<0, false><<<g, kBlock, 0, stream>>>(p);
   }
   if (p.acap > 0 || p.dcap > 0) {
     listed_backward_kernel<<<grid_for(p.n, kListAtoms), kBlock, 0, stream>>>(p);

@@ -246,6 +246,65 @@ def adjoint_leg(frames: int, traj_steps: int, every: int) -> dict:
                            "backward_over_forward": t_traj_b / t_traj_f}}
 
 
+def adjoint_melt_leg(frames: int) -> dict:
+    """Force-matching gradient cost on the 1M-bead melt's force field (LJ,
+    bonds, angles, dihedrals, exclusions; 7 parameters): per-frame kernel time
+    of the backward against the forward, and autograd forward + backward
+    against the forward."""
+    import numpy as np
+    import torch
+
+    from beads_amd import workloads as W
+    from beads_amd.adjoint import FrameModel, Params, force_matching_loss
+
+    w = W.polymer_melt()
+    print(f"bench: adjoint melt leg, {w.n} beads, {frames} frames", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(321)
+    lo = w.box_bound[0].astype(np.float64)
+    L = (w.box_bound[1] - w.box_bound[0]).astype(np.float64)
+    xs = []
+    for _ in range(frames):
+        q = w.positions.astype(np.float64) + rng.uniform(-0.01, 0.01, (w.n, 3))
+        xs.append(torch.tensor((lo + np.mod(q - lo, L)).astype(np.float32), device="cuda"))
+    with FrameModel(w) as m:
+        p = m.initial_params()
+        with torch.no_grad():
+            tp = Params(*[t.detach() * float(rng.uniform(0.96, 1.04)) for t in p.tensors()])
+            targets = [m.forces(x, tp)[0].clone() for x in xs]
+        force_matching_loss(m, xs[:1], targets[:1], p).backward()
+        torch.cuda.synchronize()
+        fwd_dev = bwd_dev = 0.0
+        for x in xs:
+            f, _ = m.forces(x, p)
+            fd, _ = m.engine.frame_timing()
+            (f.double() ** 2).sum().backward()
+            _, bd = m.engine.frame_timing()
+            fwd_dev += fd
+            bwd_dev += bd
+        for t in p.tensors():
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            force_matching_loss(m, xs, targets, p)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss = force_matching_loss(m, xs, targets, p)
+        loss.backward()
+        torch.cuda.synchronize()
+        t_fb = time.perf_counter() - t0
+    return {"workload": f"polymer melt {w.n} beads (20,000 x 50), LJ + harmonic bonds, angles, "
+                        f"dihedrals, exclusions to distance 2; 7 parameters; frames jittered "
+                        f"by up to 0.01 nm",
+            "frames": frames, "loss": float(loss.detach()),
+            "frame_forward_device_us": fwd_dev / frames * 1e6,
+            "frame_backward_device_us": bwd_dev / frames * 1e6,
+            "kernel_backward_over_forward": bwd_dev / fwd_dev,
+            "forward_s": t_fwd, "forward_backward_s": t_fb,
+            "gradient_over_forward": t_fb / t_fwd}
+
+
 def measured_traffic(workload: str, mean_neighbors: float):
     """HBM bytes per force-kernel launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration) and the
@@ -414,6 +473,7 @@ def main() -> None:
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(args.adjoint_frames, args.traj_steps, 20)
+        adjoint["melt"] = adjoint_melt_leg(8)
 
     cpu = None
     if not args.no_cpu_baseline:
