@@ -385,7 +385,8 @@ def main() -> None:
                     help="skip the 4,000-bead FCC fluid and 4M-bead mixture lines")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serial-steps", type=int, default=100)
-    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--replicas", type=int, default=512,
+                    help="LJ replicas of the batched leg (0: skip it)")
     ap.add_argument("--melt-replicas", type=int, default=128,
                     help="bead-spring melt replicas of the batched-melt leg (0: skip it)")
     ap.add_argument("--adjoint-frames", type=int, default=64)
@@ -464,7 +465,7 @@ def main() -> None:
     eng.close()
 
     batched = None
-    if not args.no_batch:
+    if not args.no_batch and args.replicas > 0:
         batched = batched_leg(args.batch_steps, args.serial_steps, args.replicas)
     batched_melts = None
     if not args.no_batch and args.melt_replicas > 0:
