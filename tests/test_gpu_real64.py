@@ -211,6 +211,36 @@ def test_batched_replicas_match_single_double_engines():
         assert np.max(np.abs(dx)) <= RUN_RTOL * float(L.max())
 
 
+def test_batched_bonded_replicas_match_single_double_engines():
+    """Melts with bonds, angles, dihedrals and exclusions (1,200 beads on the
+    cell path, reordering every 2nd build; 216 beads on two cells per axis)
+    beside a pair-only box, batched in binary64 in both modes: after 20 steps
+    positions and the four energies match single binary64 engines."""
+    ws = [W.polymer_melt(n_chains=40, beads_per_chain=30, seed=11, jitter=0.05)
+          .with_(sort_every_rebuild=2).as_float64(jitter=1e-4, seed=1),
+          W.polymer_melt(n_chains=18, beads_per_chain=12, density=7.5, seed=13, jitter=0.05)
+          .as_float64(jitter=1e-4, seed=2),
+          W.lj_box(500, 0.8, 1.0, seed=14).as_float64(jitter=1e-4, seed=3)]
+    steps = 20
+    for config in (1, 3):
+        with Batch(ws, precision="double") as b:
+            b.set_config(config)
+            b.run(steps, observables_on_last=True)
+            got = [(b.state(r), b.stats(r)) for r in range(len(ws))]
+        for w, (g, gs) in zip(ws, got):
+            with Engine(w, precision="double") as e:
+                e.initialize(observables=True)
+                e.advance(steps, observables_on_last=True)
+                st, ss = e.state(), e.stats()
+            assert (gs.build_count, gs.reorder_count) == (ss.build_count, ss.reorder_count)
+            a, c = _by_tag(g), _by_tag(st)
+            L = w.box_bound[1] - w.box_bound[0]
+            dx = (a["positions"] + a["images"] * L) - (c["positions"] + c["images"] * L)
+            assert np.max(np.abs(dx)) <= RUN_RTOL * float(L.max()), (config, w.name)
+            for q in range(4):
+                assert abs(gs.pe[q] - ss.pe[q]) <= RUN_RTOL * max(1.0, abs(ss.pe[q])), (q, w.name)
+
+
 def test_trajectory_and_snapshot_are_binary64_through_the_public_api(tmp_path):
     """BEADS_AMD_PRECISION=double selects _native_real64: the public builders
     validate binary64 arrays and the run writes real size 8 files whose
