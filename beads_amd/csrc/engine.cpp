@@ -373,7 +373,7 @@ void validate_spec(const beads_spec_t& s) {
   constexpr uint64_t kMaxParticles = (uint64_t{1} << 32) / sizeof(real4) - 1;
   require(s.n_particles <= kMaxParticles,
           "system.n_particles must be at most " + std::to_string(kMaxParticles) +
-              " in this build.",
+              " with " + kRealPrecision + "-precision storage.",
           BEADS_E_OVERFLOW);
   require(s.box_bound && s.positions && s.velocities && s.masses && s.types && s.tags &&
               s.molecule_ids && s.images,

@@ -122,7 +122,7 @@ def test_particle_count_limit_of_the_gather_offsets(precision, limit):
     handle = ctypes.c_void_p()
     assert lib.beads_engine_create(ctypes.byref(spec), ctypes.byref(handle)) == 3
     assert lib.beads_last_error().decode() == (
-        f"system.n_particles must be at most {limit} in this build.")
+        f"system.n_particles must be at most {limit} with {precision}-precision storage.")
     assert not handle.value
     del keep
 
